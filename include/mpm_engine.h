@@ -1,0 +1,240 @@
+/*
+ * mpm_engine.h -- C ABI of the MI355X (gfx950) MLS-MPM substep engine.
+ *
+ * The engine replaces the inner stepping of the reference crate's
+ * `Simulation` (core/src/mpm/solver.rs:324-611) and keeps the same state,
+ * ledgers, step semantics and errors.  Particle state lives on the GPU in
+ * f32 structure-of-arrays form between calls; host arrays are borrowed only
+ * for the duration of a call and are f64, laid out as the reference stores
+ * them (one array per field; vectors as 3 doubles, matrices as 9 doubles in
+ * row-major order).
+ *
+ * Status codes: 0 = ok; > 0 = a rejected trial step, numbered like the
+ * reference's StepFailure / ConstitutiveError variants (solver.rs:249-259,
+ * constitutive.rs:41-66); < 0 = SolverError or API misuse (solver.rs:128-142).
+ * No C++ exception crosses this ABI.  One engine per host thread; an engine
+ * is not re-entrant.
+ */
+#ifndef MPM_ENGINE_H
+#define MPM_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+enum {
+    MPM_OK = 0,
+    MPM_FAIL_INVERTED = 1,        /* ConstitutiveError::InvertedDeformation */
+    MPM_FAIL_EIGEN = 2,           /* ConstitutiveError::EigenFailed */
+    MPM_FAIL_STRETCH = 3,         /* ConstitutiveError::DegenerateStretch */
+    MPM_FAIL_RETURN = 4,          /* ConstitutiveError::ReturnNotConverged */
+    MPM_FAIL_NONFINITE = 5,       /* ConstitutiveError::NonFinite */
+    MPM_FAIL_NONFINITE_GRID = 6,  /* StepFailure::NonFiniteGrid */
+    MPM_FAIL_DISPLACEMENT = 7,    /* StepFailure::Displacement */
+    MPM_FAIL_PELLET = 8,          /* StepFailure::PelletNonFinite */
+    MPM_FAIL_TRACTION = 9,        /* StepFailure::NonFiniteTractionEnergy */
+    MPM_ERR_MIN_STEP = -1,        /* SolverError::MinimumStepReached */
+    MPM_ERR_INVALID = -2,         /* SolverError::InvalidState / bad argument */
+    MPM_ERR_DEVICE = -3,          /* HIP runtime failure */
+    MPM_ERR_CAPACITY = -4         /* more particles or pellets than allocated */
+};
+
+/* Material, constitutive.rs:111-131.  kind: 0 paste, 1 liquid.  tait_gamma is
+ * an extension for the liquid: p = (K/g)(J^-g - 1); 1.0 is the reference's
+ * linear law p = K(1/J - 1) (constitutive.rs:574). */
+typedef struct {
+    int kind;
+    double density;
+    double shear_modulus;
+    double lame_lambda;
+    double bulk_modulus;
+    double yield_stress;
+    double consistency;
+    double flow_index;
+    double tait_gamma;
+} mpm_material;
+
+/* SolverConfig, solver.rs:49-62.  face_friction extends the single wall
+ * coefficient to one per face (x-, x+, y-, y+, z-, z+); +inf is a sticky
+ * face, 0 a free-slip face; six copies of wall_friction is the reference. */
+typedef struct {
+    double gravity[3];
+    double wall_friction;
+    double cfl;
+    double max_dt;
+    double min_dt;
+    double contact_stiffness;
+    double contact_restitution;
+    double contact_friction;
+    double face_friction[6];
+} mpm_config;
+
+/* GridLayout::new arguments, grid.rs:37-86. */
+typedef struct {
+    double domain[3];
+    double spacing;
+    size_t max_nodes;
+} mpm_layout;
+
+/* ParticleSet, particles.rs:11-32 (SoA, f64, borrowed). */
+typedef struct {
+    size_t n;
+    uint32_t* id;
+    double* position;       /* n x 3 */
+    double* velocity;       /* n x 3 */
+    double* affine;         /* n x 9 */
+    double* deformation;    /* n x 9 */
+    double* volume_ratio;   /* n */
+    double* kirchhoff;      /* n x 9 */
+    double* plastic_strain; /* n */
+    double* mass;           /* n */
+    double* volume0;        /* n */
+} mpm_particles;
+
+/* Ledger, solver.rs:67-124 (field order as in the reference). */
+typedef struct {
+    double initial_mass;
+    double outflow_mass;
+    double initial_momentum[3];
+    double initial_mechanical_energy;
+    double gravity_impulse[3];
+    double wall_impulse[3];
+    double pellet_wall_impulse[3];
+    double coupling_impulse[3];
+    double coupling_angular_impulse[3];
+    double wall_normal_projection_energy;
+    double wall_friction_dissipation;
+    double wall_normal_traction_energy;
+    double pellet_wall_work;
+    double coupling_grid_energy;
+    double coupling_pellet_energy;
+    double plastic_dissipation;
+    double outflow_momentum[3];
+} mpm_ledger;
+
+/* Integration history, solver.rs:338-351 (min_dt_used = +inf before the first step). */
+typedef struct {
+    double time;
+    uint64_t step_count;
+    uint64_t limited_steps;
+    uint64_t rejected_steps;
+    double dt_scale;
+    double min_dt_used;
+    double max_dt_used;
+} mpm_history;
+
+/* Pellet, rigid.rs:30-49; orientation as the quaternion (i, j, k, w). */
+typedef struct {
+    double radius, length, mass;
+    double inertia_body[3];
+    double position[3];
+    double orientation[4];
+    double velocity[3];
+    double angular_momentum[3];
+    int sphere_count;
+    double sphere_offsets[64];
+} mpm_pellet;
+
+/* One trial step's outcome (solver.rs:593-611). */
+typedef struct {
+    int accepted;
+    int failure;      /* MPM_FAIL_* when rejected */
+    double max_cells; /* largest particle displacement in cells */
+    double dt;
+} mpm_step_report;
+
+/* FixtureSpec, fixtures.rs:94-115 (seeded inputs; jitter is the placement
+ * jitter as a fraction of the sub-cell spacing, 0.1 in the reference). */
+typedef struct {
+    int fixture; /* 0 slump, 1 hydrostatic, 2 dam_break, 3 coupled_patch */
+    mpm_material material;
+    double grid_spacing;
+    double domain[3];
+    double initial_size[3];
+    double initial_velocity[3];
+    uint64_t seed;
+    double gravity_norm;
+    double jitter;
+} mpm_fixture;
+
+typedef struct mpm_engine mpm_engine;
+
+/* ABI version of the loaded library. */
+int mpm_abi_version(void);
+
+/* Material constructors: Material::paste, ::paste_from_shear_rheology and
+ * ::liquid, constitutive.rs:146-223. */
+void mpm_material_paste(double density, double young, double poisson, double yield_stress,
+                        double consistency, double flow_index, mpm_material* out);
+void mpm_material_paste_shear(double density, double young, double poisson, double shear_yield,
+                              double shear_consistency, double flow_index, mpm_material* out);
+void mpm_material_liquid(double density, double young, double poisson, double viscosity,
+                         mpm_material* out);
+
+/* FixtureSpec::particle_count and ::place_particles with SplitMix64,
+ * fixtures.rs:120-133, 272-349 and rng.rs:35-47.  `out` holds caller-allocated
+ * arrays of mpm_fixture_particle_count() entries. */
+size_t mpm_fixture_particle_count(const mpm_fixture* spec);
+int mpm_fixture_place(const mpm_fixture* spec, mpm_particles* out);
+
+/* Simulation::new, solver.rs:357-390: allocate device storage for `capacity`
+ * particles and `max_pellets` pellets on the current HIP device. */
+int mpm_create(const mpm_material* material, const mpm_config* config, const mpm_layout* layout,
+               size_t capacity, size_t max_pellets, mpm_engine** out);
+void mpm_destroy(mpm_engine* engine);
+const char* mpm_last_error(const mpm_engine* engine);
+
+/* Replace the state (restore from a snapshot, research.rs:324-368).  A NULL
+ * ledger seeds it as Simulation::new does; a NULL history starts at t = 0. */
+int mpm_set_state(mpm_engine* engine, const mpm_particles* particles, const mpm_pellet* pellets,
+                  size_t n_pellets, const mpm_ledger* ledger, const mpm_history* history);
+/* Copy the state out (snapshot, research.rs:308-321), in the reference's
+ * particle order.  Arrays are caller-allocated with mpm_particle_count() entries. */
+int mpm_get_state(mpm_engine* engine, mpm_particles* particles, mpm_pellet* pellets,
+                  size_t pellet_capacity, mpm_ledger* ledger, mpm_history* history);
+size_t mpm_particle_count(const mpm_engine* engine);
+size_t mpm_pellet_count(const mpm_engine* engine);
+int mpm_set_config(mpm_engine* engine, const mpm_config* config);
+
+/* Simulation::try_step, solver.rs:535-611: one trial step of size dt;
+ * commits only if every check passes.  Returns 0 or the failure code. */
+int mpm_try_step(mpm_engine* engine, double dt, mpm_step_report* report);
+/* Simulation::advance_while / advance_to, solver.rs:402-472.  keep_going may
+ * be NULL; *reached is 0 when it stopped the loop at a resumable boundary. */
+int mpm_advance_while(mpm_engine* engine, double target, int (*keep_going)(void*), void* ctx,
+                      int* reached);
+int mpm_advance_to(mpm_engine* engine, double target);
+/* Simulation::stability_limit, solver.rs:477-498. */
+double mpm_stability_limit(mpm_engine* engine);
+
+/* Residual and energy diagnostics, solver.rs:1002-1099. */
+double mpm_mass_residual(mpm_engine* engine);
+int mpm_momentum_residual(mpm_engine* engine, double* out3);
+int mpm_energies(mpm_engine* engine, double* kinetic, double* potential, double* elastic);
+
+/* Batched constitutive update on the device for parity tests: update_paste
+ * (constitutive.rs:456-531) for n trial deformation gradients (n x 9, f64 in
+ * and out), or update_liquid (557-589) for n (J, C) pairs.  codes[i] is 0 or
+ * the failure code of particle i. */
+int mpm_paste_update_batch(const mpm_material* material, double dt, size_t n, const double* trial,
+                           double* deformation, double* kirchhoff, double* increment,
+                           double* dissipation_density, int* codes);
+int mpm_liquid_update_batch(const mpm_material* material, double dt, size_t n,
+                            const double* volume_ratio, const double* gradient, double* new_ratio,
+                            double* kirchhoff, int* codes);
+
+/* Per-kernel timing of the last mpm_advance_* call, for the benchmark: the
+ * average device time of the fused grid-to-particle-to-grid kernel in ms and
+ * its launch count. */
+int mpm_profile_g2p2g(mpm_engine* engine, double* avg_ms, uint64_t* launches);
+void mpm_enable_profiling(mpm_engine* engine, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

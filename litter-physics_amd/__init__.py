@@ -1,0 +1,470 @@
+"""MI355X (gfx950) MLS-MPM substep engine: Python host binding.
+
+`Simulation` mirrors the reference crate's `Simulation` (core/src/mpm/solver.rs:324-1140):
+`advance_to`, `advance_while`, `try_step`, `stability_limit`, the residual and energy
+getters and the public state.  Every call goes through the C ABI of
+include/mpm_engine.h into libmpm_amd.so; the particle and grid work runs only in the
+HIP kernels.  There is no CPU path: without the built library or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpm_amd.so")
+
+dp = C.POINTER(C.c_double)
+u32p = C.POINTER(C.c_uint32)
+
+
+class MpmError(RuntimeError):
+    """A SolverError (negative status) or API misuse."""
+
+
+class StepRejected(RuntimeError):
+    """A trial step failed (StepFailure, positive status)."""
+
+    def __init__(self, code):
+        super().__init__(FAILURES.get(code, str(code)))
+        self.code = code
+
+
+FAILURES = {1: "inverted deformation", 2: "eigen-decomposition failed", 3: "degenerate stretch",
+            4: "return not converged", 5: "non-finite constitutive quantity", 6: "non-finite grid velocity",
+            7: "displacement over one cell", 8: "pellet non-finite", 9: "non-finite traction energy"}
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_int), ("density", C.c_double), ("shear_modulus", C.c_double),
+                ("lame_lambda", C.c_double), ("bulk_modulus", C.c_double), ("yield_stress", C.c_double),
+                ("consistency", C.c_double), ("flow_index", C.c_double), ("tait_gamma", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("gravity", C.c_double * 3), ("wall_friction", C.c_double), ("cfl", C.c_double),
+                ("max_dt", C.c_double), ("min_dt", C.c_double), ("contact_stiffness", C.c_double),
+                ("contact_restitution", C.c_double), ("contact_friction", C.c_double),
+                ("face_friction", C.c_double * 6)]
+
+
+class Layout(C.Structure):
+    _fields_ = [("domain", C.c_double * 3), ("spacing", C.c_double), ("max_nodes", C.c_size_t)]
+
+
+class Particles(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("id", u32p), ("position", dp), ("velocity", dp), ("affine", dp),
+                ("deformation", dp), ("volume_ratio", dp), ("kirchhoff", dp), ("plastic_strain", dp),
+                ("mass", dp), ("volume0", dp)]
+
+
+class Ledger(C.Structure):
+    _fields_ = [("initial_mass", C.c_double), ("outflow_mass", C.c_double),
+                ("initial_momentum", C.c_double * 3), ("initial_mechanical_energy", C.c_double),
+                ("gravity_impulse", C.c_double * 3), ("wall_impulse", C.c_double * 3),
+                ("pellet_wall_impulse", C.c_double * 3), ("coupling_impulse", C.c_double * 3),
+                ("coupling_angular_impulse", C.c_double * 3),
+                ("wall_normal_projection_energy", C.c_double), ("wall_friction_dissipation", C.c_double),
+                ("wall_normal_traction_energy", C.c_double), ("pellet_wall_work", C.c_double),
+                ("coupling_grid_energy", C.c_double), ("coupling_pellet_energy", C.c_double),
+                ("plastic_dissipation", C.c_double), ("outflow_momentum", C.c_double * 3)]
+
+    def ledgered_energy(self):
+        """Ledger::ledgered_energy, solver.rs:168-175."""
+        return (self.wall_normal_projection_energy - self.wall_friction_dissipation
+                + self.wall_normal_traction_energy + self.pellet_wall_work + self.coupling_grid_energy
+                + self.coupling_pellet_energy - self.plastic_dissipation)
+
+
+class History(C.Structure):
+    _fields_ = [("time", C.c_double), ("step_count", C.c_uint64), ("limited_steps", C.c_uint64),
+                ("rejected_steps", C.c_uint64), ("dt_scale", C.c_double), ("min_dt_used", C.c_double),
+                ("max_dt_used", C.c_double)]
+
+
+class Pellet(C.Structure):
+    _fields_ = [("radius", C.c_double), ("length", C.c_double), ("mass", C.c_double),
+                ("inertia_body", C.c_double * 3), ("position", C.c_double * 3),
+                ("orientation", C.c_double * 4), ("velocity", C.c_double * 3),
+                ("angular_momentum", C.c_double * 3), ("sphere_count", C.c_int),
+                ("sphere_offsets", C.c_double * 64)]
+
+
+class StepReport(C.Structure):
+    _fields_ = [("accepted", C.c_int), ("failure", C.c_int), ("max_cells", C.c_double), ("dt", C.c_double)]
+
+
+class FixtureSpec(C.Structure):
+    _fields_ = [("fixture", C.c_int), ("material", Material), ("grid_spacing", C.c_double),
+                ("domain", C.c_double * 3), ("initial_size", C.c_double * 3),
+                ("initial_velocity", C.c_double * 3), ("seed", C.c_uint64), ("gravity_norm", C.c_double),
+                ("jitter", C.c_double)]
+
+
+KEEP_GOING = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+_lib = None
+
+
+def build(verbose=False):
+    from importlib import util
+    spec = util.spec_from_file_location("_mpm_build", os.path.join(HERE, "build.py"))
+    mod = util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.build(verbose=verbose)
+
+
+def lib():
+    """Load libmpm_amd.so (built by build.py); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MpmError(f"{LIB_PATH} is not built; run litter-physics_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "mpm_abi_version": (C.c_int, []),
+        "mpm_material_paste": (None, [C.c_double] * 6 + [C.POINTER(Material)]),
+        "mpm_material_paste_shear": (None, [C.c_double] * 6 + [C.POINTER(Material)]),
+        "mpm_material_liquid": (None, [C.c_double] * 4 + [C.POINTER(Material)]),
+        "mpm_fixture_particle_count": (C.c_size_t, [C.POINTER(FixtureSpec)]),
+        "mpm_fixture_place": (C.c_int, [C.POINTER(FixtureSpec), C.POINTER(Particles)]),
+        "mpm_create": (C.c_int, [C.POINTER(Material), C.POINTER(Config), C.POINTER(Layout), C.c_size_t,
+                                 C.c_size_t, C.POINTER(C.c_void_p)]),
+        "mpm_destroy": (None, [C.c_void_p]),
+        "mpm_last_error": (C.c_char_p, [C.c_void_p]),
+        "mpm_set_state": (C.c_int, [C.c_void_p, C.POINTER(Particles), C.POINTER(Pellet), C.c_size_t,
+                                    C.POINTER(Ledger), C.POINTER(History)]),
+        "mpm_get_state": (C.c_int, [C.c_void_p, C.POINTER(Particles), C.POINTER(Pellet), C.c_size_t,
+                                    C.POINTER(Ledger), C.POINTER(History)]),
+        "mpm_particle_count": (C.c_size_t, [C.c_void_p]),
+        "mpm_pellet_count": (C.c_size_t, [C.c_void_p]),
+        "mpm_set_config": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
+        "mpm_try_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(StepReport)]),
+        "mpm_advance_while": (C.c_int, [C.c_void_p, C.c_double, KEEP_GOING, C.c_void_p, C.POINTER(C.c_int)]),
+        "mpm_advance_to": (C.c_int, [C.c_void_p, C.c_double]),
+        "mpm_stability_limit": (C.c_double, [C.c_void_p]),
+        "mpm_mass_residual": (C.c_double, [C.c_void_p]),
+        "mpm_momentum_residual": (C.c_int, [C.c_void_p, dp]),
+        "mpm_energies": (C.c_int, [C.c_void_p, dp, dp, dp]),
+        "mpm_paste_update_batch": (C.c_int, [C.POINTER(Material), C.c_double, C.c_size_t, dp, dp, dp, dp, dp,
+                                             C.POINTER(C.c_int)]),
+        "mpm_liquid_update_batch": (C.c_int, [C.POINTER(Material), C.c_double, C.c_size_t, dp, dp, dp, dp,
+                                              C.POINTER(C.c_int)]),
+        "mpm_profile_g2p2g": (C.c_int, [C.c_void_p, dp, C.POINTER(C.c_uint64)]),
+        "mpm_enable_profiling": (None, [C.c_void_p, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = [
+    "mpm_abi_version", "mpm_material_paste", "mpm_material_paste_shear", "mpm_material_liquid",
+    "mpm_fixture_particle_count", "mpm_fixture_place", "mpm_create", "mpm_destroy", "mpm_last_error",
+    "mpm_set_state", "mpm_get_state", "mpm_particle_count", "mpm_pellet_count", "mpm_set_config",
+    "mpm_try_step", "mpm_advance_while", "mpm_advance_to", "mpm_stability_limit", "mpm_mass_residual",
+    "mpm_momentum_residual", "mpm_energies", "mpm_paste_update_batch", "mpm_liquid_update_batch",
+    "mpm_profile_g2p2g", "mpm_enable_profiling",
+]
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(dp)
+    if a.dtype == np.uint32:
+        return a.ctypes.data_as(u32p)
+    raise TypeError(a.dtype)
+
+
+# ------------------------------------------------------------------ inputs
+
+def paste(density, young, poisson, yield_stress, consistency, flow_index, shear_units=False):
+    """Material::paste / ::paste_from_shear_rheology, constitutive.rs:146-197."""
+    m = Material()
+    fn = lib().mpm_material_paste_shear if shear_units else lib().mpm_material_paste
+    fn(density, young, poisson, yield_stress, consistency, flow_index, C.byref(m))
+    return m
+
+
+def liquid(density, young, poisson, viscosity, tait_gamma=1.0):
+    """Material::liquid, constitutive.rs:211-223, with the Tait exponent."""
+    m = Material()
+    lib().mpm_material_liquid(density, young, poisson, viscosity, C.byref(m))
+    m.tait_gamma = tait_gamma
+    return m
+
+
+def config(gravity=(0.0, 0.0, -9.81), wall_friction=0.4, cfl=0.3, max_dt=1e-3, min_dt=1e-9, stiffness=1000.0,
+           restitution=0.2, contact_friction=0.4, face_friction=None):
+    """SolverConfig, solver.rs:49-62 (face_friction: x-, x+, y-, y+, z-, z+)."""
+    c = Config()
+    c.gravity[:] = list(gravity)
+    c.wall_friction = wall_friction
+    c.cfl = cfl
+    c.max_dt = max_dt
+    c.min_dt = min_dt
+    c.contact_stiffness = stiffness
+    c.contact_restitution = restitution
+    c.contact_friction = contact_friction
+    c.face_friction[:] = [wall_friction] * 6 if face_friction is None else list(face_friction)
+    return c
+
+
+@dataclass
+class State:
+    """ParticleSet (particles.rs:11-32) as f64 numpy arrays."""
+    id: np.ndarray
+    pos: np.ndarray
+    vel: np.ndarray
+    affine: np.ndarray
+    deformation: np.ndarray
+    volume_ratio: np.ndarray
+    kirchhoff: np.ndarray
+    plastic: np.ndarray
+    mass: np.ndarray
+    volume0: np.ndarray
+
+    @classmethod
+    def empty(cls, n):
+        return cls(np.zeros(n, np.uint32), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3)),
+                   np.zeros((n, 3, 3)), np.zeros(n), np.zeros((n, 3, 3)), np.zeros(n), np.zeros(n), np.zeros(n))
+
+    def __len__(self):
+        return self.pos.shape[0]
+
+    def to_c(self):
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (self.pos, self.vel, self.affine, self.deformation, self.volume_ratio, self.kirchhoff,
+                       self.plastic, self.mass, self.volume0)]
+        self._ids = np.ascontiguousarray(self.id, dtype=np.uint32)
+        k = self._keep
+        return Particles(len(self), _p(self._ids), _p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), _p(k[5]),
+                         _p(k[6]), _p(k[7]), _p(k[8]))
+
+    def by_id(self):
+        """Copy sorted by particle id (for comparisons across engines)."""
+        o = np.argsort(self.id, kind="stable")
+        return State(*(getattr(self, f)[o] for f in
+                       ("id", "pos", "vel", "affine", "deformation", "volume_ratio", "kirchhoff", "plastic",
+                        "mass", "volume0")))
+
+
+FIXTURES = {"slump": 0, "hydrostatic": 1, "dam_break": 2, "coupled_patch": 3}
+
+
+def place_particles(kind, material, spacing, domain, initial_size, initial_velocity=(0.0, 0.0, 0.0), seed=0,
+                    jitter=0.1, gravity=9.81) -> State:
+    """FixtureSpec::place_particles with SplitMix64 jitter (fixtures.rs:272-349)."""
+    f = FixtureSpec()
+    f.fixture = FIXTURES[kind]
+    f.material = material
+    f.grid_spacing = spacing
+    f.domain[:] = list(domain)
+    f.initial_size[:] = list(initial_size)
+    f.initial_velocity[:] = list(initial_velocity)
+    f.seed = seed
+    f.gravity_norm = gravity
+    f.jitter = jitter
+    n = lib().mpm_fixture_particle_count(C.byref(f))
+    st = State.empty(n)
+    cp = st.to_c()
+    rc = lib().mpm_fixture_place(C.byref(f), C.byref(cp))
+    if rc != 0:
+        raise MpmError(f"fixture placement failed ({rc})")
+    for name, arr in zip(("pos", "vel", "affine", "deformation", "volume_ratio", "kirchhoff", "plastic", "mass",
+                          "volume0"), st._keep):
+        setattr(st, name, arr)
+    st.id = st._ids
+    return st
+
+
+def new_pellet(radius, length, density, position, orientation=(0.0, 0.0, 0.0, 1.0)):
+    """Pellet::new, rigid.rs:115-182 (multisphere proxy of the cylinder)."""
+    if not (radius > 0 and length >= 2 * radius and density > 0):
+        raise ValueError("invalid pellet geometry")
+    p = Pellet()
+    mass = density * np.pi * radius * radius * length
+    p.radius, p.length, p.mass = radius, length, mass
+    perp = mass * (3.0 * radius * radius + length * length) / 12.0
+    p.inertia_body[:] = [0.5 * mass * radius * radius, perp, perp]
+    p.position[:] = list(position)
+    p.orientation[:] = list(orientation)
+    half = 0.5 * length - radius
+    count = 1 if half <= 0 else int(np.ceil(2.0 * half / radius)) + 1
+    if count > 64:
+        raise ValueError("pellet proxy exceeds the sphere cap")
+    p.sphere_count = count
+    for i in range(count):
+        p.sphere_offsets[i] = 0.0 if count == 1 else -half + 2.0 * half * i / (count - 1)
+    return p
+
+
+# -------------------------------------------------------------- simulation
+
+class Simulation:
+    """Device-resident MLS-MPM state behind the reference's step API."""
+
+    def __init__(self, material, cfg, domain, spacing, state: State, pellets=(), capacity=None, max_pellets=None,
+                 max_nodes=1 << 40, ledger=None, history=None):
+        L = lib()
+        layout = Layout()
+        layout.domain[:] = list(domain)
+        layout.spacing = spacing
+        layout.max_nodes = max_nodes
+        self.material = material
+        self.cfg = cfg
+        self.domain = tuple(domain)
+        self.spacing = spacing
+        cap = max(len(state), 1) if capacity is None else capacity
+        mp = len(pellets) if max_pellets is None else max_pellets
+        h = C.c_void_p()
+        rc = L.mpm_create(C.byref(material), C.byref(cfg), C.byref(layout), cap, mp, C.byref(h))
+        if rc != 0:
+            raise MpmError(f"mpm_create failed ({rc})")
+        self.h = h
+        self.set_state(state, pellets, ledger, history)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.mpm_destroy(h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc < 0:
+            raise MpmError(lib().mpm_last_error(self.h).decode())
+        return rc
+
+    def set_state(self, state: State, pellets=(), ledger=None, history=None):
+        cp = state.to_c()
+        arr = (Pellet * max(len(pellets), 1))(*pellets) if pellets else None
+        self._check(lib().mpm_set_state(self.h, C.byref(cp), arr, len(pellets),
+                                        C.byref(ledger) if ledger is not None else None,
+                                        C.byref(history) if history is not None else None))
+
+    def set_config(self, cfg):
+        self.cfg = cfg
+        self._check(lib().mpm_set_config(self.h, C.byref(cfg)))
+
+    def __len__(self):
+        return lib().mpm_particle_count(self.h)
+
+    def state(self) -> State:
+        st = State.empty(len(self))
+        cp = st.to_c()
+        self._check(lib().mpm_get_state(self.h, C.byref(cp), None, 0, None, None))
+        for name, arr in zip(("pos", "vel", "affine", "deformation", "volume_ratio", "kirchhoff", "plastic",
+                              "mass", "volume0"), st._keep):
+            setattr(st, name, arr)
+        st.id = st._ids
+        return st
+
+    def pellets(self):
+        n = lib().mpm_pellet_count(self.h)
+        arr = (Pellet * max(n, 1))()
+        self._check(lib().mpm_get_state(self.h, None, arr, n, None, None))
+        return [arr[i] for i in range(n)]
+
+    def ledger(self) -> Ledger:
+        led = Ledger()
+        self._check(lib().mpm_get_state(self.h, None, None, 0, C.byref(led), None))
+        return led
+
+    def history(self) -> History:
+        hist = History()
+        self._check(lib().mpm_get_state(self.h, None, None, 0, None, C.byref(hist)))
+        return hist
+
+    @property
+    def time(self):
+        return self.history().time
+
+    def try_step(self, dt) -> StepReport:
+        """Simulation::try_step (solver.rs:535-611): commit or leave the state untouched."""
+        rep = StepReport()
+        self._check(lib().mpm_try_step(self.h, dt, C.byref(rep)))
+        return rep
+
+    def advance_to(self, target):
+        """Simulation::advance_to, solver.rs:402-404."""
+        self._check(lib().mpm_advance_to(self.h, target))
+
+    def advance_while(self, target, should_continue) -> bool:
+        """Simulation::advance_while, solver.rs:413-472; False at a resumable budget stop."""
+        cb = KEEP_GOING(lambda _ctx: 1 if should_continue() else 0)
+        reached = C.c_int()
+        self._check(lib().mpm_advance_while(self.h, target, cb, None, C.byref(reached)))
+        return bool(reached.value)
+
+    def stability_limit(self):
+        return lib().mpm_stability_limit(self.h)
+
+    def mass_residual(self):
+        return lib().mpm_mass_residual(self.h)
+
+    def momentum_residual(self):
+        out = np.zeros(3)
+        self._check(lib().mpm_momentum_residual(self.h, _p(out)))
+        return out
+
+    def energies(self):
+        k, p, e = C.c_double(), C.c_double(), C.c_double()
+        self._check(lib().mpm_energies(self.h, C.byref(k), C.byref(p), C.byref(e)))
+        return k.value, p.value, e.value
+
+    def mechanical_energy(self):
+        return sum(self.energies())
+
+    def energy_residual(self):
+        """Simulation::energy_residual, solver.rs:1095-1099."""
+        led = self.ledger()
+        return self.mechanical_energy() - led.initial_mechanical_energy - led.ledgered_energy()
+
+    def enable_profiling(self, on=True):
+        lib().mpm_enable_profiling(self.h, 1 if on else 0)
+
+    def g2p2g_profile(self):
+        ms = C.c_double()
+        n = C.c_uint64()
+        self._check(lib().mpm_profile_g2p2g(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def paste_update_batch(material, dt, trial):
+    """update_paste (constitutive.rs:456-531) on the device for a batch of trial F."""
+    trial = np.ascontiguousarray(trial, dtype=np.float64).reshape(-1, 9)
+    n = trial.shape[0]
+    f = np.zeros((n, 9))
+    tau = np.zeros((n, 9))
+    inc = np.zeros(n)
+    dd = np.zeros(n)
+    codes = np.zeros(n, np.int32)
+    rc = lib().mpm_paste_update_batch(C.byref(material), dt, n, _p(trial), _p(f), _p(tau), _p(inc), _p(dd),
+                                      codes.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        raise MpmError(f"paste batch failed ({rc})")
+    return f.reshape(n, 3, 3), tau.reshape(n, 3, 3), inc, dd, codes
+
+
+def liquid_update_batch(material, dt, volume_ratio, gradient):
+    """update_liquid (constitutive.rs:557-589) on the device for a batch of (J, C)."""
+    j = np.ascontiguousarray(volume_ratio, dtype=np.float64).reshape(-1)
+    g = np.ascontiguousarray(gradient, dtype=np.float64).reshape(-1, 9)
+    n = j.shape[0]
+    nj = np.zeros(n)
+    tau = np.zeros((n, 9))
+    codes = np.zeros(n, np.int32)
+    rc = lib().mpm_liquid_update_batch(C.byref(material), dt, n, _p(j), _p(g), _p(nj), _p(tau),
+                                       codes.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        raise MpmError(f"liquid batch failed ({rc})")
+    return nj, tau.reshape(n, 3, 3), codes

@@ -1,0 +1,252 @@
+"""Parity of the HIP engine against the f64 oracle, through the C ABI.
+
+Tolerances (stated per test) follow the engine's contract: positions and
+velocities within 1e-4 relative (normwise: largest error over the largest
+magnitude) after 100 substeps, stress and plastic dissipation within 1e-3,
+mass and momentum ledgers to f32 round-off.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def as_engine(E, obj, cls):
+    out = cls()
+    for name, _ in obj._fields_:
+        val = getattr(obj, name)
+        if hasattr(val, "__len__"):
+            getattr(out, name)[:] = list(val)
+        else:
+            setattr(out, name, val)
+    return out
+
+
+def engine_from_oracle(E, osim, cfg=None, pellets=True):
+    st = osim.state()
+    est = E.State(st.id.copy(), st.pos, st.vel, st.affine, st.deformation, st.volume_ratio, st.kirchhoff,
+                  st.plastic, st.mass, st.volume0)
+    m = as_engine(E, osim_material(osim), E.Material)
+    c = as_engine(E, cfg if cfg is not None else osim.config(), E.Config)
+    spacing, cells = osim.layout()
+    domain = [cells[a] * spacing for a in range(3)]
+    pl = [as_engine(E, p, E.Pellet) for p in osim.pellets()] if pellets else []
+    return E.Simulation(m, c, domain, spacing, est, pellets=pl)
+
+
+_materials = {}
+
+
+def osim_material(osim):
+    return _materials[id(osim)]
+
+
+def build_oracle(spec):
+    s = O.Sim.from_fixture(spec)
+    _materials[id(s)] = spec.material
+    return s
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def compare_states(gpu, ref, pos_tol=1e-4, vel_tol=1e-4, stress_tol=1e-3):
+    g = gpu.state().by_id()
+    r = ref.state()
+    o = np.argsort(r.id)
+    assert np.array_equal(g.id, r.id[o])
+    ep = rel(g.pos, r.pos[o])
+    ev = rel(g.vel, r.vel[o])
+    et = rel(g.kirchhoff, r.kirchhoff[o])
+    print(f"parity: pos {ep:.2e} vel {ev:.2e} stress {et:.2e}", flush=True)
+    assert ep < pos_tol, ep
+    assert ev < vel_tol, ev
+    assert et < stress_tol, et
+    return g, r, o
+
+
+# ---------------------------------------------------------------- kernels
+
+def test_paste_batch_matches_oracle(engine):
+    rng = np.random.default_rng(3)
+    m = O.paste(1200.0, 138461.538, 0.384615, 50.0, 5.0, 0.5, shear_units=True)
+    em = as_engine(engine, m, engine.Material)
+    n = 4096
+    scale = np.where(rng.random(n) < 0.5, 2e-4, 4e-3)[:, None, None]
+    trial = np.eye(3)[None] + rng.standard_normal((n, 3, 3)) * scale
+    trial[0] = np.eye(3) * 0.99  # hydrostatic compression
+    dt = 6.7e-5
+    f, tau, inc, dd, codes = engine.paste_update_batch(em, dt, trial)
+    ref_f = np.zeros_like(f)
+    ref_t = np.zeros_like(tau)
+    ref_i = np.zeros(n)
+    for p in range(n):
+        rc, rf, rt, ri, rdd, req = O.update_paste(trial[p], m, dt)
+        assert rc == codes[p] == 0
+        ref_f[p], ref_t[p], ref_i[p] = rf, rt, ri
+    yielded = ref_i > 0
+    assert 0.2 < yielded.mean() < 0.8
+    # Stress: each particle within 1e-3 of its own stress scale (or of the yield stress).
+    denom = np.maximum(np.abs(ref_t).reshape(n, -1).max(1), m.yield_stress)
+    err = np.abs(tau - ref_t).reshape(n, -1).max(1) / denom
+    print("paste batch stress err", err.max())
+    assert err.max() < 1e-3
+    assert np.abs(f - ref_f).max() < 2e-6
+    ierr = np.abs(inc - ref_i)[yielded] / ref_i[yielded]
+    assert ierr.max() < 1e-3
+
+
+def test_manuscript_return_on_device(engine):
+    # core/tests/formal_examples.rs:12-26 through the f32 kernel: q = 78.0059568837,
+    # plastic increment 0.001595234493, each within 1e-6 relative.
+    m = engine.paste(1000.0, 1000.0, 0.2, 30.0, 10.0, 0.6, shear_units=True)
+    strain = 80.0 / (2.0 * math.sqrt(3.0) * m.shear_modulus)
+    trial = np.diag([math.exp(strain), math.exp(-strain), 1.0])[None]
+    f, tau, inc, dd, codes = engine.paste_update_batch(m, 0.0014, trial)
+    assert codes[0] == 0
+    dev = tau[0] - np.eye(3) * np.trace(tau[0]) / 3.0
+    q = math.sqrt(1.5) * np.linalg.norm(dev)
+    assert abs(q - 78.0059568837) / 78.0059568837 < 1e-5
+    assert abs(inc[0] - 0.001595234493) / 0.001595234493 < 1e-5
+    assert abs(np.linalg.det(f[0]) - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("gamma", [1.0, 7.0])
+def test_liquid_batch_matches_oracle(engine, gamma):
+    rng = np.random.default_rng(5)
+    m = O.liquid(1000.0, 3.6e5, 0.2, 1e-3, tait_gamma=gamma)
+    em = as_engine(engine, m, engine.Material)
+    n = 2048
+    j = 1.0 + rng.standard_normal(n) * 2e-3
+    grad = rng.standard_normal((n, 3, 3)) * 5.0
+    dt = 3e-5
+    nj, tau, codes = engine.liquid_update_batch(em, dt, j, grad)
+    for p in range(0, n, 7):
+        rc, rj, rt, rp = O.update_liquid(j[p], grad[p], m, dt)
+        assert rc == codes[p] == 0
+        assert abs(nj[p] - rj) < 1e-7
+        assert np.abs(tau[p] - rt).max() < 1e-3 * max(np.abs(rt).max(), 1.0)
+
+
+# ---------------------------------------------------------------- substeps
+
+def fixed_dt_run(E, spec, steps, dt=None):
+    osim = build_oracle(spec)
+    gsim = engine_from_oracle(E, osim)
+    if dt is None:
+        dt = min(osim.stability_limit(), spec.config.max_dt)
+    for k in range(steps):
+        rc = osim.try_step(dt)
+        rep = gsim.try_step(dt)
+        assert rc == 0 and rep.accepted == 1, (k, rc, rep.failure)
+    return osim, gsim, dt
+
+
+def test_slump_cfg0_parity_100_substeps(engine):
+    # 49,152-particle Herschel-Bulkley column (32^3 grid, dx 2.5 mm), 100 substeps.
+    spec = cases.slump_cfg0()
+    osim, gsim, dt = fixed_dt_run(engine, spec, 100)
+    g, r, o = compare_states(gsim, osim)
+    lo, lg = osim.ledger(), gsim.ledger()
+    assert lo.plastic_dissipation > 0.0
+    assert abs(lg.plastic_dissipation - lo.plastic_dissipation) / lo.plastic_dissipation < 1e-3
+    assert abs(gsim.mass_residual()) <= 1e-7 * lo.initial_mass
+    scale = np.linalg.norm(lo.gravity_impulse)
+    assert np.linalg.norm(gsim.momentum_residual()) < 1e-5 * scale
+    for a in range(3):
+        assert abs(lg.wall_impulse[a] - lo.wall_impulse[a]) < 1e-3 * scale
+    # Plastic strain of yielded particles.
+    eps_err = np.abs(g.plastic - r.plastic[o]).max() / max(r.plastic.max(), 1e-30)
+    assert eps_err < 1e-3
+
+
+def test_hydrostatic_column_matches_golden(engine):
+    # docs/hydrostatic-balance.md:26-31: max 0.011513856585915002, rms 0.0034430221485745437
+    spec = cases.water_column([0.02] * 3, 0.01, 0.002, 5e-5)
+    osim = build_oracle(spec)
+    gsim = engine_from_oracle(engine, osim)
+    gsim.advance_to(0.01)
+    h = gsim.history()
+    assert h.step_count == 200 and h.rejected_steps == 0 and h.limited_steps == 0
+    st = gsim.state()
+    e = cases.hydrostatic_errors(cases.water(), st.pos, st.volume_ratio, 0.01)
+    mx, rms = np.abs(e).max(), math.sqrt(np.mean(e * e))
+    print(f"hydrostatic gpu max {mx!r} rms {rms!r}")
+    assert abs(mx - 0.011513856585915002) < 2e-4
+    assert abs(rms - 0.0034430221485745437) < 5e-5
+    assert np.abs(e[st.pos[:, 2] < 0.002]).max() < 2e-4
+    led = gsim.ledger()
+    assert abs(gsim.mass_residual()) <= 1e-7 * led.initial_mass
+    assert np.linalg.norm(gsim.momentum_residual()) < 1e-5 * np.linalg.norm(led.gravity_impulse)
+    assert abs(led.wall_normal_traction_energy - 1.0219846729e-8) / 1.0219846729e-8 < 2e-2
+
+
+def test_adaptive_advance_matches_oracle(engine):
+    spec = cases.slump_cfg0(max_dt=1e-3)
+    osim = build_oracle(spec)
+    gsim = engine_from_oracle(engine, osim)
+    osim.advance_to(4e-3)
+    gsim.advance_to(4e-3)
+    ho, hg = osim.history(), gsim.history()
+    assert hg.step_count == ho.step_count and hg.rejected_steps == ho.rejected_steps
+    assert abs(hg.min_dt_used - ho.min_dt_used) / ho.min_dt_used < 1e-4
+    compare_states(gsim, osim)
+
+
+def test_rejected_step_leaves_state_untouched(engine):
+    spec = cases.slump_cfg0()
+    osim = build_oracle(spec)
+    gsim = engine_from_oracle(engine, osim)
+    gsim.advance_to(1e-3)
+    before = gsim.state()
+    led = bytes(gsim.ledger())
+    rep = gsim.try_step(0.05)
+    assert rep.accepted == 0 and rep.failure in (1, 2, 3, 4, 5, 7)
+    after = gsim.state()
+    assert bytes(gsim.ledger()) == led
+    assert np.array_equal(after.pos, before.pos) and np.array_equal(after.kirchhoff, before.kirchhoff)
+    # The engine recovers: the next regular step is accepted.
+    gsim.advance_to(1.5e-3)
+
+
+@pytest.mark.parametrize("gamma", [1.0, 7.0])
+def test_dam_break_liquid_parity(engine, gamma):
+    spec = cases.dam_break_cfg1(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=gamma)
+    osim, gsim, dt = fixed_dt_run(engine, spec, 100)
+    g, r, o = compare_states(gsim, osim)
+    assert np.abs(g.volume_ratio - r.volume_ratio[o]).max() < 1e-5
+
+
+def test_coupled_patch_pellet_parity(engine):
+    spec, _, _ = cases.research_coupled_patch()
+    osim, gsim, dt = fixed_dt_run(engine, spec, 100, dt=5e-5)
+    compare_states(gsim, osim)
+    po, pg = osim.pellets()[0], gsim.pellets()[0]
+    assert rel(np.array(pg.position[:]), np.array(po.position[:])) < 1e-5
+    vo = np.array(po.velocity[:])
+    assert np.abs(np.array(pg.velocity[:]) - vo).max() < 1e-3 * max(np.abs(vo).max(), 1e-3)
+    lo, lg = osim.ledger(), gsim.ledger()
+    ci = np.array(lo.coupling_impulse[:])
+    assert np.abs(np.array(lg.coupling_impulse[:]) - ci).max() < 2e-3 * np.abs(ci).max()
+
+
+def test_sorted_order_follows_reference_key(engine):
+    spec = cases.slump_cfg0()
+    osim = build_oracle(spec)
+    gsim = engine_from_oracle(engine, osim)
+    dt = 0.5 * osim.stability_limit()
+    cfg = osim.config()
+    cfg.max_dt = dt
+    osim.set_config(cfg)
+    gsim.set_config(as_engine(engine, cfg, engine.Config))
+    osim.advance_to(32 * dt)
+    gsim.advance_to(32 * dt)
+    assert gsim.history().step_count == 32
+    same = np.mean(gsim.state().id == osim.state().id)
+    assert same > 0.99, same
