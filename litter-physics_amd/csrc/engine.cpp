@@ -423,23 +423,25 @@ struct mpm_engine {
             PelletDev& d = dev[k];
             V3 ax = p.axis(), om = p.omega(), lo, hi;
             p.bounding_box(lo, hi);
-            double margin = 1e-6 * dx;
+            double margin = 0.0;
             for (int a = 0; a < 3; ++a) {
-                d.pos[a] = (float)p.pos[a];
-                d.axis[a] = (float)ax[a];
-                d.vel[a] = (float)p.vel[a];
-                d.omega[a] = (float)om[a];
-                d.lo[a] = (float)(lo[a] - margin);
-                d.hi[a] = (float)(hi[a] + margin);
+                d.pos[a] = p.pos[a];
+                d.axis[a] = ax[a];
+                d.vel[a] = p.vel[a];
+                d.omega[a] = om[a];
+                d.lo[a] = lo[a];
+                d.hi[a] = hi[a];
             }
-            d.radius = (float)p.radius;
+            d.radius = p.radius;
             d.spheres = (int)p.offsets.size();
-            d.half_span = 0.0f;
-            for (size_t s = 0; s < p.offsets.size() && s < (size_t)kMaxSpheres; ++s) d.offsets[s] = (float)p.offsets[s];
+            d.pad = 0;
+            for (size_t s = 0; s < p.offsets.size() && s < (size_t)kMaxSpheres; ++s) d.offsets[s] = p.offsets[s];
             int b0[3], b1[3];
             bool empty = false;
             for (int a = 0; a < 3; ++a) {
-                double i0 = std::ceil((lo[a] - margin) / dx + 1.0), i1 = std::floor((hi[a] + margin) / dx + 1.0);
+                // Candidate node coordinates with one node of slack (the kernel re-tests the box exactly).
+                double i0 = std::floor((lo[a] - margin) / dx + 1.0) - 1.0,
+                       i1 = std::ceil((hi[a] + margin) / dx + 1.0) + 1.0;
                 i0 = std::max(i0, 0.0);
                 i1 = std::min(i1, (double)(cells[a] + 2));
                 if (!(i0 <= i1)) empty = true;
@@ -599,8 +601,8 @@ int mpm_engine::try_step(double dt, mpm_step_report* report) {
     if (!pellets.empty()) {
         int rc = upload_pellets();
         if (rc) return rc;
-        launch_couple(stream, P, nblocks, grid[gcur], gv, na, d_pel, (int)pellets.size(), d_bstart, d_blist, d_acc,
-                      d_st, (float)cfg.wall_friction);
+        launch_couple(stream, P, dx, nblocks, grid[gcur], gv, na, d_pel, (int)pellets.size(), d_bstart, d_blist,
+                      d_acc, d_st, cfg.wall_friction);
         HIP_OK(hipGetLastError());
     }
     launch_g2p2g(stream, P, fdt, (int)n, slab[cur], slab[cur ^ 1], cap, shared[sh], gv, na, grid[gcur ^ 1], d_st,

@@ -88,17 +88,19 @@ struct PelletAccum {
     double constrained;
 };
 
-// f32 pellet geometry for the coupling kernel (state at the start of the step).
+// Pellet geometry for the coupling kernel (state at the start of the step), in
+// f64 so that the inside test phi <= 0 on the node lattice matches the
+// reference's f64 decision.
 struct PelletDev {
-    float pos[3];
-    float axis[3];
-    float vel[3];
-    float omega[3];
-    float radius;
-    float half_span;
+    double pos[3];
+    double axis[3];
+    double vel[3];
+    double omega[3];
+    double radius;
+    double lo[3], hi[3];
     int spheres;
-    float lo[3], hi[3];
-    float offsets[kMaxSpheres];
+    int pad;
+    double offsets[kMaxSpheres];
 };
 
 struct GridPtrs {
@@ -140,9 +142,9 @@ void launch_traction(hipStream_t s, const Params& P, float dt, int n, const floa
                      StepStatus* st);
 void launch_grid_update(hipStream_t s, const Params& P, float dt, int nblocks, GridPtrs a, float* trS, float* trT,
                         float* gv, size_t nodes_alloc, GridPtrs clear, StepStatus* st);
-void launch_couple(hipStream_t s, const Params& P, int nblocks, GridPtrs a, float* gv, size_t nodes_alloc,
+void launch_couple(hipStream_t s, const Params& P, double dx, int nblocks, GridPtrs a, float* gv, size_t nodes_alloc,
                    const PelletDev* pellets, int npellets, const int* block_start, const int* block_list,
-                   PelletAccum* acc, StepStatus* st, float friction);
+                   PelletAccum* acc, StepStatus* st, double friction);
 void launch_g2p2g(hipStream_t s, const Params& P, float dt, int n, const float* src, float* dst, size_t cap,
                   const float* shared, const float* gv, size_t nodes_alloc, GridPtrs out, StepStatus* st,
                   hipEvent_t start, hipEvent_t stop);

@@ -537,21 +537,22 @@ __global__ __launch_bounds__(64) void grid_update_kernel(Params P, float dt, Gri
 
 // couple_grid (rigid.rs:385-438) for every pellet listed on this node block, in
 // pellet index order, so a node sees the pellets one after another as in a
-// sequential loop over pellets.
-__global__ __launch_bounds__(64) void couple_kernel(Params P, GridPtrs A, float* gv, size_t na,
+// sequential loop over pellets.  Geometry and the projection run in f64 with the
+// reference's operation order; node positions are (i - 1) dx exactly.
+__global__ __launch_bounds__(64) void couple_kernel(Params P, double dx, GridPtrs A, float* gv, size_t na,
                                                     const PelletDev* __restrict__ pellets,
                                                     const int* __restrict__ bstart, const int* __restrict__ blist,
-                                                    PelletAccum* acc, float friction) {
+                                                    PelletAccum* acc, double friction) {
     int b = blockIdx.x, l = threadIdx.x;
     int n = (b << 6) | l;
     int s0 = bstart[b], s1 = bstart[b + 1];
     if (s0 == s1 || !A.flag[b]) return;
-    float m = A.m[n];
-    if (!(m > 0.0f)) return;
+    double m = A.m[n];
+    if (!(m > 0.0)) return;
     int ci, cj, ck;
     node_coords(P, n, ci, cj, ck);
-    float node[3] = {(ci - 1.0f) * P.dx, (cj - 1.0f) * P.dx, (ck - 1.0f) * P.dx};
-    float v[3] = {gv[n], gv[na + n], gv[2 * na + n]};
+    double node[3] = {((double)ci - 1.0) * dx, ((double)cj - 1.0) * dx, ((double)ck - 1.0) * dx};
+    double v[3] = {gv[n], gv[na + n], gv[2 * na + n]};
     bool changed = false;
     for (int s = s0; s < s1; ++s) {
         int pi = blist[s];
@@ -559,51 +560,48 @@ __global__ __launch_bounds__(64) void couple_kernel(Params P, GridPtrs A, float*
         if (node[0] < pl.lo[0] || node[0] > pl.hi[0] || node[1] < pl.lo[1] || node[1] > pl.hi[1] ||
             node[2] < pl.lo[2] || node[2] > pl.hi[2])
             continue;
-        float best = INFINITY, nrm[3] = {0.0f, 0.0f, 1.0f};
+        double best = INFINITY, nrm[3] = {0.0, 0.0, 1.0};
         for (int k = 0; k < pl.spheres; ++k) {
-            float o = pl.offsets[k];
-            float dlt[3] = {node[0] - (pl.pos[0] + pl.axis[0] * o), node[1] - (pl.pos[1] + pl.axis[1] * o),
-                            node[2] - (pl.pos[2] + pl.axis[2] * o)};
-            float dist = sqrtf(dlt[0] * dlt[0] + dlt[1] * dlt[1] + dlt[2] * dlt[2]);
-            float phi = dist - pl.radius;
+            double o = pl.offsets[k];
+            double dlt[3] = {node[0] - (pl.pos[0] + pl.axis[0] * o), node[1] - (pl.pos[1] + pl.axis[1] * o),
+                             node[2] - (pl.pos[2] + pl.axis[2] * o)};
+            double dist = sqrt(dlt[0] * dlt[0] + dlt[1] * dlt[1] + dlt[2] * dlt[2]);
+            double phi = dist - pl.radius;
             if (phi < best) {
                 best = phi;
-                if (dist > 1e-12f) {
+                if (dist > 1e-12) {
                     nrm[0] = dlt[0] / dist;
                     nrm[1] = dlt[1] / dist;
                     nrm[2] = dlt[2] / dist;
                 } else {
-                    nrm[0] = 0.0f;
-                    nrm[1] = 0.0f;
-                    nrm[2] = 1.0f;
+                    nrm[0] = 0.0;
+                    nrm[1] = 0.0;
+                    nrm[2] = 1.0;
                 }
             }
         }
-        if (best > 0.0f) continue;
-        float r[3] = {node[0] - pl.pos[0], node[1] - pl.pos[1], node[2] - pl.pos[2]};
-        float body[3] = {pl.vel[0] + pl.omega[1] * r[2] - pl.omega[2] * r[1],
-                         pl.vel[1] + pl.omega[2] * r[0] - pl.omega[0] * r[2],
-                         pl.vel[2] + pl.omega[0] * r[1] - pl.omega[1] * r[0]};
-        float rel[3] = {v[0] - body[0], v[1] - body[1], v[2] - body[2]};
-        float ns = rel[0] * nrm[0] + rel[1] * nrm[1] + rel[2] * nrm[2];
-        if (ns >= 0.0f) continue;
-        float t[3] = {rel[0] - nrm[0] * ns, rel[1] - nrm[1] * ns, rel[2] - nrm[2] * ns};
-        float ts = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-        float red = friction * (-ns);
-        float f = (ts <= red || ts < 1e-14f) ? 0.0f : (ts - red) / ts;
-        float nv[3] = {body[0] + t[0] * f, body[1] + t[1] * f, body[2] + t[2] * f};
-        double dp[3] = {((double)nv[0] - v[0]) * m, ((double)nv[1] - v[1]) * m, ((double)nv[2] - v[2]) * m};
-        double e = 0.5 * (double)m *
-                   (((double)nv[0] * nv[0] + (double)nv[1] * nv[1] + (double)nv[2] * nv[2]) -
-                    ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]));
+        if (best > 0.0) continue;
+        double r[3] = {node[0] - pl.pos[0], node[1] - pl.pos[1], node[2] - pl.pos[2]};
+        double body[3] = {pl.vel[0] + (pl.omega[1] * r[2] - pl.omega[2] * r[1]),
+                          pl.vel[1] + (pl.omega[2] * r[0] - pl.omega[0] * r[2]),
+                          pl.vel[2] + (pl.omega[0] * r[1] - pl.omega[1] * r[0])};
+        double rel[3] = {v[0] - body[0], v[1] - body[1], v[2] - body[2]};
+        double ns = rel[0] * nrm[0] + rel[1] * nrm[1] + rel[2] * nrm[2];
+        if (ns >= 0.0) continue;
+        double t[3] = {rel[0] - nrm[0] * ns, rel[1] - nrm[1] * ns, rel[2] - nrm[2] * ns};
+        double ts = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        double red = friction * (-ns);
+        double f = (ts <= red || ts < 1e-14) ? 0.0 : (ts - red) / ts;
+        double nv[3] = {body[0] + t[0] * f, body[1] + t[1] * f, body[2] + t[2] * f};
+        double dp[3] = {(nv[0] - v[0]) * m, (nv[1] - v[1]) * m, (nv[2] - v[2]) * m};
+        double e = 0.5 * m * ((nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) - (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
         PelletAccum* a = acc + pi;
         atomicAdd(&a->impulse[0], -dp[0]);
         atomicAdd(&a->impulse[1], -dp[1]);
         atomicAdd(&a->impulse[2], -dp[2]);
-        double lx = r[0], ly = r[1], lz = r[2];
-        atomicAdd(&a->angular[0], -(ly * dp[2] - lz * dp[1]));
-        atomicAdd(&a->angular[1], -(lz * dp[0] - lx * dp[2]));
-        atomicAdd(&a->angular[2], -(lx * dp[1] - ly * dp[0]));
+        atomicAdd(&a->angular[0], -(r[1] * dp[2] - r[2] * dp[1]));
+        atomicAdd(&a->angular[1], -(r[2] * dp[0] - r[0] * dp[2]));
+        atomicAdd(&a->angular[2], -(r[0] * dp[1] - r[1] * dp[0]));
         atomicAdd(&a->grid_energy, e);
         atomicAdd(&a->constrained, 1.0);
         v[0] = nv[0];
@@ -612,9 +610,9 @@ __global__ __launch_bounds__(64) void couple_kernel(Params P, GridPtrs A, float*
         changed = true;
     }
     if (changed) {
-        gv[n] = v[0];
-        gv[na + n] = v[1];
-        gv[2 * na + n] = v[2];
+        gv[n] = (float)v[0];
+        gv[na + n] = (float)v[1];
+        gv[2 * na + n] = (float)v[2];
     }
 }
 
@@ -807,10 +805,10 @@ void launch_grid_update(hipStream_t s, const Params& P, float dt, int nblocks, G
     grid_update_kernel<<<nblocks, 64, 0, s>>>(P, dt, a, trS, trT, gv, na, clear, st);
 }
 
-void launch_couple(hipStream_t s, const Params& P, int nblocks, GridPtrs a, float* gv, size_t na,
+void launch_couple(hipStream_t s, const Params& P, double dx, int nblocks, GridPtrs a, float* gv, size_t na,
                    const PelletDev* pellets, int npellets, const int* bstart, const int* blist, PelletAccum* acc,
-                   StepStatus*, float friction) {
-    if (npellets > 0) couple_kernel<<<nblocks, 64, 0, s>>>(P, a, gv, na, pellets, bstart, blist, acc, friction);
+                   StepStatus*, double friction) {
+    if (npellets > 0) couple_kernel<<<nblocks, 64, 0, s>>>(P, dx, a, gv, na, pellets, bstart, blist, acc, friction);
 }
 
 void launch_g2p2g(hipStream_t s, const Params& P, float dt, int n, const float* src, float* dst, size_t cap,
