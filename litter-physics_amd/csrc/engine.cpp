@@ -264,6 +264,28 @@ struct mpm_engine {
     int* d_bstart = nullptr;
     int* d_blist = nullptr;
     size_t blist_cap = 0;
+    // per-workgroup partials of the step reductions
+    int ngwg = 0, npwg = 0;
+    double* gpart = nullptr;
+    int* gbad = nullptr;
+    float* pmax = nullptr;
+    double* pdiss = nullptr;
+    unsigned long long* pfail = nullptr;
+    int* pesc = nullptr;
+    // particles within half a cell of a face, one list per particle slab
+    int* nw_list[2] = {nullptr, nullptr};
+    int* nw_count = nullptr;  // two counters
+    int nw_host = 0;
+    // reference order keys (kept with the shared slab)
+    uint64_t* refkey[2] = {nullptr, nullptr};
+    // workgroup work items: runs of particles in one or two z-adjacent 4^3-cell blocks
+    int2* d_work = nullptr;
+    int nwork = 0;
+    size_t work_cap = 0;
+    uint32_t* run_unique = nullptr;
+    uint32_t* run_count = nullptr;
+    int* d_nruns = nullptr;
+    int cb[3] = {0, 0, 0};  // cell blocks per axis
     // sort
     uint64_t* keys[2] = {nullptr, nullptr};
     uint32_t* order[2] = {nullptr, nullptr};
@@ -340,6 +362,8 @@ struct mpm_engine {
             HIP_OK(hipMemset(shared[b], 0, sizeof(float) * NSF * cap));
             HIP_OK(hipMalloc(&ids[b], sizeof(uint32_t) * cap));
             HIP_OK(hipMalloc(&keys[b], sizeof(uint64_t) * cap));
+            HIP_OK(hipMalloc(&refkey[b], sizeof(uint64_t) * cap));
+            HIP_OK(hipMalloc(&nw_list[b], sizeof(int) * cap));
             HIP_OK(hipMalloc(&order[b], sizeof(uint32_t) * cap));
         }
         nblocks = P.nb[0] * P.nb[1] * P.nb[2];
@@ -358,6 +382,22 @@ struct mpm_engine {
         HIP_OK(hipMalloc(&trT, sizeof(float) * 3 * na));
         HIP_OK(hipMemset(trT, 0, sizeof(float) * 3 * na));
         HIP_OK(hipMalloc(&d_st, sizeof(StepStatus)));
+        HIP_OK(hipMalloc(&nw_count, sizeof(int) * 2));
+        HIP_OK(hipMemset(nw_count, 0, sizeof(int) * 2));
+        ngwg = grid_workgroups(nblocks);
+        for (int a = 0; a < 3; ++a) cb[a] = (cells[a] + 4) / 4;
+        work_cap = (size_t)cb[0] * cb[1] * cb[2] + cap / (size_t)g2p2g_chunk() + 16;
+        npwg = (int)work_cap;
+        HIP_OK(hipMalloc(&d_work, sizeof(int2) * work_cap));
+        HIP_OK(hipMalloc(&run_unique, sizeof(uint32_t) * cap));
+        HIP_OK(hipMalloc(&run_count, sizeof(uint32_t) * cap));
+        HIP_OK(hipMalloc(&d_nruns, sizeof(int)));
+        HIP_OK(hipMalloc(&gpart, sizeof(double) * 12 * ngwg));
+        HIP_OK(hipMalloc(&gbad, sizeof(int) * ngwg));
+        HIP_OK(hipMalloc(&pmax, sizeof(float) * 2 * npwg));
+        HIP_OK(hipMalloc(&pdiss, sizeof(double) * npwg));
+        HIP_OK(hipMalloc(&pfail, sizeof(unsigned long long) * npwg));
+        HIP_OK(hipMalloc(&pesc, sizeof(int) * npwg));
         HIP_OK(hipHostMalloc(&h_st, sizeof(StepStatus), hipHostMallocDefault));
         if (max_pellets) {
             HIP_OK(hipMalloc(&d_pel, sizeof(PelletDev) * max_pellets));
@@ -365,10 +405,10 @@ struct mpm_engine {
             HIP_OK(hipHostMalloc(&h_acc, sizeof(PelletAccum) * max_pellets, hipHostMallocDefault));
             HIP_OK(hipMalloc(&d_bstart, sizeof(int) * (nblocks + 1)));
         }
-        uint64_t kmax = (uint64_t)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1);
+        uint64_t kmax = (uint64_t)((cells[0] + 4) / 4) * ((cells[1] + 4) / 4) * ((cells[2] + 4) / 4) * 64;
         key_bits = 32;
         while ((1ull << (key_bits - 32)) < kmax) ++key_bits;
-        sort_bytes = sort_temp_bytes((int)cap, key_bits);
+        sort_bytes = std::max(sort_temp_bytes((int)cap, key_bits), rle_temp_bytes((int)cap));
         HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
         HIP_OK(hipStreamCreate(&stream));
         HIP_OK(hipEventCreate(&ev0));
@@ -382,6 +422,8 @@ struct mpm_engine {
             (void)hipFree(shared[b]);
             (void)hipFree(ids[b]);
             (void)hipFree(keys[b]);
+            (void)hipFree(refkey[b]);
+            (void)hipFree(nw_list[b]);
             (void)hipFree(order[b]);
             (void)hipFree(gmem[b]);
             (void)hipFree(gflag[b]);
@@ -390,6 +432,17 @@ struct mpm_engine {
         (void)hipFree(trS);
         (void)hipFree(trT);
         (void)hipFree(d_st);
+        (void)hipFree(nw_count);
+        (void)hipFree(gpart);
+        (void)hipFree(d_work);
+        (void)hipFree(run_unique);
+        (void)hipFree(run_count);
+        (void)hipFree(d_nruns);
+        (void)hipFree(gbad);
+        (void)hipFree(pmax);
+        (void)hipFree(pdiss);
+        (void)hipFree(pfail);
+        (void)hipFree(pesc);
         if (h_st) (void)hipHostFree(h_st);
         (void)hipFree(d_pel);
         (void)hipFree(d_acc);
@@ -408,7 +461,19 @@ struct mpm_engine {
         HIP_OK(hipMemsetAsync(gflag[gcur], 0, sizeof(int) * nblocks, stream));
         launch_p2g(stream, P, (int)n, slab[cur], cap, shared[sh], grid[gcur]);
         HIP_OK(hipGetLastError());
+        int rc = rebuild_nearwall();
+        if (rc) return rc;
         grid_valid = true;
+        return MPM_OK;
+    }
+
+    // Near-wall particle list of the committed slab.
+    int rebuild_nearwall() {
+        HIP_OK(hipMemsetAsync(nw_count + cur, 0, sizeof(int), stream));
+        launch_nearwall(stream, P, (int)n, slab[cur], cap, nw_list[cur], nw_count + cur);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(&nw_host, nw_count + cur, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
         return MPM_OK;
     }
 
@@ -489,25 +554,78 @@ struct mpm_engine {
         return limit;
     }
 
-    // solver.rs:501-517: sort by (clamped cell, id); the grid sums are order free.
-    int sort_particles() {
-        if (n < 2) return MPM_OK;
-        launch_sort_keys_cap(stream, P, (int)n, slab[cur], cap, ids[sh], keys[0], order[0]);
+    // Sort by 4^3-cell block (the device order) and rebuild the work items.  At the
+    // reference's 32-step cadence (solver.rs:501-517) the reference order key is
+    // refreshed too; keep_ref re-sorts for locality only, leaving the reference
+    // order of the state unchanged.  The grid sums are order free.
+    int sort_particles(int keep_ref = 0) {
+        if (n == 0) {
+            nwork = 0;
+            return MPM_OK;
+        }
+        launch_sort_keys(stream, P, (int)n, slab[cur], cap, ids[sh], keys[0], order[0], refkey[sh], keep_ref);
         launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)n, key_bits);
         launch_permute(stream, (int)n, order[1], slab[cur], slab[cur ^ 1], cap, shared[sh], shared[sh ^ 1], ids[sh],
-                       ids[sh ^ 1]);
+                       ids[sh ^ 1], refkey[sh], refkey[sh ^ 1]);
+        launch_block_runs(stream, sort_temp, sort_bytes, (int)n, keys[1], order[0], run_unique, run_count, d_nruns);
         HIP_OK(hipGetLastError());
         cur ^= 1;
         sh ^= 1;
-        return MPM_OK;
+        int nruns = 0;
+        HIP_OK(hipMemcpyAsync(&nruns, d_nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        std::vector<uint32_t> uq(nruns), cnt(nruns);
+        HIP_OK(hipMemcpyAsync(uq.data(), run_unique, sizeof(uint32_t) * nruns, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(cnt.data(), run_count, sizeof(uint32_t) * nruns, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        // Merge at most two z-adjacent blocks per work item; split runs longer than a chunk.
+        const int chunk = g2p2g_chunk();
+        std::vector<int2> work;
+        work.reserve(nruns + n / chunk + 1);
+        int start = 0;
+        int cur_start = -1, cur_count = 0, cur_blocks = 0;
+        uint32_t prev = 0;
+        auto flush = [&]() {
+            if (cur_start >= 0) work.push_back(make_int2(cur_start, cur_count));
+            cur_start = -1;
+            cur_count = 0;
+            cur_blocks = 0;
+        };
+        for (int r = 0; r < nruns; ++r) {
+            int c = (int)cnt[r];
+            uint32_t b = uq[r];
+            bool adjacent = cur_blocks == 1 && b == prev + 1 && (b % (uint32_t)cb[2]) != 0 && cur_count + c <= chunk;
+            if (!adjacent) flush();
+            if (c > chunk) {
+                for (int off = 0; off < c; off += chunk) work.push_back(make_int2(start + off, std::min(chunk, c - off)));
+            } else if (adjacent) {
+                cur_count += c;
+                cur_blocks = 2;
+            } else {
+                cur_start = start;
+                cur_count = c;
+                cur_blocks = 1;
+            }
+            prev = b;
+            start += c;
+        }
+        flush();
+        if (work.size() > work_cap) return fail(MPM_ERR_CAPACITY, "work list overflow");
+        nwork = (int)work.size();
+        HIP_OK(hipMemcpyAsync(d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return rebuild_nearwall();
     }
 
     int try_step(double dt, mpm_step_report* report);
     int remove_escaped();
+    std::vector<uint64_t> host_ref;
     int download(std::vector<float>& s, std::vector<float>& shv, std::vector<uint32_t>& id) {
         s.resize(NPF * n);
         shv.resize(NSF * n);
         id.resize(n);
+        host_ref.resize(n);
+        if (n) HIP_OK(hipMemcpyAsync(host_ref.data(), refkey[sh], sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
         for (int f = 0; f < NPF; ++f)
             HIP_OK(hipMemcpyAsync(s.data() + f * n, slab[cur] + f * cap, sizeof(float) * n, hipMemcpyDeviceToHost,
                                   stream));
@@ -528,6 +646,7 @@ struct mpm_engine {
             HIP_OK(hipMemcpyAsync(shared[sh] + f * cap, shv.data() + f * n, sizeof(float) * n,
                                   hipMemcpyHostToDevice, stream));
         if (n) HIP_OK(hipMemcpyAsync(ids[sh], id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
+        if (n) HIP_OK(hipMemcpyAsync(refkey[sh], host_ref.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
         return MPM_OK;
     }
@@ -552,6 +671,7 @@ int mpm_engine::remove_escaped() {
     size_t keep = 0;
     std::vector<float> s2(NPF * n), sh2(NSF * n);
     std::vector<uint32_t> id2(n);
+    std::vector<uint64_t> ref2(n);
     double out_mass = 0.0;
     V3 out_mom{{0, 0, 0}};
     for (size_t p = 0; p < n; ++p) {
@@ -571,6 +691,7 @@ int mpm_engine::remove_escaped() {
         for (int f = 0; f < NPF; ++f) s2[f * n + keep] = s[f * n + p];
         for (int f = 0; f < NSF; ++f) sh2[f * n + keep] = shv[f * n + p];
         id2[keep] = id[p];
+        ref2[keep] = host_ref[p];
         ++keep;
     }
     size_t old = n;
@@ -579,10 +700,14 @@ int mpm_engine::remove_escaped() {
     for (int f = 0; f < NPF; ++f) std::copy(s2.begin() + f * old, s2.begin() + f * old + n, s3.begin() + f * n);
     for (int f = 0; f < NSF; ++f) std::copy(sh2.begin() + f * old, sh2.begin() + f * old + n, sh3.begin() + f * n);
     id2.resize(n);
+    ref2.resize(n);
+    host_ref = ref2;
     ledger.outflow_mass += out_mass;
     particle_mass -= out_mass;
     for (int a = 0; a < 3; ++a) ledger.outflow_momentum[a] += out_mom[a];
-    return upload(s3, sh3, id2);
+    rc = upload(s3, sh3, id2);
+    if (rc) return rc;
+    return sort_particles(1);
 }
 
 // Simulation::try_step_captured (solver.rs:540-611) on the device.
@@ -593,20 +718,23 @@ int mpm_engine::try_step(double dt, mpm_step_report* report) {
         if (rc) return rc;
     }
     float fdt = (float)dt;
-    launch_reset_status(stream, d_st);
-    launch_traction(stream, P, fdt, (int)n, slab[cur], cap, shared[sh], grid[gcur], trS, trT, na, d_st);
-    launch_grid_update(stream, P, fdt, nblocks, grid[gcur], trS, trT, gv, na, grid[gcur ^ 1], d_st);
+    launch_reset_status(stream, d_st, nw_count + (cur ^ 1));
+    launch_traction(stream, P, fdt, nw_list[cur], nw_count + cur, nw_host, slab[cur], cap, shared[sh], grid[gcur],
+                    trS, trT, na);
+    launch_grid_update(stream, P, fdt, nblocks, grid[gcur], trS, trT, gv, na, grid[gcur ^ 1], gpart, gbad);
     HIP_OK(hipGetLastError());
     std::vector<Pellet> trial = pellets;
     if (!pellets.empty()) {
         int rc = upload_pellets();
         if (rc) return rc;
         launch_couple(stream, P, dx, nblocks, grid[gcur], gv, na, d_pel, (int)pellets.size(), d_bstart, d_blist,
-                      d_acc, d_st, cfg.wall_friction);
+                      d_acc, cfg.wall_friction);
         HIP_OK(hipGetLastError());
     }
-    launch_g2p2g(stream, P, fdt, (int)n, slab[cur], slab[cur ^ 1], cap, shared[sh], gv, na, grid[gcur ^ 1], d_st,
-                 profiling ? ev0 : nullptr, profiling ? ev1 : nullptr);
+    launch_g2p2g(stream, P, fdt, (int)n, slab[cur], slab[cur ^ 1], cap, shared[sh], gv, na, grid[gcur ^ 1], d_work,
+                 nwork, pmax, pdiss, pfail, pesc, nw_list[cur ^ 1], nw_count + (cur ^ 1), profiling ? ev0 : nullptr,
+                 profiling ? ev1 : nullptr);
+    launch_finalize(stream, ngwg, gpart, gbad, nwork, pmax, pdiss, pfail, pesc, nw_count + (cur ^ 1), d_st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(h_st, d_st, sizeof(StepStatus), hipMemcpyDeviceToHost, stream));
     if (!pellets.empty())
@@ -674,6 +802,7 @@ int mpm_engine::try_step(double dt, mpm_step_report* report) {
     }
     pellets = trial;
     max_speed = (double)bits_float(st.max_speed_bits);
+    nw_host = st.nearwall;
     if (report) report->accepted = 1;
     if (st.escaped) return remove_escaped();
     return MPM_OK;
@@ -918,7 +1047,12 @@ int mpm_set_state(mpm_engine* e, const mpm_particles* ps, const mpm_pellet* pell
     }
     e->cur = 0;
     e->sh = 0;
+    // Until the first sort the reference order is the given order.
+    e->host_ref.resize(n);
+    for (size_t p = 0; p < n; ++p) e->host_ref[p] = p;
     int rc = e->upload(s, shv, id);
+    if (rc) return rc;
+    rc = e->sort_particles(1);
     if (rc) return rc;
     e->pellets.clear();
     for (size_t k = 0; k < n_pellets; ++k) {
@@ -991,6 +1125,23 @@ int mpm_get_state(mpm_engine* e, mpm_particles* ps, mpm_pellet* pellets, size_t 
         size_t n = e->n;
         ps->n = n;
         bool fresh = e->steps_since_set == 0;
+        // Hand the particles back in the reference's order (sorted by the key of the
+        // last 32-step sort, solver.rs:501-517), whatever the device order is.
+        std::vector<size_t> order(n);
+        for (size_t k = 0; k < n; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return e->host_ref[a] < e->host_ref[b]; });
+        std::vector<float> so(NPF * n), sho(NSF * n);
+        std::vector<uint32_t> ido(n);
+        for (size_t k = 0; k < n; ++k) {
+            size_t q = order[k];
+            for (int f = 0; f < NPF; ++f) so[f * n + k] = s[f * n + q];
+            for (int f = 0; f < NSF; ++f) sho[f * n + k] = shv[f * n + q];
+            ido[k] = id[q];
+        }
+        s.swap(so);
+        shv.swap(sho);
+        id.swap(ido);
         for (size_t p = 0; p < n; ++p) {
             if (ps->id) ps->id[p] = id[p];
             for (int a = 0; a < 3; ++a) {

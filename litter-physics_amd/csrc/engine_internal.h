@@ -76,7 +76,7 @@ struct StepStatus {
     unsigned long long fail;     // (particle << 8) | code of the first failure
     int grid_nonfinite;
     int escaped;
-    int traction_nonfinite;
+    int nearwall;  // particles of the trial state within half a cell of a face
     int pad;
 };
 
@@ -130,28 +130,35 @@ __host__ __device__ inline void node_coords(const Params& P, int index, int& i, 
 }
 
 // Host launchers (kernels.hip).
-struct Launch {
-    hipStream_t stream;
-};
-
-void launch_reset_status(hipStream_t s, StepStatus* st);
+int g2p2g_chunk();
+int grid_workgroups(int nblocks);
+size_t rle_temp_bytes(int n);
+void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
+                       uint32_t* ids_tmp, uint32_t* unique, uint32_t* counts, int* nruns);
+void launch_reset_status(hipStream_t s, StepStatus* st, int* nw_count);
 void launch_p2g(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const float* shared,
                 GridPtrs g);
-void launch_traction(hipStream_t s, const Params& P, float dt, int n, const float* slab, size_t cap,
-                     const float* shared, GridPtrs g, float* trS, float* trT, size_t nodes_alloc,
-                     StepStatus* st);
+void launch_nearwall(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, int* list, int* count);
+void launch_traction(hipStream_t s, const Params& P, float dt, const int* list, const int* count, int max_count,
+                     const float* slab, size_t cap, const float* shared, GridPtrs g, float* trS, float* trT,
+                     size_t nodes_alloc);
 void launch_grid_update(hipStream_t s, const Params& P, float dt, int nblocks, GridPtrs a, float* trS, float* trT,
-                        float* gv, size_t nodes_alloc, GridPtrs clear, StepStatus* st);
+                        float* gv, size_t nodes_alloc, GridPtrs clear, double* partials, int* bad);
 void launch_couple(hipStream_t s, const Params& P, double dx, int nblocks, GridPtrs a, float* gv, size_t nodes_alloc,
                    const PelletDev* pellets, int npellets, const int* block_start, const int* block_list,
-                   PelletAccum* acc, StepStatus* st, double friction);
+                   PelletAccum* acc, double friction);
 void launch_g2p2g(hipStream_t s, const Params& P, float dt, int n, const float* src, float* dst, size_t cap,
-                  const float* shared, const float* gv, size_t nodes_alloc, GridPtrs out, StepStatus* st,
+                  const float* shared, const float* gv, size_t nodes_alloc, GridPtrs out, const int2* work, int nwork,
+                  float* pmax, double* pdiss, unsigned long long* pfail, int* pesc, int* nw_list, int* nw_count,
                   hipEvent_t start, hipEvent_t stop);
-void launch_sort_keys_cap(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const uint32_t* id,
-                          uint64_t* keys, uint32_t* order);
+void launch_finalize(hipStream_t s, int ngrid, const double* gpart, const int* gbad, int npart, const float* pmax,
+                     const double* pdiss, const unsigned long long* pfail, const int* pesc, const int* nw_count,
+                     StepStatus* st);
+void launch_sort_keys(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const uint32_t* id,
+                      uint64_t* keys, uint32_t* order, uint64_t* refkey, int keep_ref);
 void launch_permute(hipStream_t s, int n, const uint32_t* order, const float* src, float* dst, size_t cap,
-                    const float* shared_src, float* shared_dst, const uint32_t* id_src, uint32_t* id_dst);
+                    const float* shared_src, float* shared_dst, const uint32_t* id_src, uint32_t* id_dst,
+                    const uint64_t* ref_src, uint64_t* ref_dst);
 size_t sort_temp_bytes(int n, int end_bit);
 void launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
                        const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit);

@@ -14,6 +14,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 
 #include "engine_internal.h"
@@ -330,11 +331,12 @@ __device__ inline int liquid_update(const Params& P, float dt, float jm1, const 
 
 // ---------------------------------------------------------------- kernels
 
-__global__ void reset_status_kernel(StepStatus* st) {
+__global__ void reset_status_kernel(StepStatus* st, int* nw_count) {
     if (threadIdx.x == 0) {
         StepStatus z = {};
         z.fail = ~0ull;
         *st = z;
+        *nw_count = 0;
     }
 }
 
@@ -354,20 +356,33 @@ __global__ __launch_bounds__(kBlock) void p2g_kernel(Params P, int n, const floa
                      C, tau, shared[SVOL0 * cap + p]);
 }
 
+__device__ inline bool near_wall(const Params& P, float x, float y, float z) {
+    float half = 0.5f * P.dx;
+    return x < half || y < half || z < half || x > P.extent[0] - half || y > P.extent[1] - half ||
+           z > P.extent[2] - half;
+}
+
+// Indices of the particles within half a cell of a face (the only ones the
+// wall traction can touch).
+__global__ __launch_bounds__(kBlock) void nearwall_kernel(Params P, int n, const float* __restrict__ slab,
+                                                          size_t cap, int* list, int* count) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (near_wall(P, slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p])) list[atomicAdd(count, 1)] = p;
+}
+
 // wall_traction_to_grid (solver.rs:725-819).  Each deposit adds to the signed
 // traction impulse (trS) and the Coulomb budget (trT) of its node; the grid
 // update books the telescoped kinetic-energy jump per node.
-__global__ __launch_bounds__(kBlock) void traction_kernel(Params P, float dt, int n, const float* __restrict__ slab,
+__global__ __launch_bounds__(kBlock) void traction_kernel(Params P, float dt, const int* __restrict__ list,
+                                                          const int* __restrict__ count, const float* __restrict__ slab,
                                                           size_t cap, const float* __restrict__ shared, GridPtrs g,
                                                           float* trS, float* trT, size_t na) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= *count) return;
+    int p = list[q];
     float pos[3] = {slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]};
     float half = 0.5f * P.dx;
-    bool near = false;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) near |= (pos[a] < half) || (pos[a] > P.extent[a] - half);
-    if (!near) return;
     float mass = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
     float k4 = 4.0f * P.inv_dx * P.inv_dx;
     float force_scale = -dt * k4;
@@ -430,108 +445,127 @@ __device__ inline void reduce_tangential(float* v, float reduction) {
     }
 }
 
-// grid_update (solver.rs:822-892).  One wave per 64-node block.
-__global__ __launch_bounds__(64) void grid_update_kernel(Params P, float dt, GridPtrs A, float* trS, float* trT,
-                                                         float* gv, size_t na, GridPtrs B, StepStatus* st) {
-    int b = blockIdx.x, l = threadIdx.x;
-    int n = (b << 6) | l;
-    if (B.flag[b]) {
-        B.m[n] = 0.0f;
-        B.px[n] = 0.0f;
-        B.py[n] = 0.0f;
-        B.pz[n] = 0.0f;
-        B.fx[n] = 0.0f;
-        B.fy[n] = 0.0f;
-        B.fz[n] = 0.0f;
-        __syncthreads();
-        if (l == 0) B.flag[b] = 0;
+// Block-wide sum of NV doubles held by every thread; thread 0 gets the totals.
+template <int NV>
+__device__ inline void block_sum(double* vals, double* scratch) {
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) vals[i] = wave_sum(vals[i]);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) scratch[wave * NV + i] = vals[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int waves = blockDim.x >> 6;
+        for (int i = 0; i < NV; ++i) {
+            double t = 0.0;
+            for (int w = 0; w < waves; ++w) t += scratch[w * NV + i];
+            vals[i] = t;
+        }
     }
-    if (!A.flag[b]) return;
-    float m = A.m[n];
-    double imp[3] = {0, 0, 0}, timp[3] = {0, 0, 0}, proj = 0, fric = 0, tren = 0, gm[3] = {0, 0, 0};
+}
+
+// grid_update (solver.rs:822-892).  Four 64-node blocks per workgroup, one per
+// wavefront; ledger sums go to per-workgroup partials (no same-address atomics).
+constexpr int kGridVals = 12;
+__global__ __launch_bounds__(256) void grid_update_kernel(Params P, float dt, GridPtrs A, float* trS, float* trT,
+                                                          float* gv, size_t na, GridPtrs B, int nblocks,
+                                                          double* partials, int* bad_partials) {
+    __shared__ double scratch[4 * kGridVals];
+    __shared__ int sbad[4];
+    int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int b = blockIdx.x * 4 + wave;
+    double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int bad = 0;
-    float k4 = 4.0f * P.inv_dx * P.inv_dx;
-    if (m > 0.0f) {
-        float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
-        float v[3];
-        float tb[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float D = trS[a * na + n];
-            tb[a] = trT[a * na + n];
-            if (D != 0.0f) {
-                // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
-                tren += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
-                timp[a] += D;
-            }
-            v[a] = (p[a] + D) / m + P.gravity[a] * dt;
-            trS[a * na + n] = 0.0f;
-            trT[a * na + n] = 0.0f;
+    if (b < nblocks) {
+        int n = (b << 6) | l;
+        if (B.flag[b]) {
+            B.m[n] = 0.0f;
+            B.px[n] = 0.0f;
+            B.py[n] = 0.0f;
+            B.pz[n] = 0.0f;
+            B.fx[n] = 0.0f;
+            B.fy[n] = 0.0f;
+            B.fz[n] = 0.0f;
+            if (l == 0) B.flag[b] = 0;
         }
-        if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
-            bad = 1;
-            v[0] = v[1] = v[2] = 0.0f;
-        } else {
-            float before[3] = {v[0], v[1], v[2]};
-            int c[3];
-            node_coords(P, n, c[0], c[1], c[2]);
-            for (int axis = 0; axis < 3; ++axis) {
-                int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
-                if (sign == 0) continue;
-                float ns = v[axis] * (float)sign;
-                if (ns >= 0.0f) continue;
-                proj -= 0.5 * (double)m * (double)ns * (double)ns;
-                v[axis] = 0.0f;
-                float t[3] = {v[0], v[1], v[2]};
-                double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
-                double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                fric += 0.5 * (double)m * (tn - vn);
-            }
-            for (int axis = 0; axis < 3; ++axis) {
-                float reaction = tb[axis];
-                if (reaction <= 0.0f) continue;
-                float normal = v[axis];
-                float t[3] = {v[0], v[1], v[2]};
-                t[axis] = 0.0f;
-                double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
-                reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
-                double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                fric += 0.5 * (double)m * (tn - rn);
-                v[0] = t[0];
-                v[1] = t[1];
-                v[2] = t[2];
-                v[axis] = normal;
-            }
+        if (A.flag[b]) {
+            float m = A.m[n];
+            float k4 = 4.0f * P.inv_dx * P.inv_dx;
+            if (m > 0.0f) {
+                float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
+                float v[3], tb[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                imp[a] = ((double)v[a] - (double)before[a]) * (double)m;
-                gm[a] = (double)v[a] * (double)m;
+                for (int a = 0; a < 3; ++a) {
+                    float D = trS[a * na + n];
+                    tb[a] = trT[a * na + n];
+                    if (D != 0.0f) {
+                        // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
+                        vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
+                        vals[3 + a] += D;
+                        trS[a * na + n] = 0.0f;
+                    }
+                    if (tb[a] != 0.0f) trT[a * na + n] = 0.0f;
+                    v[a] = (p[a] + D) / m + P.gravity[a] * dt;
+                }
+                if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
+                    bad = 1;
+                    v[0] = v[1] = v[2] = 0.0f;
+                } else {
+                    float before[3] = {v[0], v[1], v[2]};
+                    int c[3];
+                    node_coords(P, n, c[0], c[1], c[2]);
+                    for (int axis = 0; axis < 3; ++axis) {
+                        int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
+                        if (sign == 0) continue;
+                        float ns = v[axis] * (float)sign;
+                        if (ns >= 0.0f) continue;
+                        vals[6] -= 0.5 * (double)m * (double)ns * (double)ns;
+                        v[axis] = 0.0f;
+                        double tn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                        reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
+                        double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                        vals[7] += 0.5 * (double)m * (tn - vn);
+                    }
+                    for (int axis = 0; axis < 3; ++axis) {
+                        float reaction = tb[axis];
+                        if (reaction <= 0.0f) continue;
+                        float normal = v[axis];
+                        float t[3] = {v[0], v[1], v[2]};
+                        t[axis] = 0.0f;
+                        double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                        int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
+                        reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
+                        double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                        vals[7] += 0.5 * (double)m * (tn - rn);
+                        v[0] = t[0];
+                        v[1] = t[1];
+                        v[2] = t[2];
+                        v[axis] = normal;
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        vals[a] = ((double)v[a] - (double)before[a]) * (double)m;
+                        vals[9 + a] = (double)v[a] * (double)m;
+                    }
+                }
+                gv[n] = v[0];
+                gv[na + n] = v[1];
+                gv[2 * na + n] = v[2];
+            } else {
+                gv[n] = 0.0f;
+                gv[na + n] = 0.0f;
+                gv[2 * na + n] = 0.0f;
             }
         }
-        gv[n] = v[0];
-        gv[na + n] = v[1];
-        gv[2 * na + n] = v[2];
-    } else {
-        gv[n] = 0.0f;
-        gv[na + n] = 0.0f;
-        gv[2 * na + n] = 0.0f;
     }
-    double vals[13] = {imp[0], imp[1], imp[2], timp[0], timp[1], timp[2], proj, fric, tren, gm[0], gm[1], gm[2], 0};
-#pragma unroll
-    for (int i = 0; i < 12; ++i) vals[i] = wave_sum(vals[i]);
     int nbad = wave_sum_i(bad);
-    if (l == 0) {
-        for (int a = 0; a < 3; ++a) {
-            if (vals[a] != 0.0) atomicAdd(&st->wall_impulse[a], vals[a]);
-            if (vals[3 + a] != 0.0) atomicAdd(&st->traction_impulse[a], vals[3 + a]);
-            if (vals[9 + a] != 0.0) atomicAdd(&st->grid_momentum[a], vals[9 + a]);
-        }
-        if (vals[6] != 0.0) atomicAdd(&st->projection, vals[6]);
-        if (vals[7] != 0.0) atomicAdd(&st->friction, vals[7]);
-        if (vals[8] != 0.0) atomicAdd(&st->traction_energy, vals[8]);
-        if (nbad) atomicAdd(&st->grid_nonfinite, nbad);
+    if (l == 0) sbad[wave] = nbad;
+    block_sum<kGridVals>(vals, scratch);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < kGridVals; ++i) partials[blockIdx.x * kGridVals + i] = vals[i];
+        bad_partials[blockIdx.x] = sbad[0] + sbad[1] + sbad[2] + sbad[3];
     }
 }
 
@@ -616,145 +650,559 @@ __global__ __launch_bounds__(64) void couple_kernel(Params P, double dx, GridPtr
     }
 }
 
+// ------------------------------------------------------------ fused G2P2G
+
+// Tile geometry of one workgroup's chunk of particles.  Gather tile: the nodes
+// of every current stencil; scatter tile: one cell wider on each side, since an
+// accepted step moves a particle by at most one cell.
+struct Tile {
+    int org[3];  // gather origin (lowest stencil base)
+    int gd[3];   // gather dims
+    int sd[3];   // scatter dims (origin org - 1)
+    int tiled;
+};
+
+__device__ inline void gather_global(const Params& P, const float* __restrict__ gv, size_t na, const Stencil& s,
+                                     float* v, float* B) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            float wab = s.w[0][a] * s.w[1][b];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float w = wab * s.w[2][c];
+                int ni = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
+                float nv[3] = {gv[ni] * w, gv[na + ni] * w, gv[2 * na + ni] * w};
+                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    v[i] += nv[i];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
+                }
+            }
+        }
+}
+
+__device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil& s, float* v, float* B) {
+    int li = s.base[0] - t.org[0], lj = s.base[1] - t.org[1], lk = s.base[2] - t.org[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            float wab = s.w[0][a] * s.w[1][b];
+            int row = ((li + a) * t.gd[1] + (lj + b)) * t.gd[2] + lk;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float w = wab * s.w[2][c];
+                float4 g = sg[row + c];
+                float nv[3] = {g.x * w, g.y * w, g.z * w};
+                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    v[i] += nv[i];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
+                }
+            }
+        }
+}
+
+// Select among three register values by a runtime index (no array indexing, which
+// would place the stencil in scratch memory).
+__device__ inline float sel3(int i, float x0, float x1, float x2) { return i == 0 ? x0 : (i == 1 ? x1 : x2); }
+
+// deposit_particles for one particle into the f64 LDS accumulator tile (LDS
+// f64 atomic adds run at close to full LDS rate on CDNA4; f32 ones do not).
+// Lanes walk the nine (a, b) stencil columns starting at a lane-dependent
+// column so that particles of one cell, which share most nodes, hit different
+// LDS words.
+template <int SCAP>
+__device__ inline void scatter_lds(double* ss, const int* so, const int* sd, const Params& P, const Stencil& s,
+                                   float m, const float* v, const float* C, const float* tau, float v0, int rot) {
+    float mc[9], vt[6];
+    float mdx = m * P.dx, vdx = v0 * P.dx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
+    float mv0 = m * v[0], mv1 = m * v[1], mv2 = m * v[2];
+    int li = s.base[0] - so[0], lj = s.base[1] - so[1], lk = s.base[2] - so[2];
+    for (int kk = 0; kk < 9; ++kk) {
+        int ab = kk + rot;
+        ab = ab >= 9 ? ab - 9 : ab;
+        int a = ab >= 6 ? 2 : (ab >= 3 ? 1 : 0);
+        int b = ab - 3 * a;
+        float wab = sel3(a, s.w[0][0], s.w[0][1], s.w[0][2]) * sel3(b, s.w[1][0], s.w[1][1], s.w[1][2]);
+        float dx = sel3(a, s.d[0][0], s.d[0][1], s.d[0][2]), dy = sel3(b, s.d[1][0], s.d[1][1], s.d[1][2]);
+        int row = ((li + a) * sd[1] + (lj + b)) * sd[2] + lk;
+        float px0 = mv0 + mc[0] * dx + mc[1] * dy, py0 = mv1 + mc[3] * dx + mc[4] * dy,
+              pz0 = mv2 + mc[6] * dx + mc[7] * dy;
+        float fx0 = vt[0] * dx + vt[3] * dy, fy0 = vt[3] * dx + vt[1] * dy, fz0 = vt[4] * dx + vt[5] * dy;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float w = wab * s.w[2][c];
+            float dz = s.d[2][c];
+            int idx = row + c;
+            atomicAdd(ss + 0 * SCAP + idx, (double)(w * m));
+            atomicAdd(ss + 1 * SCAP + idx, (double)(w * (px0 + mc[2] * dz)));
+            atomicAdd(ss + 2 * SCAP + idx, (double)(w * (py0 + mc[5] * dz)));
+            atomicAdd(ss + 3 * SCAP + idx, (double)(w * (pz0 + mc[8] * dz)));
+            atomicAdd(ss + 4 * SCAP + idx, (double)(w * (fx0 + vt[4] * dz)));
+            atomicAdd(ss + 5 * SCAP + idx, (double)(w * (fy0 + vt[5] * dz)));
+            atomicAdd(ss + 6 * SCAP + idx, (double)(w * (fz0 + vt[2] * dz)));
+        }
+    }
+}
+
+__device__ inline int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Wave-aggregated append of this lane's index when `pred` holds.
+__device__ inline void wave_append(bool pred, int value, int* list, int* count) {
+    unsigned long long mask = __ballot(pred);
+    if (!mask) return;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
 // grid_to_particle + constitutive update (solver.rs:895-977) fused with the
-// next step's particle-to-grid deposit into the other accumulator set.
+// next step's particle-to-grid deposit into the other accumulator set.  A
+// workgroup owns one or two cell blocks of particles (a work item).  It stages
+// the node velocities of their stencils in LDS, updates every particle, then
+// deposits the updated particles into an f64 LDS tile covering exactly their
+// new stencils and flushes it with one global atomic per touched node and field.
+// Work items whose tiles exceed the LDS budget take the direct global path.
+template <int CHUNK, int GCAP, int SCAP>
 __global__ __launch_bounds__(kBlock) void g2p2g_kernel(Params P, float dt, int n, const float* __restrict__ src,
                                                        float* __restrict__ dst, size_t cap,
                                                        const float* __restrict__ shared,
                                                        const float* __restrict__ gv, size_t na, GridPtrs out,
-                                                       StepStatus* st) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    float speed = 0.0f, disp = 0.0f;
-    double diss = 0.0;
-    unsigned long long fail = ~0ull;
-    int escaped = 0;
-    if (p < n) {
-        float x = src[PX * cap + p], y = src[PY * cap + p], z = src[PZ * cap + p];
-        Stencil s;
-        if (!make_stencil(P, x, y, z, s)) {
-            // Outside the padded grid: frozen for this trial (solver.rs:905-917).
-            for (int f = 0; f < NPF; ++f) dst[f * cap + p] = src[f * cap + p];
-            escaped = 1;
-        } else {
-            float v[3] = {0.0f, 0.0f, 0.0f};
-            float B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                                                       const int2* __restrict__ work, float* pmax, double* pdiss,
+                                                       unsigned long long* pfail, int* pesc, int* nw_list,
+                                                       int* nw_count) {
+    constexpr int R = CHUNK / kBlock;
+    __shared__ float4 sg[GCAP];
+    __shared__ double ss[7 * SCAP];
+    __shared__ int sbox[4][6];
+    __shared__ Tile tile;
+    __shared__ int sorg[3], sdim[3], stiled;
+    __shared__ double sdiss[4];
+    __shared__ float sred[4][2];
+    __shared__ unsigned long long sfail[4];
+    __shared__ int sesc[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int2 item = work[blockIdx.x];
+    const int chunk0 = item.x;
+    const int chunk_end = min(item.x + item.y, n);
+
+    // Phase A: stencil bases of the current positions and their bounding box.
+    float px[R], py[R], pz[R];
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+    for (int r = 0; r < R; ++r) {
+        int p = chunk0 + r * kBlock + tid;
+        px[r] = py[r] = pz[r] = 0.0f;
+        if (p < chunk_end) {
+            px[r] = src[PX * cap + p];
+            py[r] = src[PY * cap + p];
+            pz[r] = src[PZ * cap + p];
+            float xs[3] = {px[r] * P.inv_dx, py[r] * P.inv_dx, pz[r] * P.inv_dx};
 #pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    float wab = s.w[0][a] * s.w[1][b];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        float w = wab * s.w[2][c];
-                        int ni = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
-                        float nv[3] = {gv[ni] * w, gv[na + ni] * w, gv[2 * na + ni] * w};
-                        float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
-#pragma unroll
-                        for (int i = 0; i < 3; ++i) {
-                            v[i] += nv[i];
-#pragma unroll
-                            for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
-                        }
-                    }
-                }
-            float C[9];
-            float k4 = 4.0f * P.inv_dx;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) C[i] = B[i] * k4;
-            float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
-            float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            speed = vn;
-            disp = vn * dt * P.inv_dx;
-            float tau[6];
-            float inc = 0.0f, dd = 0.0f;
-            int rc;
-            if (P.kind == 0) {
-                float H[9], Ht[9], Hn[9];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) H[i] = src[(H00 + i) * cap + p];
-                // (I + dt C)(I + H) - I = H + dt C + dt C H
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j)
-                        Ht[3 * i + j] = H[3 * i + j] + dt * C[3 * i + j] +
-                                        dt * (C[3 * i] * H[j] + C[3 * i + 1] * H[3 + j] + C[3 * i + 2] * H[6 + j]);
-                rc = paste_update(P, dt, Ht, Hn, tau, inc, dd);
-                if (rc == F_OK) {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) dst[(H00 + i) * cap + p] = Hn[i];
-                    diss = (double)dd * (double)shared[SVOL0 * cap + p];
-                }
-            } else {
-                float jn;
-                rc = liquid_update(P, dt, src[JM1 * cap + p], C, jn, tau);
-                if (rc == F_OK) dst[JM1 * cap + p] = jn;
-            }
-            if (rc != F_OK) {
-                fail = ((unsigned long long)p << 8) | (unsigned long long)rc;
-            } else {
-                dst[PX * cap + p] = nx;
-                dst[PY * cap + p] = ny;
-                dst[PZ * cap + p] = nz;
-                dst[VX * cap + p] = v[0];
-                dst[VY * cap + p] = v[1];
-                dst[VZ * cap + p] = v[2];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
-#pragma unroll
-                for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
-                dst[EPS * cap + p] = src[EPS * cap + p] + inc;
-                Stencil s2;
-                if (make_stencil(P, nx, ny, nz, s2)) {
-                    scatter_particle(P, out, s2, shared[SMASS * cap + p], v[0], v[1], v[2], C, tau,
-                                     shared[SVOL0 * cap + p]);
-                } else {
-                    escaped = 1;
+            for (int a = 0; a < 3; ++a) {
+                float bf = floorf(xs[a] - 0.5f) + 1.0f;
+                if (bf >= 0.0f && bf + 2.0f < (float)P.nodes[a]) {
+                    int bi = (int)bf;
+                    lo[a] = min(lo[a], bi);
+                    hi[a] = max(hi[a], bi);
                 }
             }
         }
     }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min_i(lo[a]);
+        hi[a] = wave_max_i(hi[a]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            sbox[wave][a] = lo[a];
+            sbox[wave][3 + a] = hi[a];
+        }
+    __syncthreads();
+    if (tid == 0) {
+        Tile t;
+        int L[3], H[3];
+        for (int a = 0; a < 3; ++a) {
+            L[a] = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
+            H[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
+        }
+        bool ok = L[0] <= H[0] && L[1] <= H[1] && L[2] <= H[2];
+        int gsz = 1;
+        for (int a = 0; a < 3; ++a) {
+            t.org[a] = ok ? L[a] : 0;
+            t.gd[a] = ok ? H[a] - L[a] + 3 : 0;
+            t.sd[a] = 0;
+            gsz *= t.gd[a];
+        }
+        t.tiled = ok && gsz <= GCAP;
+        tile = t;
+    }
+    __syncthreads();
+    const Tile t = tile;
+    if (t.tiled) {
+        int gsz = t.gd[0] * t.gd[1] * t.gd[2];
+        for (int q = tid; q < gsz; q += kBlock) {
+            int k = q % t.gd[2], rest = q / t.gd[2];
+            int j = rest % t.gd[1], i = rest / t.gd[1];
+            int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
+            sg[q] = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
+        }
+    }
+    __syncthreads();
+
+    // Phase B: G2P, advection, constitutive update; outputs go straight to the
+    // trial slab.  The new stencil base of each updated particle is kept packed.
+    float speed = 0.0f, disp = 0.0f;
+    double diss = 0.0;
+    unsigned long long fail = ~0ull;
+    int escaped = 0;
+    int nbase[R];
+    lo[0] = lo[1] = lo[2] = INT_MAX;
+    hi[0] = hi[1] = hi[2] = INT_MIN;
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        int p = chunk0 + r * kBlock + tid;
+        bool nw = false;
+        nbase[r] = -1;
+        if (p < chunk_end) {
+            float x = px[r], y = py[r], z = pz[r];
+            Stencil s;
+            if (!make_stencil(P, x, y, z, s)) {
+                // Outside the padded grid: frozen for this trial (solver.rs:905-917).
+                for (int f = 0; f < NPF; ++f) dst[f * cap + p] = src[f * cap + p];
+                escaped += 1;
+            } else {
+                float v[3] = {0.0f, 0.0f, 0.0f};
+                float B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                if (t.tiled)
+                    gather_lds(sg, t, s, v, B);
+                else
+                    gather_global(P, gv, na, s, v, B);
+                float C[9];
+                float k4 = 4.0f * P.inv_dx;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) C[i] = B[i] * k4;
+                float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
+                float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                speed = fmaxf(speed, vn);
+                disp = fmaxf(disp, vn * dt * P.inv_dx);
+                float tau[6];
+                float inc = 0.0f, dd = 0.0f;
+                int rc;
+                if (P.kind == 0) {
+                    float H[9], Ht[9], Hn[9];
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) H[i] = src[(H00 + i) * cap + p];
+                    // (I + dt C)(I + H) - I = H + dt C + dt C H
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+                            Ht[3 * i + j] = H[3 * i + j] + dt * C[3 * i + j] +
+                                            dt * (C[3 * i] * H[j] + C[3 * i + 1] * H[3 + j] + C[3 * i + 2] * H[6 + j]);
+                    rc = paste_update(P, dt, Ht, Hn, tau, inc, dd);
+                    if (rc == F_OK) {
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) dst[(H00 + i) * cap + p] = Hn[i];
+                        diss += (double)dd * (double)shared[SVOL0 * cap + p];
+                        dst[EPS * cap + p] = src[EPS * cap + p] + inc;
+                    }
+                } else {
+                    float jn;
+                    rc = liquid_update(P, dt, src[JM1 * cap + p], C, jn, tau);
+                    if (rc == F_OK) dst[JM1 * cap + p] = jn;
+                }
+                if (rc != F_OK) {
+                    unsigned long long f = ((unsigned long long)p << 8) | (unsigned long long)rc;
+                    fail = f < fail ? f : fail;
+                } else {
+                    dst[PX * cap + p] = nx;
+                    dst[PY * cap + p] = ny;
+                    dst[PZ * cap + p] = nz;
+                    dst[VX * cap + p] = v[0];
+                    dst[VY * cap + p] = v[1];
+                    dst[VZ * cap + p] = v[2];
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
+                    float ns[3] = {nx * P.inv_dx, ny * P.inv_dx, nz * P.inv_dx};
+                    int nb[3];
+                    bool valid = true;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        float bf = floorf(ns[a] - 0.5f) + 1.0f;
+                        valid = valid && bf >= 0.0f && bf + 2.0f < (float)P.nodes[a];
+                        nb[a] = valid ? (int)bf : 0;
+                    }
+                    if (valid) {
+                        nbase[r] = (nb[0] << 20) | (nb[1] << 10) | nb[2];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            lo[a] = min(lo[a], nb[a]);
+                            hi[a] = max(hi[a], nb[a]);
+                        }
+                        nw = near_wall(P, nx, ny, nz);
+                    } else {
+                        escaped += 1;
+                    }
+                }
+            }
+        }
+        wave_append(nw, p, nw_list, nw_count);
+    }
+
+    // Phase C: exact tile of the new stencils.
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min_i(lo[a]);
+        hi[a] = wave_max_i(hi[a]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            sbox[wave][a] = lo[a];
+            sbox[wave][3 + a] = hi[a];
+        }
+    __syncthreads();
+    if (tid == 0) {
+        int L[3], H[3];
+        for (int a = 0; a < 3; ++a) {
+            L[a] = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
+            H[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
+        }
+        bool ok = L[0] <= H[0] && L[1] <= H[1] && L[2] <= H[2];
+        int ssz = 1;
+        for (int a = 0; a < 3; ++a) {
+            sorg[a] = ok ? L[a] : 0;
+            sdim[a] = ok ? H[a] - L[a] + 3 : 0;
+            ssz *= sdim[a];
+        }
+        stiled = ok && ssz <= SCAP;
+    }
+    __syncthreads();
+    const int so[3] = {sorg[0], sorg[1], sorg[2]};
+    const int sd[3] = {sdim[0], sdim[1], sdim[2]};
+    const bool tiled2 = stiled != 0;
+    const int ssz = sd[0] * sd[1] * sd[2];
+    if (tiled2)
+        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0.0;
+    __syncthreads();
+
+    // Phase D: deposit the updated particles (each thread re-reads its own writes).
+    const int rot = lane & 7;
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        if (nbase[r] < 0) continue;
+        int p = chunk0 + r * kBlock + tid;
+        float x = dst[PX * cap + p], y = dst[PY * cap + p], z = dst[PZ * cap + p];
+        Stencil s2;
+        make_stencil(P, x, y, z, s2);
+        float v[3] = {dst[VX * cap + p], dst[VY * cap + p], dst[VZ * cap + p]};
+        float C[9], tau[6];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) C[i] = dst[(C00 + i) * cap + p];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) tau[i] = dst[(TXX + i) * cap + p];
+        float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
+        if (tiled2)
+            scatter_lds<SCAP>(ss, so, sd, P, s2, m, v, C, tau, v0, rot);
+        else
+            scatter_particle(P, out, s2, m, v[0], v[1], v[2], C, tau, v0);
+    }
+    __syncthreads();
+    if (tiled2) {
+        for (int q = tid; q < ssz; q += kBlock) {
+            double m = ss[q];
+            if (m == 0.0) continue;
+            int k = q % sd[2], rest = q / sd[2];
+            int j = rest % sd[1], i = rest / sd[1];
+            int ni = node_index(P, so[0] + i, so[1] + j, so[2] + k);
+            atomicAdd(out.m + ni, (float)m);
+            atomicAdd(out.px + ni, (float)ss[1 * SCAP + q]);
+            atomicAdd(out.py + ni, (float)ss[2 * SCAP + q]);
+            atomicAdd(out.pz + ni, (float)ss[3 * SCAP + q]);
+            atomicAdd(out.fx + ni, (float)ss[4 * SCAP + q]);
+            atomicAdd(out.fy + ni, (float)ss[5 * SCAP + q]);
+            atomicAdd(out.fz + ni, (float)ss[6 * SCAP + q]);
+            out.flag[ni >> 6] = 1;
+        }
+    }
+    // Per-workgroup partials of the step reductions.
     speed = wave_max(speed);
     disp = wave_max(disp);
     diss = wave_sum(diss);
     fail = wave_min_u64(fail);
     escaped = wave_sum_i(escaped);
-    if (lane0()) {
-        if (speed > 0.0f) atomicMax(&st->max_speed_bits, __float_as_uint(speed));
-        if (disp > 0.0f) atomicMax(&st->max_disp_bits, __float_as_uint(disp));
-        if (diss != 0.0) atomicAdd(&st->dissipation, diss);
-        if (fail != ~0ull) atomicMin(&st->fail, fail);
-        if (escaped) atomicAdd(&st->escaped, escaped);
+    if (lane == 0) {
+        sred[wave][0] = speed;
+        sred[wave][1] = disp;
+        sdiss[wave] = diss;
+        sfail[wave] = fail;
+        sesc[wave] = escaped;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float s0 = 0.0f, s1 = 0.0f;
+        double d = 0.0;
+        unsigned long long f = ~0ull;
+        int e = 0;
+        for (int w = 0; w < 4; ++w) {
+            s0 = fmaxf(s0, sred[w][0]);
+            s1 = fmaxf(s1, sred[w][1]);
+            d += sdiss[w];
+            f = sfail[w] < f ? sfail[w] : f;
+            e += sesc[w];
+        }
+        pmax[2 * blockIdx.x] = s0;
+        pmax[2 * blockIdx.x + 1] = s1;
+        pdiss[blockIdx.x] = d;
+        pfail[blockIdx.x] = f;
+        pesc[blockIdx.x] = e;
     }
 }
 
-// Sort key of solver.rs:501-517: clamped cell index (z fastest) then id.
+// Fixed-order reduction of the per-workgroup partials into the step status.
+__global__ __launch_bounds__(256) void finalize_kernel(int ngrid, const double* gpart, const int* gbad, int npart,
+                                                       const float* pmax, const double* pdiss,
+                                                       const unsigned long long* pfail, const int* pesc,
+                                                       const int* nw_count, StepStatus* st) {
+    __shared__ double sd[256 * 4];
+    __shared__ float sf[256 * 2];
+    __shared__ unsigned long long su[256];
+    __shared__ int si[256 * 2];
+    int tid = threadIdx.x;
+    double acc[kGridVals + 1];
+    for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
+    int bad = 0;
+    for (int q = tid; q < ngrid; q += 256) {
+        for (int i = 0; i < kGridVals; ++i) acc[i] += gpart[q * kGridVals + i];
+        bad += gbad[q];
+    }
+    float smax = 0.0f, dmax = 0.0f;
+    unsigned long long f = ~0ull;
+    int esc = 0;
+    for (int q = tid; q < npart; q += 256) {
+        smax = fmaxf(smax, pmax[2 * q]);
+        dmax = fmaxf(dmax, pmax[2 * q + 1]);
+        acc[kGridVals] += pdiss[q];
+        f = pfail[q] < f ? pfail[q] : f;
+        esc += pesc[q];
+    }
+    // Tree over threads, one quantity group at a time.
+    for (int i = 0; i <= kGridVals; i += 4) {
+        for (int k = 0; k < 4; ++k) sd[tid * 4 + k] = (i + k <= kGridVals) ? acc[i + k] : 0.0;
+        __syncthreads();
+        for (int stride = 128; stride > 0; stride >>= 1) {
+            if (tid < stride)
+                for (int k = 0; k < 4; ++k) sd[tid * 4 + k] += sd[(tid + stride) * 4 + k];
+            __syncthreads();
+        }
+        if (tid == 0)
+            for (int k = 0; k < 4; ++k)
+                if (i + k <= kGridVals) acc[i + k] = sd[k];
+        __syncthreads();
+    }
+    sf[tid * 2] = smax;
+    sf[tid * 2 + 1] = dmax;
+    su[tid] = f;
+    si[tid * 2] = bad;
+    si[tid * 2 + 1] = esc;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (tid < stride) {
+            sf[tid * 2] = fmaxf(sf[tid * 2], sf[(tid + stride) * 2]);
+            sf[tid * 2 + 1] = fmaxf(sf[tid * 2 + 1], sf[(tid + stride) * 2 + 1]);
+            su[tid] = su[tid + stride] < su[tid] ? su[tid + stride] : su[tid];
+            si[tid * 2] += si[(tid + stride) * 2];
+            si[tid * 2 + 1] += si[(tid + stride) * 2 + 1];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int a = 0; a < 3; ++a) {
+            st->wall_impulse[a] = acc[a];
+            st->traction_impulse[a] = acc[3 + a];
+            st->grid_momentum[a] = acc[9 + a];
+        }
+        st->projection = acc[6];
+        st->friction = acc[7];
+        st->traction_energy = acc[8];
+        st->dissipation = acc[kGridVals];
+        st->max_speed_bits = __float_as_uint(sf[0]);
+        st->max_disp_bits = __float_as_uint(sf[1]);
+        st->fail = su[0];
+        st->grid_nonfinite = si[0];
+        st->escaped = si[1];
+        st->nearwall = *nw_count;
+    }
+}
+
+// Sort keys every 32 accepted steps (solver.rs:501-517).  The device order
+// groups cells in 4x4x4-cell blocks (compact workgroup chunks); the reference's
+// order key (clamped cell, z fastest, then id) is kept per particle so that the
+// state is handed back in the reference's order.
 __global__ void sort_keys_kernel(Params P, int n, const float* __restrict__ slab, size_t cap,
-                                 const uint32_t* __restrict__ id, uint64_t* keys, uint32_t* order) {
+                                 const uint32_t* __restrict__ id, uint64_t* keys, uint32_t* order, uint64_t* refkey,
+                                 int keep_ref) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    uint64_t cell = 0;
+    uint64_t c[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float sc = floorf(slab[(PX + a) * cap + p] / P.dx);
         sc = isnan(sc) ? 0.0f : fmaxf(sc, 0.0f);
-        uint64_t c = sc >= (float)P.cells[a] ? (uint64_t)P.cells[a] : (uint64_t)sc;
-        cell = cell * (uint64_t)(P.cells[a] + 1) + c;
+        c[a] = sc >= (float)P.cells[a] ? (uint64_t)P.cells[a] : (uint64_t)sc;
     }
-    keys[p] = (cell << 32) | (uint64_t)id[p];
+    uint64_t ref = (c[0] * (uint64_t)(P.cells[1] + 1) + c[1]) * (uint64_t)(P.cells[2] + 1) + c[2];
+    uint64_t nb1 = (uint64_t)(P.cells[1] + 4) / 4, nb2 = (uint64_t)(P.cells[2] + 4) / 4;
+    uint64_t blk = ((c[0] >> 2) * nb1 + (c[1] >> 2)) * nb2 + (c[2] >> 2);
+    uint64_t within = ((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3);
+    keys[p] = ((blk * 64 + within) << 32) | (uint64_t)id[p];
     order[p] = (uint32_t)p;
+    if (!keep_ref) refkey[p] = (ref << 32) | (uint64_t)id[p];
+}
+
+__global__ void block_ids_kernel(int n, const uint64_t* __restrict__ keys, uint32_t* out) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = (uint32_t)(keys[p] >> 38);
 }
 
 __global__ void permute_kernel(int n, const uint32_t* __restrict__ order, const float* __restrict__ src,
                                float* __restrict__ dst, size_t cap, const float* __restrict__ ssrc,
                                float* __restrict__ sdst, const uint32_t* __restrict__ isrc,
-                               uint32_t* __restrict__ idst) {
+                               uint32_t* __restrict__ idst, const uint64_t* __restrict__ rsrc,
+                               uint64_t* __restrict__ rdst) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t o = order[k];
     for (int f = 0; f < NPF; ++f) dst[f * cap + k] = src[f * cap + o];
     for (int f = 0; f < NSF; ++f) sdst[f * cap + k] = ssrc[f * cap + o];
     idst[k] = isrc[o];
+    rdst[k] = rsrc[o];
 }
 
 __global__ void paste_batch_kernel(Params P, float dt, int n, const float* h_in, float* h_out, float* tau,
@@ -786,59 +1234,103 @@ __global__ void liquid_batch_kernel(Params P, float dt, int n, const float* jm1,
 
 inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
 
+constexpr int kChunk = 1024;
+
+constexpr int kGatherCap = 768;
+constexpr int kScatterCap = 1024;
+
 }  // namespace
 
-void launch_reset_status(hipStream_t s, StepStatus* st) { reset_status_kernel<<<1, 64, 0, s>>>(st); }
+int g2p2g_chunk() { return kChunk; }
+int grid_workgroups(int nblocks) { return (nblocks + 3) / 4; }
+
+void launch_reset_status(hipStream_t s, StepStatus* st, int* nw_count) {
+    reset_status_kernel<<<1, 64, 0, s>>>(st, nw_count);
+}
 
 void launch_p2g(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const float* shared,
                 GridPtrs g) {
     if (n > 0) p2g_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, slab, cap, shared, g);
 }
 
-void launch_traction(hipStream_t s, const Params& P, float dt, int n, const float* slab, size_t cap,
-                     const float* shared, GridPtrs g, float* trS, float* trT, size_t na, StepStatus*) {
-    if (n > 0) traction_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, dt, n, slab, cap, shared, g, trS, trT, na);
+void launch_nearwall(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, int* list, int* count) {
+    if (n > 0) nearwall_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, slab, cap, list, count);
+}
+
+void launch_traction(hipStream_t s, const Params& P, float dt, const int* list, const int* count, int max_count,
+                     const float* slab, size_t cap, const float* shared, GridPtrs g, float* trS, float* trT,
+                     size_t na) {
+    if (max_count > 0)
+        traction_kernel<<<blocks_for(max_count), kBlock, 0, s>>>(P, dt, list, count, slab, cap, shared, g, trS, trT,
+                                                                 na);
 }
 
 void launch_grid_update(hipStream_t s, const Params& P, float dt, int nblocks, GridPtrs a, float* trS, float* trT,
-                        float* gv, size_t na, GridPtrs clear, StepStatus* st) {
-    grid_update_kernel<<<nblocks, 64, 0, s>>>(P, dt, a, trS, trT, gv, na, clear, st);
+                        float* gv, size_t na, GridPtrs clear, double* partials, int* bad) {
+    grid_update_kernel<<<grid_workgroups(nblocks), 256, 0, s>>>(P, dt, a, trS, trT, gv, na, clear, nblocks, partials,
+                                                                bad);
 }
 
 void launch_couple(hipStream_t s, const Params& P, double dx, int nblocks, GridPtrs a, float* gv, size_t na,
                    const PelletDev* pellets, int npellets, const int* bstart, const int* blist, PelletAccum* acc,
-                   StepStatus*, double friction) {
+                   double friction) {
     if (npellets > 0) couple_kernel<<<nblocks, 64, 0, s>>>(P, dx, a, gv, na, pellets, bstart, blist, acc, friction);
 }
 
 void launch_g2p2g(hipStream_t s, const Params& P, float dt, int n, const float* src, float* dst, size_t cap,
-                  const float* shared, const float* gv, size_t na, GridPtrs out, StepStatus* st, hipEvent_t start,
-                  hipEvent_t stop) {
+                  const float* shared, const float* gv, size_t na, GridPtrs out, const int2* work, int nwork,
+                  float* pmax, double* pdiss, unsigned long long* pfail, int* pesc, int* nw_list, int* nw_count,
+                  hipEvent_t start, hipEvent_t stop) {
     if (start) (void)hipEventRecord(start, s);
-    if (n > 0) g2p2g_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, dt, n, src, dst, cap, shared, gv, na, out, st);
+    if (n > 0 && nwork > 0)
+        g2p2g_kernel<kChunk, kGatherCap, kScatterCap><<<nwork, kBlock, 0, s>>>(
+            P, dt, n, src, dst, cap, shared, gv, na, out, work, pmax, pdiss, pfail, pesc, nw_list, nw_count);
     if (stop) (void)hipEventRecord(stop, s);
+}
+
+size_t rle_temp_bytes(int n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                (uint32_t*)nullptr, (int*)nullptr, n);
+    return bytes;
+}
+
+void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
+                       uint32_t* ids_tmp, uint32_t* unique, uint32_t* counts, int* nruns) {
+    if (n <= 0) return;
+    block_ids_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, sorted_keys, ids_tmp);
+    (void)hipcub::DeviceRunLengthEncode::Encode(temp, temp_bytes, ids_tmp, unique, counts, nruns, n, s);
+}
+
+void launch_finalize(hipStream_t s, int ngrid, const double* gpart, const int* gbad, int npart, const float* pmax,
+                     const double* pdiss, const unsigned long long* pfail, const int* pesc, const int* nw_count,
+                     StepStatus* st) {
+    finalize_kernel<<<1, 256, 0, s>>>(ngrid, gpart, gbad, npart, pmax, pdiss, pfail, pesc, nw_count, st);
 }
 
 size_t sort_temp_bytes(int n, int end_bit) {
     size_t bytes = 0;
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
     return bytes;
 }
 
 void launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
                        const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit) {
-    (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, end_bit, s);
+    (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, end_bit,
+                                             s);
 }
 
-void launch_sort_keys_cap(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const uint32_t* id,
-                          uint64_t* keys, uint32_t* order) {
-    if (n > 0) sort_keys_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, slab, cap, id, keys, order);
+void launch_sort_keys(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const uint32_t* id,
+                      uint64_t* keys, uint32_t* order, uint64_t* refkey, int keep_ref) {
+    if (n > 0) sort_keys_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, slab, cap, id, keys, order, refkey, keep_ref);
 }
 
 void launch_permute(hipStream_t s, int n, const uint32_t* order, const float* src, float* dst, size_t cap,
-                    const float* ssrc, float* sdst, const uint32_t* isrc, uint32_t* idst) {
-    if (n > 0) permute_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, order, src, dst, cap, ssrc, sdst, isrc, idst);
+                    const float* ssrc, float* sdst, const uint32_t* isrc, uint32_t* idst, const uint64_t* rsrc,
+                    uint64_t* rdst) {
+    if (n > 0)
+        permute_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, order, src, dst, cap, ssrc, sdst, isrc, idst, rsrc, rdst);
 }
 
 void launch_paste_batch(hipStream_t s, const Params& P, float dt, int n, const float* h_in, float* h_out, float* tau,
