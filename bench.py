@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
@@ -109,25 +111,24 @@ def main():
     sim = E.Simulation(mat, cfg, domain, dx, st)
     log("engine created")
 
-    def run(k):
-        count = [0]
-
-        def keep_going():
-            count[0] += 1
-            return count[0] <= k
-
-        h0 = sim.history().step_count
-        sim.advance_while(1e9, keep_going)
-        return sim.history().step_count - h0
-
-    run(args.warmup)
+    sim.advance_steps(args.warmup)
     log(f"warmup {args.warmup} steps done")
+    if not args.no_autotune:
+        chosen = sim.autotune(steps=16)
+        log(f"autotuned launch shape: {chosen.as_dict()}")
+    # Kernel timing pass: HIP events around every fused G2P2G launch (stream mode).
+    shape = sim.launch()
+    sim.set_launch(use_graph=0)
     sim.enable_profiling(True)
-    t0 = time.perf_counter()
-    accepted = run(args.steps)
-    elapsed = time.perf_counter() - t0
+    sim.advance_steps(args.profile_steps)
     avg_ms, launches = sim.g2p2g_profile()
-    log(f"timed {accepted} steps in {elapsed:.3f} s; g2p2g avg {avg_ms:.4f} ms over {launches} launches")
+    sim.enable_profiling(False)
+    sim.set_launch(use_graph=shape.use_graph)
+    log(f"g2p2g avg {avg_ms:.4f} ms over {launches} launches")
+    t0 = time.perf_counter()
+    accepted = sim.advance_steps(args.steps)
+    elapsed = time.perf_counter() - t0
+    log(f"timed {accepted} steps in {elapsed:.3f} s")
     h = sim.history()
     value = n * accepted / elapsed
     bytes_per_launch = PASTE_BYTES_PER_PARTICLE * n
@@ -152,7 +153,7 @@ def main():
         "config": {"workload": "HB paste column 100^3 particles (1,000,000) on a 96^3 grid, dx 2.5 mm, plain "
                                "implicit HB, CFL 0.4, sticky floor, free-slip walls, seed 0",
                    "particles": n, "grid_cells": [96, 96, 96], "rejected_steps": h.rejected_steps,
-                   "dt_last_s": h.max_dt_used},
+                   "dt_last_s": h.max_dt_used, "launch": sim.launch().as_dict()},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "kernel": "g2p2g_kernel", "kernel_ms": avg_ms,

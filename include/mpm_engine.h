@@ -228,11 +228,54 @@ int mpm_liquid_update_batch(const mpm_material* material, double dt, size_t n,
                             const double* volume_ratio, const double* gradient, double* new_ratio,
                             double* kirchhoff, int* codes);
 
-/* Per-kernel timing of the last mpm_advance_* call, for the benchmark: the
+/* Per-kernel timing since profiling was last enabled, for the benchmark: the
  * average device time of the fused grid-to-particle-to-grid kernel in ms and
  * its launch count. */
 int mpm_profile_g2p2g(mpm_engine* engine, double* avg_ms, uint64_t* launches);
 void mpm_enable_profiling(mpm_engine* engine, int on);
+
+/* ---- Batched instances.  No counterpart in the reference, whose sweeps run
+ * one Simulation per job in sequence (python/litter_physics/sweep.py:170-261).
+ * One engine holds n independent instances, each with its own material,
+ * config, grid, particles, pellets, adaptive dt, accept/reject state and
+ * ledgers; one launch sequence steps all of them.  An engine from mpm_create
+ * is a batch of one, and the single-instance calls above act on instance 0. */
+int mpm_batch_create(size_t n_instances, const mpm_material* materials, const mpm_config* configs,
+                     const mpm_layout* layouts, const size_t* capacities, const size_t* max_pellets,
+                     mpm_engine** out);
+size_t mpm_instance_count(const mpm_engine* engine);
+size_t mpm_instance_particle_count(const mpm_engine* engine, size_t instance);
+/* mpm_set_state / mpm_get_state for one instance. */
+int mpm_instance_set_state(mpm_engine* engine, size_t instance, const mpm_particles* particles,
+                           const mpm_pellet* pellets, size_t n_pellets, const mpm_ledger* ledger,
+                           const mpm_history* history);
+int mpm_instance_get_state(mpm_engine* engine, size_t instance, mpm_particles* particles,
+                           mpm_pellet* pellets, size_t pellet_capacity, mpm_ledger* ledger,
+                           mpm_history* history);
+/* Advance every instance towards targets[i] (advance_to, solver.rs:402-472,
+ * per instance) taking at most max_steps accepted steps each (0: no limit).
+ * status[i] (may be NULL): 0 target reached, 1 step limit reached,
+ * MPM_ERR_MIN_STEP when the instance aborted (its state stays consistent, as in
+ * the reference).  done_ms[i] (may be NULL): wall time in ms from the start of
+ * the call until instance i stopped, at the resolution of one host
+ * synchronisation.  Returns MPM_OK, or a negative code for misuse or a device
+ * failure. */
+int mpm_batch_advance(mpm_engine* engine, const double* targets, uint64_t max_steps, int* status,
+                      double* done_ms);
+
+/* Launch shape of the stepping loop (the autotuner's search space). */
+typedef struct {
+    int steps_per_sync;  /* trial steps enqueued between host synchronisations (1..256) */
+    int use_graph;       /* 1: replay the steps as one captured HIP graph */
+    int merge_blocks;    /* cell blocks per G2P2G work item along z (1..4) */
+    int traction_wgs;    /* workgroups of the wall-traction kernel */
+} mpm_launch;
+int mpm_get_launch(const mpm_engine* engine, mpm_launch* out);
+int mpm_set_launch(mpm_engine* engine, const mpm_launch* launch);
+/* Time `steps` trial steps of every candidate launch shape on the current
+ * state, restore the state, keep the fastest and return it in *chosen (may be
+ * NULL).  The state, ledgers and histories are unchanged afterwards. */
+int mpm_autotune(mpm_engine* engine, int steps, mpm_launch* chosen);
 
 #ifdef __cplusplus
 }

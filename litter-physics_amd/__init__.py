@@ -97,6 +97,15 @@ class StepReport(C.Structure):
     _fields_ = [("accepted", C.c_int), ("failure", C.c_int), ("max_cells", C.c_double), ("dt", C.c_double)]
 
 
+class Launch(C.Structure):
+    """mpm_launch: launch shape of the stepping loop (the autotuner's search space)."""
+    _fields_ = [("steps_per_sync", C.c_int), ("use_graph", C.c_int), ("merge_blocks", C.c_int),
+                ("traction_wgs", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class FixtureSpec(C.Structure):
     _fields_ = [("fixture", C.c_int), ("material", Material), ("grid_spacing", C.c_double),
                 ("domain", C.c_double * 3), ("initial_size", C.c_double * 3),
@@ -156,6 +165,18 @@ def lib():
                                               C.POINTER(C.c_int)]),
         "mpm_profile_g2p2g": (C.c_int, [C.c_void_p, dp, C.POINTER(C.c_uint64)]),
         "mpm_enable_profiling": (None, [C.c_void_p, C.c_int]),
+        "mpm_batch_create": (C.c_int, [C.c_size_t, C.POINTER(Material), C.POINTER(Config), C.POINTER(Layout),
+                                       C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]),
+        "mpm_instance_count": (C.c_size_t, [C.c_void_p]),
+        "mpm_instance_particle_count": (C.c_size_t, [C.c_void_p, C.c_size_t]),
+        "mpm_instance_set_state": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(Particles), C.POINTER(Pellet),
+                                             C.c_size_t, C.POINTER(Ledger), C.POINTER(History)]),
+        "mpm_instance_get_state": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(Particles), C.POINTER(Pellet),
+                                             C.c_size_t, C.POINTER(Ledger), C.POINTER(History)]),
+        "mpm_batch_advance": (C.c_int, [C.c_void_p, dp, C.c_uint64, C.POINTER(C.c_int), dp]),
+        "mpm_get_launch": (C.c_int, [C.c_void_p, C.POINTER(Launch)]),
+        "mpm_set_launch": (C.c_int, [C.c_void_p, C.POINTER(Launch)]),
+        "mpm_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Launch)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -171,7 +192,9 @@ EXPORTED = [
     "mpm_set_state", "mpm_get_state", "mpm_particle_count", "mpm_pellet_count", "mpm_set_config",
     "mpm_try_step", "mpm_advance_while", "mpm_advance_to", "mpm_stability_limit", "mpm_mass_residual",
     "mpm_momentum_residual", "mpm_energies", "mpm_paste_update_batch", "mpm_liquid_update_batch",
-    "mpm_profile_g2p2g", "mpm_enable_profiling",
+    "mpm_profile_g2p2g", "mpm_enable_profiling", "mpm_batch_create", "mpm_instance_count",
+    "mpm_instance_particle_count", "mpm_instance_set_state", "mpm_instance_get_state", "mpm_batch_advance",
+    "mpm_get_launch", "mpm_set_launch", "mpm_autotune",
 ]
 
 
@@ -405,6 +428,22 @@ class Simulation:
         self._check(lib().mpm_advance_while(self.h, target, cb, None, C.byref(reached)))
         return bool(reached.value)
 
+    def advance_steps(self, steps, target=1e9) -> int:
+        """Advance towards `target` taking at most `steps` accepted steps, with no
+        host callback between steps; returns the number of accepted steps."""
+        h0 = self.history().step_count
+        _advance(self, [target], steps)
+        return self.history().step_count - h0
+
+    def launch(self) -> Launch:
+        return _get_launch(self)
+
+    def set_launch(self, **kw):
+        _set_launch(self, **kw)
+
+    def autotune(self, steps=8) -> Launch:
+        return _autotune(self, steps)
+
     def stability_limit(self):
         return lib().mpm_stability_limit(self.h)
 
@@ -428,6 +467,134 @@ class Simulation:
         """Simulation::energy_residual, solver.rs:1095-1099."""
         led = self.ledger()
         return self.mechanical_energy() - led.initial_mechanical_energy - led.ledgered_energy()
+
+    def enable_profiling(self, on=True):
+        lib().mpm_enable_profiling(self.h, 1 if on else 0)
+
+    def g2p2g_profile(self):
+        ms = C.c_double()
+        n = C.c_uint64()
+        self._check(lib().mpm_profile_g2p2g(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def _get_launch(eng) -> Launch:
+    out = Launch()
+    eng._check(lib().mpm_get_launch(eng.h, C.byref(out)))
+    return out
+
+
+def _set_launch(eng, **kw):
+    cur = _get_launch(eng)
+    for k, v in kw.items():
+        setattr(cur, k, int(v))
+    eng._check(lib().mpm_set_launch(eng.h, C.byref(cur)))
+
+
+def _autotune(eng, steps) -> Launch:
+    out = Launch()
+    eng._check(lib().mpm_autotune(eng.h, steps, C.byref(out)))
+    return out
+
+
+def _advance(eng, targets, max_steps=0):
+    t = np.ascontiguousarray(targets, dtype=np.float64)
+    status = np.zeros(len(t), np.int32)
+    done = np.zeros(len(t))
+    eng._check(lib().mpm_batch_advance(eng.h, _p(t), int(max_steps), status.ctypes.data_as(C.POINTER(C.c_int)),
+                                       _p(done)))
+    return status, done
+
+
+class Batch:
+    """Many independent instances stepped by one launch sequence (mpm_batch_*).
+
+    Each instance has its own material, config, grid, particles, pellets, adaptive
+    dt and ledgers; `advance` moves every instance towards its own target."""
+
+    def __init__(self, specs):
+        """specs: list of dicts with material, cfg, domain, spacing, state, and
+        optionally pellets, capacity, max_pellets."""
+        L = lib()
+        n = len(specs)
+        mats = (Material * n)(*[s["material"] for s in specs])
+        cfgs = (Config * n)(*[s["cfg"] for s in specs])
+        lays = (Layout * n)()
+        caps = (C.c_size_t * n)()
+        mps = (C.c_size_t * n)()
+        for i, s in enumerate(specs):
+            lays[i].domain[:] = list(s["domain"])
+            lays[i].spacing = s["spacing"]
+            lays[i].max_nodes = s.get("max_nodes", 1 << 40)
+            caps[i] = s.get("capacity") or max(len(s["state"]), 1)
+            mps[i] = s.get("max_pellets", len(s.get("pellets", ())))
+        h = C.c_void_p()
+        rc = L.mpm_batch_create(n, mats, cfgs, lays, caps, mps, C.byref(h))
+        if rc != 0:
+            raise MpmError(f"mpm_batch_create failed ({rc})")
+        self.h = h
+        self.n = n
+        for i, s in enumerate(specs):
+            self.set_state(i, s["state"], s.get("pellets", ()))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.mpm_destroy(h)
+            self.h = None
+
+    def __len__(self):
+        return self.n
+
+    def _check(self, rc):
+        if rc < 0:
+            raise MpmError(lib().mpm_last_error(self.h).decode())
+        return rc
+
+    def set_state(self, i, state: State, pellets=(), ledger=None, history=None):
+        cp = state.to_c()
+        arr = (Pellet * max(len(pellets), 1))(*pellets) if pellets else None
+        self._check(lib().mpm_instance_set_state(self.h, i, C.byref(cp), arr, len(pellets),
+                                                 C.byref(ledger) if ledger is not None else None,
+                                                 C.byref(history) if history is not None else None))
+
+    def particle_count(self, i):
+        return lib().mpm_instance_particle_count(self.h, i)
+
+    def state(self, i) -> State:
+        st = State.empty(self.particle_count(i))
+        cp = st.to_c()
+        self._check(lib().mpm_instance_get_state(self.h, i, C.byref(cp), None, 0, None, None))
+        for name, arr in zip(("pos", "vel", "affine", "deformation", "volume_ratio", "kirchhoff", "plastic",
+                              "mass", "volume0"), st._keep):
+            setattr(st, name, arr)
+        st.id = st._ids
+        return st
+
+    def ledger(self, i) -> Ledger:
+        led = Ledger()
+        self._check(lib().mpm_instance_get_state(self.h, i, None, None, 0, C.byref(led), None))
+        return led
+
+    def history(self, i) -> History:
+        hist = History()
+        self._check(lib().mpm_instance_get_state(self.h, i, None, None, 0, None, C.byref(hist)))
+        return hist
+
+    def advance(self, targets, max_steps=0):
+        """Advance instance i towards targets[i]; returns (status, done_ms) arrays."""
+        if np.isscalar(targets):
+            targets = [targets] * self.n
+        return _advance(self, targets, max_steps)
+
+    def launch(self) -> Launch:
+        return _get_launch(self)
+
+    def set_launch(self, **kw):
+        _set_launch(self, **kw)
+
+    def autotune(self, steps=8) -> Launch:
+        return _autotune(self, steps)
 
     def enable_profiling(self, on=True):
         lib().mpm_enable_profiling(self.h, 1 if on else 0)

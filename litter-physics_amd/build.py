@@ -8,7 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmpm_amd.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "engine.cpp")]
-HEADERS = [os.path.join(HERE, "csrc", "engine_internal.h"), os.path.join(ROOT, "include", "mpm_engine.h")]
+HEADERS = [os.path.join(HERE, "csrc", "engine_internal.h"), os.path.join(HERE, "csrc", "physics.h"),
+           os.path.join(ROOT, "include", "mpm_engine.h")]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1,14 +1,18 @@
 // Host side of the MI355X MLS-MPM engine: the C ABI of include/mpm_engine.h.
 //
-// The host mirrors the reference's driver (core/src/mpm/solver.rs:357-611) in
-// f64: the adaptive-dt loop, accept/reject, ledgers, the 32-step particle sort
-// and the rigid pellets (rigid.rs), while every per-particle and per-node phase
-// runs in the HIP kernels of kernels.hip.  There is no CPU fallback for any
+// One engine holds a batch of instances (an instance is the reference's
+// Simulation, core/src/mpm/solver.rs:324-611).  Every trial step, the adaptive
+// dt, accept/reject, the ledgers and the pellets run on the device (kernels.hip);
+// the host enqueues a run of trial steps, synchronises once per run and only
+// steps in to remove particles that left the grid, to re-sort particles for
+// locality and to hand states in and out.  There is no CPU fallback for any
 // kernel: without a usable GPU every stepping entry point fails with
 // MPM_ERR_DEVICE.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,10 +25,14 @@
 
 using namespace mpm;
 
+static_assert(sizeof(mpm_ledger) == sizeof(double) * kLedger, "ledger layout");
+
 namespace {
 
-constexpr uint64_t kSortInterval = 32;  // solver.rs:39
-constexpr double kDtRecovery = 1.25;   // solver.rs:45
+constexpr int kMaxStepsPerSync = 256;
+constexpr int kLocalitySortInterval = 32;  // host re-sorts for locality every 32 trial steps
+
+// ------------------------------------------------------------ f64 host math
 
 struct V3 {
     double x[3];
@@ -32,148 +40,32 @@ struct V3 {
     double operator[](int i) const { return x[i]; }
 };
 inline V3 v3(double a, double b, double c) { return V3{{a, b, c}}; }
-inline V3 operator+(const V3& a, const V3& b) { return v3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
-inline V3 operator-(const V3& a, const V3& b) { return v3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
-inline V3 operator*(const V3& a, double s) { return v3(a[0] * s, a[1] * s, a[2] * s); }
 inline double dot(const V3& a, const V3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-inline double norm(const V3& a) { return std::sqrt(dot(a, a)); }
-inline V3 cross(const V3& a, const V3& b) {
-    return v3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
-}
-inline bool finite(const V3& a) { return std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]); }
 
-struct Quat {
-    double i, j, k, w;
-};
-Quat qmul(const Quat& a, const Quat& b) {
-    return Quat{a.w * b.i + a.i * b.w + a.j * b.k - a.k * b.j, a.w * b.j - a.i * b.k + a.j * b.w + a.k * b.i,
-                a.w * b.k + a.i * b.j - a.j * b.i + a.k * b.w, a.w * b.w - a.i * b.i - a.j * b.j - a.k * b.k};
-}
-V3 qrot(const Quat& q, const V3& v) {
-    V3 qv = v3(q.i, q.j, q.k);
-    V3 t = cross(qv, v) * 2.0;
-    return t * q.w + cross(qv, t) + v;
-}
-void qmatrix(const Quat& q, double r[9]) {
-    double i = q.i, j = q.j, k = q.k, w = q.w;
+void qmatrix(const double* q, double r[9]) {
+    double i = q[0], j = q[1], k = q[2], w = q[3];
     double ww = w * w, ii = i * i, jj = j * j, kk = k * k;
-    double ij = i * j * 2.0, wk = w * k * 2.0, wj = w * j * 2.0, ik = i * k * 2.0, jk = j * k * 2.0, wi = w * i * 2.0;
+    double ij = i * j * 2.0, wk = w * k * 2.0, wj = w * j * 2.0, ik = i * k * 2.0, jk = j * k * 2.0,
+           wi = w * i * 2.0;
     double m[9] = {ww + ii - jj - kk, ij - wk, wj + ik, wk + ij, ww - ii + jj - kk, jk - wi, ik - wj, wi + jk,
                    ww - ii - jj + kk};
     std::memcpy(r, m, sizeof(m));
 }
-V3 mv(const double r[9], const V3& x) {
-    return v3(r[0] * x[0] + r[1] * x[1] + r[2] * x[2], r[3] * x[0] + r[4] * x[1] + r[5] * x[2],
-              r[6] * x[0] + r[7] * x[1] + r[8] * x[2]);
+
+// Pellet::angular_velocity and kinetic energy, rigid.rs:193-210.
+V3 pellet_omega(const PelletState& p) {
+    double r[9];
+    qmatrix(p.q, r);
+    V3 b = v3(r[0] * p.L[0] + r[3] * p.L[1] + r[6] * p.L[2], r[1] * p.L[0] + r[4] * p.L[1] + r[7] * p.L[2],
+              r[2] * p.L[0] + r[5] * p.L[1] + r[8] * p.L[2]);
+    V3 bo = v3(b[0] / p.inertia[0], b[1] / p.inertia[1], b[2] / p.inertia[2]);
+    return v3(r[0] * bo[0] + r[1] * bo[1] + r[2] * bo[2], r[3] * bo[0] + r[4] * bo[1] + r[5] * bo[2],
+              r[6] * bo[0] + r[7] * bo[1] + r[8] * bo[2]);
 }
-V3 mtv(const double r[9], const V3& x) {
-    return v3(r[0] * x[0] + r[3] * x[1] + r[6] * x[2], r[1] * x[0] + r[4] * x[1] + r[7] * x[2],
-              r[2] * x[0] + r[5] * x[1] + r[8] * x[2]);
-}
-// Exponential of the pure quaternion a/2 (a rotation by |a| about a).
-Quat from_scaled_axis(const V3& a) {
-    V3 half = a * 0.5;
-    double nn = dot(half, half);
-    const double eps = std::numeric_limits<double>::epsilon();
-    if (nn <= eps * eps) return Quat{0, 0, 0, 1};
-    double n = std::sqrt(nn);
-    double f = std::sin(n) / n;
-    return Quat{half[0] * f, half[1] * f, half[2] * f, std::cos(n)};
-}
-
-// Rigid multisphere pellet, rigid.rs:30-344 (f64 on the host).
-struct Pellet {
-    double radius = 0, length = 0, mass = 0;
-    V3 inertia{{0, 0, 0}}, pos{{0, 0, 0}};
-    Quat q{0, 0, 0, 1};
-    V3 vel{{0, 0, 0}}, L{{0, 0, 0}};
-    std::vector<double> offsets;
-
-    V3 axis() const { return qrot(q, v3(1, 0, 0)); }
-    V3 omega() const {
-        double r[9];
-        qmatrix(q, r);
-        V3 b = mtv(r, L);
-        return mv(r, v3(b[0] / inertia[0], b[1] / inertia[1], b[2] / inertia[2]));
-    }
-    void bounding_box(V3& lo, V3& hi) const {
-        V3 ax = axis();
-        double half = 0.0;
-        for (double o : offsets) half = std::max(half, std::fabs(o));
-        V3 reach = v3(std::fabs(ax[0]) * half + radius, std::fabs(ax[1]) * half + radius,
-                      std::fabs(ax[2]) * half + radius);
-        lo = pos - reach;
-        hi = pos + reach;
-    }
-    void apply_impulse(const V3& imp, const V3& ang) {
-        vel = vel + v3(imp[0] / mass, imp[1] / mass, imp[2] / mass);
-        L = L + ang;
-    }
-    double kinetic() const { return 0.5 * mass * dot(vel, vel) + 0.5 * dot(omega(), L); }
-    bool finite_state() const {
-        return finite(pos) && finite(vel) && finite(L) && std::isfinite(q.i) && std::isfinite(q.j) &&
-               std::isfinite(q.k) && std::isfinite(q.w);
-    }
-};
-
-struct WallContact {
-    V3 impulse{{0, 0, 0}};
-    double work = 0.0;
-};
-
-double contact_damping(const mpm_config& c, double mass) {
-    double zeta;
-    if (c.contact_restitution <= 0.0)
-        zeta = 1.0;
-    else if (c.contact_restitution >= 1.0)
-        zeta = 0.0;
-    else {
-        double lg = std::log(c.contact_restitution);
-        zeta = -lg / std::sqrt(M_PI * M_PI + lg * lg);
-    }
-    return 2.0 * zeta * std::sqrt(c.contact_stiffness * mass);
-}
-
-// Pellet::integrate, rigid.rs:279-334.
-WallContact integrate(Pellet& p, double dt, const V3& g, const V3& ext, const mpm_config& c) {
-    double damping = contact_damping(c, p.mass);
-    V3 force{{0, 0, 0}}, torque{{0, 0, 0}};
-    WallContact led;
-    V3 om = p.omega(), ax = p.axis();
-    for (double o : p.offsets) {
-        V3 center = p.pos + ax * o;
-        for (int a = 0; a < 3; ++a)
-            for (int side = 0; side < 2; ++side) {
-                double face = side == 0 ? 0.0 : ext[a];
-                double sign = side == 0 ? 1.0 : -1.0;
-                double pen = p.radius - (center[a] - face) * sign;
-                if (pen <= 0.0) continue;
-                V3 nrm{{0, 0, 0}};
-                nrm[a] = sign;
-                V3 lever = center - nrm * p.radius - p.pos;
-                V3 pv = p.vel + cross(om, lever);
-                double vn = dot(pv, nrm);
-                double fn = std::max(c.contact_stiffness * pen - damping * vn, 0.0);
-                V3 t = pv - nrm * vn;
-                double ts = norm(t);
-                V3 fr{{0, 0, 0}};
-                if (ts > 1e-12) {
-                    double lim = std::min(c.contact_friction * fn, p.mass * ts / dt);
-                    fr = v3(-t[0] / ts, -t[1] / ts, -t[2] / ts) * lim;
-                }
-                V3 total = nrm * fn + fr;
-                force = force + total;
-                torque = torque + cross(lever, total);
-                led.work += dot(total, pv) * dt;
-            }
-    }
-    V3 imp = force * dt;
-    led.impulse = imp;
-    p.vel = p.vel + (g * dt + v3(imp[0] / p.mass, imp[1] / p.mass, imp[2] / p.mass));
-    p.L = p.L + torque * dt;
-    p.pos = p.pos + p.vel * dt;
-    p.q = qmul(from_scaled_axis(p.omega() * dt), p.q);
-    return led;
+double pellet_kinetic(const PelletState& p) {
+    V3 om = pellet_omega(p);
+    return 0.5 * p.mass * (p.vel[0] * p.vel[0] + p.vel[1] * p.vel[1] + p.vel[2] * p.vel[2]) +
+           0.5 * (om[0] * p.L[0] + om[1] * p.L[1] + om[2] * p.L[2]);
 }
 
 // Symmetric 3x3 cyclic Jacobi in f64 (constitutive.rs:373-425), used for the
@@ -221,116 +113,122 @@ double wave_speed(const mpm_material& m) {
 }
 double diffusivity(const mpm_material& m) { return m.kind == 0 ? 0.0 : m.consistency / m.density; }
 
-#define HIP_OK(expr)                                                    \
-    do {                                                                \
-        hipError_t e_ = (expr);                                         \
-        if (e_ != hipSuccess) {                                         \
-            return fail_device(#expr, e_);                              \
-        }                                                               \
-    } while (0)
+// 2 zeta of the pellet wall contact (rigid.rs:279-290); damping = 2 zeta sqrt(k m).
+double damping_ratio2(const mpm_config& c) {
+    double zeta;
+    if (c.contact_restitution <= 0.0)
+        zeta = 1.0;
+    else if (c.contact_restitution >= 1.0)
+        zeta = 0.0;
+    else {
+        double lg = std::log(c.contact_restitution);
+        zeta = -lg / std::sqrt(M_PI * M_PI + lg * lg);
+    }
+    return 2.0 * zeta;
+}
+
+// Host copy of one instance's particles in the f32 slab layout (field-major, n entries).
+struct HostParticles {
+    size_t n = 0;
+    std::vector<float> s, sh;
+    std::vector<uint32_t> id;
+    std::vector<uint64_t> ref;
+};
+
+// Kinetic, potential and elastic energy (solver.rs:1027-1076) from the f32 state.
+void energies_of(const mpm_material& mat, const double* g, const HostParticles& h,
+                 const std::vector<PelletState>& pellets, double* ke_out, double* pe_out, double* el_out) {
+    size_t n = h.n;
+    double ke = 0.0, pe = 0.0, el = 0.0;
+    for (size_t p = 0; p < n; ++p) {
+        double m = h.sh[SMASS * n + p], v0 = h.sh[SVOL0 * n + p];
+        double v2 = 0.0, gx = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            double v = h.s[(VX + a) * n + p];
+            v2 += v * v;
+            gx += g[a] * (double)h.s[(PX + a) * n + p];
+        }
+        ke += 0.5 * m * v2;
+        pe += -m * gx;
+        if (mat.kind == 0) {
+            double F[9], b[9], vals[3];
+            for (int a = 0; a < 9; ++a) F[a] = (a % 4 == 0 ? 1.0 : 0.0) + (double)h.s[(H00 + a) * n + p];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    b[3 * i + j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
+            if (!eigen3(b, vals)) {
+                el = std::numeric_limits<double>::quiet_NaN();
+                continue;
+            }
+            double hh[3], tr = 0.0, h2 = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                hh[a] = 0.5 * std::log(std::max(vals[a], 1e-300));
+                tr += hh[a];
+                h2 += hh[a] * hh[a];
+            }
+            el += v0 * (mat.shear_modulus * h2 + 0.5 * mat.lame_lambda * tr * tr);
+        } else {
+            double j = 1.0 + (double)h.s[JM1 * n + p];
+            double gm = mat.tait_gamma;
+            if (gm == 1.0)
+                el += v0 * mat.bulk_modulus * (j - 1.0 - std::log(j));
+            else
+                el += v0 * (mat.bulk_modulus / gm) * (j - 1.0 + (std::pow(j, 1.0 - gm) - 1.0) / (gm - 1.0));
+        }
+    }
+    for (const PelletState& p : pellets) {
+        ke += pellet_kinetic(p);
+        pe -= p.mass * (g[0] * p.pos[0] + g[1] * p.pos[1] + g[2] * p.pos[2]);
+    }
+    if (ke_out) *ke_out = ke;
+    if (pe_out) *pe_out = pe;
+    if (el_out) *el_out = el;
+}
+
+// GridLayout::new's checks, grid.rs:37-86.
+int grid_cells(const mpm_layout& l, int* cells) {
+    double spacing = l.spacing;
+    if (!(std::isfinite(spacing) && spacing > 0.0)) return MPM_ERR_INVALID;
+    size_t total = 1;
+    for (int a = 0; a < 3; ++a) {
+        double ratio = l.domain[a] / spacing;
+        double rounded = std::round(ratio);
+        if (!std::isfinite(ratio) || std::fabs(ratio - rounded) > 1e-6 * std::max(rounded, 1.0) || rounded < 2.0 ||
+            rounded > 4000.0)
+            return MPM_ERR_INVALID;
+        cells[a] = (int)rounded;
+        total *= (size_t)(cells[a] + 3);
+    }
+    if (total > l.max_nodes) return MPM_ERR_INVALID;
+    return MPM_OK;
+}
+
+int ceil_log2(uint64_t v) {
+    int b = 0;
+    while ((1ull << b) < v) ++b;
+    return b;
+}
 
 }  // namespace
 
-struct mpm_engine {
+#define HIP_OK(expr)                                      \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return fail_device(#expr, e_); \
+    } while (0)
+
+struct Instance {
     mpm_material mat{};
     mpm_config cfg{};
-    Params P{};
     double dx = 0.0;
     int cells[3] = {0, 0, 0};
-    size_t cap = 0, n = 0, max_pellets = 0;
-    // device particle storage
-    float* slab[2] = {nullptr, nullptr};
-    float* shared[2] = {nullptr, nullptr};
-    uint32_t* ids[2] = {nullptr, nullptr};
-    int cur = 0, sh = 0;
-    // device grid
-    size_t na = 0;
-    int nblocks = 0;
-    float* gmem[2] = {nullptr, nullptr};
-    int* gflag[2] = {nullptr, nullptr};
-    GridPtrs grid[2]{};
-    int gcur = 0;
+    int cb[3] = {0, 0, 0};  // 4^3-cell blocks per axis (cells 0..cells inclusive after clamping)
+    int pel_cap = 0;
+    InstParams ip{};
     bool grid_valid = false;
-    float* gv = nullptr;
-    float* trS = nullptr;
-    float* trT = nullptr;
-    StepStatus* d_st = nullptr;
-    StepStatus* h_st = nullptr;
-    // pellets
-    std::vector<Pellet> pellets;
-    PelletDev* d_pel = nullptr;
-    PelletAccum* d_acc = nullptr;
-    PelletAccum* h_acc = nullptr;
-    int* d_bstart = nullptr;
-    int* d_blist = nullptr;
-    size_t blist_cap = 0;
-    // per-workgroup partials of the step reductions
-    int ngwg = 0, npwg = 0;
-    double* gpart = nullptr;
-    int* gbad = nullptr;
-    float* pmax = nullptr;
-    double* pdiss = nullptr;
-    unsigned long long* pfail = nullptr;
-    int* pesc = nullptr;
-    // particles within half a cell of a face, one list per particle slab
-    int* nw_list[2] = {nullptr, nullptr};
-    int* nw_count = nullptr;  // two counters
-    int nw_host = 0;
-    // reference order keys (kept with the shared slab)
-    uint64_t* refkey[2] = {nullptr, nullptr};
-    // workgroup work items: runs of particles in one or two z-adjacent 4^3-cell blocks
-    int2* d_work = nullptr;
-    int nwork = 0;
-    size_t work_cap = 0;
-    uint32_t* run_unique = nullptr;
-    uint32_t* run_count = nullptr;
-    int* d_nruns = nullptr;
-    int cb[3] = {0, 0, 0};  // cell blocks per axis
-    // sort
-    uint64_t* keys[2] = {nullptr, nullptr};
-    uint32_t* order[2] = {nullptr, nullptr};
-    void* sort_temp = nullptr;
-    size_t sort_bytes = 0;
-    int key_bits = 0;
-    // host state, solver.rs:324-352
-    mpm_ledger ledger{};
-    double time = 0.0;
-    uint64_t step_count = 0, limited_steps = 0, rejected_steps = 0;
-    double dt_scale = 1.0, min_dt_used = std::numeric_limits<double>::infinity(), max_dt_used = 0.0;
-    double max_speed = 0.0;
-    double particle_mass = 0.0;  // sum of the stored (f32) particle masses, in f64
-    uint64_t steps_since_set = 0;
-    int last_failure = 0;
-    std::string err;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool profiling = false;
-    double g2p2g_ms = 0.0;
-    uint64_t g2p2g_launches = 0;
 
-    int fail_device(const char* what, hipError_t e) {
-        err = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
-        return MPM_ERR_DEVICE;
-    }
-    int fail(int code, const std::string& msg) {
-        err = msg;
-        return code;
-    }
-    GridPtrs make_grid(float* mem, int* flag) const {
-        GridPtrs g;
-        g.m = mem;
-        g.px = mem + na;
-        g.py = mem + 2 * na;
-        g.pz = mem + 3 * na;
-        g.fx = mem + 4 * na;
-        g.fy = mem + 5 * na;
-        g.fz = mem + 6 * na;
-        g.flag = flag;
-        return g;
-    }
-    V3 extents() const { return v3(cells[0] * dx, cells[1] * dx, cells[2] * dx); }
-    V3 gravity() const { return v3(cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]); }
     void fill_params() {
+        Params& P = ip.P;
         P.dx = (float)dx;
         P.inv_dx = (float)(1.0 / dx);
         for (int a = 0; a < 3; ++a) {
@@ -339,6 +237,7 @@ struct mpm_engine {
             P.nb[a] = (cells[a] + 3 + 3) / 4;
             P.extent[a] = (float)(cells[a] * dx);
             P.gravity[a] = (float)cfg.gravity[a];
+            cb[a] = (cells[a] + 4) / 4;
         }
         P.kind = mat.kind;
         P.G = (float)mat.shear_modulus;
@@ -350,80 +249,269 @@ struct mpm_engine {
         P.gamma = (float)mat.tait_gamma;
         P.visc = (float)mat.consistency;
         for (int i = 0; i < 6; ++i) P.face_mu[i] = (float)cfg.face_friction[i];
+        ip.dx = dx;
+        ip.cfl = cfg.cfl;
+        ip.max_dt = cfg.max_dt;
+        ip.min_dt = cfg.min_dt;
+        ip.wave = wave_speed(mat);
+        ip.diff = diffusivity(mat);
+        for (int a = 0; a < 3; ++a) ip.gravity[a] = cfg.gravity[a];
+        ip.wall_friction = cfg.wall_friction;
+        ip.contact_k = cfg.contact_stiffness;
+        ip.contact_damping = damping_ratio2(cfg);
+        ip.contact_friction = cfg.contact_friction;
+    }
+    int node_blocks() const { return ip.P.nb[0] * ip.P.nb[1] * ip.P.nb[2]; }
+};
+
+struct mpm_engine {
+    int ninst = 0;
+    std::vector<Instance> inst;
+    size_t cap = 0, na = 0;
+    int total_blocks = 0, ngwg = 0, total_pel = 0;
+    // particles
+    float* slab[2] = {nullptr, nullptr};
+    float* shared[2] = {nullptr, nullptr};
+    uint32_t* ids[2] = {nullptr, nullptr};
+    uint64_t* refkey[2] = {nullptr, nullptr};
+    int* nw_list[2] = {nullptr, nullptr};
+    int sh = 0;
+    // grid
+    float* gmem[2] = {nullptr, nullptr};
+    int* gflag[2] = {nullptr, nullptr};
+    float* gv = nullptr;
+    float* trS = nullptr;
+    float* trT = nullptr;
+    int* blk_inst = nullptr;
+    double* gpart = nullptr;
+    int* gbad = nullptr;
+    // instances
+    InstParams* d_ip = nullptr;
+    InstState* d_is = nullptr;
+    InstState* h_is = nullptr;  // pinned mirror
+    int* d_poffs = nullptr;
+    int* counters = nullptr;
+    // pellets
+    PelletState* d_pel = nullptr;
+    PelletGeo* d_pgeo = nullptr;
+    PelletAccum* d_pacc = nullptr;
+    PelletTrial* d_ptrial = nullptr;
+    // work items and partials
+    int4* d_work = nullptr;
+    int nwork = 0;
+    size_t work_cap = 0;
+    float* pmax = nullptr;
+    double* pdiss = nullptr;
+    unsigned long long* pfail = nullptr;
+    int* pesc = nullptr;
+    // sort
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* order[2] = {nullptr, nullptr};
+    uint64_t* run_unique = nullptr;
+    uint32_t* run_count = nullptr;
+    int* d_nruns = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_bytes = 0;
+    int cell_bits = 0, end_bit = 0;
+    bool dirty = true;  // work items and near-wall lists must be rebuilt
+    // launch
+    mpm_launch launch{32, 0, 2, 512};
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    bool graph_stale = true;
+    hipEvent_t ev0[kMaxStepsPerSync] = {}, ev1[kMaxStepsPerSync] = {};
+    bool profiling = false;
+    double g2p2g_ms = 0.0;
+    uint64_t g2p2g_launches = 0;
+    std::string err;
+
+    int fail_device(const char* what, hipError_t e) {
+        err = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+        return MPM_ERR_DEVICE;
+    }
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    bool any_pellets() const {
+        for (const Instance& in : inst)
+            if (in.ip.npel > 0) return true;
+        return false;
     }
 
-    int alloc(size_t capacity, size_t pellets_cap) {
-        cap = std::max<size_t>(capacity, 1);
-        max_pellets = pellets_cap;
+    BatchDev dev() const {
+        BatchDev B{};
+        B.ninst = ninst;
+        B.cap = cap;
+        B.na = na;
+        B.ip = d_ip;
+        B.is = d_is;
+        B.poffs = d_poffs;
         for (int b = 0; b < 2; ++b) {
-            HIP_OK(hipMalloc(&slab[b], sizeof(float) * NPF * cap));
-            HIP_OK(hipMemset(slab[b], 0, sizeof(float) * NPF * cap));
-            HIP_OK(hipMalloc(&shared[b], sizeof(float) * NSF * cap));
-            HIP_OK(hipMemset(shared[b], 0, sizeof(float) * NSF * cap));
-            HIP_OK(hipMalloc(&ids[b], sizeof(uint32_t) * cap));
-            HIP_OK(hipMalloc(&keys[b], sizeof(uint64_t) * cap));
-            HIP_OK(hipMalloc(&refkey[b], sizeof(uint64_t) * cap));
-            HIP_OK(hipMalloc(&nw_list[b], sizeof(int) * cap));
-            HIP_OK(hipMalloc(&order[b], sizeof(uint32_t) * cap));
+            B.slab[b] = slab[b];
+            B.shared[b] = shared[b];
+            B.gmem[b] = gmem[b];
+            B.gflag[b] = gflag[b];
+            B.nw_list[b] = nw_list[b];
         }
-        nblocks = P.nb[0] * P.nb[1] * P.nb[2];
-        na = (size_t)nblocks * 64;
-        for (int b = 0; b < 2; ++b) {
-            HIP_OK(hipMalloc(&gmem[b], sizeof(float) * 7 * na));
-            HIP_OK(hipMemset(gmem[b], 0, sizeof(float) * 7 * na));
-            HIP_OK(hipMalloc(&gflag[b], sizeof(int) * nblocks));
-            HIP_OK(hipMemset(gflag[b], 0, sizeof(int) * nblocks));
-            grid[b] = make_grid(gmem[b], gflag[b]);
-        }
-        HIP_OK(hipMalloc(&gv, sizeof(float) * 3 * na));
-        HIP_OK(hipMemset(gv, 0, sizeof(float) * 3 * na));
-        HIP_OK(hipMalloc(&trS, sizeof(float) * 3 * na));
-        HIP_OK(hipMemset(trS, 0, sizeof(float) * 3 * na));
-        HIP_OK(hipMalloc(&trT, sizeof(float) * 3 * na));
-        HIP_OK(hipMemset(trT, 0, sizeof(float) * 3 * na));
-        HIP_OK(hipMalloc(&d_st, sizeof(StepStatus)));
-        HIP_OK(hipMalloc(&nw_count, sizeof(int) * 2));
-        HIP_OK(hipMemset(nw_count, 0, sizeof(int) * 2));
-        ngwg = grid_workgroups(nblocks);
-        for (int a = 0; a < 3; ++a) cb[a] = (cells[a] + 4) / 4;
-        work_cap = (size_t)cb[0] * cb[1] * cb[2] + cap / (size_t)g2p2g_chunk() + 16;
-        npwg = (int)work_cap;
-        HIP_OK(hipMalloc(&d_work, sizeof(int2) * work_cap));
-        HIP_OK(hipMalloc(&run_unique, sizeof(uint32_t) * cap));
-        HIP_OK(hipMalloc(&run_count, sizeof(uint32_t) * cap));
-        HIP_OK(hipMalloc(&d_nruns, sizeof(int)));
-        HIP_OK(hipMalloc(&gpart, sizeof(double) * 12 * ngwg));
-        HIP_OK(hipMalloc(&gbad, sizeof(int) * ngwg));
-        HIP_OK(hipMalloc(&pmax, sizeof(float) * 2 * npwg));
-        HIP_OK(hipMalloc(&pdiss, sizeof(double) * npwg));
-        HIP_OK(hipMalloc(&pfail, sizeof(unsigned long long) * npwg));
-        HIP_OK(hipMalloc(&pesc, sizeof(int) * npwg));
-        HIP_OK(hipHostMalloc(&h_st, sizeof(StepStatus), hipHostMallocDefault));
-        if (max_pellets) {
-            HIP_OK(hipMalloc(&d_pel, sizeof(PelletDev) * max_pellets));
-            HIP_OK(hipMalloc(&d_acc, sizeof(PelletAccum) * max_pellets));
-            HIP_OK(hipHostMalloc(&h_acc, sizeof(PelletAccum) * max_pellets, hipHostMallocDefault));
-            HIP_OK(hipMalloc(&d_bstart, sizeof(int) * (nblocks + 1)));
-        }
-        uint64_t kmax = (uint64_t)((cells[0] + 4) / 4) * ((cells[1] + 4) / 4) * ((cells[2] + 4) / 4) * 64;
-        key_bits = 32;
-        while ((1ull << (key_bits - 32)) < kmax) ++key_bits;
-        sort_bytes = std::max(sort_temp_bytes((int)cap, key_bits), rle_temp_bytes((int)cap));
-        HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
-        HIP_OK(hipStreamCreate(&stream));
-        HIP_OK(hipEventCreate(&ev0));
-        HIP_OK(hipEventCreate(&ev1));
+        B.sh = sh;
+        B.gv = gv;
+        B.trS = trS;
+        B.trT = trT;
+        B.ids = ids[sh];
+        B.refkey = refkey[sh];
+        B.blk_inst = blk_inst;
+        B.work = d_work;
+        B.nwork = nwork;
+        B.ngwg = ngwg;
+        B.counters = counters;
+        B.gpart = gpart;
+        B.gbad = gbad;
+        B.pmax = pmax;
+        B.pdiss = pdiss;
+        B.pfail = pfail;
+        B.pesc = pesc;
+        B.pel = d_pel;
+        B.pgeo = d_pgeo;
+        B.pacc = d_pacc;
+        B.ptrial = d_ptrial;
+        B.npel = total_pel;
+        return B;
+    }
+
+    int pull() {
+        HIP_OK(hipMemcpyAsync(h_is, d_is, sizeof(InstState) * ninst, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return MPM_OK;
+    }
+    int push() {
+        HIP_OK(hipMemcpyAsync(d_is, h_is, sizeof(InstState) * ninst, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return MPM_OK;
+    }
+    int upload_params(int i) {
+        HIP_OK(hipMemcpyAsync(d_ip + i, &inst[i].ip, sizeof(InstParams), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
         return MPM_OK;
     }
 
+    template <class T>
+    int dalloc(T** p, size_t n, bool zero = true) {
+        HIP_OK(hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1)));
+        if (zero) HIP_OK(hipMemset(*p, 0, sizeof(T) * std::max<size_t>(n, 1)));
+        return MPM_OK;
+    }
+
+    int alloc(const size_t* capacities, const size_t* max_pellets) {
+        HIP_OK(hipStreamCreate(&stream));
+        std::vector<int> poffs(ninst + 1, 0);
+        std::vector<int> binst;
+        size_t total = 0, wcap = 0;
+        int max_cb = 1;
+        for (int i = 0; i < ninst; ++i) {
+            Instance& in = inst[i];
+            size_t pc = std::max<size_t>(capacities ? capacities[i] : 1, 1);
+            if (total + pc > (size_t)INT_MAX / 2) return fail(MPM_ERR_CAPACITY, "total particle capacity too large");
+            in.ip.poff = (int)total;
+            in.ip.pcap = (int)pc;
+            poffs[i] = (int)total;
+            total += pc;
+            int nb = in.node_blocks();
+            int nblk = (nb + 3) / 4 * 4;
+            in.ip.boff = total_blocks;
+            in.ip.nblk = nblk;
+            for (int b = 0; b < nblk; ++b) binst.push_back(i);
+            total_blocks += nblk;
+            in.pel_cap = (int)(max_pellets ? max_pellets[i] : 0);
+            in.ip.pel_off = total_pel;
+            in.ip.npel = 0;
+            total_pel += in.pel_cap;
+            int cbs = in.cb[0] * in.cb[1] * in.cb[2];
+            max_cb = std::max(max_cb, cbs);
+            wcap += (size_t)cbs + pc / (size_t)g2p2g_chunk() + 16;
+        }
+        poffs[ninst] = (int)total;
+        cap = total;
+        na = (size_t)total_blocks * 64;
+        ngwg = total_blocks / 4;
+        work_cap = wcap;
+        // Device-order key: (instance, cell block, cell within block); the all-ones
+        // block value marks the unused entries of an instance's range.
+        cell_bits = ceil_log2((uint64_t)max_cb + 1) + 6;
+        end_bit = cell_bits + ceil_log2((uint64_t)ninst);
+        if (end_bit > 63) return fail(MPM_ERR_CAPACITY, "too many instances or cells for the sort key");
+        for (int b = 0; b < 2; ++b) {
+            if (int rc = dalloc(&slab[b], NPF * cap)) return rc;
+            if (int rc = dalloc(&shared[b], NSF * cap)) return rc;
+            if (int rc = dalloc(&ids[b], cap)) return rc;
+            if (int rc = dalloc(&refkey[b], cap)) return rc;
+            if (int rc = dalloc(&nw_list[b], cap)) return rc;
+            if (int rc = dalloc(&keys[b], cap, false)) return rc;
+            if (int rc = dalloc(&order[b], cap, false)) return rc;
+            if (int rc = dalloc(&gmem[b], 7 * na)) return rc;
+            if (int rc = dalloc(&gflag[b], (size_t)total_blocks)) return rc;
+        }
+        if (int rc = dalloc(&gv, 3 * na)) return rc;
+        if (int rc = dalloc(&trS, 3 * na)) return rc;
+        if (int rc = dalloc(&trT, 3 * na)) return rc;
+        if (int rc = dalloc(&blk_inst, (size_t)total_blocks)) return rc;
+        HIP_OK(hipMemcpy(blk_inst, binst.data(), sizeof(int) * binst.size(), hipMemcpyHostToDevice));
+        if (int rc = dalloc(&gpart, (size_t)kGridVals * ngwg)) return rc;
+        if (int rc = dalloc(&gbad, (size_t)ngwg)) return rc;
+        if (int rc = dalloc(&d_ip, (size_t)ninst)) return rc;
+        if (int rc = dalloc(&d_is, (size_t)ninst)) return rc;
+        HIP_OK(hipHostMalloc((void**)&h_is, sizeof(InstState) * ninst, hipHostMallocDefault));
+        std::memset(h_is, 0, sizeof(InstState) * ninst);
+        for (int i = 0; i < ninst; ++i) {
+            h_is[i].mode = M_DONE;
+            h_is[i].dt_scale = 1.0;
+            h_is[i].min_dt_used = std::numeric_limits<double>::infinity();
+        }
+        if (int rc = dalloc(&d_poffs, (size_t)ninst + 1)) return rc;
+        HIP_OK(hipMemcpy(d_poffs, poffs.data(), sizeof(int) * poffs.size(), hipMemcpyHostToDevice));
+        if (int rc = dalloc(&counters, 2)) return rc;
+        if (int rc = dalloc(&d_pel, (size_t)total_pel)) return rc;
+        if (int rc = dalloc(&d_pgeo, (size_t)total_pel)) return rc;
+        if (int rc = dalloc(&d_pacc, (size_t)total_pel)) return rc;
+        if (int rc = dalloc(&d_ptrial, (size_t)total_pel)) return rc;
+        if (total_pel) {
+            std::vector<PelletState> empty(total_pel);
+            for (PelletState& p : empty) {
+                std::memset(&p, 0, sizeof(p));
+                p.inst = -1;
+            }
+            HIP_OK(hipMemcpy(d_pel, empty.data(), sizeof(PelletState) * total_pel, hipMemcpyHostToDevice));
+        }
+        if (int rc = dalloc(&d_work, work_cap)) return rc;
+        if (int rc = dalloc(&pmax, 2 * work_cap)) return rc;
+        if (int rc = dalloc(&pdiss, work_cap)) return rc;
+        if (int rc = dalloc(&pfail, work_cap)) return rc;
+        if (int rc = dalloc(&pesc, work_cap)) return rc;
+        if (int rc = dalloc(&run_unique, cap, false)) return rc;
+        if (int rc = dalloc(&run_count, cap, false)) return rc;
+        if (int rc = dalloc(&d_nruns, 1)) return rc;
+        sort_bytes = std::max(sort_temp_bytes((int)cap, end_bit), rle_temp_bytes((int)cap));
+        HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
+        for (int k = 0; k < kMaxStepsPerSync; ++k) {
+            HIP_OK(hipEventCreate(&ev0[k]));
+            HIP_OK(hipEventCreate(&ev1[k]));
+        }
+        for (int i = 0; i < ninst; ++i)
+            if (int rc = upload_params(i)) return rc;
+        return push();
+    }
+
     ~mpm_engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
         for (int b = 0; b < 2; ++b) {
             (void)hipFree(slab[b]);
             (void)hipFree(shared[b]);
             (void)hipFree(ids[b]);
-            (void)hipFree(keys[b]);
             (void)hipFree(refkey[b]);
             (void)hipFree(nw_list[b]);
+            (void)hipFree(keys[b]);
             (void)hipFree(order[b]);
             (void)hipFree(gmem[b]);
             (void)hipFree(gflag[b]);
@@ -431,401 +519,709 @@ struct mpm_engine {
         (void)hipFree(gv);
         (void)hipFree(trS);
         (void)hipFree(trT);
-        (void)hipFree(d_st);
-        (void)hipFree(nw_count);
+        (void)hipFree(blk_inst);
         (void)hipFree(gpart);
-        (void)hipFree(d_work);
-        (void)hipFree(run_unique);
-        (void)hipFree(run_count);
-        (void)hipFree(d_nruns);
         (void)hipFree(gbad);
+        (void)hipFree(d_ip);
+        (void)hipFree(d_is);
+        if (h_is) (void)hipHostFree(h_is);
+        (void)hipFree(d_poffs);
+        (void)hipFree(counters);
+        (void)hipFree(d_pel);
+        (void)hipFree(d_pgeo);
+        (void)hipFree(d_pacc);
+        (void)hipFree(d_ptrial);
+        (void)hipFree(d_work);
         (void)hipFree(pmax);
         (void)hipFree(pdiss);
         (void)hipFree(pfail);
         (void)hipFree(pesc);
-        if (h_st) (void)hipHostFree(h_st);
-        (void)hipFree(d_pel);
-        (void)hipFree(d_acc);
-        if (h_acc) (void)hipHostFree(h_acc);
-        (void)hipFree(d_bstart);
-        (void)hipFree(d_blist);
+        (void)hipFree(run_unique);
+        (void)hipFree(run_count);
+        (void)hipFree(d_nruns);
         (void)hipFree(sort_temp);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
+        for (int k = 0; k < kMaxStepsPerSync; ++k) {
+            if (ev0[k]) (void)hipEventDestroy(ev0[k]);
+            if (ev1[k]) (void)hipEventDestroy(ev1[k]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // Deposit the committed state into the current accumulator set.
-    int rebuild_grid() {
-        HIP_OK(hipMemsetAsync(gmem[gcur], 0, sizeof(float) * 7 * na, stream));
-        HIP_OK(hipMemsetAsync(gflag[gcur], 0, sizeof(int) * nblocks, stream));
-        launch_p2g(stream, P, (int)n, slab[cur], cap, shared[sh], grid[gcur]);
+    // Sort every instance's particles by 4^3-cell block (device order, for
+    // locality), rebuild the work items and the near-wall lists.  The order keys
+    // of the reference order travel with the particles.  The mirror must be
+    // current; it is current again on return.
+    int sort_all() {
+        BatchDev B = dev();
+        launch_sort_keys(stream, B, cell_bits, keys[0], order[0]);
+        launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)cap, end_bit);
+        launch_permute(stream, B, order[1], shared[sh], shared[sh ^ 1], ids[sh], ids[sh ^ 1], refkey[sh],
+                       refkey[sh ^ 1]);
+        launch_block_runs(stream, sort_temp, sort_bytes, (int)cap, keys[1], keys[0], run_unique, run_count, d_nruns);
         HIP_OK(hipGetLastError());
-        int rc = rebuild_nearwall();
-        if (rc) return rc;
-        grid_valid = true;
-        return MPM_OK;
-    }
-
-    // Near-wall particle list of the committed slab.
-    int rebuild_nearwall() {
-        HIP_OK(hipMemsetAsync(nw_count + cur, 0, sizeof(int), stream));
-        launch_nearwall(stream, P, (int)n, slab[cur], cap, nw_list[cur], nw_count + cur);
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(&nw_host, nw_count + cur, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
-        return MPM_OK;
-    }
-
-    // Pellet geometry and per-block pellet lists for the coupling kernel.
-    int upload_pellets() {
-        size_t np = pellets.size();
-        std::vector<PelletDev> dev(np);
-        std::vector<int> count(nblocks + 1, 0);
-        std::vector<std::vector<int>> ranges(np);
-        for (size_t k = 0; k < np; ++k) {
-            const Pellet& p = pellets[k];
-            PelletDev& d = dev[k];
-            V3 ax = p.axis(), om = p.omega(), lo, hi;
-            p.bounding_box(lo, hi);
-            double margin = 0.0;
-            for (int a = 0; a < 3; ++a) {
-                d.pos[a] = p.pos[a];
-                d.axis[a] = ax[a];
-                d.vel[a] = p.vel[a];
-                d.omega[a] = om[a];
-                d.lo[a] = lo[a];
-                d.hi[a] = hi[a];
-            }
-            d.radius = p.radius;
-            d.spheres = (int)p.offsets.size();
-            d.pad = 0;
-            for (size_t s = 0; s < p.offsets.size() && s < (size_t)kMaxSpheres; ++s) d.offsets[s] = p.offsets[s];
-            int b0[3], b1[3];
-            bool empty = false;
-            for (int a = 0; a < 3; ++a) {
-                // Candidate node coordinates with one node of slack (the kernel re-tests the box exactly).
-                double i0 = std::floor((lo[a] - margin) / dx + 1.0) - 1.0,
-                       i1 = std::ceil((hi[a] + margin) / dx + 1.0) + 1.0;
-                i0 = std::max(i0, 0.0);
-                i1 = std::min(i1, (double)(cells[a] + 2));
-                if (!(i0 <= i1)) empty = true;
-                b0[a] = empty ? 0 : (int)i0 >> 2;
-                b1[a] = empty ? -1 : (int)i1 >> 2;
-            }
-            if (empty) continue;
-            for (int bi = b0[0]; bi <= b1[0]; ++bi)
-                for (int bj = b0[1]; bj <= b1[1]; ++bj)
-                    for (int bk = b0[2]; bk <= b1[2]; ++bk) {
-                        int b = (bi * P.nb[1] + bj) * P.nb[2] + bk;
-                        ranges[k].push_back(b);
-                        count[b + 1] += 1;
-                    }
-        }
-        for (int b = 0; b < nblocks; ++b) count[b + 1] += count[b];
-        std::vector<int> list(std::max(count[nblocks], 1));
-        std::vector<int> fillp(count.begin(), count.end() - 1);
-        for (size_t k = 0; k < np; ++k)
-            for (int b : ranges[k]) list[fillp[b]++] = (int)k;
-        if ((size_t)list.size() > blist_cap) {
-            (void)hipFree(d_blist);
-            blist_cap = list.size() * 2;
-            HIP_OK(hipMalloc(&d_blist, sizeof(int) * blist_cap));
-        }
-        HIP_OK(hipMemcpyAsync(d_pel, dev.data(), sizeof(PelletDev) * np, hipMemcpyHostToDevice, stream));
-        HIP_OK(hipMemcpyAsync(d_bstart, count.data(), sizeof(int) * (nblocks + 1), hipMemcpyHostToDevice, stream));
-        HIP_OK(hipMemcpyAsync(d_blist, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, stream));
-        HIP_OK(hipMemsetAsync(d_acc, 0, sizeof(PelletAccum) * np, stream));
-        // The host vectors must outlive the asynchronous copies.
-        HIP_OK(hipStreamSynchronize(stream));
-        return MPM_OK;
-    }
-
-    // solver.rs:477-498 (one contact limit per pellet).
-    double stability_limit() const {
-        double limit = cfg.cfl * dx / (wave_speed(mat) + max_speed);
-        double diff = diffusivity(mat);
-        if (diff > 0.0) limit = std::min(limit, 0.25 * dx * dx / diff);
-        for (const Pellet& p : pellets) {
-            limit = std::min(limit, 0.2 * std::sqrt(p.mass / cfg.contact_stiffness));
-            double s = norm(p.vel);
-            if (s > 0.0) limit = std::min(limit, cfg.cfl * dx / s);
-        }
-        return limit;
-    }
-
-    // Sort by 4^3-cell block (the device order) and rebuild the work items.  At the
-    // reference's 32-step cadence (solver.rs:501-517) the reference order key is
-    // refreshed too; keep_ref re-sorts for locality only, leaving the reference
-    // order of the state unchanged.  The grid sums are order free.
-    int sort_particles(int keep_ref = 0) {
-        if (n == 0) {
-            nwork = 0;
-            return MPM_OK;
-        }
-        launch_sort_keys(stream, P, (int)n, slab[cur], cap, ids[sh], keys[0], order[0], refkey[sh], keep_ref);
-        launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)n, key_bits);
-        launch_permute(stream, (int)n, order[1], slab[cur], slab[cur ^ 1], cap, shared[sh], shared[sh ^ 1], ids[sh],
-                       ids[sh ^ 1], refkey[sh], refkey[sh ^ 1]);
-        launch_block_runs(stream, sort_temp, sort_bytes, (int)n, keys[1], order[0], run_unique, run_count, d_nruns);
-        HIP_OK(hipGetLastError());
-        cur ^= 1;
-        sh ^= 1;
         int nruns = 0;
         HIP_OK(hipMemcpyAsync(&nruns, d_nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
-        std::vector<uint32_t> uq(nruns), cnt(nruns);
-        HIP_OK(hipMemcpyAsync(uq.data(), run_unique, sizeof(uint32_t) * nruns, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipMemcpyAsync(cnt.data(), run_count, sizeof(uint32_t) * nruns, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
-        // Merge at most two z-adjacent blocks per work item; split runs longer than a chunk.
+        std::vector<uint64_t> uq(nruns);
+        std::vector<uint32_t> cnt(nruns);
+        if (nruns) {
+            HIP_OK(hipMemcpyAsync(uq.data(), run_unique, sizeof(uint64_t) * nruns, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipMemcpyAsync(cnt.data(), run_count, sizeof(uint32_t) * nruns, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+        }
+        sh ^= 1;
+        for (int i = 0; i < ninst; ++i) h_is[i].cur ^= 1;
+        // Work items: runs of one cell block, merged along z up to launch.merge_blocks
+        // blocks of one instance and one chunk of particles; longer runs split.
         const int chunk = g2p2g_chunk();
-        std::vector<int2> work;
-        work.reserve(nruns + n / chunk + 1);
-        int start = 0;
-        int cur_start = -1, cur_count = 0, cur_blocks = 0;
-        uint32_t prev = 0;
+        const int shift = cell_bits - 6;
+        const uint64_t bmask = (1ull << shift) - 1;
+        std::vector<int4> work;
+        work.reserve(nruns + cap / chunk + 1);
+        int start = 0, ci = -1, cstart = -1, ccount = 0, cblocks = 0;
+        uint64_t prev = 0;
         auto flush = [&]() {
-            if (cur_start >= 0) work.push_back(make_int2(cur_start, cur_count));
-            cur_start = -1;
-            cur_count = 0;
-            cur_blocks = 0;
+            if (cstart >= 0) work.push_back(make_int4(ci, cstart, ccount, 0));
+            cstart = -1;
+            ccount = 0;
+            cblocks = 0;
         };
         for (int r = 0; r < nruns; ++r) {
             int c = (int)cnt[r];
-            uint32_t b = uq[r];
-            bool adjacent = cur_blocks == 1 && b == prev + 1 && (b % (uint32_t)cb[2]) != 0 && cur_count + c <= chunk;
+            int in = (int)(uq[r] >> shift);
+            uint64_t blk = uq[r] & bmask;
+            if (blk == bmask || in >= ninst) {
+                flush();
+                start += c;
+                continue;
+            }
+            uint64_t cbz = (uint64_t)inst[in].cb[2];
+            bool adjacent = cstart >= 0 && ci == in && cblocks < launch.merge_blocks && blk == prev + 1 &&
+                            blk % cbz != 0 && ccount + c <= chunk;
             if (!adjacent) flush();
             if (c > chunk) {
-                for (int off = 0; off < c; off += chunk) work.push_back(make_int2(start + off, std::min(chunk, c - off)));
+                for (int off = 0; off < c; off += chunk)
+                    work.push_back(make_int4(in, start + off, std::min(chunk, c - off), 0));
             } else if (adjacent) {
-                cur_count += c;
-                cur_blocks = 2;
+                ccount += c;
+                cblocks += 1;
             } else {
-                cur_start = start;
-                cur_count = c;
-                cur_blocks = 1;
+                ci = in;
+                cstart = start;
+                ccount = c;
+                cblocks = 1;
             }
-            prev = b;
+            prev = blk;
             start += c;
         }
         flush();
         if (work.size() > work_cap) return fail(MPM_ERR_CAPACITY, "work list overflow");
         nwork = (int)work.size();
-        HIP_OK(hipMemcpyAsync(d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, stream));
-        HIP_OK(hipStreamSynchronize(stream));
-        return rebuild_nearwall();
+        for (int i = 0; i < ninst; ++i) {
+            h_is[i].w_off = 0;
+            h_is[i].w_cnt = 0;
+        }
+        for (int w = nwork - 1; w >= 0; --w) {
+            InstState& s = h_is[work[w].x];
+            s.w_off = w;
+            s.w_cnt += 1;
+        }
+        for (int i = 0; i < ninst; ++i) h_is[i].nw_count[h_is[i].cur] = 0;
+        if (nwork) HIP_OK(hipMemcpyAsync(d_work, work.data(), sizeof(int4) * nwork, hipMemcpyHostToDevice, stream));
+        if (int rc = push()) return rc;
+        launch_nearwall(stream, dev());
+        HIP_OK(hipGetLastError());
+        if (int rc = pull()) return rc;
+        dirty = false;
+        graph_stale = true;
+        return MPM_OK;
     }
 
-    int try_step(double dt, mpm_step_report* report);
-    int remove_escaped();
-    std::vector<uint64_t> host_ref;
-    int download(std::vector<float>& s, std::vector<float>& shv, std::vector<uint32_t>& id) {
-        s.resize(NPF * n);
-        shv.resize(NSF * n);
-        id.resize(n);
-        host_ref.resize(n);
-        if (n) HIP_OK(hipMemcpyAsync(host_ref.data(), refkey[sh], sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
+    // Deposit instance i's committed state into its current accumulator set
+    // (the reference rebuilds the grid at the start of a step from the particles).
+    int rebuild_grid(int i) {
+        Instance& in = inst[i];
+        const InstState& s = h_is[i];
+        int g = s.gcur;
+        size_t noff = (size_t)in.ip.boff * 64, nn = (size_t)in.ip.nblk * 64;
+        for (int f = 0; f < 7; ++f) HIP_OK(hipMemsetAsync(gmem[g] + f * na + noff, 0, sizeof(float) * nn, stream));
+        HIP_OK(hipMemsetAsync(gflag[g] + in.ip.boff, 0, sizeof(int) * in.ip.nblk, stream));
+        launch_p2g(stream, dev(), in.ip.P, in.ip.poff, s.n, in.ip.boff, s.cur, g);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(stream));
+        in.grid_valid = true;
+        return MPM_OK;
+    }
+
+    int prepare() {
+        if (dirty)
+            if (int rc = sort_all()) return rc;
+        for (int i = 0; i < ninst; ++i)
+            if (!inst[i].grid_valid)
+                if (int rc = rebuild_grid(i)) return rc;
+        return MPM_OK;
+    }
+
+    // Copy instance i's committed particles to the host (mirror current).
+    int download(int i, HostParticles& h) {
+        const Instance& in = inst[i];
+        size_t n = (size_t)h_is[i].n, off = (size_t)in.ip.poff;
+        int cur = h_is[i].cur;
+        h.n = n;
+        h.s.assign(NPF * n, 0.0f);
+        h.sh.assign(NSF * n, 0.0f);
+        h.id.assign(n, 0);
+        h.ref.assign(n, 0);
+        if (n == 0) return MPM_OK;
         for (int f = 0; f < NPF; ++f)
-            HIP_OK(hipMemcpyAsync(s.data() + f * n, slab[cur] + f * cap, sizeof(float) * n, hipMemcpyDeviceToHost,
-                                  stream));
-        for (int f = 0; f < NSF; ++f)
-            HIP_OK(hipMemcpyAsync(shv.data() + f * n, shared[sh] + f * cap, sizeof(float) * n,
+            HIP_OK(hipMemcpyAsync(h.s.data() + f * n, slab[cur] + f * cap + off, sizeof(float) * n,
                                   hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipMemcpyAsync(id.data(), ids[sh], sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream));
+        for (int f = 0; f < NSF; ++f)
+            HIP_OK(hipMemcpyAsync(h.sh.data() + f * n, shared[sh] + f * cap + off, sizeof(float) * n,
+                                  hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(h.id.data(), ids[sh] + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(h.ref.data(), refkey[sh] + off, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
         return MPM_OK;
     }
-    int upload(const std::vector<float>& s, const std::vector<float>& shv, const std::vector<uint32_t>& id) {
-        for (int b = 0; b < 2; ++b) {
-            for (int f = 0; f < NPF; ++f)
-                HIP_OK(hipMemcpyAsync(slab[b] + f * cap, s.data() + f * n, sizeof(float) * n, hipMemcpyHostToDevice,
-                                      stream));
+    // Write instance i's particles into both slabs (and the current shared slab).
+    int upload(int i, const HostParticles& h) {
+        const Instance& in = inst[i];
+        size_t n = h.n, off = (size_t)in.ip.poff;
+        if (n) {
+            for (int b = 0; b < 2; ++b)
+                for (int f = 0; f < NPF; ++f)
+                    HIP_OK(hipMemcpyAsync(slab[b] + f * cap + off, h.s.data() + f * n, sizeof(float) * n,
+                                          hipMemcpyHostToDevice, stream));
+            for (int f = 0; f < NSF; ++f)
+                HIP_OK(hipMemcpyAsync(shared[sh] + f * cap + off, h.sh.data() + f * n, sizeof(float) * n,
+                                      hipMemcpyHostToDevice, stream));
+            HIP_OK(hipMemcpyAsync(ids[sh] + off, h.id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
+            HIP_OK(hipMemcpyAsync(refkey[sh] + off, h.ref.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice,
+                                  stream));
         }
-        for (int f = 0; f < NSF; ++f)
-            HIP_OK(hipMemcpyAsync(shared[sh] + f * cap, shv.data() + f * n, sizeof(float) * n,
-                                  hipMemcpyHostToDevice, stream));
-        if (n) HIP_OK(hipMemcpyAsync(ids[sh], id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
-        if (n) HIP_OK(hipMemcpyAsync(refkey[sh], host_ref.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
+        h_is[i].n = (int)n;
+        dirty = true;
+        return MPM_OK;
+    }
+    int download_pellets(int i, std::vector<PelletState>& out) {
+        const Instance& in = inst[i];
+        out.resize(in.ip.npel);
+        if (in.ip.npel) {
+            HIP_OK(hipMemcpyAsync(out.data(), d_pel + in.ip.pel_off, sizeof(PelletState) * in.ip.npel,
+                                  hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+        }
+        return MPM_OK;
+    }
+
+    // Remove instance i's particles that left the padded grid (sweep_outflow,
+    // solver.rs:520-532), booking their mass and momentum as outflow.
+    int remove_escaped(int i) {
+        Instance& in = inst[i];
+        HostParticles h, k;
+        if (int rc = download(i, h)) return rc;
+        size_t n = h.n, keep = 0;
+        std::vector<char> inside(n, 1);
+        double out_mass = 0.0, out_mom[3] = {0.0, 0.0, 0.0};
+        for (size_t p = 0; p < n; ++p) {
+            for (int a = 0; a < 3; ++a) {
+                double sc = (double)h.s[(PX + a) * n + p] / in.dx;
+                double padded = std::floor(sc - 0.5) + 1.0;
+                if (std::isnan(padded) || padded < 0.0 || padded + 2.0 >= (double)(in.cells[a] + 3)) inside[p] = 0;
+            }
+            if (!inside[p]) {
+                double m = h.sh[SMASS * n + p];
+                out_mass += m;
+                for (int a = 0; a < 3; ++a) out_mom[a] += (double)h.s[(VX + a) * n + p] * m;
+            } else {
+                ++keep;
+            }
+        }
+        k.n = keep;
+        k.s.resize(NPF * keep);
+        k.sh.resize(NSF * keep);
+        k.id.resize(keep);
+        k.ref.resize(keep);
+        size_t q = 0;
+        for (size_t p = 0; p < n; ++p) {
+            if (!inside[p]) continue;
+            for (int f = 0; f < NPF; ++f) k.s[f * keep + q] = h.s[f * n + p];
+            for (int f = 0; f < NSF; ++f) k.sh[f * keep + q] = h.sh[f * n + p];
+            k.id[q] = h.id[p];
+            k.ref[q] = h.ref[p];
+            ++q;
+        }
+        InstState& s = h_is[i];
+        s.ledger[1] += out_mass;
+        for (int a = 0; a < 3; ++a) s.ledger[28 + a] += out_mom[a];
+        in.ip.particle_mass -= out_mass;
+        if (int rc = upload_params(i)) return rc;
+        return upload(i, k);
+    }
+
+    // Enqueue `k` trial steps of every running instance.
+    int enqueue(int k) {
+        BatchDev B = dev();
+        bool pel = any_pellets();
+        if (launch.use_graph && !profiling) {
+            if (graph_stale || graph_steps != k || !graph) {
+                if (graph) {
+                    HIP_OK(hipGraphExecDestroy(graph));
+                    graph = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                for (int j = 0; j < k; ++j) launch_step(stream, B, launch.traction_wgs, pel, nullptr, nullptr);
+                HIP_OK(hipStreamEndCapture(stream, &g));
+                hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_OK(e);
+                graph_steps = k;
+                graph_stale = false;
+            }
+            HIP_OK(hipGraphLaunch(graph, stream));
+        } else {
+            for (int j = 0; j < k; ++j)
+                launch_step(stream, B, launch.traction_wgs, pel, profiling ? ev0[j] : nullptr,
+                            profiling ? ev1[j] : nullptr);
+        }
+        HIP_OK(hipGetLastError());
+        return MPM_OK;
+    }
+    int collect_profile(int k) {
+        if (!profiling) return MPM_OK;
+        for (int j = 0; j < k; ++j) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, ev0[j], ev1[j]));
+            g2p2g_ms += ms;
+            g2p2g_launches += 1;
+        }
+        return MPM_OK;
+    }
+
+    std::string abort_message(int i) const {
+        const InstState& s = h_is[i];
+        char buf[320];
+        if (s.err == E_MIN_AFTER_REJECT)
+            std::snprintf(buf, sizeof(buf),
+                          "solver aborted at t='%.17g' s: step '%.17g' s below minimum after '%llu' rejections; "
+                          "last failure code %d",
+                          s.time, s.dt, (unsigned long long)s.rejected_steps, s.last_failure);
+        else
+            std::snprintf(buf, sizeof(buf), "solver aborted at t='%.17g' s: step '%.17g' s below minimum", s.time,
+                          s.dt);
+        std::string msg(buf);
+        if (ninst > 1) msg = "instance " + std::to_string(i) + ": " + msg;
+        return msg;
+    }
+
+    // Run the instances in M_RUN until none is left running, `k_sync` trial steps
+    // per host synchronisation, at most `max_rounds` synchronisations (< 0: no
+    // limit).  status/done_ms per instance (may be null).
+    int run(int k_sync, int* status, double* done_ms, int max_rounds = -1) {
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<char> recorded(ninst, 0);
+        int since_sort = 0, rounds = 0;
+        bool first_error = true;
+        for (int i = 0; i < ninst; ++i)
+            if (h_is[i].mode != M_RUN) recorded[i] = 1;
+        while (max_rounds < 0 || rounds < max_rounds) {
+            bool running = false;
+            int most = 0;  // largest step budget left among the running instances
+            for (int i = 0; i < ninst; ++i)
+                if (h_is[i].mode == M_RUN) {
+                    running = true;
+                    most = std::max(most, h_is[i].trial_dt > 0.0 ? 1 : h_is[i].budget);
+                }
+            if (!running) break;
+            // No launches past the last budgeted step (they would find nothing to do).
+            const int k = std::max(1, std::min(k_sync, most));
+            if (int rc = enqueue(k)) return rc;
+            HIP_OK(hipStreamSynchronize(stream));
+            if (int rc = collect_profile(k)) return rc;
+            if (int rc = pull()) return rc;
+            since_sort += k;
+            rounds += 1;
+            double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            bool changed = false;
+            for (int i = 0; i < ninst; ++i) {
+                InstState& s = h_is[i];
+                if (s.mode == M_PAUSE && (s.pause & P_ESCAPED)) {
+                    if (int rc = remove_escaped(i)) return rc;
+                    s.pause &= ~P_ESCAPED;
+                    s.mode = (s.pause & P_BUDGET) ? M_PAUSE : M_RUN;
+                    changed = true;
+                }
+                // The transitions the control kernel would make at the next trial step.
+                if (s.mode == M_RUN && s.trial_dt == 0.0) {
+                    if (!(s.time < s.target)) {
+                        s.time = s.target;
+                        s.mode = M_DONE;
+                        changed = true;
+                    } else if (s.budget <= 0) {
+                        s.mode = M_PAUSE;
+                        s.pause |= P_BUDGET;
+                        changed = true;
+                    }
+                }
+                if (recorded[i] || s.mode == M_RUN) continue;
+                recorded[i] = 1;
+                int st = s.mode == M_DONE ? 0 : (s.mode == M_ERROR ? MPM_ERR_MIN_STEP : 1);
+                if (st == MPM_ERR_MIN_STEP && first_error) {
+                    err = abort_message(i);
+                    first_error = false;
+                }
+                if (status) status[i] = st;
+                if (done_ms) done_ms[i] = ms;
+            }
+            if (dirty || since_sort >= kLocalitySortInterval) {
+                if (changed)
+                    if (int rc = push()) return rc;
+                if (int rc = sort_all()) return rc;
+                since_sort = 0;
+            } else if (changed) {
+                if (int rc = push()) return rc;
+            }
+        }
+        return MPM_OK;
+    }
+
+    // Host copy of Simulation::stability_limit (solver.rs:477-498).
+    double stability_limit(int i) {
+        const Instance& in = inst[i];
+        if (pull()) return std::numeric_limits<double>::quiet_NaN();
+        std::vector<PelletState> pel;
+        if (download_pellets(i, pel)) return std::numeric_limits<double>::quiet_NaN();
+        double limit = in.cfg.cfl * in.dx / (in.ip.wave + h_is[i].max_speed);
+        if (in.ip.diff > 0.0) limit = std::min(limit, 0.25 * in.dx * in.dx / in.ip.diff);
+        for (const PelletState& p : pel) {
+            limit = std::min(limit, 0.2 * std::sqrt(p.mass / in.cfg.contact_stiffness));
+            double s = std::sqrt(p.vel[0] * p.vel[0] + p.vel[1] * p.vel[1] + p.vel[2] * p.vel[2]);
+            if (s > 0.0) limit = std::min(limit, in.cfg.cfl * in.dx / s);
+        }
+        return limit;
+    }
+
+    // Device copy of every mutable buffer, for the autotuner's trial runs.
+    struct Snapshot {
+        std::vector<std::pair<void*, size_t>> bufs;  // (backup, bytes), in the order of targets()
+        std::vector<InstState> mirror;
+        int sh = 0;
+    };
+    std::vector<std::pair<void*, size_t>> targets() {
+        std::vector<std::pair<void*, size_t>> t;
+        for (int b = 0; b < 2; ++b) {
+            t.push_back({slab[b], sizeof(float) * NPF * cap});
+            t.push_back({shared[b], sizeof(float) * NSF * cap});
+            t.push_back({ids[b], sizeof(uint32_t) * cap});
+            t.push_back({refkey[b], sizeof(uint64_t) * cap});
+            t.push_back({gmem[b], sizeof(float) * 7 * na});
+            t.push_back({gflag[b], sizeof(int) * (size_t)total_blocks});
+        }
+        t.push_back({d_pel, sizeof(PelletState) * (size_t)std::max(total_pel, 1)});
+        return t;
+    }
+    int snapshot(Snapshot& snap) {
+        snap.mirror.assign(h_is, h_is + ninst);
+        snap.sh = sh;
+        for (auto& t : targets()) {
+            void* b = nullptr;
+            HIP_OK(hipMalloc(&b, t.second));
+            snap.bufs.push_back({b, t.second});
+            HIP_OK(hipMemcpyAsync(b, t.first, t.second, hipMemcpyDeviceToDevice, stream));
+        }
+        HIP_OK(hipStreamSynchronize(stream));
+        return MPM_OK;
+    }
+    int restore(const Snapshot& snap) {
+        auto t = targets();
+        for (size_t k = 0; k < t.size(); ++k)
+            HIP_OK(hipMemcpyAsync(t[k].first, snap.bufs[k].first, t[k].second, hipMemcpyDeviceToDevice, stream));
+        std::copy(snap.mirror.begin(), snap.mirror.end(), h_is);
+        sh = snap.sh;
+        dirty = true;
+        return push();
+    }
+    static void release(Snapshot& snap) {
+        for (auto& b : snap.bufs) (void)hipFree(b.first);
+        snap.bufs.clear();
+    }
+
+    // Time `steps` trial steps of every instance under each candidate launch shape,
+    // from the same state, and keep the fastest.
+    int autotune(int steps, mpm_launch* chosen) {
+        if (int rc = pull()) return rc;
+        if (int rc = prepare()) return rc;
+        Snapshot snap;
+        int rc = snapshot(snap);
+        bool prof = profiling;
+        profiling = false;
+        std::vector<mpm_launch> cands;
+        for (int k : {8, 32})
+            for (int g : {0, 1})
+                for (int m : {1, 2, 4}) cands.push_back(mpm_launch{k, g, m, launch.traction_wgs});
+        mpm_launch best = launch;
+        double best_ms = std::numeric_limits<double>::infinity();
+        for (const mpm_launch& c : cands) {
+            if (rc) break;
+            if ((rc = restore(snap))) break;
+            launch = c;
+            graph_stale = true;
+            if ((rc = prepare())) break;
+            for (int i = 0; i < ninst; ++i) {
+                InstState& s = h_is[i];
+                s.target = std::numeric_limits<double>::max();
+                s.mode = s.n > 0 ? M_RUN : M_DONE;
+                s.pause = 0;
+                s.trial_dt = 0.0;
+                s.budget = steps;
+            }
+            if ((rc = push())) break;
+            auto t0 = std::chrono::steady_clock::now();
+            if ((rc = run(std::min(c.steps_per_sync, steps), nullptr, nullptr))) break;
+            double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        if (!rc) rc = restore(snap);
+        release(snap);
+        profiling = prof;
+        launch = best;
+        graph_stale = true;
+        dirty = true;
+        if (!rc) rc = prepare();
+        if (chosen) *chosen = best;
+        return rc;
+    }
+
+    int set_state(int i, const mpm_particles* ps, const mpm_pellet* pellets, size_t n_pellets,
+                  const mpm_ledger* ledger, const mpm_history* history);
+    int get_state(int i, mpm_particles* ps, mpm_pellet* pellets, size_t pellet_capacity, mpm_ledger* ledger,
+                  mpm_history* history);
+    int energies(int i, double* ke, double* pe, double* el) {
+        if (int rc = pull()) return rc;
+        HostParticles h;
+        if (int rc = download(i, h)) return rc;
+        std::vector<PelletState> pel;
+        if (int rc = download_pellets(i, pel)) return rc;
+        energies_of(inst[i].mat, inst[i].cfg.gravity, h, pel, ke, pe, el);
         return MPM_OK;
     }
 };
 
-namespace {
-
-float bits_float(unsigned int b) {
-    float f;
-    std::memcpy(&f, &b, sizeof(f));
-    return f;
-}
-
-}  // namespace
-
-// Remove particles that left the padded grid (sweep_outflow, solver.rs:520-532).
-int mpm_engine::remove_escaped() {
-    std::vector<float> s, shv;
-    std::vector<uint32_t> id;
-    int rc = download(s, shv, id);
-    if (rc) return rc;
-    size_t keep = 0;
-    std::vector<float> s2(NPF * n), sh2(NSF * n);
-    std::vector<uint32_t> id2(n);
-    std::vector<uint64_t> ref2(n);
-    double out_mass = 0.0;
-    V3 out_mom{{0, 0, 0}};
+int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pellets, size_t n_pellets,
+                          const mpm_ledger* ledger, const mpm_history* history) {
+    Instance& in = inst[i];
+    if (!ps) return fail(MPM_ERR_INVALID, "particles are required");
+    if (ps->n > (size_t)in.ip.pcap) return fail(MPM_ERR_CAPACITY, "more particles than the instance capacity");
+    if (n_pellets > (size_t)in.pel_cap) return fail(MPM_ERR_CAPACITY, "more pellets than the instance capacity");
+    if (n_pellets && !pellets) return fail(MPM_ERR_INVALID, "pellet array is NULL");
+    size_t n = ps->n;
+    if (n && (!ps->position || !ps->mass || !ps->volume0))
+        return fail(MPM_ERR_INVALID, "position, mass and volume0 are required");
+    if (int rc = pull()) return rc;
+    HostParticles h;
+    h.n = n;
+    h.s.assign(NPF * n, 0.0f);
+    h.sh.assign(NSF * n, 0.0f);
+    h.id.resize(n);
+    h.ref.resize(n);
+    double max_speed = 0.0, mass = 0.0;
     for (size_t p = 0; p < n; ++p) {
-        bool inside = true;
+        h.id[p] = ps->id ? ps->id[p] : (uint32_t)p;
+        h.ref[p] = p;  // until the first 32-step sort the reference order is the given order
         for (int a = 0; a < 3; ++a) {
-            double sc = (double)s[(PX + a) * n + p] / dx;
-            double bf = std::floor(sc - 0.5);
-            double padded = bf + 1.0;
-            if (std::isnan(padded) || padded < 0.0 || padded + 2.0 >= (double)(cells[a] + 3)) inside = false;
+            h.s[(PX + a) * n + p] = (float)ps->position[3 * p + a];
+            h.s[(VX + a) * n + p] = ps->velocity ? (float)ps->velocity[3 * p + a] : 0.0f;
         }
-        if (!inside) {
-            double m = shv[SMASS * n + p];
-            out_mass += m;
-            for (int a = 0; a < 3; ++a) out_mom[a] += (double)s[(VX + a) * n + p] * m;
-            continue;
+        double vs = 0.0;
+        for (int a = 0; a < 3; ++a) vs += (double)h.s[(VX + a) * n + p] * (double)h.s[(VX + a) * n + p];
+        max_speed = std::max(max_speed, std::sqrt(vs));
+        for (int a = 0; a < 9; ++a) {
+            h.s[(C00 + a) * n + p] = ps->affine ? (float)ps->affine[9 * p + a] : 0.0f;
+            double F = ps->deformation ? ps->deformation[9 * p + a] : (a % 4 == 0 ? 1.0 : 0.0);
+            h.s[(H00 + a) * n + p] = (float)(F - (a % 4 == 0 ? 1.0 : 0.0));
         }
-        for (int f = 0; f < NPF; ++f) s2[f * n + keep] = s[f * n + p];
-        for (int f = 0; f < NSF; ++f) sh2[f * n + keep] = shv[f * n + p];
-        id2[keep] = id[p];
-        ref2[keep] = host_ref[p];
-        ++keep;
+        h.s[JM1 * n + p] = ps->volume_ratio ? (float)(ps->volume_ratio[p] - 1.0) : 0.0f;
+        if (ps->kirchhoff) {
+            const double* t = ps->kirchhoff + 9 * p;
+            h.s[TXX * n + p] = (float)t[0];
+            h.s[TYY * n + p] = (float)t[4];
+            h.s[TZZ * n + p] = (float)t[8];
+            h.s[TXY * n + p] = (float)(0.5 * (t[1] + t[3]));
+            h.s[TXZ * n + p] = (float)(0.5 * (t[2] + t[6]));
+            h.s[TYZ * n + p] = (float)(0.5 * (t[5] + t[7]));
+        }
+        float eps = ps->plastic_strain ? (float)ps->plastic_strain[p] : 0.0f;
+        h.s[EPS * n + p] = eps;
+        h.sh[SEPS * n + p] = eps;
+        h.sh[SMASS * n + p] = (float)ps->mass[p];
+        h.sh[SVOL0 * n + p] = (float)ps->volume0[p];
+        mass += (double)h.sh[SMASS * n + p];
     }
-    size_t old = n;
-    n = keep;
-    std::vector<float> s3(NPF * n), sh3(NSF * n);
-    for (int f = 0; f < NPF; ++f) std::copy(s2.begin() + f * old, s2.begin() + f * old + n, s3.begin() + f * n);
-    for (int f = 0; f < NSF; ++f) std::copy(sh2.begin() + f * old, sh2.begin() + f * old + n, sh3.begin() + f * n);
-    id2.resize(n);
-    ref2.resize(n);
-    host_ref = ref2;
-    ledger.outflow_mass += out_mass;
-    particle_mass -= out_mass;
-    for (int a = 0; a < 3; ++a) ledger.outflow_momentum[a] += out_mom[a];
-    rc = upload(s3, sh3, id2);
-    if (rc) return rc;
-    return sort_particles(1);
+    // Pellets: the instance's slots, unused ones marked free.
+    std::vector<PelletState> pel(in.pel_cap);
+    double contact_dt = std::numeric_limits<double>::infinity();
+    for (int k = 0; k < in.pel_cap; ++k) {
+        PelletState& d = pel[k];
+        std::memset(&d, 0, sizeof(d));
+        d.inst = -1;
+        if ((size_t)k >= n_pellets) continue;
+        const mpm_pellet& src = pellets[k];
+        d.inst = i;
+        d.radius = src.radius;
+        d.length = src.length;
+        d.mass = src.mass;
+        for (int a = 0; a < 3; ++a) {
+            d.inertia[a] = src.inertia_body[a];
+            d.pos[a] = src.position[a];
+            d.vel[a] = src.velocity[a];
+            d.L[a] = src.angular_momentum[a];
+        }
+        for (int a = 0; a < 4; ++a) d.q[a] = src.orientation[a];
+        d.spheres = std::min(std::max(src.sphere_count, 0), kMaxSpheres);
+        for (int s = 0; s < d.spheres; ++s) d.offsets[s] = src.sphere_offsets[s];
+        contact_dt = std::min(contact_dt, 0.2 * std::sqrt(d.mass / in.cfg.contact_stiffness));
+    }
+    if (in.pel_cap) {
+        HIP_OK(hipMemcpyAsync(d_pel + in.ip.pel_off, pel.data(), sizeof(PelletState) * in.pel_cap,
+                              hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    pel.resize(n_pellets);
+    in.ip.npel = (int)n_pellets;
+    in.ip.particle_mass = mass;
+    in.ip.pel_contact_dt = contact_dt;
+    if (int rc = upload_params(i)) return rc;
+    if (int rc = upload(i, h)) return rc;
+    InstState& s = h_is[i];
+    std::memset(&s, 0, sizeof(s));
+    s.n = (int)n;
+    s.mode = M_DONE;
+    s.max_speed = max_speed;
+    if (history) {
+        s.time = history->time;
+        s.step_count = history->step_count;
+        s.limited_steps = history->limited_steps;
+        s.rejected_steps = history->rejected_steps;
+        s.dt_scale = history->dt_scale;
+        s.min_dt_used = history->min_dt_used;
+        s.max_dt_used = history->max_dt_used;
+    } else {
+        s.dt_scale = 1.0;
+        s.min_dt_used = std::numeric_limits<double>::infinity();
+        s.max_dt_used = 0.0;
+    }
+    s.target = s.time;
+    if (ledger) {
+        std::memcpy(s.ledger, ledger, sizeof(mpm_ledger));
+    } else {
+        // Simulation::new seeds the initial inventory, solver.rs:357-390.
+        double mom[3] = {0.0, 0.0, 0.0};
+        for (size_t p = 0; p < n; ++p)
+            for (int a = 0; a < 3; ++a) mom[a] += (double)h.s[(VX + a) * n + p] * (double)h.sh[SMASS * n + p];
+        for (const PelletState& p : pel)
+            for (int a = 0; a < 3; ++a) mom[a] += p.vel[a] * p.mass;
+        s.ledger[0] = mass;
+        for (int a = 0; a < 3; ++a) s.ledger[2 + a] = mom[a];
+        double ke = 0, pe = 0, el = 0;
+        energies_of(in.mat, in.cfg.gravity, h, pel, &ke, &pe, &el);
+        s.ledger[5] = ke + pe + el;
+    }
+    in.grid_valid = false;
+    dirty = true;
+    graph_stale = true;
+    return push();
 }
 
-// Simulation::try_step_captured (solver.rs:540-611) on the device.
-int mpm_engine::try_step(double dt, mpm_step_report* report) {
-    if (report) *report = mpm_step_report{0, 0, 0.0, dt};
-    if (!grid_valid) {
-        int rc = rebuild_grid();
-        if (rc) return rc;
-    }
-    float fdt = (float)dt;
-    launch_reset_status(stream, d_st, nw_count + (cur ^ 1));
-    launch_traction(stream, P, fdt, nw_list[cur], nw_count + cur, nw_host, slab[cur], cap, shared[sh], grid[gcur],
-                    trS, trT, na);
-    launch_grid_update(stream, P, fdt, nblocks, grid[gcur], trS, trT, gv, na, grid[gcur ^ 1], gpart, gbad);
-    HIP_OK(hipGetLastError());
-    std::vector<Pellet> trial = pellets;
-    if (!pellets.empty()) {
-        int rc = upload_pellets();
-        if (rc) return rc;
-        launch_couple(stream, P, dx, nblocks, grid[gcur], gv, na, d_pel, (int)pellets.size(), d_bstart, d_blist,
-                      d_acc, cfg.wall_friction);
-        HIP_OK(hipGetLastError());
-    }
-    launch_g2p2g(stream, P, fdt, (int)n, slab[cur], slab[cur ^ 1], cap, shared[sh], gv, na, grid[gcur ^ 1], d_work,
-                 nwork, pmax, pdiss, pfail, pesc, nw_list[cur ^ 1], nw_count + (cur ^ 1), profiling ? ev0 : nullptr,
-                 profiling ? ev1 : nullptr);
-    launch_finalize(stream, ngwg, gpart, gbad, nwork, pmax, pdiss, pfail, pesc, nw_count + (cur ^ 1), d_st);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(h_st, d_st, sizeof(StepStatus), hipMemcpyDeviceToHost, stream));
-    if (!pellets.empty())
-        HIP_OK(hipMemcpyAsync(h_acc, d_acc, sizeof(PelletAccum) * pellets.size(), hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipStreamSynchronize(stream));
-    if (profiling) {
-        float ms = 0.0f;
-        HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
-        g2p2g_ms += ms;
-        g2p2g_launches += 1;
-    }
-    const StepStatus& st = *h_st;
-    double max_cells = (double)bits_float(st.max_disp_bits);
-    if (report) report->max_cells = max_cells;
-    int code = 0;
-    if (!std::isfinite(st.traction_energy) || !std::isfinite(ledger.wall_normal_traction_energy + st.traction_energy))
-        code = F_TRACTION;
-    else if (st.grid_nonfinite)
-        code = F_GRID;
-    std::vector<double> pellet_energy(pellets.size(), 0.0);
-    std::vector<WallContact> walls(pellets.size());
-    if (!code) {
-        V3 g = gravity(), ext = extents();
-        for (size_t k = 0; k < trial.size(); ++k) {
-            const PelletAccum& a = h_acc[k];
-            double before = trial[k].kinetic();
-            trial[k].apply_impulse(v3(a.impulse[0], a.impulse[1], a.impulse[2]),
-                                   v3(a.angular[0], a.angular[1], a.angular[2]));
-            pellet_energy[k] = trial[k].kinetic() - before;
-            walls[k] = integrate(trial[k], dt, g, ext, cfg);
-            if (!trial[k].finite_state() || !std::isfinite(pellet_energy[k])) {
-                code = F_PELLET;
-                break;
+int mpm_engine::get_state(int i, mpm_particles* ps, mpm_pellet* pellets, size_t pellet_capacity,
+                          mpm_ledger* ledger, mpm_history* history) {
+    if (int rc = pull()) return rc;
+    const Instance& in = inst[i];
+    const InstState& st = h_is[i];
+    if (ps) {
+        if (ps->n < (size_t)st.n) return fail(MPM_ERR_CAPACITY, "particle arrays are smaller than the state");
+        HostParticles h;
+        if (int rc = download(i, h)) return rc;
+        size_t n = h.n;
+        ps->n = n;
+        bool fresh = st.committed == 0;
+        // Hand the particles back in the reference's order (sorted by the key of the
+        // last 32-step sort, solver.rs:501-517), whatever the device order is.
+        std::vector<size_t> ord(n);
+        for (size_t k = 0; k < n; ++k) ord[k] = k;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return h.ref[a] < h.ref[b]; });
+        for (size_t k = 0; k < n; ++k) {
+            size_t p = ord[k];
+            if (ps->id) ps->id[k] = h.id[p];
+            for (int a = 0; a < 3; ++a) {
+                if (ps->position) ps->position[3 * k + a] = h.s[(PX + a) * n + p];
+                if (ps->velocity) ps->velocity[3 * k + a] = h.s[(VX + a) * n + p];
             }
+            for (int a = 0; a < 9; ++a) {
+                if (ps->affine) ps->affine[9 * k + a] = h.s[(C00 + a) * n + p];
+                if (ps->deformation) {
+                    double id_a = a % 4 == 0 ? 1.0 : 0.0;
+                    // A committed liquid step resets F to I (solver.rs:965).
+                    ps->deformation[9 * k + a] =
+                        (in.mat.kind == 1 && !fresh) ? id_a : id_a + (double)h.s[(H00 + a) * n + p];
+                }
+            }
+            if (ps->kirchhoff) {
+                double* t = ps->kirchhoff + 9 * k;
+                t[0] = h.s[TXX * n + p];
+                t[4] = h.s[TYY * n + p];
+                t[8] = h.s[TZZ * n + p];
+                t[1] = t[3] = h.s[TXY * n + p];
+                t[2] = t[6] = h.s[TXZ * n + p];
+                t[5] = t[7] = h.s[TYZ * n + p];
+            }
+            // A committed paste step sets J = 1 (solver.rs:950).
+            if (ps->volume_ratio)
+                ps->volume_ratio[k] = (in.mat.kind == 0 && !fresh) ? 1.0 : 1.0 + (double)h.s[JM1 * n + p];
+            if (ps->plastic_strain) ps->plastic_strain[k] = in.mat.kind == 0 ? h.s[EPS * n + p] : h.sh[SEPS * n + p];
+            if (ps->mass) ps->mass[k] = h.sh[SMASS * n + p];
+            if (ps->volume0) ps->volume0[k] = h.sh[SVOL0 * n + p];
         }
     }
-    if (!code && st.fail != ~0ull) code = (int)(st.fail & 0xff);
-    if (!code && max_cells > 1.0) code = F_DISPLACEMENT;
-    if (code) {
-        last_failure = code;
-        if (report) report->failure = code;
-        return code;
-    }
-    // commit (solver.rs:980-998 and 600-609)
-    cur ^= 1;
-    gcur ^= 1;
-    steps_since_set += 1;
-    ledger.plastic_dissipation += st.dissipation;
-    for (int a = 0; a < 3; ++a) ledger.wall_impulse[a] += st.traction_impulse[a] + st.wall_impulse[a];
-    ledger.wall_normal_projection_energy += st.projection;
-    ledger.wall_friction_dissipation += st.friction;
-    ledger.wall_normal_traction_energy += st.traction_energy;
-    for (int a = 0; a < 3; ++a) ledger.gravity_impulse[a] += cfg.gravity[a] * (particle_mass * dt);
-    for (size_t k = 0; k < trial.size(); ++k) {
-        const PelletAccum& a = h_acc[k];
-        for (int c = 0; c < 3; ++c) {
-            ledger.gravity_impulse[c] += cfg.gravity[c] * (trial[k].mass * dt);
-            ledger.pellet_wall_impulse[c] += walls[k].impulse[c];
-            ledger.coupling_impulse[c] += a.impulse[c];
-            ledger.coupling_angular_impulse[c] += a.angular[c];
+    if (pellets) {
+        std::vector<PelletState> pel;
+        if (int rc = download_pellets(i, pel)) return rc;
+        for (size_t k = 0; k < pel.size() && k < pellet_capacity; ++k) {
+            const PelletState& p = pel[k];
+            mpm_pellet& d = pellets[k];
+            d.radius = p.radius;
+            d.length = p.length;
+            d.mass = p.mass;
+            for (int a = 0; a < 3; ++a) {
+                d.inertia_body[a] = p.inertia[a];
+                d.position[a] = p.pos[a];
+                d.velocity[a] = p.vel[a];
+                d.angular_momentum[a] = p.L[a];
+            }
+            for (int a = 0; a < 4; ++a) d.orientation[a] = p.q[a];
+            d.sphere_count = p.spheres;
+            for (int s = 0; s < p.spheres; ++s) d.sphere_offsets[s] = p.offsets[s];
         }
-        ledger.pellet_wall_work += walls[k].work;
-        ledger.coupling_grid_energy += a.grid_energy;
-        ledger.coupling_pellet_energy += pellet_energy[k];
     }
-    pellets = trial;
-    max_speed = (double)bits_float(st.max_speed_bits);
-    nw_host = st.nearwall;
-    if (report) report->accepted = 1;
-    if (st.escaped) return remove_escaped();
+    if (ledger) std::memcpy(ledger, st.ledger, sizeof(mpm_ledger));
+    if (history)
+        *history = mpm_history{st.time,     st.step_count,  st.limited_steps, st.rejected_steps,
+                               st.dt_scale, st.min_dt_used, st.max_dt_used};
     return MPM_OK;
 }
 
 namespace {
-
-double particle_mass_sum(mpm_engine* e, std::vector<float>& s, std::vector<float>& shv, std::vector<uint32_t>& id) {
-    if (e->download(s, shv, id)) return std::numeric_limits<double>::quiet_NaN();
-    double m = 0.0;
-    for (size_t p = 0; p < e->n; ++p) m += shv[SMASS * e->n + p];
-    return m;
-}
 
 void material_paste(double density, double young, double poisson, double ys, double k, double n, mpm_material* out) {
     double shear = young / (2.0 * (1.0 + poisson));
     double lame = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson));
     *out = mpm_material{0, density, shear, lame, lame + 2.0 * shear / 3.0, ys, k, n, 1.0};
 }
-
-}  // namespace
-
-namespace {
 
 Params batch_params(const mpm_material* m) {
     Params P{};
@@ -849,6 +1245,8 @@ struct DevBuf {
     hipError_t alloc(size_t n) { return hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)); }
     ~DevBuf() { (void)hipFree(p); }
 };
+
+bool valid_instance(const mpm_engine* e, size_t i) { return e && i < (size_t)e->ninst; }
 
 }  // namespace
 
@@ -952,35 +1350,28 @@ int mpm_fixture_place(const mpm_fixture* f, mpm_particles* out) {
     return MPM_OK;
 }
 
-int mpm_create(const mpm_material* material, const mpm_config* config, const mpm_layout* layout, size_t capacity,
-               size_t max_pellets, mpm_engine** out) {
-    if (!material || !config || !layout || !out) return MPM_ERR_INVALID;
+int mpm_batch_create(size_t n_instances, const mpm_material* materials, const mpm_config* configs,
+                     const mpm_layout* layouts, const size_t* capacities, const size_t* max_pellets,
+                     mpm_engine** out) {
+    if (!out) return MPM_ERR_INVALID;
     *out = nullptr;
-    if (max_pellets > (size_t)kMaxPellets) return MPM_ERR_CAPACITY;
-    double spacing = layout->spacing;
-    if (!(std::isfinite(spacing) && spacing > 0.0)) return MPM_ERR_INVALID;
-    int cells[3];
-    size_t total = 1;
-    for (int a = 0; a < 3; ++a) {
-        double ratio = layout->domain[a] / spacing;
-        double rounded = std::round(ratio);
-        // grid.rs:37-86
-        if (!std::isfinite(ratio) || std::fabs(ratio - rounded) > 1e-6 * std::max(rounded, 1.0) || rounded < 2.0 ||
-            rounded > 4000.0)
-            return MPM_ERR_INVALID;
-        cells[a] = (int)rounded;
-        total *= (size_t)(cells[a] + 3);
+    if (n_instances == 0 || n_instances > 65536 || !materials || !configs || !layouts) return MPM_ERR_INVALID;
+    std::vector<Instance> inst(n_instances);
+    for (size_t i = 0; i < n_instances; ++i) {
+        if (max_pellets && max_pellets[i] > 4096) return MPM_ERR_CAPACITY;
+        int rc = grid_cells(layouts[i], inst[i].cells);
+        if (rc) return rc;
+        inst[i].mat = materials[i];
+        inst[i].cfg = configs[i];
+        inst[i].dx = layouts[i].spacing;
+        inst[i].fill_params();
     }
-    if (total > layout->max_nodes) return MPM_ERR_INVALID;
     int devices = 0;
     if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1) return MPM_ERR_DEVICE;
     mpm_engine* e = new mpm_engine();
-    e->mat = *material;
-    e->cfg = *config;
-    e->dx = spacing;
-    for (int a = 0; a < 3; ++a) e->cells[a] = cells[a];
-    e->fill_params();
-    int rc = e->alloc(capacity, max_pellets);
+    e->ninst = (int)n_instances;
+    e->inst = std::move(inst);
+    int rc = e->alloc(capacities, max_pellets);
     if (rc) {
         delete e;
         return rc;
@@ -989,273 +1380,136 @@ int mpm_create(const mpm_material* material, const mpm_config* config, const mpm
     return MPM_OK;
 }
 
+int mpm_create(const mpm_material* material, const mpm_config* config, const mpm_layout* layout, size_t capacity,
+               size_t max_pellets, mpm_engine** out) {
+    if (!material || !config || !layout || !out) return MPM_ERR_INVALID;
+    return mpm_batch_create(1, material, config, layout, &capacity, &max_pellets, out);
+}
+
 void mpm_destroy(mpm_engine* e) { delete e; }
 
 const char* mpm_last_error(const mpm_engine* e) { return e ? e->err.c_str() : "null engine"; }
 
-size_t mpm_particle_count(const mpm_engine* e) { return e ? e->n : 0; }
-size_t mpm_pellet_count(const mpm_engine* e) { return e ? e->pellets.size() : 0; }
+size_t mpm_instance_count(const mpm_engine* e) { return e ? (size_t)e->ninst : 0; }
+
+size_t mpm_instance_particle_count(const mpm_engine* e, size_t i) {
+    return valid_instance(e, i) ? (size_t)e->h_is[i].n : 0;
+}
+
+size_t mpm_particle_count(const mpm_engine* e) { return mpm_instance_particle_count(e, 0); }
+
+size_t mpm_pellet_count(const mpm_engine* e) { return e ? (size_t)e->inst[0].ip.npel : 0; }
 
 int mpm_set_config(mpm_engine* e, const mpm_config* c) {
     if (!e || !c) return MPM_ERR_INVALID;
-    e->cfg = *c;
-    e->fill_params();
-    return MPM_OK;
+    Instance& in = e->inst[0];
+    in.cfg = *c;
+    in.fill_params();
+    return e->upload_params(0);
+}
+
+int mpm_instance_set_state(mpm_engine* e, size_t i, const mpm_particles* ps, const mpm_pellet* pellets,
+                           size_t n_pellets, const mpm_ledger* ledger, const mpm_history* history) {
+    if (!valid_instance(e, i)) return MPM_ERR_INVALID;
+    return e->set_state((int)i, ps, pellets, n_pellets, ledger, history);
 }
 
 int mpm_set_state(mpm_engine* e, const mpm_particles* ps, const mpm_pellet* pellets, size_t n_pellets,
                   const mpm_ledger* ledger, const mpm_history* history) {
-    if (!e || !ps) return MPM_ERR_INVALID;
-    if (ps->n > e->cap) return e->fail(MPM_ERR_CAPACITY, "more particles than the engine capacity");
-    if (n_pellets > e->max_pellets) return e->fail(MPM_ERR_CAPACITY, "more pellets than the engine capacity");
-    size_t n = ps->n;
-    if (n && (!ps->position || !ps->mass || !ps->volume0))
-        return e->fail(MPM_ERR_INVALID, "position, mass and volume0 are required");
-    e->n = n;
-    std::vector<float> s(NPF * n, 0.0f), shv(NSF * n, 0.0f);
-    std::vector<uint32_t> id(n);
-    double max_speed = 0.0;
-    for (size_t p = 0; p < n; ++p) {
-        id[p] = ps->id ? ps->id[p] : (uint32_t)p;
-        for (int a = 0; a < 3; ++a) {
-            s[(PX + a) * n + p] = (float)ps->position[3 * p + a];
-            s[(VX + a) * n + p] = ps->velocity ? (float)ps->velocity[3 * p + a] : 0.0f;
-        }
-        double vs = 0.0;
-        for (int a = 0; a < 3; ++a) vs += (double)s[(VX + a) * n + p] * (double)s[(VX + a) * n + p];
-        max_speed = std::max(max_speed, std::sqrt(vs));
-        for (int a = 0; a < 9; ++a) {
-            s[(C00 + a) * n + p] = ps->affine ? (float)ps->affine[9 * p + a] : 0.0f;
-            double F = ps->deformation ? ps->deformation[9 * p + a] : (a % 4 == 0 ? 1.0 : 0.0);
-            s[(H00 + a) * n + p] = (float)(F - (a % 4 == 0 ? 1.0 : 0.0));
-        }
-        s[JM1 * n + p] = ps->volume_ratio ? (float)(ps->volume_ratio[p] - 1.0) : 0.0f;
-        if (ps->kirchhoff) {
-            const double* t = ps->kirchhoff + 9 * p;
-            s[TXX * n + p] = (float)t[0];
-            s[TYY * n + p] = (float)t[4];
-            s[TZZ * n + p] = (float)t[8];
-            s[TXY * n + p] = (float)(0.5 * (t[1] + t[3]));
-            s[TXZ * n + p] = (float)(0.5 * (t[2] + t[6]));
-            s[TYZ * n + p] = (float)(0.5 * (t[5] + t[7]));
-        }
-        float eps = ps->plastic_strain ? (float)ps->plastic_strain[p] : 0.0f;
-        s[EPS * n + p] = eps;
-        shv[SEPS * n + p] = eps;
-        shv[SMASS * n + p] = (float)ps->mass[p];
-        shv[SVOL0 * n + p] = (float)ps->volume0[p];
-    }
-    e->cur = 0;
-    e->sh = 0;
-    // Until the first sort the reference order is the given order.
-    e->host_ref.resize(n);
-    for (size_t p = 0; p < n; ++p) e->host_ref[p] = p;
-    int rc = e->upload(s, shv, id);
-    if (rc) return rc;
-    rc = e->sort_particles(1);
-    if (rc) return rc;
-    e->pellets.clear();
-    for (size_t k = 0; k < n_pellets; ++k) {
-        const mpm_pellet& src = pellets[k];
-        Pellet p;
-        p.radius = src.radius;
-        p.length = src.length;
-        p.mass = src.mass;
-        p.inertia = v3(src.inertia_body[0], src.inertia_body[1], src.inertia_body[2]);
-        p.pos = v3(src.position[0], src.position[1], src.position[2]);
-        p.q = Quat{src.orientation[0], src.orientation[1], src.orientation[2], src.orientation[3]};
-        p.vel = v3(src.velocity[0], src.velocity[1], src.velocity[2]);
-        p.L = v3(src.angular_momentum[0], src.angular_momentum[1], src.angular_momentum[2]);
-        int count = std::min(std::max(src.sphere_count, 0), kMaxSpheres);
-        p.offsets.assign(src.sphere_offsets, src.sphere_offsets + count);
-        e->pellets.push_back(p);
-    }
-    e->max_speed = max_speed;
-    e->particle_mass = 0.0;
-    for (size_t p = 0; p < n; ++p) e->particle_mass += shv[SMASS * n + p];
-    e->steps_since_set = 0;
-    e->grid_valid = false;
-    if (history) {
-        e->time = history->time;
-        e->step_count = history->step_count;
-        e->limited_steps = history->limited_steps;
-        e->rejected_steps = history->rejected_steps;
-        e->dt_scale = history->dt_scale;
-        e->min_dt_used = history->min_dt_used;
-        e->max_dt_used = history->max_dt_used;
-    } else {
-        e->time = 0.0;
-        e->step_count = e->limited_steps = e->rejected_steps = 0;
-        e->dt_scale = 1.0;
-        e->min_dt_used = std::numeric_limits<double>::infinity();
-        e->max_dt_used = 0.0;
-    }
-    if (ledger) {
-        e->ledger = *ledger;
-    } else {
-        // Simulation::new seeds the initial inventory, solver.rs:357-390.
-        std::memset(&e->ledger, 0, sizeof(e->ledger));
-        double mass = 0.0;
-        V3 mom{{0, 0, 0}};
-        for (size_t p = 0; p < n; ++p) {
-            double m = shv[SMASS * n + p];
-            mass += m;
-            for (int a = 0; a < 3; ++a) mom[a] += (double)s[(VX + a) * n + p] * m;
-        }
-        for (const Pellet& p : e->pellets) mom = mom + p.vel * p.mass;
-        e->ledger.initial_mass = mass;
-        for (int a = 0; a < 3; ++a) e->ledger.initial_momentum[a] = mom[a];
-        double k = 0, pe = 0, el = 0;
-        rc = mpm_energies(e, &k, &pe, &el);
-        if (rc) return rc;
-        e->ledger.initial_mechanical_energy = k + pe + el;
-    }
-    return MPM_OK;
+    return mpm_instance_set_state(e, 0, ps, pellets, n_pellets, ledger, history);
+}
+
+int mpm_instance_get_state(mpm_engine* e, size_t i, mpm_particles* ps, mpm_pellet* pellets, size_t pellet_capacity,
+                           mpm_ledger* ledger, mpm_history* history) {
+    if (!valid_instance(e, i)) return MPM_ERR_INVALID;
+    return e->get_state((int)i, ps, pellets, pellet_capacity, ledger, history);
 }
 
 int mpm_get_state(mpm_engine* e, mpm_particles* ps, mpm_pellet* pellets, size_t pellet_capacity, mpm_ledger* ledger,
                   mpm_history* history) {
-    if (!e) return MPM_ERR_INVALID;
-    if (ps) {
-        if (ps->n < e->n) return e->fail(MPM_ERR_CAPACITY, "particle arrays are smaller than the state");
-        std::vector<float> s, shv;
-        std::vector<uint32_t> id;
-        int rc = e->download(s, shv, id);
-        if (rc) return rc;
-        size_t n = e->n;
-        ps->n = n;
-        bool fresh = e->steps_since_set == 0;
-        // Hand the particles back in the reference's order (sorted by the key of the
-        // last 32-step sort, solver.rs:501-517), whatever the device order is.
-        std::vector<size_t> order(n);
-        for (size_t k = 0; k < n; ++k) order[k] = k;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](size_t a, size_t b) { return e->host_ref[a] < e->host_ref[b]; });
-        std::vector<float> so(NPF * n), sho(NSF * n);
-        std::vector<uint32_t> ido(n);
-        for (size_t k = 0; k < n; ++k) {
-            size_t q = order[k];
-            for (int f = 0; f < NPF; ++f) so[f * n + k] = s[f * n + q];
-            for (int f = 0; f < NSF; ++f) sho[f * n + k] = shv[f * n + q];
-            ido[k] = id[q];
-        }
-        s.swap(so);
-        shv.swap(sho);
-        id.swap(ido);
-        for (size_t p = 0; p < n; ++p) {
-            if (ps->id) ps->id[p] = id[p];
-            for (int a = 0; a < 3; ++a) {
-                if (ps->position) ps->position[3 * p + a] = s[(PX + a) * n + p];
-                if (ps->velocity) ps->velocity[3 * p + a] = s[(VX + a) * n + p];
-            }
-            for (int a = 0; a < 9; ++a) {
-                if (ps->affine) ps->affine[9 * p + a] = s[(C00 + a) * n + p];
-                if (ps->deformation) {
-                    double id_a = a % 4 == 0 ? 1.0 : 0.0;
-                    // A committed liquid step resets F to I (solver.rs:965).
-                    ps->deformation[9 * p + a] = (e->mat.kind == 1 && !fresh) ? id_a : id_a + s[(H00 + a) * n + p];
-                }
-            }
-            if (ps->kirchhoff) {
-                double* t = ps->kirchhoff + 9 * p;
-                t[0] = s[TXX * n + p];
-                t[4] = s[TYY * n + p];
-                t[8] = s[TZZ * n + p];
-                t[1] = t[3] = s[TXY * n + p];
-                t[2] = t[6] = s[TXZ * n + p];
-                t[5] = t[7] = s[TYZ * n + p];
-            }
-            // A committed paste step sets J = 1 (solver.rs:950).
-            if (ps->volume_ratio)
-                ps->volume_ratio[p] = (e->mat.kind == 0 && !fresh) ? 1.0 : 1.0 + (double)s[JM1 * n + p];
-            if (ps->plastic_strain) ps->plastic_strain[p] = e->mat.kind == 0 ? s[EPS * n + p] : shv[SEPS * n + p];
-            if (ps->mass) ps->mass[p] = shv[SMASS * n + p];
-            if (ps->volume0) ps->volume0[p] = shv[SVOL0 * n + p];
-        }
-    }
-    if (pellets) {
-        for (size_t k = 0; k < e->pellets.size() && k < pellet_capacity; ++k) {
-            const Pellet& p = e->pellets[k];
-            mpm_pellet& d = pellets[k];
-            d.radius = p.radius;
-            d.length = p.length;
-            d.mass = p.mass;
-            for (int a = 0; a < 3; ++a) {
-                d.inertia_body[a] = p.inertia[a];
-                d.position[a] = p.pos[a];
-                d.velocity[a] = p.vel[a];
-                d.angular_momentum[a] = p.L[a];
-            }
-            d.orientation[0] = p.q.i;
-            d.orientation[1] = p.q.j;
-            d.orientation[2] = p.q.k;
-            d.orientation[3] = p.q.w;
-            d.sphere_count = (int)p.offsets.size();
-            for (size_t s = 0; s < p.offsets.size(); ++s) d.sphere_offsets[s] = p.offsets[s];
-        }
-    }
-    if (ledger) *ledger = e->ledger;
-    if (history)
-        *history = mpm_history{e->time,     e->step_count,  e->limited_steps, e->rejected_steps,
-                               e->dt_scale, e->min_dt_used, e->max_dt_used};
-    return MPM_OK;
+    return mpm_instance_get_state(e, 0, ps, pellets, pellet_capacity, ledger, history);
 }
 
+// Simulation::try_step, solver.rs:535-611: one bare trial step of instance 0.
 int mpm_try_step(mpm_engine* e, double dt, mpm_step_report* report) {
     if (!e) return MPM_ERR_INVALID;
+    if (report) *report = mpm_step_report{0, 0, 0.0, dt};
     if (!(std::isfinite(dt) && dt > 0.0)) return e->fail(MPM_ERR_INVALID, "dt must be finite and positive");
-    int rc = e->try_step(dt, report);
-    if (rc == MPM_OK) {
-        // try_step is the bare trial (solver.rs:535-537); the sort cadence belongs to advance_while.
+    if (int rc = e->pull()) return rc;
+    if (int rc = e->prepare()) return rc;
+    for (int i = 0; i < e->ninst; ++i) {
+        InstState& s = e->h_is[i];
+        s.mode = i == 0 ? M_RUN : M_DONE;
+        s.pause = 0;
+        s.trial_dt = i == 0 ? dt : 0.0;
     }
-    return rc;
+    if (int rc = e->push()) return rc;
+    if (int rc = e->enqueue(1)) return rc;
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(MPM_ERR_DEVICE, "device failure in try_step");
+    if (int rc = e->collect_profile(1)) return rc;
+    if (int rc = e->pull()) return rc;
+    InstState& s = e->h_is[0];
+    int code = s.last_failure;
+    if (report) {
+        report->accepted = code == 0;
+        report->failure = code;
+        report->max_cells = (double)s.st.max_disp;
+    }
+    if (s.mode == M_PAUSE && (s.pause & P_ESCAPED)) {
+        if (int rc = e->remove_escaped(0)) return rc;
+    }
+    s.mode = M_DONE;
+    s.pause = 0;
+    s.trial_dt = 0.0;
+    if (int rc = e->push()) return rc;
+    if (e->dirty)
+        if (int rc = e->sort_all()) return rc;
+    return code;
 }
 
-// Simulation::advance_while, solver.rs:413-472.
+// Simulation::advance_while, solver.rs:413-472, for instance 0.  With a
+// callback, it is asked before every trial step as in the reference; without
+// one, trial steps run in device-driven batches.
 int mpm_advance_while(mpm_engine* e, double target, int (*keep_going)(void*), void* ctx, int* reached) {
     if (!e) return MPM_ERR_INVALID;
     if (reached) *reached = 0;
-    if (!std::isfinite(target) || target < e->time) return e->fail(MPM_ERR_INVALID, "invalid target time");
-    while (e->time < target) {
-        if (keep_going && !keep_going(ctx)) return MPM_OK;
-        double remaining = target - e->time;
-        double limit = e->stability_limit();
-        double scaled = e->dt_scale * limit;
-        bool limited = scaled < e->cfg.max_dt;
-        double dt = std::min(std::min(scaled, e->cfg.max_dt), remaining);
-        if (remaining - dt < 1e-12 * std::max(target, 1.0)) dt = remaining;
-        if (!std::isfinite(dt) || dt <= 0.0 || (dt < e->cfg.min_dt && remaining > e->cfg.min_dt)) {
-            char buf[256];
-            std::snprintf(buf, sizeof(buf), "solver aborted at t='%.17g' s: step '%.17g' s below minimum", e->time, dt);
-            return e->fail(MPM_ERR_MIN_STEP, buf);
-        }
-        int rc = e->try_step(dt, nullptr);
-        if (rc < 0) return rc;
-        if (rc == MPM_OK) {
-            e->time += dt;
-            e->step_count += 1;
-            e->limited_steps += limited ? 1 : 0;
-            e->min_dt_used = std::min(e->min_dt_used, dt);
-            e->max_dt_used = std::max(e->max_dt_used, dt);
-            e->dt_scale = std::min(e->dt_scale * kDtRecovery, 1.0);
-            if (e->step_count % kSortInterval == 0) {
-                rc = e->sort_particles();
-                if (rc) return rc;
+    if (int rc = e->pull()) return rc;
+    InstState& s0 = e->h_is[0];
+    if (!std::isfinite(target) || target < s0.time) return e->fail(MPM_ERR_INVALID, "invalid target time");
+    if (int rc = e->prepare()) return rc;
+    for (int i = 1; i < e->ninst; ++i) e->h_is[i].mode = M_DONE;
+    int status = 0;
+    if (!keep_going) {
+        s0.target = target;
+        s0.mode = M_RUN;
+        s0.pause = 0;
+        s0.trial_dt = 0.0;
+        s0.budget = INT_MAX;
+        if (int rc = e->push()) return rc;
+        if (int rc = e->run(e->launch.steps_per_sync, &status, nullptr)) return rc;
+    } else {
+        while (true) {
+            InstState& s = e->h_is[0];
+            if (!(s.time < target)) {
+                s.time = target;
+                s.mode = M_DONE;
+                status = 0;
+                if (int rc = e->push()) return rc;
+                break;
             }
-        } else {
-            e->rejected_steps += 1;
-            e->dt_scale *= 0.5;
-            if (e->dt_scale * limit < e->cfg.min_dt) {
-                char buf[256];
-                std::snprintf(buf, sizeof(buf),
-                              "solver aborted at t='%.17g' s: step '%.17g' s below minimum after '%llu' rejections; "
-                              "last failure code %d",
-                              e->time, dt, (unsigned long long)e->rejected_steps, rc);
-                return e->fail(MPM_ERR_MIN_STEP, buf);
-            }
+            if (!keep_going(ctx)) return MPM_OK;
+            s.target = target;
+            s.mode = M_RUN;
+            s.pause = 0;
+            s.trial_dt = 0.0;
+            s.budget = 1;
+            if (int rc = e->push()) return rc;
+            status = 0;
+            if (int rc = e->run(1, &status, nullptr, 1)) return rc;
+            if (status == MPM_ERR_MIN_STEP) break;
         }
     }
-    e->time = target;
+    if (status == MPM_ERR_MIN_STEP) return MPM_ERR_MIN_STEP;
     if (reached) *reached = 1;
     return MPM_OK;
 }
@@ -1265,91 +1519,64 @@ int mpm_advance_to(mpm_engine* e, double target) {
     return mpm_advance_while(e, target, nullptr, nullptr, &reached);
 }
 
-double mpm_stability_limit(mpm_engine* e) { return e ? e->stability_limit() : std::numeric_limits<double>::quiet_NaN(); }
+int mpm_batch_advance(mpm_engine* e, const double* targets, uint64_t max_steps, int* status, double* done_ms) {
+    if (!e || !targets) return MPM_ERR_INVALID;
+    if (int rc = e->pull()) return rc;
+    for (int i = 0; i < e->ninst; ++i)
+        if (!std::isfinite(targets[i]) || targets[i] < e->h_is[i].time)
+            return e->fail(MPM_ERR_INVALID, "invalid target time for instance " + std::to_string(i));
+    if (int rc = e->prepare()) return rc;
+    int budget = max_steps == 0 || max_steps > (uint64_t)INT_MAX ? INT_MAX : (int)max_steps;
+    for (int i = 0; i < e->ninst; ++i) {
+        InstState& s = e->h_is[i];
+        s.target = targets[i];
+        s.mode = M_RUN;
+        s.pause = 0;
+        s.trial_dt = 0.0;
+        s.budget = budget;
+        if (status) status[i] = 0;
+        if (done_ms) done_ms[i] = 0.0;
+    }
+    if (int rc = e->push()) return rc;
+    return e->run(e->launch.steps_per_sync, status, done_ms);
+}
+
+double mpm_stability_limit(mpm_engine* e) {
+    return e ? e->stability_limit(0) : std::numeric_limits<double>::quiet_NaN();
+}
 
 double mpm_mass_residual(mpm_engine* e) {
-    if (!e) return std::numeric_limits<double>::quiet_NaN();
-    std::vector<float> s, shv;
-    std::vector<uint32_t> id;
-    double m = particle_mass_sum(e, s, shv, id);
-    return m + e->ledger.outflow_mass - e->ledger.initial_mass;
+    if (!e || e->pull()) return std::numeric_limits<double>::quiet_NaN();
+    HostParticles h;
+    if (e->download(0, h)) return std::numeric_limits<double>::quiet_NaN();
+    double m = 0.0;
+    for (size_t p = 0; p < h.n; ++p) m += h.sh[SMASS * h.n + p];
+    const double* L = e->h_is[0].ledger;
+    return m + L[1] - L[0];
 }
 
 int mpm_momentum_residual(mpm_engine* e, double* out) {
     if (!e || !out) return MPM_ERR_INVALID;
-    std::vector<float> s, shv;
-    std::vector<uint32_t> id;
-    int rc = e->download(s, shv, id);
-    if (rc) return rc;
-    size_t n = e->n;
-    V3 p{{0, 0, 0}};
+    if (int rc = e->pull()) return rc;
+    HostParticles h;
+    if (int rc = e->download(0, h)) return rc;
+    std::vector<PelletState> pel;
+    if (int rc = e->download_pellets(0, pel)) return rc;
+    size_t n = h.n;
+    double p[3] = {0.0, 0.0, 0.0};
     for (size_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) p[a] += (double)s[(VX + a) * n + i] * (double)shv[SMASS * n + i];
-    for (const Pellet& pl : e->pellets) p = p + pl.vel * pl.mass;
-    const mpm_ledger& l = e->ledger;
-    for (int a = 0; a < 3; ++a)
-        out[a] = p[a] - l.initial_momentum[a] - l.gravity_impulse[a] + l.outflow_momentum[a] - l.wall_impulse[a] -
-                 l.pellet_wall_impulse[a];
+        for (int a = 0; a < 3; ++a) p[a] += (double)h.s[(VX + a) * n + i] * (double)h.sh[SMASS * n + i];
+    for (const PelletState& pl : pel)
+        for (int a = 0; a < 3; ++a) p[a] += pl.vel[a] * pl.mass;
+    const double* L = e->h_is[0].ledger;
+    for (int a = 0; a < 3; ++a) out[a] = p[a] - L[2 + a] - L[6 + a] + L[28 + a] - L[9 + a] - L[12 + a];
     return MPM_OK;
 }
 
-// Kinetic, potential and elastic energy (solver.rs:1027-1076) from the f32 state.
 int mpm_energies(mpm_engine* e, double* kinetic, double* potential, double* elastic) {
     if (!e) return MPM_ERR_INVALID;
-    std::vector<float> s, shv;
-    std::vector<uint32_t> id;
-    int rc = e->download(s, shv, id);
-    if (rc) return rc;
-    size_t n = e->n;
-    double ke = 0.0, pe = 0.0, el = 0.0;
-    V3 g = e->gravity();
-    bool fresh = e->steps_since_set == 0;
-    for (size_t p = 0; p < n; ++p) {
-        double m = shv[SMASS * n + p], v0 = shv[SVOL0 * n + p];
-        double v2 = 0.0, gx = 0.0;
-        for (int a = 0; a < 3; ++a) {
-            double v = s[(VX + a) * n + p];
-            v2 += v * v;
-            gx += g[a] * (double)s[(PX + a) * n + p];
-        }
-        ke += 0.5 * m * v2;
-        pe += -m * gx;
-        if (e->mat.kind == 0) {
-            double F[9], b[9], vals[3];
-            for (int a = 0; a < 9; ++a) F[a] = (a % 4 == 0 ? 1.0 : 0.0) + (double)s[(H00 + a) * n + p];
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) b[3 * i + j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
-            if (!eigen3(b, vals)) {
-                el = std::numeric_limits<double>::quiet_NaN();
-                continue;
-            }
-            double h[3], tr = 0.0, h2 = 0.0;
-            for (int a = 0; a < 3; ++a) {
-                h[a] = 0.5 * std::log(std::max(vals[a], 1e-300));
-                tr += h[a];
-                h2 += h[a] * h[a];
-            }
-            el += v0 * (e->mat.shear_modulus * h2 + 0.5 * e->mat.lame_lambda * tr * tr);
-        } else {
-            double j = 1.0 + (double)s[JM1 * n + p];
-            double gm = e->mat.tait_gamma;
-            if (gm == 1.0)
-                el += v0 * e->mat.bulk_modulus * (j - 1.0 - std::log(j));
-            else
-                el += v0 * (e->mat.bulk_modulus / gm) * (j - 1.0 + (std::pow(j, 1.0 - gm) - 1.0) / (gm - 1.0));
-        }
-    }
-    (void)fresh;
-    for (const Pellet& p : e->pellets) {
-        ke += p.kinetic();
-        pe -= p.mass * dot(g, p.pos);
-    }
-    if (kinetic) *kinetic = ke;
-    if (potential) *potential = pe;
-    if (elastic) *elastic = el;
-    return MPM_OK;
+    return e->energies(0, kinetic, potential, elastic);
 }
-
 
 int mpm_paste_update_batch(const mpm_material* material, double dt, size_t n, const double* trial, double* deformation,
                            double* kirchhoff, double* increment, double* dissipation_density, int* codes) {
@@ -1443,6 +1670,29 @@ int mpm_profile_g2p2g(mpm_engine* e, double* avg_ms, uint64_t* launches) {
     if (!e) return MPM_ERR_INVALID;
     if (avg_ms) *avg_ms = e->g2p2g_launches ? e->g2p2g_ms / (double)e->g2p2g_launches : 0.0;
     if (launches) *launches = e->g2p2g_launches;
+    return MPM_OK;
+}
+
+int mpm_autotune(mpm_engine* e, int steps, mpm_launch* chosen) {
+    if (!e || steps < 1) return MPM_ERR_INVALID;
+    return e->autotune(steps, chosen);
+}
+
+int mpm_get_launch(const mpm_engine* e, mpm_launch* out) {
+    if (!e || !out) return MPM_ERR_INVALID;
+    *out = e->launch;
+    return MPM_OK;
+}
+
+int mpm_set_launch(mpm_engine* e, const mpm_launch* l) {
+    if (!e || !l) return MPM_ERR_INVALID;
+    if (l->steps_per_sync < 1 || l->steps_per_sync > kMaxStepsPerSync || l->merge_blocks < 1 ||
+        l->merge_blocks > 4 || l->traction_wgs < 1 || l->traction_wgs > 65536)
+        return e->fail(MPM_ERR_INVALID, "launch shape out of range");
+    bool resort = l->merge_blocks != e->launch.merge_blocks;
+    e->launch = *l;
+    e->graph_stale = true;
+    if (resort) e->dirty = true;
     return MPM_OK;
 }
 

@@ -1,18 +1,25 @@
 // Internal types shared by the MLS-MPM HIP kernels and the host engine.
 //
+// The engine steps a batch of independent instances (one instance is the
+// reference's Simulation); every kernel covers all instances of the batch, and
+// each instance carries its own adaptive dt, accept/reject state and ledgers on
+// the device, so one launch sequence advances every instance by one trial step.
+//
 // Device layout (HBM):
-//  * particles: one f32 slab of NPF fields x capacity, structure of arrays
-//    (field f of particle p at slab[f * cap + p]); two slabs alternate as the
-//    committed state and the trial state.  The deformation gradient is held as
-//    H = F - I and the liquid volume ratio as J - 1 so that the small elastic
-//    strains of a paste near yield keep f32 precision.  Kirchhoff stress is
-//    symmetric and stored as six components.  Mass, reference volume and id
-//    never change during a step and live in a separate single slab.
-//  * grid: dense padded node array (grid.rs:12-21) re-laid in 4x4x4-node
-//    blocks of 64 consecutive nodes, so one wavefront owns one block.  Each of
-//    two accumulator sets holds mass, APIC momentum sum(w (m v + m C d)) and the
-//    dt-free stress sum(w V0 tau d); the momentum of the reference's P2G is
-//    mom - dt (4/dx^2) force.  A per-block flag marks blocks that received mass.
+//  * particles: two f32 slabs of NPF fields x total capacity (structure of
+//    arrays, field f of particle p at slab[f * cap + p]); instance i owns the
+//    range [poff_i, poff_i + pcap_i) of both.  Per instance, one slab holds the
+//    committed state and the other receives the trial step; acceptance flips
+//    the instance's selector.  The deformation gradient is held as H = F - I and
+//    the liquid volume ratio as J - 1 so that small elastic strains keep f32
+//    precision; Kirchhoff stress is symmetric (six components).  Mass, reference
+//    volume, a liquid's plastic strain, id and the reference-order key never
+//    change within a step and live in a separate slab pair swapped by the sort.
+//  * grid: each instance's dense padded node array (grid.rs:12-21) re-laid in
+//    4x4x4-node blocks of 64 consecutive nodes (one wavefront per block); the
+//    instances' blocks are concatenated.  Two accumulator sets hold mass, APIC
+//    momentum sum(w (m v + m C d)) and the dt-free stress sum(w V0 tau d); the
+//    reference's P2G momentum is mom - dt (4/dx^2) force.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,15 +53,23 @@ enum Fail {
     F_TRACTION = 9
 };
 
-constexpr int kMaxPellets = 1024;
-constexpr int kMaxSpheres = 64;
+// Instance modes.
+enum Mode { M_RUN = 0, M_DONE = 1, M_PAUSE = 2, M_ERROR = 3 };
+// Pause reasons (bit flags).
+enum { P_ESCAPED = 1, P_BUDGET = 2 };
+// Error kinds of an instance in M_ERROR.
+enum { E_MIN_STEP = 1, E_MIN_AFTER_REJECT = 2 };
 
-// Launch-constant parameters (f32 copies of the f64 host values).
+constexpr int kMaxSpheres = 64;
+constexpr int kLedger = 31;  // doubles of mpm_ledger
+constexpr int kGridVals = 12;
+
+// Launch-constant parameters of one instance (f32 copies of the f64 host values).
 struct Params {
     float dx, inv_dx;
     int cells[3];
     int nodes[3];
-    int nb[3];  // blocks per axis
+    int nb[3];  // node blocks per axis
     float extent[3];
     float gravity[3];
     int kind;  // 0 paste, 1 liquid
@@ -62,7 +77,20 @@ struct Params {
     float face_mu[6];
 };
 
-// Accumulators of one trial step.  Reset before every attempt.
+// Constant per instance (uploaded when the instance is configured).
+struct InstParams {
+    Params P;
+    double dx, cfl, max_dt, min_dt, wave, diff;
+    double gravity[3];
+    double wall_friction, contact_k, contact_damping, contact_friction;
+    double particle_mass;   // sum of the stored particle masses (f64)
+    double pel_contact_dt;  // min over the instance's pellets of 0.2 sqrt(m / k)
+    int poff, pcap;         // particle range of the slabs
+    int boff, nblk;         // node-block range (nblk a multiple of 4)
+    int pel_off, npel;      // pellet range
+};
+
+// Accumulators of one trial step.
 struct StepStatus {
     double wall_impulse[3];      // sum over nodes of (v' - v) m, walls and budget
     double traction_impulse[3];  // sum of signed traction deposits
@@ -70,13 +98,33 @@ struct StepStatus {
     double friction;             // wall_friction_dissipation increment
     double traction_energy;      // wall_normal_traction_energy increment
     double dissipation;          // plastic dissipation increment
-    double grid_momentum[3];     // sum of m v' over nodes after walls and pellets
-    unsigned int max_speed_bits;
-    unsigned int max_disp_bits;
+    double grid_momentum[3];     // sum of m v' over nodes after the walls
+    float max_speed;
+    float max_disp;
     unsigned long long fail;     // (particle << 8) | code of the first failure
     int grid_nonfinite;
     int escaped;
-    int nearwall;  // particles of the trial state within half a cell of a face
+    int pellet_bad;
+    int pad;
+};
+
+// Device-resident state of one instance (solver.rs:324-352 plus the driver's loop state).
+struct InstState {
+    double ledger[kLedger];
+    double time, target, dt, limit, dt_scale, max_speed, min_dt_used, max_dt_used;
+    double trial_dt;  // > 0: one bare trial step of this size (try_step), no time advance
+    unsigned long long step_count, limited_steps, rejected_steps;
+    StepStatus st;
+    int n;            // particle count
+    int cur, gcur;    // slab / accumulator-set selectors
+    int nw_count[2];  // near-wall list length per slab
+    int mode, pause, err, last_failure;
+    int active, limited;
+    int budget;       // accepted steps left before a pause (host-set)
+    int w_off, w_cnt; // work items
+    int nw_base;      // inclusive end of the instance's range in the near-wall prefix (traction)
+    int refsort;      // the accepted step hit the 32-step cadence: refresh the order keys
+    int committed;    // a trial step was accepted since the state was set
     int pad;
 };
 
@@ -88,19 +136,31 @@ struct PelletAccum {
     double constrained;
 };
 
-// Pellet geometry for the coupling kernel (state at the start of the step), in
-// f64 so that the inside test phi <= 0 on the node lattice matches the
-// reference's f64 decision.
-struct PelletDev {
-    double pos[3];
-    double axis[3];
-    double vel[3];
-    double omega[3];
-    double radius;
-    double lo[3], hi[3];
-    int spheres;
-    int pad;
+// Pellet state (rigid.rs:30-49), f64; orientation quaternion (i, j, k, w).
+struct PelletState {
+    double radius, length, mass;
+    double inertia[3];
+    double pos[3], q[4], vel[3], L[3];
+    int inst, spheres;
     double offsets[kMaxSpheres];
+};
+
+// Start-of-step geometry of a pellet for the coupling kernel.
+struct PelletGeo {
+    double pos[3], axis[3], vel[3], omega[3];
+    double lo[3], hi[3];
+    double radius;
+    int spheres, inst;
+};
+
+// Trial pellet after coupling and integration (committed on acceptance).
+struct PelletTrial {
+    double pos[3], q[4], vel[3], L[3];
+    double energy;  // kinetic-energy change from the coupling impulse
+    double wall_impulse[3];
+    double wall_work;
+    int bad;
+    int pad;
 };
 
 struct GridPtrs {
@@ -113,6 +173,57 @@ struct GridPtrs {
     float* fz;
     int* flag;
 };
+
+// Device pointers of the whole batch.
+struct BatchDev {
+    int ninst;
+    size_t cap;  // slab stride (total particle capacity)
+    size_t na;   // total node count (all instances)
+    InstParams* ip;
+    InstState* is;
+    const int* poffs;  // particle-range starts of the instances (ninst + 1 entries, the last = cap)
+    float* slab[2];
+    float* shared[2];
+    int sh;       // shared-slab selector (host controlled)
+    float* gmem[2];  // 7 * na floats each
+    int* gflag[2];   // one per node block
+    float* gv;       // 3 * na
+    float* trS;      // 3 * na
+    float* trT;      // 3 * na
+    uint32_t* ids;     // current shared-slab ids and reference-order keys
+    uint64_t* refkey;
+    int* nw_list[2];  // per slab, instance ranges like the particles
+    int* blk_inst;    // node block -> instance
+    int4* work;       // (instance, start, count, 0)
+    int nwork;        // number of work items
+    int ngwg;         // grid-update workgroups (all instances)
+    int* counters;    // [0] unused, [1] near-wall entries of the running instances
+    double* gpart;    // grid partials, field-major: gpart[v * ngwg + workgroup]
+    int* gbad;
+    float* pmax;      // g2p2g partials per work item
+    double* pdiss;
+    unsigned long long* pfail;
+    int* pesc;
+    PelletState* pel;
+    PelletGeo* pgeo;
+    PelletAccum* pacc;
+    PelletTrial* ptrial;
+    int npel;
+};
+
+__host__ __device__ inline GridPtrs grid_set(const BatchDev& B, int set) {
+    GridPtrs g;
+    float* m = B.gmem[set];
+    g.m = m;
+    g.px = m + B.na;
+    g.py = m + 2 * B.na;
+    g.pz = m + 3 * B.na;
+    g.fx = m + 4 * B.na;
+    g.fy = m + 5 * B.na;
+    g.fz = m + 6 * B.na;
+    g.flag = B.gflag[set];
+    return g;
+}
 
 __host__ __device__ inline int node_index(const Params& P, int i, int j, int k) {
     int b = ((i >> 2) * P.nb[1] + (j >> 2)) * P.nb[2] + (k >> 2);
@@ -131,37 +242,29 @@ __host__ __device__ inline void node_coords(const Params& P, int index, int& i, 
 
 // Host launchers (kernels.hip).
 int g2p2g_chunk();
-int grid_workgroups(int nblocks);
-size_t rle_temp_bytes(int n);
-void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
-                       uint32_t* ids_tmp, uint32_t* unique, uint32_t* counts, int* nruns);
-void launch_reset_status(hipStream_t s, StepStatus* st, int* nw_count);
-void launch_p2g(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const float* shared,
-                GridPtrs g);
-void launch_nearwall(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, int* list, int* count);
-void launch_traction(hipStream_t s, const Params& P, float dt, const int* list, const int* count, int max_count,
-                     const float* slab, size_t cap, const float* shared, GridPtrs g, float* trS, float* trT,
-                     size_t nodes_alloc);
-void launch_grid_update(hipStream_t s, const Params& P, float dt, int nblocks, GridPtrs a, float* trS, float* trT,
-                        float* gv, size_t nodes_alloc, GridPtrs clear, double* partials, int* bad);
-void launch_couple(hipStream_t s, const Params& P, double dx, int nblocks, GridPtrs a, float* gv, size_t nodes_alloc,
-                   const PelletDev* pellets, int npellets, const int* block_start, const int* block_list,
-                   PelletAccum* acc, double friction);
-void launch_g2p2g(hipStream_t s, const Params& P, float dt, int n, const float* src, float* dst, size_t cap,
-                  const float* shared, const float* gv, size_t nodes_alloc, GridPtrs out, const int2* work, int nwork,
-                  float* pmax, double* pdiss, unsigned long long* pfail, int* pesc, int* nw_list, int* nw_count,
-                  hipEvent_t start, hipEvent_t stop);
-void launch_finalize(hipStream_t s, int ngrid, const double* gpart, const int* gbad, int npart, const float* pmax,
-                     const double* pdiss, const unsigned long long* pfail, const int* pesc, const int* nw_count,
-                     StepStatus* st);
-void launch_sort_keys(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const uint32_t* id,
-                      uint64_t* keys, uint32_t* order, uint64_t* refkey, int keep_ref);
-void launch_permute(hipStream_t s, int n, const uint32_t* order, const float* src, float* dst, size_t cap,
-                    const float* shared_src, float* shared_dst, const uint32_t* id_src, uint32_t* id_dst,
-                    const uint64_t* ref_src, uint64_t* ref_dst);
+// One trial step of every running instance: control, traction, grid update,
+// pellet coupling, fused G2P2G, pellet integration, reductions, commit and the
+// 32-step order keys.
+void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, hipEvent_t g2p2g_start,
+                 hipEvent_t g2p2g_stop);
+void launch_p2g(hipStream_t s, const BatchDev& B, const Params& P, int poff, int n, int boff, int cur, int gcur);
+// Near-wall lists of every instance's committed slab (counters cleared by the caller).
+void launch_nearwall(hipStream_t s, const BatchDev& B);
+// Device-order keys of every slab entry: (instance << cell_bits) | 4^3-cell block
+// key; entries past an instance's particle count sort to the end of its range.
+void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t* keys, uint32_t* order);
+// Gather every slab entry into the sorted order (each instance from its
+// committed slab into the other one; shared slabs, ids and order keys likewise).
+void launch_permute(hipStream_t s, const BatchDev& B, const uint32_t* order, const float* shared_src,
+                    float* shared_dst, const uint32_t* id_src, uint32_t* id_dst, const uint64_t* ref_src,
+                    uint64_t* ref_dst);
 size_t sort_temp_bytes(int n, int end_bit);
+size_t rle_temp_bytes(int n);
 void launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
                        const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit);
+// Runs of equal (instance, cell block) in the sorted keys.
+void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
+                       uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns);
 void launch_paste_batch(hipStream_t s, const Params& P, float dt, int n, const float* h_in, float* h_out,
                         float* tau, float* inc, float* dd, int* codes);
 void launch_liquid_batch(hipStream_t s, const Params& P, float dt, int n, const float* jm1, const float* c,

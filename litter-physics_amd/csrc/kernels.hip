@@ -1,562 +1,371 @@
-// HIP kernels of the MLS-MPM substep for CDNA4 (gfx950).
+// HIP kernels of the batched MLS-MPM substep for CDNA4 (gfx950).
 //
-// One substep (core/src/mpm/solver.rs:540-611) maps to:
-//   traction     -- wall_traction_to_grid (solver.rs:725-819) on particles near a face
-//   grid_update  -- grid_update (solver.rs:822-892) plus the ledger sums, one wave per
-//                   4x4x4-node block; also clears the other accumulator set
-//   couple       -- couple_grid (rigid.rs:385-438) for every pellet, in index order
-//   g2p2g        -- grid_to_particle (solver.rs:895-977) with update_paste / update_liquid
-//                   (constitutive.rs:456-589), fused with the next step's
-//                   deposit_particles (solver.rs:632-661) into the other accumulator set.
-// The deposit is split into the dt-free APIC momentum and the dt-free stress sum so
-// that it can run before the next step's dt is known.
+// One trial step of every running instance (core/src/mpm/solver.rs:413-611) is a
+// fixed launch sequence with no host round trip:
+//   control      -- advance_while's dt choice (solver.rs:421-443) and the stability
+//                   limit (477-498) per instance; pellet geometry for the coupling
+//   traction     -- wall_traction_to_grid (solver.rs:725-819) over the near-wall lists
+//   grid_update  -- grid_update (solver.rs:822-892), one wavefront per 4x4x4-node block
+//   couple       -- couple_grid (rigid.rs:385-438) for each instance's pellets in order
+//   g2p2g        -- grid_to_particle with update_paste / update_liquid (solver.rs:895-977,
+//                   constitutive.rs:456-589) fused with the next step's deposit
+//                   (solver.rs:632-661) into the other accumulator set
+//   pellet       -- Pellet::apply_impulse and ::integrate (rigid.rs:251-334) on trial copies
+//   finalize     -- fixed-order reduction of the per-workgroup partials per instance
+//   commit       -- accept/reject, commit and ledgers (solver.rs:444-467, 593-611, 980-998)
+//   refkeys      -- the 32-step reference order keys (solver.rs:501-517)
+// The deposit is split into the dt-free APIC momentum and the dt-free stress sum
+// so that it can run before the next step's dt is known.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
-#include <cfloat>
-#include <climits>
-#include <cmath>
-
-#include "engine_internal.h"
+#include "physics.h"
 
 namespace mpm {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kChunk = 1024;
+constexpr int kGatherCap = 768;
+constexpr int kScatterCap = 1024;
+constexpr double kDtRecovery = 1.25;              // solver.rs:45
+constexpr unsigned long long kSortInterval = 32;  // solver.rs:39
 
-struct Stencil {
-    int base[3];
-    float w[3][3];
-    float d[3][3];  // node minus particle in cells
-};
+// ------------------------------------------------------------ pellets (f64)
 
-// grid.rs:151-183 in f32: base = floor(x/dx - 1/2) + 1 (padded), quadratic B-spline weights.
-__device__ inline bool make_stencil(const Params& P, float x, float y, float z, Stencil& s) {
-    float xs[3] = {x * P.inv_dx, y * P.inv_dx, z * P.inv_dx};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float bf = floorf(xs[a] - 0.5f);
-        float padded = bf + 1.0f;
-        if (!(padded >= 0.0f) || padded + 2.0f >= (float)P.nodes[a]) return false;
-        float fx = xs[a] - bf;
-        float t0 = 1.5f - fx, t1 = fx - 1.0f, t2 = fx - 0.5f;
-        s.w[a][0] = 0.5f * (t0 * t0);
-        s.w[a][1] = 0.75f - t1 * t1;
-        s.w[a][2] = 0.5f * (t2 * t2);
-        s.base[a] = (int)padded;
-        s.d[a][0] = -fx;
-        s.d[a][1] = 1.0f - fx;
-        s.d[a][2] = 2.0f - fx;
+// nalgebra's UnitQuaternion (i, j, k, w) acting on a vector.
+__device__ inline void qrot(const double* q, const double* v, double* out) {
+    double t[3] = {2.0 * (q[1] * v[2] - q[2] * v[1]), 2.0 * (q[2] * v[0] - q[0] * v[2]),
+                   2.0 * (q[0] * v[1] - q[1] * v[0])};
+    double c[3] = {q[1] * t[2] - q[2] * t[1], q[2] * t[0] - q[0] * t[2], q[0] * t[1] - q[1] * t[0]};
+    for (int a = 0; a < 3; ++a) out[a] = t[a] * q[3] + c[a] + v[a];
+}
+
+__device__ inline void qmatrix(const double* q, double* r) {
+    double i = q[0], j = q[1], k = q[2], w = q[3];
+    double ww = w * w, ii = i * i, jj = j * j, kk = k * k;
+    double ij = i * j * 2.0, wk = w * k * 2.0, wj = w * j * 2.0, ik = i * k * 2.0, jk = j * k * 2.0,
+           wi = w * i * 2.0;
+    r[0] = ww + ii - jj - kk;
+    r[1] = ij - wk;
+    r[2] = wj + ik;
+    r[3] = wk + ij;
+    r[4] = ww - ii + jj - kk;
+    r[5] = jk - wi;
+    r[6] = ik - wj;
+    r[7] = wi + jk;
+    r[8] = ww - ii - jj + kk;
+}
+
+// Pellet::angular_velocity, rigid.rs:193-198: R I^-1 R^T L.
+__device__ inline void pellet_omega(const double* q, const double* L, const double* inertia, double* om) {
+    double r[9];
+    qmatrix(q, r);
+    double b[3] = {r[0] * L[0] + r[3] * L[1] + r[6] * L[2], r[1] * L[0] + r[4] * L[1] + r[7] * L[2],
+                   r[2] * L[0] + r[5] * L[1] + r[8] * L[2]};
+    double bo[3] = {b[0] / inertia[0], b[1] / inertia[1], b[2] / inertia[2]};
+    for (int a = 0; a < 3; ++a) om[a] = r[3 * a] * bo[0] + r[3 * a + 1] * bo[1] + r[3 * a + 2] * bo[2];
+}
+
+__device__ inline double pellet_kinetic(double mass, const double* vel, const double* q, const double* L,
+                                        const double* inertia) {
+    double om[3];
+    pellet_omega(q, L, inertia, om);
+    return 0.5 * mass * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]) +
+           0.5 * (om[0] * L[0] + om[1] * L[1] + om[2] * L[2]);
+}
+
+// Instance owning slab entry p (binary search over the range starts).
+__device__ inline int instance_of(const BatchDev& B, int p) {
+    int lo = 0, hi = B.ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (B.poffs[mid] <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
-    return true;
+    return lo;
 }
 
-__device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ inline int wave_sum_i(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ inline unsigned long long wave_min_u64(unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
+// ------------------------------------------------------------ control
+
+// Simulation::stability_limit, solver.rs:477-498.
+__device__ inline double stability_limit(const BatchDev& B, const InstParams& ip, const InstState& s) {
+    double limit = ip.cfl * ip.dx / (ip.wave + s.max_speed);
+    if (ip.diff > 0.0) limit = fmin(limit, 0.25 * ip.dx * ip.dx / ip.diff);
+    if (ip.npel > 0) {
+        limit = fmin(limit, ip.pel_contact_dt);
+        for (int k = 0; k < ip.npel; ++k) {
+            const PelletState& p = B.pel[ip.pel_off + k];
+            double sp = sqrt(p.vel[0] * p.vel[0] + p.vel[1] * p.vel[1] + p.vel[2] * p.vel[2]);
+            if (sp > 0.0) limit = fmin(limit, ip.cfl * ip.dx / sp);
+        }
     }
-    return v;
+    return limit;
 }
 
-__device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
-
-// ------------------------------------------------------------ scatter
-
-// deposit_particles (solver.rs:632-661) for one particle: mass w m, momentum
-// w (m v + m C d) and stress w V0 tau d, d = x_node - x_p in metres.
-__device__ inline void scatter_particle(const Params& P, GridPtrs g, const Stencil& s, float m, float vx, float vy,
-                                        float vz, const float* C, const float* tau, float v0) {
-    // Precompute m C dx and V0 tau dx (d in cells times dx is metres).
-    float mc[9], vt[6];
-    float mdx = m * P.dx, vdx = v0 * P.dx;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
-    float mvx = m * vx, mvy = m * vy, mvz = m * vz;
-    // Block flags of the (up to 8) node blocks the stencil touches.
-    int b0[3], b1[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        b0[a] = s.base[a] >> 2;
-        b1[a] = (s.base[a] + 2) >> 2;
-    }
-    for (int bi = b0[0]; bi <= b1[0]; ++bi)
-        for (int bj = b0[1]; bj <= b1[1]; ++bj)
-            for (int bk = b0[2]; bk <= b1[2]; ++bk) g.flag[(bi * P.nb[1] + bj) * P.nb[2] + bk] = 1;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            float wab = s.w[0][a] * s.w[1][b];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float w = wab * s.w[2][c];
-                float dx = s.d[0][a], dy = s.d[1][b], dz = s.d[2][c];
-                int n = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
-                float px = w * (mvx + mc[0] * dx + mc[1] * dy + mc[2] * dz);
-                float py = w * (mvy + mc[3] * dx + mc[4] * dy + mc[5] * dz);
-                float pz = w * (mvz + mc[6] * dx + mc[7] * dy + mc[8] * dz);
-                float fx = w * (vt[0] * dx + vt[3] * dy + vt[4] * dz);
-                float fy = w * (vt[3] * dx + vt[1] * dy + vt[5] * dz);
-                float fz = w * (vt[4] * dx + vt[5] * dy + vt[2] * dz);
-                atomicAdd(g.m + n, w * m);
-                atomicAdd(g.px + n, px);
-                atomicAdd(g.py + n, py);
-                atomicAdd(g.pz + n, pz);
-                atomicAdd(g.fx + n, fx);
-                atomicAdd(g.fy + n, fy);
-                atomicAdd(g.fz + n, fz);
+__global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
+    for (int t = threadIdx.x; t < B.ninst; t += 256) {
+        InstState& s = B.is[t];
+        const InstParams& ip = B.ip[t];
+        s.active = 0;
+        s.refsort = 0;
+        if (s.mode == M_RUN) {
+            if (s.trial_dt > 0.0) {
+                // A bare trial step of a given size (solver.rs:535-537).
+                s.dt = s.trial_dt;
+                s.limit = 0.0;
+                s.limited = 0;
+                s.active = 1;
+            } else if (!(s.time < s.target)) {
+                s.time = s.target;  // solver.rs:470
+                s.mode = M_DONE;
+            } else if (s.budget <= 0) {
+                s.mode = M_PAUSE;
+                s.pause |= P_BUDGET;
+            } else {
+                // solver.rs:425-443
+                double remaining = s.target - s.time;
+                double limit = stability_limit(B, ip, s);
+                double scaled = s.dt_scale * limit;
+                int limited = scaled < ip.max_dt;
+                double dt = fmin(fmin(scaled, ip.max_dt), remaining);
+                if (remaining - dt < 1e-12 * fmax(s.target, 1.0)) dt = remaining;
+                if (!isfinite(dt) || dt <= 0.0 || (dt < ip.min_dt && remaining > ip.min_dt)) {
+                    s.mode = M_ERROR;
+                    s.err = E_MIN_STEP;
+                    s.dt = dt;
+                } else {
+                    s.dt = dt;
+                    s.limit = limit;
+                    s.limited = limited;
+                    s.active = 1;
+                }
             }
         }
-    }
-}
-
-// ------------------------------------------------------------ constitutive
-
-__device__ inline float det3(const float* f) {
-    float m23 = f[4] * f[8] - f[7] * f[5];
-    float m13 = f[3] * f[8] - f[6] * f[5];
-    float m12 = f[3] * f[7] - f[6] * f[4];
-    return f[0] * m23 - f[1] * m13 + f[2] * m12;
-}
-
-// Cyclic Jacobi on a symmetric 3x3 (constitutive.rs:373-425), pairs (0,1),(0,2),(1,2).
-// Stops when the off-diagonal norm falls below 1e-12 of the Frobenius norm (the
-// f64 reference's 1e-30 is below f32 resolution).
-__device__ inline void jacobi_rotate(float* a, float* v, int p, int q) {
-    float apq = a[3 * p + q];
-    if (apq == 0.0f) return;
-    float theta = (a[3 * q + q] - a[3 * p + p]) / (2.0f * apq);
-    float t = fabsf(theta) > 1e15f ? 0.5f / theta
-                                   : copysignf(1.0f, theta) / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
-    float c = 1.0f / sqrtf(t * t + 1.0f);
-    float s = t * c;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        float akp = a[3 * k + p], akq = a[3 * k + q];
-        a[3 * k + p] = c * akp - s * akq;
-        a[3 * k + q] = s * akp + c * akq;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        float apk = a[3 * p + k], aqk = a[3 * q + k];
-        a[3 * p + k] = c * apk - s * aqk;
-        a[3 * q + k] = s * apk + c * aqk;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        float vkp = v[3 * k + p], vkq = v[3 * k + q];
-        v[3 * k + p] = c * vkp - s * vkq;
-        v[3 * k + q] = s * vkp + c * vkq;
-    }
-}
-
-__device__ inline int eigen_sym3(float* a, float* v) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) v[i] = (i % 4 == 0) ? 1.0f : 0.0f;
-    float scale = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) scale += a[i] * a[i];
-    if (!isfinite(scale)) return F_EIGEN;
-    scale = fmaxf(scale, FLT_MIN);
-    for (int sweep = 0; sweep < 24; ++sweep) {
-        float off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
-        if (off <= 1e-12f * scale) return F_OK;
-        jacobi_rotate(a, v, 0, 1);
-        jacobi_rotate(a, v, 0, 2);
-        jacobi_rotate(a, v, 1, 2);
-    }
-    float off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
-    return off <= 1e-8f * scale ? F_OK : F_EIGEN;
-}
-
-// Herschel-Bulkley implicit return (constitutive.rs:280-362), solved for the
-// overstress d = q - sigma_y on [0, q_tr - sigma_y] with the same bracketed Newton;
-// tolerance 2.5e-7 of the trial stress (a few f32 ulps).
-__device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, float& inc) {
-    float G3 = 3.0f * P.G;
-    if (!isfinite(qtr)) return F_NONFINITE;
-    if (qtr <= P.ys) {
-        q = qtr;
-        inc = 0.0f;
-        return F_OK;
-    }
-    float avail = qtr - P.ys;
-    if (P.K <= 0.0f) {
-        q = P.ys;
-        inc = avail / G3;
-        return F_OK;
-    }
-    float vs = G3 * dt;
-    float inv_k = 1.0f / P.K;
-    float lo = 0.0f, hi = avail, d = 0.5f * avail;
-    float tol = 2.5e-7f * fmaxf(qtr, 1.0f);
-    float r = d - avail + vs * powf(d * inv_k, P.inv_n);
-    for (int it = 0; it < 100; ++it) {
-        if (fabsf(r) <= tol || (hi - lo) <= tol) {
-            q = P.ys + d;
-            inc = (avail - d) / G3;
-            return F_OK;
+        if (s.active) {
+            StepStatus z = {};
+            z.fail = ~0ull;
+            s.st = z;
+            s.nw_count[s.cur ^ 1] = 0;
         }
-        if (r > 0.0f)
-            hi = d;
-        else
-            lo = d;
-        float over = d * inv_k;
-        float deriv = over > 0.0f ? 1.0f + vs * P.inv_n * powf(over, P.inv_n - 1.0f) * inv_k : INFINITY;
-        float nd = d - r / deriv;
-        d = (isfinite(nd) && nd > lo && nd < hi) ? nd : 0.5f * (lo + hi);
-        r = d - avail + vs * powf(d * inv_k, P.inv_n);
-        if (!isfinite(r)) return F_NONFINITE;
     }
-    return F_RETURN;
-}
-
-// update_paste (constitutive.rs:456-531) on H = F - I:
-//   b - I = H + H^T + H H^T, Hencky e = log1p(beta)/2, radial return on dev(e),
-//   F' = F + U diag(expm1((alpha - 1) dev e)) U^T F, tau = U diag(2G e' + lambda tr e) U^T.
-__device__ inline int paste_update(const Params& P, float dt, const float* Ht, float* Hn, float* tau, float& inc,
-                                   float& dd) {
-    float F[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) F[i] = Ht[i] + ((i % 4 == 0) ? 1.0f : 0.0f);
-    float det = det3(F);
-    if (!isfinite(det)) return F_NONFINITE;
-    if (det <= 0.0f) return F_INVERTED;
-    float B[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            B[3 * i + j] = Ht[3 * i + j] + Ht[3 * j + i] + Ht[3 * i] * Ht[3 * j] + Ht[3 * i + 1] * Ht[3 * j + 1] +
-                           Ht[3 * i + 2] * Ht[3 * j + 2];
-    float U[9];
-    int rc = eigen_sym3(B, U);
-    if (rc) return rc;
-    float beta[3] = {B[0], B[4], B[8]};
-    float e[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float s = sqrtf(fmaxf(1.0f + beta[a], 0.0f));
-        if (!isfinite(s) || s < 1e-6f) return F_STRETCH;
-        e[a] = 0.5f * log1pf(beta[a]);
-    }
-    float vol = e[0] + e[1] + e[2];
-    float mean = vol / 3.0f;
-    float dev[3] = {e[0] - mean, e[1] - mean, e[2] - mean};
-    float dn = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
-    float qtr = 1.2247448713915890f * 2.0f * P.G * dn;
-    float alpha = 1.0f, q = qtr;
-    inc = 0.0f;
-    if (qtr > P.ys && dn > 0.0f) {
-        rc = hb_return(P, dt, qtr, q, inc);
-        if (rc) return rc;
-        alpha = q / qtr;
-    }
-    float rm1[3], pr[3];
-    float lv = P.lam * vol;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        rm1[a] = expm1f((alpha - 1.0f) * dev[a]);
-        pr[a] = 2.0f * P.G * (alpha * dev[a] + mean) + lv;
-    }
-    // M = U diag(rm1) U^T, H' = H + M (I + H)
-    float M[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            M[3 * i + j] = U[3 * i] * rm1[0] * U[3 * j] + U[3 * i + 1] * rm1[1] * U[3 * j + 1] +
-                           U[3 * i + 2] * rm1[2] * U[3 * j + 2];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            Hn[3 * i + j] = Ht[3 * i + j] + M[3 * i + j] + M[3 * i] * Ht[j] + M[3 * i + 1] * Ht[3 + j] +
-                            M[3 * i + 2] * Ht[6 + j];
-    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        int i = ii[t], j = jj[t];
-        tau[t] = U[3 * i] * pr[0] * U[3 * j] + U[3 * i + 1] * pr[1] * U[3 * j + 1] + U[3 * i + 2] * pr[2] * U[3 * j + 2];
-        if (!isfinite(tau[t])) return F_NONFINITE;
-    }
-    dd = q * inc;
-    return F_OK;
-}
-
-// update_liquid (constitutive.rs:557-589) on J - 1, with the Tait pressure
-// p = (K/g) expm1(-g log1p(J - 1)); g = 1 gives the reference's K (1/J - 1).
-__device__ inline int liquid_update(const Params& P, float dt, float jm1, const float* C, float& jn, float* tau) {
-    float tr = C[0] + C[4] + C[8];
-    float g = dt * tr;
-    jn = jm1 + g + g * jm1;
-    float J = 1.0f + jn;
-    if (!isfinite(jn)) return F_NONFINITE;
-    if (J <= 1e-6f) return F_INVERTED;
-    float p = P.gamma == 1.0f ? -P.bulk * jn / J : (P.bulk / P.gamma) * expm1f(-P.gamma * log1pf(jn));
-    float c2 = 2.0f * P.visc * J;
-    float m3 = tr / 3.0f;
-    float pj = -p * J;
-    tau[0] = pj + c2 * (C[0] - m3);
-    tau[1] = pj + c2 * (C[4] - m3);
-    tau[2] = pj + c2 * (C[8] - m3);
-    tau[3] = c2 * 0.5f * (C[1] + C[3]);
-    tau[4] = c2 * 0.5f * (C[2] + C[6]);
-    tau[5] = c2 * 0.5f * (C[5] + C[7]);
-#pragma unroll
-    for (int t = 0; t < 6; ++t)
-        if (!isfinite(tau[t])) return F_NONFINITE;
-    return F_OK;
-}
-
-// ---------------------------------------------------------------- kernels
-
-__global__ void reset_status_kernel(StepStatus* st, int* nw_count) {
-    if (threadIdx.x == 0) {
-        StepStatus z = {};
-        z.fail = ~0ull;
-        *st = z;
-        *nw_count = 0;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void p2g_kernel(Params P, int n, const float* __restrict__ slab, size_t cap,
-                                                     const float* __restrict__ shared, GridPtrs g) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float x = slab[PX * cap + p], y = slab[PY * cap + p], z = slab[PZ * cap + p];
-    Stencil s;
-    if (!make_stencil(P, x, y, z, s)) return;
-    float C[9], tau[6];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) C[i] = slab[(C00 + i) * cap + p];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) tau[i] = slab[(TXX + i) * cap + p];
-    scatter_particle(P, g, s, shared[SMASS * cap + p], slab[VX * cap + p], slab[VY * cap + p], slab[VZ * cap + p],
-                     C, tau, shared[SVOL0 * cap + p]);
-}
-
-__device__ inline bool near_wall(const Params& P, float x, float y, float z) {
-    float half = 0.5f * P.dx;
-    return x < half || y < half || z < half || x > P.extent[0] - half || y > P.extent[1] - half ||
-           z > P.extent[2] - half;
-}
-
-// Indices of the particles within half a cell of a face (the only ones the
-// wall traction can touch).
-__global__ __launch_bounds__(kBlock) void nearwall_kernel(Params P, int n, const float* __restrict__ slab,
-                                                          size_t cap, int* list, int* count) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    if (near_wall(P, slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p])) list[atomicAdd(count, 1)] = p;
-}
-
-// wall_traction_to_grid (solver.rs:725-819).  Each deposit adds to the signed
-// traction impulse (trS) and the Coulomb budget (trT) of its node; the grid
-// update books the telescoped kinetic-energy jump per node.
-__global__ __launch_bounds__(kBlock) void traction_kernel(Params P, float dt, const int* __restrict__ list,
-                                                          const int* __restrict__ count, const float* __restrict__ slab,
-                                                          size_t cap, const float* __restrict__ shared, GridPtrs g,
-                                                          float* trS, float* trT, size_t na) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= *count) return;
-    int p = list[q];
-    float pos[3] = {slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]};
-    float half = 0.5f * P.dx;
-    float mass = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
-    float k4 = 4.0f * P.inv_dx * P.inv_dx;
-    float force_scale = -dt * k4;
-    for (int axis = 0; axis < 3; ++axis) {
-        float face;
-        if (pos[axis] < half)
-            face = 0.0f;
-        else if (pos[axis] > P.extent[axis] - half)
-            face = P.extent[axis];
-        else
-            continue;
-        float img[3] = {pos[0], pos[1], pos[2]};
-        img[axis] = 2.0f * face - pos[axis];
-        Stencil s;
-        if (!make_stencil(P, img[0], img[1], img[2], s)) continue;
-        int plane = face == 0.0f ? 1 : P.cells[axis] + 1;
-        float inward = face == 0.0f ? 1.0f : -1.0f;
-        float g_a = P.gravity[axis];
-        float normal_stress = slab[(TXX + axis) * cap + p] * v0 - mass * g_a * (img[axis] - pos[axis]);
-        if (normal_stress >= 0.0f) continue;
-        float weight_momentum = mass * g_a * dt;
-        const float* mom = axis == 0 ? g.px : (axis == 1 ? g.py : g.pz);
-        const float* frc = axis == 0 ? g.fx : (axis == 1 ? g.fy : g.fz);
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b)
-                for (int c = 0; c < 3; ++c) {
-                    int co[3] = {s.base[0] + a, s.base[1] + b, s.base[2] + c};
-                    if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
-                    float w = s.w[0][a] * s.w[1][b] * s.w[2][c];
-                    if (w == 0.0f) continue;
-                    int pc[3] = {co[0], co[1], co[2]};
-                    pc[axis] = plane;
-                    int pi = node_index(P, pc[0], pc[1], pc[2]);
-                    float pm = g.m[pi];
-                    if (pm <= 0.0f) continue;
-                    float trial = (mom[pi] - dt * k4 * frc[pi]) / pm + g_a * dt;
-                    if (trial * inward >= 0.0f) continue;
-                    int off = axis == 0 ? a : (axis == 1 ? b : c);
-                    float distance = s.d[axis][off] * P.dx;
-                    float reaction = (normal_stress * distance * force_scale + weight_momentum) * w * inward;
-                    if (reaction <= 0.0f) continue;
-                    int ni = node_index(P, co[0], co[1], co[2]);
-                    if (g.m[ni] <= 0.0f) continue;
-                    atomicAdd(trS + axis * na + ni, reaction * inward);
-                    atomicAdd(trT + axis * na + ni, reaction);
-                }
-    }
-}
-
-__device__ inline void reduce_tangential(float* v, float reduction) {
-    if (reduction == 0.0f) return;
-    float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    if (speed <= reduction) {
-        v[0] = v[1] = v[2] = 0.0f;
-    } else {
-        float f = (speed - reduction) / speed;
-        v[0] *= f;
-        v[1] *= f;
-        v[2] *= f;
-    }
-}
-
-// Block-wide sum of NV doubles held by every thread; thread 0 gets the totals.
-template <int NV>
-__device__ inline void block_sum(double* vals, double* scratch) {
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) vals[i] = wave_sum(vals[i]);
-    if (lane == 0)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) scratch[wave * NV + i] = vals[i];
     __syncthreads();
     if (threadIdx.x == 0) {
-        int waves = blockDim.x >> 6;
-        for (int i = 0; i < NV; ++i) {
-            double t = 0.0;
-            for (int w = 0; w < waves; ++w) t += scratch[w * NV + i];
-            vals[i] = t;
+        int acc = 0;
+        for (int t = 0; t < B.ninst; ++t) {
+            InstState& s = B.is[t];
+            acc += s.active ? s.nw_count[s.cur] : 0;
+            s.nw_base = acc;
+        }
+        B.counters[1] = acc;
+    }
+    // Start-of-step pellet geometry and zeroed coupling accumulators.
+    for (int k = threadIdx.x; k < B.npel; k += 256) {
+        const PelletState& p = B.pel[k];
+        if (p.inst < 0 || !B.is[p.inst].active) continue;
+        PelletGeo g;
+        const double xh[3] = {1.0, 0.0, 0.0};
+        qrot(p.q, xh, g.axis);
+        pellet_omega(p.q, p.L, p.inertia, g.omega);
+        double half = 0.0;
+        for (int q = 0; q < p.spheres; ++q) half = fmax(half, fabs(p.offsets[q]));
+        for (int a = 0; a < 3; ++a) {
+            g.pos[a] = p.pos[a];
+            g.vel[a] = p.vel[a];
+            double reach = fabs(g.axis[a]) * half + p.radius;
+            g.lo[a] = p.pos[a] - reach;
+            g.hi[a] = p.pos[a] + reach;
+        }
+        g.radius = p.radius;
+        g.spheres = p.spheres;
+        g.inst = p.inst;
+        B.pgeo[k] = g;
+        PelletAccum z = {};
+        B.pacc[k] = z;
+    }
+}
+
+// ------------------------------------------------------------ traction
+
+// wall_traction_to_grid (solver.rs:725-819) over the near-wall lists of the
+// running instances, concatenated.  Each deposit adds to the signed traction
+// impulse (trS) and the Coulomb budget (trT) of its node; the grid update books
+// the telescoped kinetic-energy jump per node.
+__global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
+    const int total = B.counters[1];
+    const float* shared = B.shared[B.sh];
+    const size_t cap = B.cap, na = B.na;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        int lo = 0, hi = B.ninst - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (B.is[mid].nw_base > q)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const InstState& s = B.is[lo];
+        const InstParams& ip = B.ip[lo];
+        const Params& P = ip.P;
+        int local = q - (s.nw_base - s.nw_count[s.cur]);
+        int p = B.nw_list[s.cur][ip.poff + local];
+        const float* slab = B.slab[s.cur];
+        const float dt = (float)s.dt;
+        GridPtrs g = grid_set(B, s.gcur);
+        const size_t noff = (size_t)ip.boff * 64;
+        float pos[3] = {slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]};
+        float half = 0.5f * P.dx;
+        float mass = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
+        float k4 = 4.0f * P.inv_dx * P.inv_dx;
+        float force_scale = -dt * k4;
+        for (int axis = 0; axis < 3; ++axis) {
+            float face;
+            if (pos[axis] < half)
+                face = 0.0f;
+            else if (pos[axis] > P.extent[axis] - half)
+                face = P.extent[axis];
+            else
+                continue;
+            float img[3] = {pos[0], pos[1], pos[2]};
+            img[axis] = 2.0f * face - pos[axis];
+            Stencil st;
+            if (!make_stencil(P, img[0], img[1], img[2], st)) continue;
+            int plane = face == 0.0f ? 1 : P.cells[axis] + 1;
+            float inward = face == 0.0f ? 1.0f : -1.0f;
+            float g_a = P.gravity[axis];
+            float normal_stress = slab[(TXX + axis) * cap + p] * v0 - mass * g_a * (img[axis] - pos[axis]);
+            if (normal_stress >= 0.0f) continue;
+            float weight_momentum = mass * g_a * dt;
+            const float* mom = axis == 0 ? g.px : (axis == 1 ? g.py : g.pz);
+            const float* frc = axis == 0 ? g.fx : (axis == 1 ? g.fy : g.fz);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    for (int c = 0; c < 3; ++c) {
+                        int co[3] = {st.base[0] + a, st.base[1] + b, st.base[2] + c};
+                        if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
+                        float w = st.w[0][a] * st.w[1][b] * st.w[2][c];
+                        if (w == 0.0f) continue;
+                        int pc[3] = {co[0], co[1], co[2]};
+                        pc[axis] = plane;
+                        size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
+                        float pm = g.m[pi];
+                        if (pm <= 0.0f) continue;
+                        float trial = (mom[pi] - dt * k4 * frc[pi]) / pm + g_a * dt;
+                        if (trial * inward >= 0.0f) continue;
+                        int off = axis == 0 ? a : (axis == 1 ? b : c);
+                        float distance = st.d[axis][off] * P.dx;
+                        float reaction = (normal_stress * distance * force_scale + weight_momentum) * w * inward;
+                        if (reaction <= 0.0f) continue;
+                        size_t ni = noff + node_index(P, co[0], co[1], co[2]);
+                        if (g.m[ni] <= 0.0f) continue;
+                        atomicAdd(B.trS + axis * na + ni, reaction * inward);
+                        atomicAdd(B.trT + axis * na + ni, reaction);
+                    }
         }
     }
 }
 
-// grid_update (solver.rs:822-892).  Four 64-node blocks per workgroup, one per
-// wavefront; ledger sums go to per-workgroup partials (no same-address atomics).
-constexpr int kGridVals = 12;
-__global__ __launch_bounds__(256) void grid_update_kernel(Params P, float dt, GridPtrs A, float* trS, float* trT,
-                                                          float* gv, size_t na, GridPtrs B, int nblocks,
-                                                          double* partials, int* bad_partials) {
+// ------------------------------------------------------------ grid update
+
+// grid_update (solver.rs:822-892).  Four 64-node blocks of one instance per
+// workgroup, one per wavefront; ledger sums go to per-workgroup partials (no
+// same-address atomics).  The other accumulator set is cleared on the way for
+// the trial deposit.
+__global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
     __shared__ double scratch[4 * kGridVals];
     __shared__ int sbad[4];
-    int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    int b = blockIdx.x * 4 + wave;
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wave;
+    const int inst = B.blk_inst[blockIdx.x * 4];
+    const InstState& s = B.is[inst];
+    if (!s.active) return;
+    const InstParams& ip = B.ip[inst];
+    const Params& P = ip.P;
+    GridPtrs A = grid_set(B, s.gcur), C = grid_set(B, s.gcur ^ 1);
+    const float dt = (float)s.dt;
+    const size_t na = B.na;
     double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int bad = 0;
-    if (b < nblocks) {
-        int n = (b << 6) | l;
-        if (B.flag[b]) {
-            B.m[n] = 0.0f;
-            B.px[n] = 0.0f;
-            B.py[n] = 0.0f;
-            B.pz[n] = 0.0f;
-            B.fx[n] = 0.0f;
-            B.fy[n] = 0.0f;
-            B.fz[n] = 0.0f;
-            if (l == 0) B.flag[b] = 0;
-        }
-        if (A.flag[b]) {
-            float m = A.m[n];
-            float k4 = 4.0f * P.inv_dx * P.inv_dx;
-            if (m > 0.0f) {
-                float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
-                float v[3], tb[3];
+    const size_t n = ((size_t)b << 6) | l;
+    if (C.flag[b]) {
+        C.m[n] = 0.0f;
+        C.px[n] = 0.0f;
+        C.py[n] = 0.0f;
+        C.pz[n] = 0.0f;
+        C.fx[n] = 0.0f;
+        C.fy[n] = 0.0f;
+        C.fz[n] = 0.0f;
+        if (l == 0) C.flag[b] = 0;
+    }
+    if (A.flag[b]) {
+        float m = A.m[n];
+        float k4 = 4.0f * P.inv_dx * P.inv_dx;
+        if (m > 0.0f) {
+            float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
+            float v[3], tb[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float D = B.trS[a * na + n];
+                tb[a] = B.trT[a * na + n];
+                if (D != 0.0f) {
+                    // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
+                    vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
+                    vals[3 + a] += D;
+                    B.trS[a * na + n] = 0.0f;
+                }
+                if (tb[a] != 0.0f) B.trT[a * na + n] = 0.0f;
+                v[a] = (p[a] + D) / m + P.gravity[a] * dt;
+            }
+            if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
+                bad = 1;
+                v[0] = v[1] = v[2] = 0.0f;
+            } else {
+                float before[3] = {v[0], v[1], v[2]};
+                int c[3];
+                node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
+                for (int axis = 0; axis < 3; ++axis) {
+                    int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
+                    if (sign == 0) continue;
+                    float ns = v[axis] * (float)sign;
+                    if (ns >= 0.0f) continue;
+                    vals[6] -= 0.5 * (double)m * (double)ns * (double)ns;
+                    v[axis] = 0.0f;
+                    double tn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                    reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
+                    double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                    vals[7] += 0.5 * (double)m * (tn - vn);
+                }
+                for (int axis = 0; axis < 3; ++axis) {
+                    float reaction = tb[axis];
+                    if (reaction <= 0.0f) continue;
+                    float normal = v[axis];
+                    float t[3] = {v[0], v[1], v[2]};
+                    t[axis] = 0.0f;
+                    double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                    int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
+                    reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
+                    double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                    vals[7] += 0.5 * (double)m * (tn - rn);
+                    v[0] = t[0];
+                    v[1] = t[1];
+                    v[2] = t[2];
+                    v[axis] = normal;
+                }
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    float D = trS[a * na + n];
-                    tb[a] = trT[a * na + n];
-                    if (D != 0.0f) {
-                        // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
-                        vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
-                        vals[3 + a] += D;
-                        trS[a * na + n] = 0.0f;
-                    }
-                    if (tb[a] != 0.0f) trT[a * na + n] = 0.0f;
-                    v[a] = (p[a] + D) / m + P.gravity[a] * dt;
+                    vals[a] = ((double)v[a] - (double)before[a]) * (double)m;
+                    vals[9 + a] = (double)v[a] * (double)m;
                 }
-                if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
-                    bad = 1;
-                    v[0] = v[1] = v[2] = 0.0f;
-                } else {
-                    float before[3] = {v[0], v[1], v[2]};
-                    int c[3];
-                    node_coords(P, n, c[0], c[1], c[2]);
-                    for (int axis = 0; axis < 3; ++axis) {
-                        int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
-                        if (sign == 0) continue;
-                        float ns = v[axis] * (float)sign;
-                        if (ns >= 0.0f) continue;
-                        vals[6] -= 0.5 * (double)m * (double)ns * (double)ns;
-                        v[axis] = 0.0f;
-                        double tn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                        reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
-                        double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                        vals[7] += 0.5 * (double)m * (tn - vn);
-                    }
-                    for (int axis = 0; axis < 3; ++axis) {
-                        float reaction = tb[axis];
-                        if (reaction <= 0.0f) continue;
-                        float normal = v[axis];
-                        float t[3] = {v[0], v[1], v[2]};
-                        t[axis] = 0.0f;
-                        double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                        int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
-                        reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
-                        double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                        vals[7] += 0.5 * (double)m * (tn - rn);
-                        v[0] = t[0];
-                        v[1] = t[1];
-                        v[2] = t[2];
-                        v[axis] = normal;
-                    }
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        vals[a] = ((double)v[a] - (double)before[a]) * (double)m;
-                        vals[9 + a] = (double)v[a] * (double)m;
-                    }
-                }
-                gv[n] = v[0];
-                gv[na + n] = v[1];
-                gv[2 * na + n] = v[2];
-            } else {
-                gv[n] = 0.0f;
-                gv[na + n] = 0.0f;
-                gv[2 * na + n] = 0.0f;
             }
+            B.gv[n] = v[0];
+            B.gv[na + n] = v[1];
+            B.gv[2 * na + n] = v[2];
+        } else {
+            B.gv[n] = 0.0f;
+            B.gv[na + n] = 0.0f;
+            B.gv[2 * na + n] = 0.0f;
         }
     }
     int nbad = wave_sum_i(bad);
@@ -564,236 +373,243 @@ __global__ __launch_bounds__(256) void grid_update_kernel(Params P, float dt, Gr
     block_sum<kGridVals>(vals, scratch);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < kGridVals; ++i) partials[blockIdx.x * kGridVals + i] = vals[i];
-        bad_partials[blockIdx.x] = sbad[0] + sbad[1] + sbad[2] + sbad[3];
+        for (int i = 0; i < kGridVals; ++i) B.gpart[(size_t)i * B.ngwg + blockIdx.x] = vals[i];
+        B.gbad[blockIdx.x] = sbad[0] + sbad[1] + sbad[2] + sbad[3];
     }
 }
 
-// couple_grid (rigid.rs:385-438) for every pellet listed on this node block, in
-// pellet index order, so a node sees the pellets one after another as in a
-// sequential loop over pellets.  Geometry and the projection run in f64 with the
+// ------------------------------------------------------------ pellet coupling
+
+// couple_grid (rigid.rs:385-438) for the pellets of the node block's instance in
+// pellet index order, so a node sees the pellets one after another as in the
+// sequential loop over pellets.  The wavefront first tests 64 pellets at a time
+// against the block's node box, then walks the hits in index order.  f64 with the
 // reference's operation order; node positions are (i - 1) dx exactly.
-__global__ __launch_bounds__(64) void couple_kernel(Params P, double dx, GridPtrs A, float* gv, size_t na,
-                                                    const PelletDev* __restrict__ pellets,
-                                                    const int* __restrict__ bstart, const int* __restrict__ blist,
-                                                    PelletAccum* acc, double friction) {
-    int b = blockIdx.x, l = threadIdx.x;
-    int n = (b << 6) | l;
-    int s0 = bstart[b], s1 = bstart[b + 1];
-    if (s0 == s1 || !A.flag[b]) return;
-    double m = A.m[n];
-    if (!(m > 0.0)) return;
+__global__ __launch_bounds__(64) void couple_kernel(BatchDev B) {
+    const int b = blockIdx.x, l = threadIdx.x;
+    const int inst = B.blk_inst[b];
+    const InstState& s = B.is[inst];
+    const InstParams& ip = B.ip[inst];
+    if (!s.active || ip.npel == 0) return;
+    GridPtrs A = grid_set(B, s.gcur);
+    if (!A.flag[b]) return;
+    const size_t n = ((size_t)b << 6) | l;
+    const size_t na = B.na;
+    const double m = A.m[n];
     int ci, cj, ck;
-    node_coords(P, n, ci, cj, ck);
-    double node[3] = {((double)ci - 1.0) * dx, ((double)cj - 1.0) * dx, ((double)ck - 1.0) * dx};
-    double v[3] = {gv[n], gv[na + n], gv[2 * na + n]};
+    node_coords(ip.P, (int)((size_t)(b - ip.boff) << 6) | l, ci, cj, ck);
+    const double dx = ip.dx;
+    const double node[3] = {((double)ci - 1.0) * dx, ((double)cj - 1.0) * dx, ((double)ck - 1.0) * dx};
+    // Node box of the block (node coordinates 4 bi .. 4 bi + 3 per axis).
+    double blo[3], bhi[3];
+    {
+        int c0[3];
+        node_coords(ip.P, (b - ip.boff) << 6, c0[0], c0[1], c0[2]);
+        for (int a = 0; a < 3; ++a) {
+            blo[a] = ((double)c0[a] - 1.0) * dx;
+            bhi[a] = ((double)c0[a] + 2.0) * dx;
+        }
+    }
+    double v[3] = {B.gv[n], B.gv[na + n], B.gv[2 * na + n]};
     bool changed = false;
-    for (int s = s0; s < s1; ++s) {
-        int pi = blist[s];
-        const PelletDev& pl = pellets[pi];
-        if (node[0] < pl.lo[0] || node[0] > pl.hi[0] || node[1] < pl.lo[1] || node[1] > pl.hi[1] ||
-            node[2] < pl.lo[2] || node[2] > pl.hi[2])
-            continue;
-        double best = INFINITY, nrm[3] = {0.0, 0.0, 1.0};
-        for (int k = 0; k < pl.spheres; ++k) {
-            double o = pl.offsets[k];
-            double dlt[3] = {node[0] - (pl.pos[0] + pl.axis[0] * o), node[1] - (pl.pos[1] + pl.axis[1] * o),
-                             node[2] - (pl.pos[2] + pl.axis[2] * o)};
-            double dist = sqrt(dlt[0] * dlt[0] + dlt[1] * dlt[1] + dlt[2] * dlt[2]);
-            double phi = dist - pl.radius;
-            if (phi < best) {
-                best = phi;
-                if (dist > 1e-12) {
-                    nrm[0] = dlt[0] / dist;
-                    nrm[1] = dlt[1] / dist;
-                    nrm[2] = dlt[2] / dist;
-                } else {
-                    nrm[0] = 0.0;
-                    nrm[1] = 0.0;
-                    nrm[2] = 1.0;
+    for (int c0 = 0; c0 < ip.npel; c0 += 64) {
+        bool hit = false;
+        if (c0 + l < ip.npel) {
+            const PelletGeo& g = B.pgeo[ip.pel_off + c0 + l];
+            hit = !(bhi[0] < g.lo[0] || blo[0] > g.hi[0] || bhi[1] < g.lo[1] || blo[1] > g.hi[1] ||
+                    bhi[2] < g.lo[2] || blo[2] > g.hi[2]);
+        }
+        unsigned long long mask = __ballot(hit);
+        while (mask) {
+            int k = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            const int pi = ip.pel_off + c0 + k;
+            const PelletGeo& g = B.pgeo[pi];
+            if (!(m > 0.0)) continue;
+            if (node[0] < g.lo[0] || node[0] > g.hi[0] || node[1] < g.lo[1] || node[1] > g.hi[1] ||
+                node[2] < g.lo[2] || node[2] > g.hi[2])
+                continue;
+            const double* offs = B.pel[pi].offsets;
+            double best = INFINITY, nrm[3] = {0.0, 0.0, 1.0};
+            for (int q = 0; q < g.spheres; ++q) {
+                double o = offs[q];
+                double dlt[3] = {node[0] - (g.pos[0] + g.axis[0] * o), node[1] - (g.pos[1] + g.axis[1] * o),
+                                 node[2] - (g.pos[2] + g.axis[2] * o)};
+                double dist = sqrt(dlt[0] * dlt[0] + dlt[1] * dlt[1] + dlt[2] * dlt[2]);
+                double phi = dist - g.radius;
+                if (phi < best) {
+                    best = phi;
+                    if (dist > 1e-12) {
+                        nrm[0] = dlt[0] / dist;
+                        nrm[1] = dlt[1] / dist;
+                        nrm[2] = dlt[2] / dist;
+                    } else {
+                        nrm[0] = 0.0;
+                        nrm[1] = 0.0;
+                        nrm[2] = 1.0;
+                    }
                 }
             }
+            if (best > 0.0) continue;
+            double r[3] = {node[0] - g.pos[0], node[1] - g.pos[1], node[2] - g.pos[2]};
+            double body[3] = {g.vel[0] + (g.omega[1] * r[2] - g.omega[2] * r[1]),
+                              g.vel[1] + (g.omega[2] * r[0] - g.omega[0] * r[2]),
+                              g.vel[2] + (g.omega[0] * r[1] - g.omega[1] * r[0])};
+            double rel[3] = {v[0] - body[0], v[1] - body[1], v[2] - body[2]};
+            double ns = rel[0] * nrm[0] + rel[1] * nrm[1] + rel[2] * nrm[2];
+            if (ns >= 0.0) continue;
+            double t[3] = {rel[0] - nrm[0] * ns, rel[1] - nrm[1] * ns, rel[2] - nrm[2] * ns};
+            double ts = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+            double red = ip.wall_friction * (-ns);
+            double f = (ts <= red || ts < 1e-14) ? 0.0 : (ts - red) / ts;
+            double nv[3] = {body[0] + t[0] * f, body[1] + t[1] * f, body[2] + t[2] * f};
+            double dp[3] = {(nv[0] - v[0]) * m, (nv[1] - v[1]) * m, (nv[2] - v[2]) * m};
+            double e = 0.5 * m *
+                       ((nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) - (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+            PelletAccum* acc = B.pacc + pi;
+            atomicAdd(&acc->impulse[0], -dp[0]);
+            atomicAdd(&acc->impulse[1], -dp[1]);
+            atomicAdd(&acc->impulse[2], -dp[2]);
+            atomicAdd(&acc->angular[0], -(r[1] * dp[2] - r[2] * dp[1]));
+            atomicAdd(&acc->angular[1], -(r[2] * dp[0] - r[0] * dp[2]));
+            atomicAdd(&acc->angular[2], -(r[0] * dp[1] - r[1] * dp[0]));
+            atomicAdd(&acc->grid_energy, e);
+            atomicAdd(&acc->constrained, 1.0);
+            v[0] = nv[0];
+            v[1] = nv[1];
+            v[2] = nv[2];
+            changed = true;
         }
-        if (best > 0.0) continue;
-        double r[3] = {node[0] - pl.pos[0], node[1] - pl.pos[1], node[2] - pl.pos[2]};
-        double body[3] = {pl.vel[0] + (pl.omega[1] * r[2] - pl.omega[2] * r[1]),
-                          pl.vel[1] + (pl.omega[2] * r[0] - pl.omega[0] * r[2]),
-                          pl.vel[2] + (pl.omega[0] * r[1] - pl.omega[1] * r[0])};
-        double rel[3] = {v[0] - body[0], v[1] - body[1], v[2] - body[2]};
-        double ns = rel[0] * nrm[0] + rel[1] * nrm[1] + rel[2] * nrm[2];
-        if (ns >= 0.0) continue;
-        double t[3] = {rel[0] - nrm[0] * ns, rel[1] - nrm[1] * ns, rel[2] - nrm[2] * ns};
-        double ts = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-        double red = friction * (-ns);
-        double f = (ts <= red || ts < 1e-14) ? 0.0 : (ts - red) / ts;
-        double nv[3] = {body[0] + t[0] * f, body[1] + t[1] * f, body[2] + t[2] * f};
-        double dp[3] = {(nv[0] - v[0]) * m, (nv[1] - v[1]) * m, (nv[2] - v[2]) * m};
-        double e = 0.5 * m * ((nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) - (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-        PelletAccum* a = acc + pi;
-        atomicAdd(&a->impulse[0], -dp[0]);
-        atomicAdd(&a->impulse[1], -dp[1]);
-        atomicAdd(&a->impulse[2], -dp[2]);
-        atomicAdd(&a->angular[0], -(r[1] * dp[2] - r[2] * dp[1]));
-        atomicAdd(&a->angular[1], -(r[2] * dp[0] - r[0] * dp[2]));
-        atomicAdd(&a->angular[2], -(r[0] * dp[1] - r[1] * dp[0]));
-        atomicAdd(&a->grid_energy, e);
-        atomicAdd(&a->constrained, 1.0);
-        v[0] = nv[0];
-        v[1] = nv[1];
-        v[2] = nv[2];
-        changed = true;
     }
     if (changed) {
-        gv[n] = (float)v[0];
-        gv[na + n] = (float)v[1];
-        gv[2 * na + n] = (float)v[2];
+        B.gv[n] = (float)v[0];
+        B.gv[na + n] = (float)v[1];
+        B.gv[2 * na + n] = (float)v[2];
     }
+}
+
+// Pellet::apply_impulse and ::integrate (rigid.rs:251-334) on a trial copy,
+// committed with the step.
+__global__ void pellet_kernel(BatchDev B) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B.npel) return;
+    const PelletState& p = B.pel[k];
+    if (p.inst < 0) return;
+    const InstState& s = B.is[p.inst];
+    if (!s.active) return;
+    const InstParams& ip = B.ip[p.inst];
+    const PelletAccum& acc = B.pacc[k];
+    PelletTrial t;
+    double vel[3], L[3], q[4], pos[3];
+    for (int a = 0; a < 3; ++a) {
+        vel[a] = p.vel[a];
+        L[a] = p.L[a];
+        pos[a] = p.pos[a];
+    }
+    for (int a = 0; a < 4; ++a) q[a] = p.q[a];
+    double before = pellet_kinetic(p.mass, vel, q, L, p.inertia);
+    for (int a = 0; a < 3; ++a) {
+        vel[a] = vel[a] + acc.impulse[a] / p.mass;
+        L[a] = L[a] + acc.angular[a];
+    }
+    t.energy = pellet_kinetic(p.mass, vel, q, L, p.inertia) - before;
+    // rigid.rs:279-334
+    const double dt = s.dt;
+    const double damping = ip.contact_damping * sqrt(ip.contact_k * p.mass);
+    double force[3] = {0, 0, 0}, torque[3] = {0, 0, 0}, work = 0.0;
+    double om[3], axis[3];
+    pellet_omega(q, L, p.inertia, om);
+    const double xh[3] = {1.0, 0.0, 0.0};
+    qrot(q, xh, axis);
+    const double ext[3] = {ip.P.cells[0] * ip.dx, ip.P.cells[1] * ip.dx, ip.P.cells[2] * ip.dx};
+    for (int sph = 0; sph < p.spheres; ++sph) {
+        double o = p.offsets[sph];
+        double center[3] = {pos[0] + axis[0] * o, pos[1] + axis[1] * o, pos[2] + axis[2] * o};
+        for (int a = 0; a < 3; ++a)
+            for (int side = 0; side < 2; ++side) {
+                double face = side == 0 ? 0.0 : ext[a];
+                double sign = side == 0 ? 1.0 : -1.0;
+                double pen = p.radius - (center[a] - face) * sign;
+                if (pen <= 0.0) continue;
+                double nrm[3] = {0, 0, 0};
+                nrm[a] = sign;
+                double lever[3] = {center[0] - nrm[0] * p.radius - pos[0], center[1] - nrm[1] * p.radius - pos[1],
+                                   center[2] - nrm[2] * p.radius - pos[2]};
+                double pv[3] = {vel[0] + (om[1] * lever[2] - om[2] * lever[1]),
+                                vel[1] + (om[2] * lever[0] - om[0] * lever[2]),
+                                vel[2] + (om[0] * lever[1] - om[1] * lever[0])};
+                double vn = pv[0] * nrm[0] + pv[1] * nrm[1] + pv[2] * nrm[2];
+                double fn = fmax(ip.contact_k * pen - damping * vn, 0.0);
+                double tg[3] = {pv[0] - nrm[0] * vn, pv[1] - nrm[1] * vn, pv[2] - nrm[2] * vn};
+                double ts = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+                double fr[3] = {0, 0, 0};
+                if (ts > 1e-12) {
+                    double lim = fmin(ip.contact_friction * fn, p.mass * ts / dt);
+                    for (int c = 0; c < 3; ++c) fr[c] = -tg[c] / ts * lim;
+                }
+                double total[3] = {nrm[0] * fn + fr[0], nrm[1] * fn + fr[1], nrm[2] * fn + fr[2]};
+                for (int c = 0; c < 3; ++c) force[c] += total[c];
+                torque[0] += lever[1] * total[2] - lever[2] * total[1];
+                torque[1] += lever[2] * total[0] - lever[0] * total[2];
+                torque[2] += lever[0] * total[1] - lever[1] * total[0];
+                work += (total[0] * pv[0] + total[1] * pv[1] + total[2] * pv[2]) * dt;
+            }
+    }
+    for (int a = 0; a < 3; ++a) {
+        double imp = force[a] * dt;
+        t.wall_impulse[a] = imp;
+        vel[a] = vel[a] + (ip.gravity[a] * dt + imp / p.mass);
+        L[a] = L[a] + torque[a] * dt;
+        pos[a] = pos[a] + vel[a] * dt;
+    }
+    t.wall_work = work;
+    // orientation = UnitQuaternion::from_scaled_axis(omega dt) * orientation
+    double omn[3];
+    pellet_omega(q, L, p.inertia, omn);
+    double half[3] = {omn[0] * dt * 0.5, omn[1] * dt * 0.5, omn[2] * dt * 0.5};
+    double nn = half[0] * half[0] + half[1] * half[1] + half[2] * half[2];
+    double inc[4] = {0.0, 0.0, 0.0, 1.0};
+    const double eps = 2.220446049250313e-16;
+    if (nn > eps * eps) {
+        double nr = sqrt(nn);
+        double f = sin(nr) / nr;
+        inc[0] = half[0] * f;
+        inc[1] = half[1] * f;
+        inc[2] = half[2] * f;
+        inc[3] = cos(nr);
+    }
+    double nq[4] = {inc[3] * q[0] + inc[0] * q[3] + inc[1] * q[2] - inc[2] * q[1],
+                    inc[3] * q[1] - inc[0] * q[2] + inc[1] * q[3] + inc[2] * q[0],
+                    inc[3] * q[2] + inc[0] * q[1] - inc[1] * q[0] + inc[2] * q[3],
+                    inc[3] * q[3] - inc[0] * q[0] - inc[1] * q[1] - inc[2] * q[2]};
+    bool finite = isfinite(t.energy);
+    for (int a = 0; a < 3; ++a) {
+        t.pos[a] = pos[a];
+        t.vel[a] = vel[a];
+        t.L[a] = L[a];
+        finite = finite && isfinite(pos[a]) && isfinite(vel[a]) && isfinite(L[a]);
+    }
+    for (int a = 0; a < 4; ++a) {
+        t.q[a] = nq[a];
+        finite = finite && isfinite(nq[a]);
+    }
+    t.bad = finite ? 0 : 1;
+    t.pad = 0;
+    B.ptrial[k] = t;
 }
 
 // ------------------------------------------------------------ fused G2P2G
 
-// Tile geometry of one workgroup's chunk of particles.  Gather tile: the nodes
-// of every current stencil; scatter tile: one cell wider on each side, since an
-// accepted step moves a particle by at most one cell.
-struct Tile {
-    int org[3];  // gather origin (lowest stencil base)
-    int gd[3];   // gather dims
-    int sd[3];   // scatter dims (origin org - 1)
-    int tiled;
-};
-
-__device__ inline void gather_global(const Params& P, const float* __restrict__ gv, size_t na, const Stencil& s,
-                                     float* v, float* B) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            float wab = s.w[0][a] * s.w[1][b];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float w = wab * s.w[2][c];
-                int ni = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
-                float nv[3] = {gv[ni] * w, gv[na + ni] * w, gv[2 * na + ni] * w};
-                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    v[i] += nv[i];
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
-                }
-            }
-        }
-}
-
-__device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil& s, float* v, float* B) {
-    int li = s.base[0] - t.org[0], lj = s.base[1] - t.org[1], lk = s.base[2] - t.org[2];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            float wab = s.w[0][a] * s.w[1][b];
-            int row = ((li + a) * t.gd[1] + (lj + b)) * t.gd[2] + lk;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float w = wab * s.w[2][c];
-                float4 g = sg[row + c];
-                float nv[3] = {g.x * w, g.y * w, g.z * w};
-                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    v[i] += nv[i];
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
-                }
-            }
-        }
-}
-
-// Select among three register values by a runtime index (no array indexing, which
-// would place the stencil in scratch memory).
-__device__ inline float sel3(int i, float x0, float x1, float x2) { return i == 0 ? x0 : (i == 1 ? x1 : x2); }
-
-// deposit_particles for one particle into the f64 LDS accumulator tile (LDS
-// f64 atomic adds run at close to full LDS rate on CDNA4; f32 ones do not).
-// Lanes walk the nine (a, b) stencil columns starting at a lane-dependent
-// column so that particles of one cell, which share most nodes, hit different
-// LDS words.
-template <int SCAP>
-__device__ inline void scatter_lds(double* ss, const int* so, const int* sd, const Params& P, const Stencil& s,
-                                   float m, const float* v, const float* C, const float* tau, float v0, int rot) {
-    float mc[9], vt[6];
-    float mdx = m * P.dx, vdx = v0 * P.dx;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
-    float mv0 = m * v[0], mv1 = m * v[1], mv2 = m * v[2];
-    int li = s.base[0] - so[0], lj = s.base[1] - so[1], lk = s.base[2] - so[2];
-    for (int kk = 0; kk < 9; ++kk) {
-        int ab = kk + rot;
-        ab = ab >= 9 ? ab - 9 : ab;
-        int a = ab >= 6 ? 2 : (ab >= 3 ? 1 : 0);
-        int b = ab - 3 * a;
-        float wab = sel3(a, s.w[0][0], s.w[0][1], s.w[0][2]) * sel3(b, s.w[1][0], s.w[1][1], s.w[1][2]);
-        float dx = sel3(a, s.d[0][0], s.d[0][1], s.d[0][2]), dy = sel3(b, s.d[1][0], s.d[1][1], s.d[1][2]);
-        int row = ((li + a) * sd[1] + (lj + b)) * sd[2] + lk;
-        float px0 = mv0 + mc[0] * dx + mc[1] * dy, py0 = mv1 + mc[3] * dx + mc[4] * dy,
-              pz0 = mv2 + mc[6] * dx + mc[7] * dy;
-        float fx0 = vt[0] * dx + vt[3] * dy, fy0 = vt[3] * dx + vt[1] * dy, fz0 = vt[4] * dx + vt[5] * dy;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float w = wab * s.w[2][c];
-            float dz = s.d[2][c];
-            int idx = row + c;
-            atomicAdd(ss + 0 * SCAP + idx, (double)(w * m));
-            atomicAdd(ss + 1 * SCAP + idx, (double)(w * (px0 + mc[2] * dz)));
-            atomicAdd(ss + 2 * SCAP + idx, (double)(w * (py0 + mc[5] * dz)));
-            atomicAdd(ss + 3 * SCAP + idx, (double)(w * (pz0 + mc[8] * dz)));
-            atomicAdd(ss + 4 * SCAP + idx, (double)(w * (fx0 + vt[4] * dz)));
-            atomicAdd(ss + 5 * SCAP + idx, (double)(w * (fy0 + vt[5] * dz)));
-            atomicAdd(ss + 6 * SCAP + idx, (double)(w * (fz0 + vt[2] * dz)));
-        }
-    }
-}
-
-__device__ inline int wave_min_i(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline int wave_max_i(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Wave-aggregated append of this lane's index when `pred` holds.
-__device__ inline void wave_append(bool pred, int value, int* list, int* count) {
-    unsigned long long mask = __ballot(pred);
-    if (!mask) return;
-    int lane = threadIdx.x & 63;
-    int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-}
-
 // grid_to_particle + constitutive update (solver.rs:895-977) fused with the
 // next step's particle-to-grid deposit into the other accumulator set.  A
-// workgroup owns one or two cell blocks of particles (a work item).  It stages
-// the node velocities of their stencils in LDS, updates every particle, then
-// deposits the updated particles into an f64 LDS tile covering exactly their
-// new stencils and flushes it with one global atomic per touched node and field.
-// Work items whose tiles exceed the LDS budget take the direct global path.
+// workgroup owns one work item: the particles of one or two z-adjacent cell
+// blocks of one instance.  It stages the node velocities of their stencils in
+// LDS, updates every particle, then deposits the updated particles into an f64
+// LDS tile covering exactly their new stencils and flushes it with one global
+// atomic per touched node and field.  Work items whose tiles exceed the LDS
+// budget take the direct global path.
 template <int CHUNK, int GCAP, int SCAP>
-__global__ __launch_bounds__(kBlock) void g2p2g_kernel(Params P, float dt, int n, const float* __restrict__ src,
-                                                       float* __restrict__ dst, size_t cap,
-                                                       const float* __restrict__ shared,
-                                                       const float* __restrict__ gv, size_t na, GridPtrs out,
-                                                       const int2* __restrict__ work, float* pmax, double* pdiss,
-                                                       unsigned long long* pfail, int* pesc, int* nw_list,
-                                                       int* nw_count) {
+__global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
     __shared__ float4 sg[GCAP];
     __shared__ double ss[7 * SCAP];
@@ -804,10 +620,33 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(Params P, float dt, int n
     __shared__ float sred[4][2];
     __shared__ unsigned long long sfail[4];
     __shared__ int sesc[4];
+    const int4 item = B.work[blockIdx.x];
+    const int inst = item.x;
+    InstState& is = B.is[inst];
+    if (!is.active) return;
+    const InstParams& ip = B.ip[inst];
+    const Params& P = ip.P;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int2 item = work[blockIdx.x];
-    const int chunk0 = item.x;
-    const int chunk_end = min(item.x + item.y, n);
+    const int chunk0 = item.y;
+    const int chunk_end = item.y + item.z;
+    const float dt = (float)is.dt;
+    const size_t cap = B.cap, na = B.na;
+    const float* __restrict__ src = B.slab[is.cur];
+    float* __restrict__ dst = B.slab[is.cur ^ 1];
+    const float* __restrict__ shared = B.shared[B.sh];
+    const size_t noff = (size_t)ip.boff * 64;
+    const float* __restrict__ gv = B.gv + noff;
+    GridPtrs out = grid_set(B, is.gcur ^ 1);
+    out.m += noff;
+    out.px += noff;
+    out.py += noff;
+    out.pz += noff;
+    out.fx += noff;
+    out.fy += noff;
+    out.fz += noff;
+    out.flag += ip.boff;
+    int* nw_list = B.nw_list[is.cur ^ 1] + ip.poff;
+    int* nw_count = &is.nw_count[is.cur ^ 1];
 
     // Phase A: stencil bases of the current positions and their bounding box.
     float px[R], py[R], pz[R];
@@ -897,15 +736,15 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(Params P, float dt, int n
                 escaped += 1;
             } else {
                 float v[3] = {0.0f, 0.0f, 0.0f};
-                float B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                float Bm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                 if (t.tiled)
-                    gather_lds(sg, t, s, v, B);
+                    gather_lds(sg, t, s, v, Bm);
                 else
-                    gather_global(P, gv, na, s, v, B);
+                    gather_global(P, gv, na, s, v, Bm);
                 float C[9];
                 float k4 = 4.0f * P.inv_dx;
 #pragma unroll
-                for (int i = 0; i < 9; ++i) C[i] = B[i] * k4;
+                for (int i = 0; i < 9; ++i) C[i] = Bm[i] * k4;
                 float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
                 float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
                 speed = fmaxf(speed, vn);
@@ -937,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(Params P, float dt, int n
                     if (rc == F_OK) dst[JM1 * cap + p] = jn;
                 }
                 if (rc != F_OK) {
-                    unsigned long long f = ((unsigned long long)p << 8) | (unsigned long long)rc;
+                    unsigned long long f = ((unsigned long long)(p - ip.poff) << 8) | (unsigned long long)rc;
                     fail = f < fail ? f : fail;
                 } else {
                     dst[PX * cap + p] = nx;
@@ -1051,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(Params P, float dt, int n
             out.flag[ni >> 6] = 1;
         }
     }
-    // Per-workgroup partials of the step reductions.
+    // Per-work-item partials of the step reductions.
     speed = wave_max(speed);
     disp = wave_max(disp);
     diss = wave_sum(diss);
@@ -1077,53 +916,51 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(Params P, float dt, int n
             f = sfail[w] < f ? sfail[w] : f;
             e += sesc[w];
         }
-        pmax[2 * blockIdx.x] = s0;
-        pmax[2 * blockIdx.x + 1] = s1;
-        pdiss[blockIdx.x] = d;
-        pfail[blockIdx.x] = f;
-        pesc[blockIdx.x] = e;
+        B.pmax[2 * blockIdx.x] = s0;
+        B.pmax[2 * blockIdx.x + 1] = s1;
+        B.pdiss[blockIdx.x] = d;
+        B.pfail[blockIdx.x] = f;
+        B.pesc[blockIdx.x] = e;
     }
 }
 
-// Fixed-order reduction of the per-workgroup partials into the step status.
-__global__ __launch_bounds__(256) void finalize_kernel(int ngrid, const double* gpart, const int* gbad, int npart,
-                                                       const float* pmax, const double* pdiss,
-                                                       const unsigned long long* pfail, const int* pesc,
-                                                       const int* nw_count, StepStatus* st) {
-    __shared__ double sd[256 * 4];
+// ------------------------------------------------------------ reductions and commit
+
+// Fixed-order reduction of one instance's partials into its step status.
+__global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
+    __shared__ double sd[256];
     __shared__ float sf[256 * 2];
     __shared__ unsigned long long su[256];
     __shared__ int si[256 * 2];
-    int tid = threadIdx.x;
+    const int inst = blockIdx.x, tid = threadIdx.x;
+    InstState& s = B.is[inst];
+    if (!s.active) return;
+    const InstParams& ip = B.ip[inst];
+    const int g0 = ip.boff / 4, g1 = (ip.boff + ip.nblk) / 4;
     double acc[kGridVals + 1];
     for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
     int bad = 0;
-    for (int q = tid; q < ngrid; q += 256) {
-        for (int i = 0; i < kGridVals; ++i) acc[i] += gpart[q * kGridVals + i];
-        bad += gbad[q];
+    for (int q = g0 + tid; q < g1; q += 256) {
+#pragma unroll
+        for (int i = 0; i < kGridVals; ++i) acc[i] += B.gpart[(size_t)i * B.ngwg + q];
+        bad += B.gbad[q];
     }
     float smax = 0.0f, dmax = 0.0f;
     unsigned long long f = ~0ull;
     int esc = 0;
-    for (int q = tid; q < npart; q += 256) {
-        smax = fmaxf(smax, pmax[2 * q]);
-        dmax = fmaxf(dmax, pmax[2 * q + 1]);
-        acc[kGridVals] += pdiss[q];
-        f = pfail[q] < f ? pfail[q] : f;
-        esc += pesc[q];
+    for (int q = s.w_off + tid; q < s.w_off + s.w_cnt; q += 256) {
+        smax = fmaxf(smax, B.pmax[2 * q]);
+        dmax = fmaxf(dmax, B.pmax[2 * q + 1]);
+        acc[kGridVals] += B.pdiss[q];
+        f = B.pfail[q] < f ? B.pfail[q] : f;
+        esc += B.pesc[q];
     }
-    // Tree over threads, one quantity group at a time.
-    for (int i = 0; i <= kGridVals; i += 4) {
-        for (int k = 0; k < 4; ++k) sd[tid * 4 + k] = (i + k <= kGridVals) ? acc[i + k] : 0.0;
+    double tot[kGridVals + 1];
+    for (int i = 0; i <= kGridVals; ++i) {
+        double v = wave_sum(acc[i]);
+        if ((tid & 63) == 0) sd[tid >> 6] = v;
         __syncthreads();
-        for (int stride = 128; stride > 0; stride >>= 1) {
-            if (tid < stride)
-                for (int k = 0; k < 4; ++k) sd[tid * 4 + k] += sd[(tid + stride) * 4 + k];
-            __syncthreads();
-        }
-        if (tid == 0)
-            for (int k = 0; k < 4; ++k)
-                if (i + k <= kGridVals) acc[i + k] = sd[k];
+        tot[i] = sd[0] + sd[1] + sd[2] + sd[3];
         __syncthreads();
     }
     sf[tid * 2] = smax;
@@ -1143,33 +980,121 @@ __global__ __launch_bounds__(256) void finalize_kernel(int ngrid, const double* 
         __syncthreads();
     }
     if (tid == 0) {
+        StepStatus& st = s.st;
         for (int a = 0; a < 3; ++a) {
-            st->wall_impulse[a] = acc[a];
-            st->traction_impulse[a] = acc[3 + a];
-            st->grid_momentum[a] = acc[9 + a];
+            st.wall_impulse[a] = tot[a];
+            st.traction_impulse[a] = tot[3 + a];
+            st.grid_momentum[a] = tot[9 + a];
         }
-        st->projection = acc[6];
-        st->friction = acc[7];
-        st->traction_energy = acc[8];
-        st->dissipation = acc[kGridVals];
-        st->max_speed_bits = __float_as_uint(sf[0]);
-        st->max_disp_bits = __float_as_uint(sf[1]);
-        st->fail = su[0];
-        st->grid_nonfinite = si[0];
-        st->escaped = si[1];
-        st->nearwall = *nw_count;
+        st.projection = tot[6];
+        st.friction = tot[7];
+        st.traction_energy = tot[8];
+        st.dissipation = tot[kGridVals];
+        st.max_speed = sf[0];
+        st.max_disp = sf[1];
+        st.fail = su[0];
+        st.grid_nonfinite = si[0];
+        st.escaped = si[1];
+        int pb = 0;
+        for (int k = 0; k < ip.npel; ++k) pb |= B.ptrial[ip.pel_off + k].bad;
+        st.pellet_bad = pb;
     }
 }
 
-// Sort keys every 32 accepted steps (solver.rs:501-517).  The device order
-// groups cells in 4x4x4-cell blocks (compact workgroup chunks); the reference's
-// order key (clamped cell, z fastest, then id) is kept per particle so that the
-// state is handed back in the reference's order.
-__global__ void sort_keys_kernel(Params P, int n, const float* __restrict__ slab, size_t cap,
-                                 const uint32_t* __restrict__ id, uint64_t* keys, uint32_t* order, uint64_t* refkey,
-                                 int keep_ref) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+// Accept or reject every running instance's trial (solver.rs:444-467, 593-611,
+// 980-998): failures in the order the reference checks them, then the commit and
+// the ledger sums in the reference's order.
+__global__ void commit_kernel(BatchDev B) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= B.ninst) return;
+    InstState& s = B.is[inst];
+    if (!s.active) return;
+    const InstParams& ip = B.ip[inst];
+    const StepStatus& st = s.st;
+    double* L = s.ledger;
+    int code = 0;
+    if (!isfinite(st.traction_energy) || !isfinite(L[23] + st.traction_energy))
+        code = F_TRACTION;
+    else if (st.grid_nonfinite)
+        code = F_GRID;
+    else if (st.pellet_bad)
+        code = F_PELLET;
+    else if (st.fail != ~0ull)
+        code = (int)(st.fail & 0xff);
+    else if ((double)st.max_disp > 1.0)
+        code = F_DISPLACEMENT;
+    s.last_failure = code;
+    const bool trial_only = s.trial_dt > 0.0;
+    if (code) {
+        if (trial_only) {
+            s.trial_dt = 0.0;
+            s.mode = M_DONE;
+            return;
+        }
+        s.rejected_steps += 1;
+        s.dt_scale *= 0.5;
+        if (s.dt_scale * s.limit < ip.min_dt) {
+            s.mode = M_ERROR;
+            s.err = E_MIN_AFTER_REJECT;
+        }
+        return;
+    }
+    const double dt = s.dt;
+    s.cur ^= 1;
+    s.gcur ^= 1;
+    s.committed = 1;
+    // commit, solver.rs:992-997
+    L[27] += st.dissipation;
+    for (int a = 0; a < 3; ++a) L[9 + a] += st.traction_impulse[a] + st.wall_impulse[a];
+    L[21] += st.projection;
+    L[22] += st.friction;
+    L[23] += st.traction_energy;
+    for (int a = 0; a < 3; ++a) L[6 + a] += ip.gravity[a] * (ip.particle_mass * dt);
+    // pellets, solver.rs:600-609
+    for (int k = 0; k < ip.npel; ++k) {
+        const int pi = ip.pel_off + k;
+        PelletState& p = B.pel[pi];
+        const PelletTrial& t = B.ptrial[pi];
+        const PelletAccum& a = B.pacc[pi];
+        for (int c = 0; c < 3; ++c) {
+            L[6 + c] += ip.gravity[c] * (p.mass * dt);
+            L[12 + c] += t.wall_impulse[c];
+            L[15 + c] += a.impulse[c];
+            L[18 + c] += a.angular[c];
+        }
+        L[24] += t.wall_work;
+        L[25] += a.grid_energy;
+        L[26] += t.energy;
+        for (int c = 0; c < 3; ++c) {
+            p.pos[c] = t.pos[c];
+            p.vel[c] = t.vel[c];
+            p.L[c] = t.L[c];
+        }
+        for (int c = 0; c < 4; ++c) p.q[c] = t.q[c];
+    }
+    s.max_speed = (double)st.max_speed;
+    if (st.escaped) {
+        s.mode = M_PAUSE;
+        s.pause |= P_ESCAPED;
+    }
+    if (trial_only) {
+        s.trial_dt = 0.0;
+        if (s.mode == M_RUN) s.mode = M_DONE;
+        return;
+    }
+    // advance_while bookkeeping, solver.rs:446-454
+    s.time += dt;
+    s.step_count += 1;
+    s.limited_steps += s.limited ? 1 : 0;
+    s.min_dt_used = fmin(s.min_dt_used, dt);
+    s.max_dt_used = fmax(s.max_dt_used, dt);
+    s.dt_scale = fmin(s.dt_scale * kDtRecovery, 1.0);
+    s.budget -= 1;
+    if (s.step_count % kSortInterval == 0) s.refsort = 1;
+}
+
+// Reference order key (clamped cell index, z fastest, then id; solver.rs:501-517).
+__device__ inline uint64_t ref_cell(const Params& P, const float* slab, size_t cap, int p) {
     uint64_t c[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -1177,28 +1102,102 @@ __global__ void sort_keys_kernel(Params P, int n, const float* __restrict__ slab
         sc = isnan(sc) ? 0.0f : fmaxf(sc, 0.0f);
         c[a] = sc >= (float)P.cells[a] ? (uint64_t)P.cells[a] : (uint64_t)sc;
     }
-    uint64_t ref = (c[0] * (uint64_t)(P.cells[1] + 1) + c[1]) * (uint64_t)(P.cells[2] + 1) + c[2];
-    uint64_t nb1 = (uint64_t)(P.cells[1] + 4) / 4, nb2 = (uint64_t)(P.cells[2] + 4) / 4;
-    uint64_t blk = ((c[0] >> 2) * nb1 + (c[1] >> 2)) * nb2 + (c[2] >> 2);
-    uint64_t within = ((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3);
-    keys[p] = ((blk * 64 + within) << 32) | (uint64_t)id[p];
-    order[p] = (uint32_t)p;
-    if (!keep_ref) refkey[p] = (ref << 32) | (uint64_t)id[p];
+    return (c[0] * (uint64_t)(P.cells[1] + 1) + c[1]) * (uint64_t)(P.cells[2] + 1) + c[2];
 }
 
-__global__ void block_ids_kernel(int n, const uint64_t* __restrict__ keys, uint32_t* out) {
+// Order keys at the 32-step cadence, for the instances whose accepted step hit it.
+__global__ __launch_bounds__(kBlock) void refkey_kernel(BatchDev B) {
+    const int4 item = B.work[blockIdx.x];
+    const InstState& s = B.is[item.x];
+    if (!s.refsort) return;
+    const Params& P = B.ip[item.x].P;
+    const float* slab = B.slab[s.cur];
+    for (int p = item.y + threadIdx.x; p < item.y + item.z; p += kBlock)
+        B.refkey[p] = (ref_cell(P, slab, B.cap, p) << 32) | (uint64_t)B.ids[p];
+}
+
+// ------------------------------------------------------------ setup kernels
+
+__global__ __launch_bounds__(kBlock) void p2g_kernel(Params P, int n, const float* __restrict__ slab, size_t cap,
+                                                     const float* __restrict__ shared, GridPtrs g) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) out[p] = (uint32_t)(keys[p] >> 38);
+    if (p >= n) return;
+    float x = slab[PX * cap + p], y = slab[PY * cap + p], z = slab[PZ * cap + p];
+    Stencil s;
+    if (!make_stencil(P, x, y, z, s)) return;
+    float C[9], tau[6];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C[i] = slab[(C00 + i) * cap + p];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) tau[i] = slab[(TXX + i) * cap + p];
+    scatter_particle(P, g, s, shared[SMASS * cap + p], slab[VX * cap + p], slab[VY * cap + p], slab[VZ * cap + p],
+                     C, tau, shared[SVOL0 * cap + p]);
 }
 
-__global__ void permute_kernel(int n, const uint32_t* __restrict__ order, const float* __restrict__ src,
-                               float* __restrict__ dst, size_t cap, const float* __restrict__ ssrc,
+// Near-wall lists of the committed slabs, one workgroup per work item.
+__global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
+    const int4 item = B.work[blockIdx.x];
+    InstState& s = B.is[item.x];
+    const InstParams& ip = B.ip[item.x];
+    const float* slab = B.slab[s.cur];
+    int* list = B.nw_list[s.cur] + ip.poff;
+    int* count = &s.nw_count[s.cur];
+    // Every lane of a wavefront takes part in the aggregated append.
+    for (int base = item.y; base < item.y + item.z; base += kBlock) {
+        int p = base + threadIdx.x;
+        bool nw = p < item.y + item.z &&
+                  near_wall(ip.P, slab[PX * B.cap + p], slab[PY * B.cap + p], slab[PZ * B.cap + p]);
+        wave_append(nw, p, list, count);
+    }
+}
+
+// Device order: 4x4x4-cell blocks (compact workgroup tiles), instances in
+// range order; entries past an instance's particle count go to the end of its
+// range.  Radix sort is stable, so equal keys keep their order.
+__global__ void sort_keys_kernel(BatchDev B, int cell_bits, uint64_t* keys, uint32_t* order) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int)B.cap) return;
+    const int inst = instance_of(B, p);
+    const InstState& s = B.is[inst];
+    const Params& P = B.ip[inst].P;
+    uint64_t key = ((uint64_t)1 << cell_bits) - 1;
+    if (p - B.poffs[inst] < s.n) {
+        const float* slab = B.slab[s.cur];
+        uint64_t c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float sc = floorf(slab[(PX + a) * B.cap + p] / P.dx);
+            sc = isnan(sc) ? 0.0f : fmaxf(sc, 0.0f);
+            c[a] = sc >= (float)P.cells[a] ? (uint64_t)P.cells[a] : (uint64_t)sc;
+        }
+        uint64_t nb1 = (uint64_t)(P.cells[1] + 4) / 4, nb2 = (uint64_t)(P.cells[2] + 4) / 4;
+        uint64_t blk = ((c[0] >> 2) * nb1 + (c[1] >> 2)) * nb2 + (c[2] >> 2);
+        uint64_t within = ((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3);
+        key = blk * 64 + within;
+    }
+    keys[p] = ((uint64_t)inst << cell_bits) | key;
+    order[p] = (uint32_t)p;
+}
+
+__global__ void block_ids_kernel(int n, const uint64_t* __restrict__ keys, uint64_t* out) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = keys[p] >> 6;
+}
+
+// Gather every slab entry into sorted order: from each instance's committed
+// slab into its other slab (the host then flips the instance's selector), and
+// the shared slab pair, ids and order keys likewise.
+__global__ void permute_kernel(BatchDev B, const uint32_t* __restrict__ order, const float* __restrict__ ssrc,
                                float* __restrict__ sdst, const uint32_t* __restrict__ isrc,
                                uint32_t* __restrict__ idst, const uint64_t* __restrict__ rsrc,
                                uint64_t* __restrict__ rdst) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    uint32_t o = order[k];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int)B.cap) return;
+    const size_t cap = B.cap;
+    const int cur = B.is[instance_of(B, k)].cur;
+    const float* src = B.slab[cur];
+    float* dst = B.slab[cur ^ 1];
+    const uint32_t o = order[k];
     for (int f = 0; f < NPF; ++f) dst[f * cap + k] = src[f * cap + o];
     for (int f = 0; f < NSF; ++f) sdst[f * cap + k] = ssrc[f * cap + o];
     idst[k] = isrc[o];
@@ -1232,80 +1231,55 @@ __global__ void liquid_batch_kernel(Params P, float dt, int n, const float* jm1,
     for (int i = 0; i < 6; ++i) tau[6 * p + i] = rc ? 0.0f : t[i];
 }
 
-inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
-
-constexpr int kChunk = 1024;
-
-constexpr int kGatherCap = 768;
-constexpr int kScatterCap = 1024;
+inline int blocks_for(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
 
 int g2p2g_chunk() { return kChunk; }
-int grid_workgroups(int nblocks) { return (nblocks + 3) / 4; }
 
-void launch_reset_status(hipStream_t s, StepStatus* st, int* nw_count) {
-    reset_status_kernel<<<1, 64, 0, s>>>(st, nw_count);
+void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, hipEvent_t g2p2g_start,
+                 hipEvent_t g2p2g_stop) {
+    control_kernel<<<1, 256, 0, s>>>(B);
+    traction_kernel<<<traction_wgs, kBlock, 0, s>>>(B);
+    grid_update_kernel<<<B.ngwg, 256, 0, s>>>(B);
+    if (pellets) couple_kernel<<<B.ngwg * 4, 64, 0, s>>>(B);
+    if (g2p2g_start) (void)hipEventRecord(g2p2g_start, s);
+    if (B.nwork > 0) g2p2g_kernel<kChunk, kGatherCap, kScatterCap><<<B.nwork, kBlock, 0, s>>>(B);
+    if (g2p2g_stop) (void)hipEventRecord(g2p2g_stop, s);
+    if (pellets) pellet_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
+    finalize_kernel<<<B.ninst, 256, 0, s>>>(B);
+    commit_kernel<<<(B.ninst + 63) / 64, 64, 0, s>>>(B);
+    if (B.nwork > 0) refkey_kernel<<<B.nwork, kBlock, 0, s>>>(B);
 }
 
-void launch_p2g(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const float* shared,
-                GridPtrs g) {
-    if (n > 0) p2g_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, slab, cap, shared, g);
-}
-
-void launch_nearwall(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, int* list, int* count) {
-    if (n > 0) nearwall_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, slab, cap, list, count);
-}
-
-void launch_traction(hipStream_t s, const Params& P, float dt, const int* list, const int* count, int max_count,
-                     const float* slab, size_t cap, const float* shared, GridPtrs g, float* trS, float* trT,
-                     size_t na) {
-    if (max_count > 0)
-        traction_kernel<<<blocks_for(max_count), kBlock, 0, s>>>(P, dt, list, count, slab, cap, shared, g, trS, trT,
-                                                                 na);
-}
-
-void launch_grid_update(hipStream_t s, const Params& P, float dt, int nblocks, GridPtrs a, float* trS, float* trT,
-                        float* gv, size_t na, GridPtrs clear, double* partials, int* bad) {
-    grid_update_kernel<<<grid_workgroups(nblocks), 256, 0, s>>>(P, dt, a, trS, trT, gv, na, clear, nblocks, partials,
-                                                                bad);
-}
-
-void launch_couple(hipStream_t s, const Params& P, double dx, int nblocks, GridPtrs a, float* gv, size_t na,
-                   const PelletDev* pellets, int npellets, const int* bstart, const int* blist, PelletAccum* acc,
-                   double friction) {
-    if (npellets > 0) couple_kernel<<<nblocks, 64, 0, s>>>(P, dx, a, gv, na, pellets, bstart, blist, acc, friction);
-}
-
-void launch_g2p2g(hipStream_t s, const Params& P, float dt, int n, const float* src, float* dst, size_t cap,
-                  const float* shared, const float* gv, size_t na, GridPtrs out, const int2* work, int nwork,
-                  float* pmax, double* pdiss, unsigned long long* pfail, int* pesc, int* nw_list, int* nw_count,
-                  hipEvent_t start, hipEvent_t stop) {
-    if (start) (void)hipEventRecord(start, s);
-    if (n > 0 && nwork > 0)
-        g2p2g_kernel<kChunk, kGatherCap, kScatterCap><<<nwork, kBlock, 0, s>>>(
-            P, dt, n, src, dst, cap, shared, gv, na, out, work, pmax, pdiss, pfail, pesc, nw_list, nw_count);
-    if (stop) (void)hipEventRecord(stop, s);
-}
-
-size_t rle_temp_bytes(int n) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                (uint32_t*)nullptr, (int*)nullptr, n);
-    return bytes;
-}
-
-void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
-                       uint32_t* ids_tmp, uint32_t* unique, uint32_t* counts, int* nruns) {
+void launch_p2g(hipStream_t s, const BatchDev& B, const Params& P, int poff, int n, int boff, int cur, int gcur) {
     if (n <= 0) return;
-    block_ids_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, sorted_keys, ids_tmp);
-    (void)hipcub::DeviceRunLengthEncode::Encode(temp, temp_bytes, ids_tmp, unique, counts, nruns, n, s);
+    GridPtrs g = grid_set(B, gcur);
+    const size_t noff = (size_t)boff * 64;
+    g.m += noff;
+    g.px += noff;
+    g.py += noff;
+    g.pz += noff;
+    g.fx += noff;
+    g.fy += noff;
+    g.fz += noff;
+    g.flag += boff;
+    p2g_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, B.slab[cur] + poff, B.cap, B.shared[B.sh] + poff, g);
 }
 
-void launch_finalize(hipStream_t s, int ngrid, const double* gpart, const int* gbad, int npart, const float* pmax,
-                     const double* pdiss, const unsigned long long* pfail, const int* pesc, const int* nw_count,
-                     StepStatus* st) {
-    finalize_kernel<<<1, 256, 0, s>>>(ngrid, gpart, gbad, npart, pmax, pdiss, pfail, pesc, nw_count, st);
+void launch_nearwall(hipStream_t s, const BatchDev& B) {
+    if (B.nwork > 0) nearwall_kernel<<<B.nwork, kBlock, 0, s>>>(B);
+}
+
+void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t* keys, uint32_t* order) {
+    sort_keys_kernel<<<blocks_for(B.cap), kBlock, 0, s>>>(B, cell_bits, keys, order);
+}
+
+void launch_permute(hipStream_t s, const BatchDev& B, const uint32_t* order, const float* shared_src,
+                    float* shared_dst, const uint32_t* id_src, uint32_t* id_dst, const uint64_t* ref_src,
+                    uint64_t* ref_dst) {
+    permute_kernel<<<blocks_for(B.cap), kBlock, 0, s>>>(B, order, shared_src, shared_dst, id_src, id_dst, ref_src,
+                                                        ref_dst);
 }
 
 size_t sort_temp_bytes(int n, int end_bit) {
@@ -1315,22 +1289,24 @@ size_t sort_temp_bytes(int n, int end_bit) {
     return bytes;
 }
 
+size_t rle_temp_bytes(int n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                (uint32_t*)nullptr, (int*)nullptr, n);
+    return bytes;
+}
+
 void launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
                        const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit) {
     (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, end_bit,
                                              s);
 }
 
-void launch_sort_keys(hipStream_t s, const Params& P, int n, const float* slab, size_t cap, const uint32_t* id,
-                      uint64_t* keys, uint32_t* order, uint64_t* refkey, int keep_ref) {
-    if (n > 0) sort_keys_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, slab, cap, id, keys, order, refkey, keep_ref);
-}
-
-void launch_permute(hipStream_t s, int n, const uint32_t* order, const float* src, float* dst, size_t cap,
-                    const float* ssrc, float* sdst, const uint32_t* isrc, uint32_t* idst, const uint64_t* rsrc,
-                    uint64_t* rdst) {
-    if (n > 0)
-        permute_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, order, src, dst, cap, ssrc, sdst, isrc, idst, rsrc, rdst);
+void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
+                       uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns) {
+    if (n <= 0) return;
+    block_ids_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, sorted_keys, ids_tmp);
+    (void)hipcub::DeviceRunLengthEncode::Encode(temp, temp_bytes, ids_tmp, unique, counts, nruns, n, s);
 }
 
 void launch_paste_batch(hipStream_t s, const Params& P, float dt, int n, const float* h_in, float* h_out, float* tau,

@@ -1,0 +1,493 @@
+// Device-side physics of the MLS-MPM substep, shared by the kernels:
+// stencils (grid.rs:151-183), the constitutive updates (constitutive.rs:280-589),
+// the wall velocity law (solver.rs:1145-1155) and the gather/scatter helpers
+// of the fused grid-to-particle-to-grid kernel.  f32 throughout, f64 where a
+// reduction or LDS accumulation needs it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "engine_internal.h"
+
+namespace mpm {
+
+constexpr int kBlock = 256;
+
+struct Stencil {
+    int base[3];
+    float w[3][3];
+    float d[3][3];  // node minus particle in cells
+};
+
+// grid.rs:151-183 in f32: base = floor(x/dx - 1/2) + 1 (padded), quadratic B-spline weights.
+__device__ inline bool make_stencil(const Params& P, float x, float y, float z, Stencil& s) {
+    float xs[3] = {x * P.inv_dx, y * P.inv_dx, z * P.inv_dx};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float bf = floorf(xs[a] - 0.5f);
+        float padded = bf + 1.0f;
+        if (!(padded >= 0.0f) || padded + 2.0f >= (float)P.nodes[a]) return false;
+        float fx = xs[a] - bf;
+        float t0 = 1.5f - fx, t1 = fx - 1.0f, t2 = fx - 0.5f;
+        s.w[a][0] = 0.5f * (t0 * t0);
+        s.w[a][1] = 0.75f - t1 * t1;
+        s.w[a][2] = 0.5f * (t2 * t2);
+        s.base[a] = (int)padded;
+        s.d[a][0] = -fx;
+        s.d[a][1] = 1.0f - fx;
+        s.d[a][2] = 2.0f - fx;
+    }
+    return true;
+}
+
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
+
+// ------------------------------------------------------------ scatter
+
+// deposit_particles (solver.rs:632-661) for one particle: mass w m, momentum
+// w (m v + m C d) and stress w V0 tau d, d = x_node - x_p in metres.
+__device__ inline void scatter_particle(const Params& P, GridPtrs g, const Stencil& s, float m, float vx, float vy,
+                                        float vz, const float* C, const float* tau, float v0) {
+    // Precompute m C dx and V0 tau dx (d in cells times dx is metres).
+    float mc[9], vt[6];
+    float mdx = m * P.dx, vdx = v0 * P.dx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
+    float mvx = m * vx, mvy = m * vy, mvz = m * vz;
+    // Block flags of the (up to 8) node blocks the stencil touches.
+    int b0[3], b1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b0[a] = s.base[a] >> 2;
+        b1[a] = (s.base[a] + 2) >> 2;
+    }
+    for (int bi = b0[0]; bi <= b1[0]; ++bi)
+        for (int bj = b0[1]; bj <= b1[1]; ++bj)
+            for (int bk = b0[2]; bk <= b1[2]; ++bk) g.flag[(bi * P.nb[1] + bj) * P.nb[2] + bk] = 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            float wab = s.w[0][a] * s.w[1][b];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float w = wab * s.w[2][c];
+                float dx = s.d[0][a], dy = s.d[1][b], dz = s.d[2][c];
+                int n = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
+                float px = w * (mvx + mc[0] * dx + mc[1] * dy + mc[2] * dz);
+                float py = w * (mvy + mc[3] * dx + mc[4] * dy + mc[5] * dz);
+                float pz = w * (mvz + mc[6] * dx + mc[7] * dy + mc[8] * dz);
+                float fx = w * (vt[0] * dx + vt[3] * dy + vt[4] * dz);
+                float fy = w * (vt[3] * dx + vt[1] * dy + vt[5] * dz);
+                float fz = w * (vt[4] * dx + vt[5] * dy + vt[2] * dz);
+                atomicAdd(g.m + n, w * m);
+                atomicAdd(g.px + n, px);
+                atomicAdd(g.py + n, py);
+                atomicAdd(g.pz + n, pz);
+                atomicAdd(g.fx + n, fx);
+                atomicAdd(g.fy + n, fy);
+                atomicAdd(g.fz + n, fz);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ constitutive
+
+__device__ inline float det3(const float* f) {
+    float m23 = f[4] * f[8] - f[7] * f[5];
+    float m13 = f[3] * f[8] - f[6] * f[5];
+    float m12 = f[3] * f[7] - f[6] * f[4];
+    return f[0] * m23 - f[1] * m13 + f[2] * m12;
+}
+
+// Cyclic Jacobi on a symmetric 3x3 (constitutive.rs:373-425), pairs (0,1),(0,2),(1,2).
+// Stops when the off-diagonal norm falls below 1e-12 of the Frobenius norm (the
+// f64 reference's 1e-30 is below f32 resolution).
+__device__ inline void jacobi_rotate(float* a, float* v, int p, int q) {
+    float apq = a[3 * p + q];
+    if (apq == 0.0f) return;
+    float theta = (a[3 * q + q] - a[3 * p + p]) / (2.0f * apq);
+    float t = fabsf(theta) > 1e15f ? 0.5f / theta
+                                   : copysignf(1.0f, theta) / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+    float c = 1.0f / sqrtf(t * t + 1.0f);
+    float s = t * c;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float akp = a[3 * k + p], akq = a[3 * k + q];
+        a[3 * k + p] = c * akp - s * akq;
+        a[3 * k + q] = s * akp + c * akq;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float apk = a[3 * p + k], aqk = a[3 * q + k];
+        a[3 * p + k] = c * apk - s * aqk;
+        a[3 * q + k] = s * apk + c * aqk;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float vkp = v[3 * k + p], vkq = v[3 * k + q];
+        v[3 * k + p] = c * vkp - s * vkq;
+        v[3 * k + q] = s * vkp + c * vkq;
+    }
+}
+
+__device__ inline int eigen_sym3(float* a, float* v) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+    float scale = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) scale += a[i] * a[i];
+    if (!isfinite(scale)) return F_EIGEN;
+    scale = fmaxf(scale, FLT_MIN);
+    for (int sweep = 0; sweep < 24; ++sweep) {
+        float off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
+        if (off <= 1e-12f * scale) return F_OK;
+        jacobi_rotate(a, v, 0, 1);
+        jacobi_rotate(a, v, 0, 2);
+        jacobi_rotate(a, v, 1, 2);
+    }
+    float off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
+    return off <= 1e-8f * scale ? F_OK : F_EIGEN;
+}
+
+// Herschel-Bulkley implicit return (constitutive.rs:280-362), solved for the
+// overstress d = q - sigma_y on [0, q_tr - sigma_y] with the same bracketed Newton;
+// tolerance 2.5e-7 of the trial stress (a few f32 ulps).
+__device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, float& inc) {
+    float G3 = 3.0f * P.G;
+    if (!isfinite(qtr)) return F_NONFINITE;
+    if (qtr <= P.ys) {
+        q = qtr;
+        inc = 0.0f;
+        return F_OK;
+    }
+    float avail = qtr - P.ys;
+    if (P.K <= 0.0f) {
+        q = P.ys;
+        inc = avail / G3;
+        return F_OK;
+    }
+    float vs = G3 * dt;
+    float inv_k = 1.0f / P.K;
+    float lo = 0.0f, hi = avail, d = 0.5f * avail;
+    float tol = 2.5e-7f * fmaxf(qtr, 1.0f);
+    float r = d - avail + vs * powf(d * inv_k, P.inv_n);
+    for (int it = 0; it < 100; ++it) {
+        if (fabsf(r) <= tol || (hi - lo) <= tol) {
+            q = P.ys + d;
+            inc = (avail - d) / G3;
+            return F_OK;
+        }
+        if (r > 0.0f)
+            hi = d;
+        else
+            lo = d;
+        float over = d * inv_k;
+        float deriv = over > 0.0f ? 1.0f + vs * P.inv_n * powf(over, P.inv_n - 1.0f) * inv_k : INFINITY;
+        float nd = d - r / deriv;
+        d = (isfinite(nd) && nd > lo && nd < hi) ? nd : 0.5f * (lo + hi);
+        r = d - avail + vs * powf(d * inv_k, P.inv_n);
+        if (!isfinite(r)) return F_NONFINITE;
+    }
+    return F_RETURN;
+}
+
+// update_paste (constitutive.rs:456-531) on H = F - I:
+//   b - I = H + H^T + H H^T, Hencky e = log1p(beta)/2, radial return on dev(e),
+//   F' = F + U diag(expm1((alpha - 1) dev e)) U^T F, tau = U diag(2G e' + lambda tr e) U^T.
+__device__ inline int paste_update(const Params& P, float dt, const float* Ht, float* Hn, float* tau, float& inc,
+                                   float& dd) {
+    float F[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = Ht[i] + ((i % 4 == 0) ? 1.0f : 0.0f);
+    float det = det3(F);
+    if (!isfinite(det)) return F_NONFINITE;
+    if (det <= 0.0f) return F_INVERTED;
+    float B[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            B[3 * i + j] = Ht[3 * i + j] + Ht[3 * j + i] + Ht[3 * i] * Ht[3 * j] + Ht[3 * i + 1] * Ht[3 * j + 1] +
+                           Ht[3 * i + 2] * Ht[3 * j + 2];
+    float U[9];
+    int rc = eigen_sym3(B, U);
+    if (rc) return rc;
+    float beta[3] = {B[0], B[4], B[8]};
+    float e[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float s = sqrtf(fmaxf(1.0f + beta[a], 0.0f));
+        if (!isfinite(s) || s < 1e-6f) return F_STRETCH;
+        e[a] = 0.5f * log1pf(beta[a]);
+    }
+    float vol = e[0] + e[1] + e[2];
+    float mean = vol / 3.0f;
+    float dev[3] = {e[0] - mean, e[1] - mean, e[2] - mean};
+    float dn = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+    float qtr = 1.2247448713915890f * 2.0f * P.G * dn;
+    float alpha = 1.0f, q = qtr;
+    inc = 0.0f;
+    if (qtr > P.ys && dn > 0.0f) {
+        rc = hb_return(P, dt, qtr, q, inc);
+        if (rc) return rc;
+        alpha = q / qtr;
+    }
+    float rm1[3], pr[3];
+    float lv = P.lam * vol;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        rm1[a] = expm1f((alpha - 1.0f) * dev[a]);
+        pr[a] = 2.0f * P.G * (alpha * dev[a] + mean) + lv;
+    }
+    // M = U diag(rm1) U^T, H' = H + M (I + H)
+    float M[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            M[3 * i + j] = U[3 * i] * rm1[0] * U[3 * j] + U[3 * i + 1] * rm1[1] * U[3 * j + 1] +
+                           U[3 * i + 2] * rm1[2] * U[3 * j + 2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Hn[3 * i + j] = Ht[3 * i + j] + M[3 * i + j] + M[3 * i] * Ht[j] + M[3 * i + 1] * Ht[3 + j] +
+                            M[3 * i + 2] * Ht[6 + j];
+    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        int i = ii[t], j = jj[t];
+        tau[t] = U[3 * i] * pr[0] * U[3 * j] + U[3 * i + 1] * pr[1] * U[3 * j + 1] + U[3 * i + 2] * pr[2] * U[3 * j + 2];
+        if (!isfinite(tau[t])) return F_NONFINITE;
+    }
+    dd = q * inc;
+    return F_OK;
+}
+
+// update_liquid (constitutive.rs:557-589) on J - 1, with the Tait pressure
+// p = (K/g) expm1(-g log1p(J - 1)); g = 1 gives the reference's K (1/J - 1).
+__device__ inline int liquid_update(const Params& P, float dt, float jm1, const float* C, float& jn, float* tau) {
+    float tr = C[0] + C[4] + C[8];
+    float g = dt * tr;
+    jn = jm1 + g + g * jm1;
+    float J = 1.0f + jn;
+    if (!isfinite(jn)) return F_NONFINITE;
+    if (J <= 1e-6f) return F_INVERTED;
+    float p = P.gamma == 1.0f ? -P.bulk * jn / J : (P.bulk / P.gamma) * expm1f(-P.gamma * log1pf(jn));
+    float c2 = 2.0f * P.visc * J;
+    float m3 = tr / 3.0f;
+    float pj = -p * J;
+    tau[0] = pj + c2 * (C[0] - m3);
+    tau[1] = pj + c2 * (C[4] - m3);
+    tau[2] = pj + c2 * (C[8] - m3);
+    tau[3] = c2 * 0.5f * (C[1] + C[3]);
+    tau[4] = c2 * 0.5f * (C[2] + C[6]);
+    tau[5] = c2 * 0.5f * (C[5] + C[7]);
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+        if (!isfinite(tau[t])) return F_NONFINITE;
+    return F_OK;
+}
+
+// Within half a cell of a face: the only particles the wall traction can touch.
+__device__ inline bool near_wall(const Params& P, float x, float y, float z) {
+    float half = 0.5f * P.dx;
+    return x < half || y < half || z < half || x > P.extent[0] - half || y > P.extent[1] - half ||
+           z > P.extent[2] - half;
+}
+
+__device__ inline void reduce_tangential(float* v, float reduction) {
+    if (reduction == 0.0f) return;
+    float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (speed <= reduction) {
+        v[0] = v[1] = v[2] = 0.0f;
+    } else {
+        float f = (speed - reduction) / speed;
+        v[0] *= f;
+        v[1] *= f;
+        v[2] *= f;
+    }
+}
+
+// Block-wide sum of NV doubles held by every thread; thread 0 gets the totals.
+template <int NV>
+__device__ inline void block_sum(double* vals, double* scratch) {
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) vals[i] = wave_sum(vals[i]);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) scratch[wave * NV + i] = vals[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int waves = blockDim.x >> 6;
+        for (int i = 0; i < NV; ++i) {
+            double t = 0.0;
+            for (int w = 0; w < waves; ++w) t += scratch[w * NV + i];
+            vals[i] = t;
+        }
+    }
+}
+
+// ------------------------------------------------------------ fused G2P2G
+
+// Tile geometry of one workgroup's chunk of particles.  Gather tile: the nodes
+// of every current stencil; scatter tile: one cell wider on each side, since an
+// accepted step moves a particle by at most one cell.
+struct Tile {
+    int org[3];  // gather origin (lowest stencil base)
+    int gd[3];   // gather dims
+    int sd[3];   // scatter dims (origin org - 1)
+    int tiled;
+};
+
+__device__ inline void gather_global(const Params& P, const float* __restrict__ gv, size_t na, const Stencil& s,
+                                     float* v, float* B) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            float wab = s.w[0][a] * s.w[1][b];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float w = wab * s.w[2][c];
+                int ni = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
+                float nv[3] = {gv[ni] * w, gv[na + ni] * w, gv[2 * na + ni] * w};
+                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    v[i] += nv[i];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
+                }
+            }
+        }
+}
+
+__device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil& s, float* v, float* B) {
+    int li = s.base[0] - t.org[0], lj = s.base[1] - t.org[1], lk = s.base[2] - t.org[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            float wab = s.w[0][a] * s.w[1][b];
+            int row = ((li + a) * t.gd[1] + (lj + b)) * t.gd[2] + lk;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float w = wab * s.w[2][c];
+                float4 g = sg[row + c];
+                float nv[3] = {g.x * w, g.y * w, g.z * w};
+                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    v[i] += nv[i];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
+                }
+            }
+        }
+}
+
+// Select among three register values by a runtime index (no array indexing, which
+// would place the stencil in scratch memory).
+__device__ inline float sel3(int i, float x0, float x1, float x2) { return i == 0 ? x0 : (i == 1 ? x1 : x2); }
+
+// deposit_particles for one particle into the f64 LDS accumulator tile (LDS
+// f64 atomic adds run at close to full LDS rate on CDNA4; f32 ones do not).
+// Lanes walk the nine (a, b) stencil columns starting at a lane-dependent
+// column so that particles of one cell, which share most nodes, hit different
+// LDS words.
+template <int SCAP>
+__device__ inline void scatter_lds(double* ss, const int* so, const int* sd, const Params& P, const Stencil& s,
+                                   float m, const float* v, const float* C, const float* tau, float v0, int rot) {
+    float mc[9], vt[6];
+    float mdx = m * P.dx, vdx = v0 * P.dx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
+    float mv0 = m * v[0], mv1 = m * v[1], mv2 = m * v[2];
+    int li = s.base[0] - so[0], lj = s.base[1] - so[1], lk = s.base[2] - so[2];
+    for (int kk = 0; kk < 9; ++kk) {
+        int ab = kk + rot;
+        ab = ab >= 9 ? ab - 9 : ab;
+        int a = ab >= 6 ? 2 : (ab >= 3 ? 1 : 0);
+        int b = ab - 3 * a;
+        float wab = sel3(a, s.w[0][0], s.w[0][1], s.w[0][2]) * sel3(b, s.w[1][0], s.w[1][1], s.w[1][2]);
+        float dx = sel3(a, s.d[0][0], s.d[0][1], s.d[0][2]), dy = sel3(b, s.d[1][0], s.d[1][1], s.d[1][2]);
+        int row = ((li + a) * sd[1] + (lj + b)) * sd[2] + lk;
+        float px0 = mv0 + mc[0] * dx + mc[1] * dy, py0 = mv1 + mc[3] * dx + mc[4] * dy,
+              pz0 = mv2 + mc[6] * dx + mc[7] * dy;
+        float fx0 = vt[0] * dx + vt[3] * dy, fy0 = vt[3] * dx + vt[1] * dy, fz0 = vt[4] * dx + vt[5] * dy;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float w = wab * s.w[2][c];
+            float dz = s.d[2][c];
+            int idx = row + c;
+            atomicAdd(ss + 0 * SCAP + idx, (double)(w * m));
+            atomicAdd(ss + 1 * SCAP + idx, (double)(w * (px0 + mc[2] * dz)));
+            atomicAdd(ss + 2 * SCAP + idx, (double)(w * (py0 + mc[5] * dz)));
+            atomicAdd(ss + 3 * SCAP + idx, (double)(w * (pz0 + mc[8] * dz)));
+            atomicAdd(ss + 4 * SCAP + idx, (double)(w * (fx0 + vt[4] * dz)));
+            atomicAdd(ss + 5 * SCAP + idx, (double)(w * (fy0 + vt[5] * dz)));
+            atomicAdd(ss + 6 * SCAP + idx, (double)(w * (fz0 + vt[2] * dz)));
+        }
+    }
+}
+
+__device__ inline int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Wave-aggregated append of this lane's index when `pred` holds.
+__device__ inline void wave_append(bool pred, int value, int* list, int* count) {
+    unsigned long long mask = __ballot(pred);
+    if (!mask) return;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+}  // namespace mpm
