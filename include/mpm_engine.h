@@ -277,6 +277,56 @@ int mpm_set_launch(mpm_engine* engine, const mpm_launch* launch);
  * NULL).  The state, ledgers and histories are unchanged afterwards. */
 int mpm_autotune(mpm_engine* engine, int steps, mpm_launch* chosen);
 
+/* ---- Learned surrogate of the Herschel-Bulkley return map.  No counterpart
+ * in the reference; the exact return (solve_herschel_bulkley_return,
+ * constitutive.rs:280-362) stays the fallback and the judge of every accepted
+ * prediction.  The network is 12 -> 64 -> 64 -> 64 -> 7 with SiLU, bf16
+ * weights on MFMA with f32 accumulation; its f32 master parameters are, in
+ * order, W1[12][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][7] b4[7]
+ * (W[k][n] maps input k to output n).  Inputs and outputs are the encoding
+ * documented with the log below. */
+typedef struct mpm_mlp mpm_mlp;
+size_t mpm_mlp_param_count(void);
+/* params NULL: Xavier-uniform weights from SplitMix64(seed), zero biases. */
+int mpm_mlp_create(const float* params, uint64_t seed, mpm_mlp** out);
+void mpm_mlp_destroy(mpm_mlp* mlp);
+const char* mpm_mlp_last_error(const mpm_mlp* mlp);
+int mpm_mlp_get_params(mpm_mlp* mlp, float* out);
+int mpm_mlp_set_params(mpm_mlp* mlp, const float* in);
+/* y (n x 7) = network(x (n x 12)), on the device. */
+int mpm_mlp_forward(mpm_mlp* mlp, size_t n, const float* x, float* y);
+/* Loss sum_i sum_u w_u (y_iu - t_iu)^2 / n (w NULL: all ones) and its gradient
+ * with respect to the master parameters, from the MFMA backward pass. */
+int mpm_mlp_loss_grad(mpm_mlp* mlp, size_t n, const float* x, const float* t,
+                      const float* loss_weights, double* loss, float* grad);
+/* Adam on the device (cosine decay from lr to lr/100), minibatches of `batch`
+ * samples in an order shuffled per epoch from SplitMix64(seed); losses[e]
+ * (may be NULL) receives the mean loss of epoch e. */
+int mpm_mlp_train(mpm_mlp* mlp, size_t n, const float* x, const float* t, const float* loss_weights,
+                  int epochs, int batch, double lr, uint64_t seed, double* losses);
+/* Use the network in the paste return of every instance (NULL: exact return
+ * only).  A prediction is refined by polish_steps Newton corrections and
+ * accepted when the return equation's residual |f(q)| / max(q_tr, 1) is at
+ * most `tolerance`; otherwise the exact return runs.  The network must stay
+ * alive while it is in use.  Resets the counters of mpm_surrogate_stats. */
+int mpm_set_surrogate(mpm_engine* engine, const mpm_mlp* mlp, double tolerance, int polish_steps);
+/* Yielded particles offered to the surrogate, and those whose prediction was accepted. */
+int mpm_surrogate_stats(mpm_engine* engine, uint64_t* considered, uint64_t* accepted);
+/* Log exact returns of yielded paste particles during stepping (capacity
+ * samples; 0 turns logging off).  A sample is 12 inputs, 7 targets and 6
+ * auxiliary values:
+ *   inputs  0-5 trial deviatoric Kirchhoff stress / q_tr (xx yy zz xy xz yz),
+ *           6 log10((q_tr - sigma_y) / (3 G dt)) / 4,
+ *           7 log1p(1e4 dt eps_dot) / 5 (eps_dot: equivalent deviatoric rate of sym C),
+ *           8 asinh(p / sigma_y) / 4, 9 sigma_y / q_tr, 10 log10(K / (3 G dt)) / 4, 11 n;
+ *   targets 0-5 returned deviatoric stress / q_tr, 6 (q - sigma_y) / (q_tr - sigma_y);
+ *   aux     q_tr, sigma_y, 3 G dt, K, n, q. */
+int mpm_log_returns(mpm_engine* engine, size_t capacity);
+/* Copy out up to `capacity` logged samples (arrays n x 12, n x 7, n x 6; any
+ * may be NULL) and clear the log. */
+int mpm_take_returns(mpm_engine* engine, float* inputs, float* targets, float* aux, size_t capacity,
+                     size_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpm_amd.so")
 
 dp = C.POINTER(C.c_double)
+fp = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
 
 
@@ -177,6 +178,20 @@ def lib():
         "mpm_get_launch": (C.c_int, [C.c_void_p, C.POINTER(Launch)]),
         "mpm_set_launch": (C.c_int, [C.c_void_p, C.POINTER(Launch)]),
         "mpm_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Launch)]),
+        "mpm_mlp_param_count": (C.c_size_t, []),
+        "mpm_mlp_create": (C.c_int, [fp, C.c_uint64, C.POINTER(C.c_void_p)]),
+        "mpm_mlp_destroy": (None, [C.c_void_p]),
+        "mpm_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+        "mpm_mlp_get_params": (C.c_int, [C.c_void_p, fp]),
+        "mpm_mlp_set_params": (C.c_int, [C.c_void_p, fp]),
+        "mpm_mlp_forward": (C.c_int, [C.c_void_p, C.c_size_t, fp, fp]),
+        "mpm_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_size_t, fp, fp, fp, dp, fp]),
+        "mpm_mlp_train": (C.c_int, [C.c_void_p, C.c_size_t, fp, fp, fp, C.c_int, C.c_int, C.c_double, C.c_uint64,
+                                    dp]),
+        "mpm_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int]),
+        "mpm_surrogate_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "mpm_log_returns": (C.c_int, [C.c_void_p, C.c_size_t]),
+        "mpm_take_returns": (C.c_int, [C.c_void_p, fp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -194,7 +209,9 @@ EXPORTED = [
     "mpm_momentum_residual", "mpm_energies", "mpm_paste_update_batch", "mpm_liquid_update_batch",
     "mpm_profile_g2p2g", "mpm_enable_profiling", "mpm_batch_create", "mpm_instance_count",
     "mpm_instance_particle_count", "mpm_instance_set_state", "mpm_instance_get_state", "mpm_batch_advance",
-    "mpm_get_launch", "mpm_set_launch", "mpm_autotune",
+    "mpm_get_launch", "mpm_set_launch", "mpm_autotune", "mpm_mlp_param_count", "mpm_mlp_create", "mpm_mlp_destroy",
+    "mpm_mlp_last_error", "mpm_mlp_get_params", "mpm_mlp_set_params", "mpm_mlp_forward", "mpm_mlp_loss_grad",
+    "mpm_mlp_train", "mpm_set_surrogate", "mpm_surrogate_stats", "mpm_log_returns", "mpm_take_returns",
 ]
 
 
@@ -203,6 +220,8 @@ def _p(a):
         return None
     if a.dtype == np.float64:
         return a.ctypes.data_as(dp)
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(fp)
     if a.dtype == np.uint32:
         return a.ctypes.data_as(u32p)
     raise TypeError(a.dtype)
@@ -444,6 +463,21 @@ class Simulation:
     def autotune(self, steps=8) -> Launch:
         return _autotune(self, steps)
 
+    def set_surrogate(self, net, tolerance=1e-4, polish=2):
+        """Use `net` for the paste return (None: exact only); see mpm_set_surrogate."""
+        _set_surrogate(self, net, tolerance, polish)
+
+    def surrogate_stats(self):
+        """(yielded particles offered to the surrogate, predictions accepted)."""
+        return _surrogate_stats(self)
+
+    def log_returns(self, capacity):
+        _log_returns(self, capacity)
+
+    def take_returns(self, capacity):
+        """(inputs n x 12, targets n x 7, aux n x 6) of the logged exact returns."""
+        return _take_returns(self, capacity)
+
     def stability_limit(self):
         return lib().mpm_stability_limit(self.h)
 
@@ -476,6 +510,96 @@ class Simulation:
         n = C.c_uint64()
         self._check(lib().mpm_profile_g2p2g(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if cols is None else a.reshape(-1, cols)
+
+
+class Network:
+    """Return-map surrogate network on the device (mpm_mlp_*): 12 -> 64 -> 64 -> 64 -> 7, SiLU,
+    bf16 MFMA with f32 accumulation; f32 master parameters in the layout of mpm_engine.h."""
+
+    def __init__(self, params=None, seed=0):
+        h = C.c_void_p()
+        p = None if params is None else _f32(params)
+        rc = lib().mpm_mlp_create(_p(p), seed, C.byref(h))
+        if rc != 0:
+            raise MpmError(f"mpm_mlp_create failed ({rc})")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.mpm_mlp_destroy(h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc < 0:
+            raise MpmError(lib().mpm_mlp_last_error(self.h).decode())
+        return rc
+
+    @staticmethod
+    def param_count():
+        return lib().mpm_mlp_param_count()
+
+    def params(self):
+        out = np.zeros(self.param_count(), np.float32)
+        self._check(lib().mpm_mlp_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.param_count()
+        self._check(lib().mpm_mlp_set_params(self.h, _p(p)))
+
+    def forward(self, x):
+        x = _f32(x, 12)
+        y = np.zeros((x.shape[0], 7), np.float32)
+        self._check(lib().mpm_mlp_forward(self.h, x.shape[0], _p(x), _p(y)))
+        return y
+
+    def loss_grad(self, x, t, weights=None):
+        x, t = _f32(x, 12), _f32(t, 7)
+        w = None if weights is None else _f32(weights)
+        g = np.zeros(self.param_count(), np.float32)
+        loss = C.c_double()
+        self._check(lib().mpm_mlp_loss_grad(self.h, x.shape[0], _p(x), _p(t), _p(w), C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train(self, x, t, weights=None, epochs=1, batch=4096, lr=1e-3, seed=0):
+        x, t = _f32(x, 12), _f32(t, 7)
+        w = None if weights is None else _f32(weights)
+        losses = np.zeros(epochs)
+        self._check(lib().mpm_mlp_train(self.h, x.shape[0], _p(x), _p(t), _p(w), epochs, batch, lr, seed,
+                                        _p(losses)))
+        return losses
+
+
+def _set_surrogate(eng, net, tolerance=1e-4, polish=2):
+    eng._surrogate = net  # the network must outlive its use by the engine
+    eng._check(lib().mpm_set_surrogate(eng.h, net.h if net is not None else None, tolerance, polish))
+
+
+def _surrogate_stats(eng):
+    a, b = C.c_uint64(), C.c_uint64()
+    eng._check(lib().mpm_surrogate_stats(eng.h, C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def _log_returns(eng, capacity):
+    eng._check(lib().mpm_log_returns(eng.h, capacity))
+
+
+def _take_returns(eng, capacity):
+    x = np.zeros((capacity, 12), np.float32)
+    t = np.zeros((capacity, 7), np.float32)
+    aux = np.zeros((capacity, 6), np.float32)
+    n = C.c_size_t()
+    eng._check(lib().mpm_take_returns(eng.h, _p(x), _p(t), _p(aux), capacity, C.byref(n)))
+    k = n.value
+    return x[:k], t[:k], aux[:k]
 
 
 def _get_launch(eng) -> Launch:
@@ -595,6 +719,21 @@ class Batch:
 
     def autotune(self, steps=8) -> Launch:
         return _autotune(self, steps)
+
+    def set_surrogate(self, net, tolerance=1e-4, polish=2):
+        """Use `net` for the paste return (None: exact only); see mpm_set_surrogate."""
+        _set_surrogate(self, net, tolerance, polish)
+
+    def surrogate_stats(self):
+        """(yielded particles offered to the surrogate, predictions accepted)."""
+        return _surrogate_stats(self)
+
+    def log_returns(self, capacity):
+        _log_returns(self, capacity)
+
+    def take_returns(self, capacity):
+        """(inputs n x 12, targets n x 7, aux n x 6) of the logged exact returns."""
+        return _take_returns(self, capacity)
 
     def enable_profiling(self, on=True):
         lib().mpm_enable_profiling(self.h, 1 if on else 0)

@@ -324,6 +324,14 @@ struct mpm_engine {
     bool profiling = false;
     double g2p2g_ms = 0.0;
     uint64_t g2p2g_launches = 0;
+    // return-map surrogate and the log of exact returns
+    const mpm_mlp* mlp = nullptr;
+    float mlp_tol = 1e-4f;
+    int mlp_polish = 2;
+    unsigned long long* mlp_stats = nullptr;
+    float* rlog = nullptr;
+    int* rlog_count = nullptr;
+    int rlog_cap = 0;
     std::string err;
 
     int fail_device(const char* what, hipError_t e) {
@@ -377,6 +385,14 @@ struct mpm_engine {
         B.pacc = d_pacc;
         B.ptrial = d_ptrial;
         B.npel = total_pel;
+        B.mlp_fwd = mlp ? mlp_device_fragments(mlp) : nullptr;
+        B.mlp_bias = mlp ? mlp_device_biases(mlp) : nullptr;
+        B.mlp_tol = mlp_tol;
+        B.mlp_polish = mlp_polish;
+        B.mlp_stats = mlp_stats;
+        B.rlog = rlog;
+        B.rlog_count = rlog_count;
+        B.rlog_cap = rlog_cap;
         return B;
     }
 
@@ -491,6 +507,7 @@ struct mpm_engine {
         if (int rc = dalloc(&run_unique, cap, false)) return rc;
         if (int rc = dalloc(&run_count, cap, false)) return rc;
         if (int rc = dalloc(&d_nruns, 1)) return rc;
+        if (int rc = dalloc(&mlp_stats, 2)) return rc;
         sort_bytes = std::max(sort_temp_bytes((int)cap, end_bit), rle_temp_bytes((int)cap));
         HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
         for (int k = 0; k < kMaxStepsPerSync; ++k) {
@@ -539,6 +556,9 @@ struct mpm_engine {
         (void)hipFree(run_unique);
         (void)hipFree(run_count);
         (void)hipFree(d_nruns);
+        (void)hipFree(mlp_stats);
+        (void)hipFree(rlog);
+        (void)hipFree(rlog_count);
         (void)hipFree(sort_temp);
         for (int k = 0; k < kMaxStepsPerSync; ++k) {
             if (ev0[k]) (void)hipEventDestroy(ev0[k]);
@@ -1670,6 +1690,68 @@ int mpm_profile_g2p2g(mpm_engine* e, double* avg_ms, uint64_t* launches) {
     if (!e) return MPM_ERR_INVALID;
     if (avg_ms) *avg_ms = e->g2p2g_launches ? e->g2p2g_ms / (double)e->g2p2g_launches : 0.0;
     if (launches) *launches = e->g2p2g_launches;
+    return MPM_OK;
+}
+
+int mpm_set_surrogate(mpm_engine* e, const mpm_mlp* mlp, double tolerance, int polish_steps) {
+    if (!e) return MPM_ERR_INVALID;
+    if (mlp && (!(tolerance > 0.0) || polish_steps < 0 || polish_steps > 8))
+        return e->fail(MPM_ERR_INVALID, "surrogate tolerance or correction count out of range");
+    e->mlp = mlp;
+    e->mlp_tol = (float)tolerance;
+    e->mlp_polish = polish_steps;
+    e->graph_stale = true;
+    if (hipMemset(e->mlp_stats, 0, 2 * sizeof(unsigned long long)) != hipSuccess)
+        return e->fail(MPM_ERR_DEVICE, "device failure clearing the surrogate counters");
+    return MPM_OK;
+}
+
+int mpm_surrogate_stats(mpm_engine* e, uint64_t* considered, uint64_t* accepted) {
+    if (!e) return MPM_ERR_INVALID;
+    unsigned long long h[2] = {0, 0};
+    if (hipMemcpy(h, e->mlp_stats, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+        return e->fail(MPM_ERR_DEVICE, "device failure reading the surrogate counters");
+    if (considered) *considered = h[0];
+    if (accepted) *accepted = h[1];
+    return MPM_OK;
+}
+
+int mpm_log_returns(mpm_engine* e, size_t capacity) {
+    if (!e || capacity > (size_t)INT32_MAX / 2) return MPM_ERR_INVALID;
+    (void)hipFree(e->rlog);
+    (void)hipFree(e->rlog_count);
+    e->rlog = nullptr;
+    e->rlog_count = nullptr;
+    e->rlog_cap = 0;
+    e->graph_stale = true;
+    if (capacity == 0) return MPM_OK;
+    if (hipMalloc(&e->rlog, sizeof(float) * kLogStride * capacity) != hipSuccess ||
+        hipMalloc(&e->rlog_count, sizeof(int)) != hipSuccess || hipMemset(e->rlog_count, 0, sizeof(int)) != hipSuccess)
+        return e->fail(MPM_ERR_DEVICE, "cannot allocate the return log");
+    e->rlog_cap = (int)capacity;
+    return MPM_OK;
+}
+
+int mpm_take_returns(mpm_engine* e, float* inputs, float* targets, float* aux, size_t capacity, size_t* n) {
+    if (!e || !n) return MPM_ERR_INVALID;
+    *n = 0;
+    if (!e->rlog) return MPM_OK;
+    int count = 0;
+    if (hipMemcpy(&count, e->rlog_count, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return e->fail(MPM_ERR_DEVICE, "device failure reading the return log");
+    size_t k = std::min({(size_t)std::max(count, 0), (size_t)e->rlog_cap, capacity});
+    std::vector<float> buf(k * kLogStride);
+    if (k && hipMemcpy(buf.data(), e->rlog, sizeof(float) * buf.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return e->fail(MPM_ERR_DEVICE, "device failure reading the return log");
+    for (size_t i = 0; i < k; ++i) {
+        const float* r = buf.data() + i * kLogStride;
+        if (inputs) std::memcpy(inputs + i * 12, r, sizeof(float) * 12);
+        if (targets) std::memcpy(targets + i * 7, r + 12, sizeof(float) * 7);
+        if (aux) std::memcpy(aux + i * 6, r + 19, sizeof(float) * 6);
+    }
+    if (hipMemset(e->rlog_count, 0, sizeof(int)) != hipSuccess)
+        return e->fail(MPM_ERR_DEVICE, "device failure clearing the return log");
+    *n = k;
     return MPM_OK;
 }
 

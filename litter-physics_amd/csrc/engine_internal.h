@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct mpm_mlp;
+
 namespace mpm {
 
 enum PField {
@@ -209,7 +211,23 @@ struct BatchDev {
     PelletAccum* pacc;
     PelletTrial* ptrial;
     int npel;
+    // Return-map surrogate (null: exact return only): packed bf16 forward
+    // fragments and biases, acceptance tolerance on the residual, Newton
+    // corrections after the prediction, counters (considered, accepted).
+    const uint16_t* mlp_fwd;
+    const float* mlp_bias;
+    float mlp_tol;
+    int mlp_polish;
+    unsigned long long* mlp_stats;
+    // Log of exact returns (null: off): kLogStride floats per yielded particle.
+    float* rlog;
+    int* rlog_count;
+    int rlog_cap;
 };
+
+// Logged exact return: 12 surrogate inputs, 7 targets, then q_tr, sigma_y,
+// 3 G dt, K, n and the returned q.
+constexpr int kLogStride = 25;
 
 __host__ __device__ inline GridPtrs grid_set(const BatchDev& B, int set) {
     GridPtrs g;
@@ -239,6 +257,10 @@ __host__ __device__ inline void node_coords(const Params& P, int index, int& i, 
     j = bj * 4 + ((l >> 2) & 3);
     k = bk * 4 + (l & 3);
 }
+
+// Device buffers of a network (mlp.hip).
+const uint16_t* mlp_device_fragments(const mpm_mlp* mlp);
+const float* mlp_device_biases(const mpm_mlp* mlp);
 
 // Host launchers (kernels.hip).
 int g2p2g_chunk();

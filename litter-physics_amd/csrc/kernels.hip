@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include "mlp.h"
 #include "physics.h"
 
 namespace mpm {
@@ -711,6 +712,18 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
             sg[q] = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
         }
     }
+    // The surrogate's weights and per-wavefront staging live in the scatter
+    // tile's LDS, which is unused until phase C.
+    const bool use_mlp = B.mlp_fwd != nullptr && P.kind == 0;
+    uint16_t* mlp_w = reinterpret_cast<uint16_t*>(ss);
+    float* mlp_b = reinterpret_cast<float*>(mlp_w + kFwdFrags * kFragElems);
+    uint16_t* mlp_in = reinterpret_cast<uint16_t*>(mlp_b + kBiasPacked) + wave * (64 * 16 + 2 * 64);
+    float* mlp_out = reinterpret_cast<float*>(mlp_in + 64 * 16);
+    if (use_mlp) {
+        for (int q = tid; q < kFwdFrags * kFragElems / 8; q += kBlock)
+            reinterpret_cast<uint4*>(mlp_w)[q] = reinterpret_cast<const uint4*>(B.mlp_fwd)[q];
+        for (int q = tid; q < kBiasPacked; q += kBlock) mlp_b[q] = B.mlp_bias[q];
+    }
     __syncthreads();
 
     // Phase B: G2P, advection, constitutive update; outputs go straight to the
@@ -719,41 +732,43 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
     double diss = 0.0;
     unsigned long long fail = ~0ull;
     int escaped = 0;
+    unsigned long long mlp_tried = 0, mlp_taken = 0;
     int nbase[R];
     lo[0] = lo[1] = lo[2] = INT_MAX;
     hi[0] = hi[1] = hi[2] = INT_MIN;
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
-        int p = chunk0 + r * kBlock + tid;
+        const int p = chunk0 + r * kBlock + tid;
         bool nw = false;
         nbase[r] = -1;
+        // Trial: gather, advection, trial deformation (or the whole liquid update).
+        bool live = false, yielded = false;
+        int rc = F_OK;
+        const float x = px[r], y = py[r], z = pz[r];
+        float v[3] = {0.0f, 0.0f, 0.0f}, C[9], Ht[9], tau[6];
+        float jn = 0.0f;
+        PasteTrial ts;
         if (p < chunk_end) {
-            float x = px[r], y = py[r], z = pz[r];
             Stencil s;
             if (!make_stencil(P, x, y, z, s)) {
                 // Outside the padded grid: frozen for this trial (solver.rs:905-917).
                 for (int f = 0; f < NPF; ++f) dst[f * cap + p] = src[f * cap + p];
                 escaped += 1;
             } else {
-                float v[3] = {0.0f, 0.0f, 0.0f};
+                live = true;
                 float Bm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                 if (t.tiled)
                     gather_lds(sg, t, s, v, Bm);
                 else
                     gather_global(P, gv, na, s, v, Bm);
-                float C[9];
-                float k4 = 4.0f * P.inv_dx;
+                const float k4 = 4.0f * P.inv_dx;
 #pragma unroll
                 for (int i = 0; i < 9; ++i) C[i] = Bm[i] * k4;
-                float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
                 float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
                 speed = fmaxf(speed, vn);
                 disp = fmaxf(disp, vn * dt * P.inv_dx);
-                float tau[6];
-                float inc = 0.0f, dd = 0.0f;
-                int rc;
                 if (P.kind == 0) {
-                    float H[9], Ht[9], Hn[9];
+                    float H[9];
 #pragma unroll
                     for (int i = 0; i < 9; ++i) H[i] = src[(H00 + i) * cap + p];
                     // (I + dt C)(I + H) - I = H + dt C + dt C H
@@ -763,56 +778,139 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
                         for (int j = 0; j < 3; ++j)
                             Ht[3 * i + j] = H[3 * i + j] + dt * C[3 * i + j] +
                                             dt * (C[3 * i] * H[j] + C[3 * i + 1] * H[3 + j] + C[3 * i + 2] * H[6 + j]);
-                    rc = paste_update(P, dt, Ht, Hn, tau, inc, dd);
-                    if (rc == F_OK) {
-#pragma unroll
-                        for (int i = 0; i < 9; ++i) dst[(H00 + i) * cap + p] = Hn[i];
-                        diss += (double)dd * (double)shared[SVOL0 * cap + p];
-                        dst[EPS * cap + p] = src[EPS * cap + p] + inc;
-                    }
+                    rc = paste_trial(P, Ht, ts);
+                    yielded = rc == F_OK && paste_yields(P, ts);
                 } else {
-                    float jn;
                     rc = liquid_update(P, dt, src[JM1 * cap + p], C, jn, tau);
-                    if (rc == F_OK) dst[JM1 * cap + p] = jn;
                 }
-                if (rc != F_OK) {
-                    unsigned long long f = ((unsigned long long)(p - ip.poff) << 8) | (unsigned long long)rc;
-                    fail = f < fail ? f : fail;
+            }
+        }
+        // Surrogate return for the wavefront's yielded particles.  Everything in
+        // this block is uniform across the wavefront, so the MFMAs run with all lanes.
+        float q = 0.0f;
+        bool settled = false;
+        if (use_mlp) {
+            const bool want = yielded && P.K > 0.0f;
+            const unsigned long long mask = __ballot(want);
+            if (mask) {
+                float f[kMlpIn];
+                if (want) {
+                    surrogate_features(P, dt, ts, C, f);
                 } else {
-                    dst[PX * cap + p] = nx;
-                    dst[PY * cap + p] = ny;
-                    dst[PZ * cap + p] = nz;
-                    dst[VX * cap + p] = v[0];
-                    dst[VY * cap + p] = v[1];
-                    dst[VZ * cap + p] = v[2];
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
+                    for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
+                }
+                uint16_t* row = mlp_in + lane * 16;
 #pragma unroll
-                    for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
-                    float ns[3] = {nx * P.inv_dx, ny * P.inv_dx, nz * P.inv_dx};
-                    int nb[3];
-                    bool valid = true;
+                for (int i = 0; i < 16; ++i) row[i] = i < kMlpIn ? bf16_bits(f[i]) : (uint16_t)0;
+                wave_lds_sync();
+#pragma unroll 1
+                for (int mt = 0; mt < 4; ++mt) {
+                    const int smp = mt * 16 + (lane & 15);
+                    const bf16x4 xb = *reinterpret_cast<const bf16x4*>(mlp_in + smp * 16 + 4 * (lane >> 4));
+                    MlpTile tl;
+                    mlp_tile_forward<false>(mlp_w, mlp_b, xb, lane, tl);
+                    // Output 6 (the retained overstress fraction): lane group 1, element 2.
+                    if ((lane >> 4) == 1) mlp_out[smp] = tl.y[2];
+                }
+                wave_lds_sync();
+                if (want) settled = surrogate_polish(P, dt, ts.qtr, mlp_out[lane], B.mlp_polish, q) <= B.mlp_tol;
+                mlp_tried += __popcll(mask);
+                mlp_taken += __popcll(__ballot(settled));
+            }
+        }
+        // Finish: exact return where the surrogate was not used or not accepted.
+        if (live) {
+            if (P.kind == 0) {
+                float inc = 0.0f, Hn[9];
+                if (rc == F_OK) {
+                    if (!yielded)
+                        q = ts.qtr;
+                    else if (settled)
+                        inc = (ts.qtr - q) / (3.0f * P.G);
+                    else
+                        rc = hb_return(P, dt, ts.qtr, q, inc);
+                }
+                if (rc == F_OK) rc = paste_finish(P, ts, Ht, q, yielded, Hn, tau);
+                if (rc == F_OK) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) dst[(H00 + i) * cap + p] = Hn[i];
+                    diss += (double)(q * inc) * (double)shared[SVOL0 * cap + p];
+                    dst[EPS * cap + p] = src[EPS * cap + p] + inc;
+                }
+            } else if (rc == F_OK) {
+                dst[JM1 * cap + p] = jn;
+            }
+            if (rc != F_OK) {
+                unsigned long long f = ((unsigned long long)(p - ip.poff) << 8) | (unsigned long long)rc;
+                fail = f < fail ? f : fail;
+            } else {
+                const float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
+                dst[PX * cap + p] = nx;
+                dst[PY * cap + p] = ny;
+                dst[PZ * cap + p] = nz;
+                dst[VX * cap + p] = v[0];
+                dst[VY * cap + p] = v[1];
+                dst[VZ * cap + p] = v[2];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
+                float ns[3] = {nx * P.inv_dx, ny * P.inv_dx, nz * P.inv_dx};
+                int nb[3];
+                bool valid = true;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    float bf = floorf(ns[a] - 0.5f) + 1.0f;
+                    valid = valid && bf >= 0.0f && bf + 2.0f < (float)P.nodes[a];
+                    nb[a] = valid ? (int)bf : 0;
+                }
+                if (valid) {
+                    nbase[r] = (nb[0] << 20) | (nb[1] << 10) | nb[2];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        float bf = floorf(ns[a] - 0.5f) + 1.0f;
-                        valid = valid && bf >= 0.0f && bf + 2.0f < (float)P.nodes[a];
-                        nb[a] = valid ? (int)bf : 0;
+                        lo[a] = min(lo[a], nb[a]);
+                        hi[a] = max(hi[a], nb[a]);
                     }
-                    if (valid) {
-                        nbase[r] = (nb[0] << 20) | (nb[1] << 10) | nb[2];
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            lo[a] = min(lo[a], nb[a]);
-                            hi[a] = max(hi[a], nb[a]);
-                        }
-                        nw = near_wall(P, nx, ny, nz);
-                    } else {
-                        escaped += 1;
-                    }
+                    nw = near_wall(P, nx, ny, nz);
+                } else {
+                    escaped += 1;
                 }
             }
         }
         wave_append(nw, p, nw_list, nw_count);
+        // Log of exact returns (training data of the surrogate).
+        if (B.rlog) {
+            const bool lg = live && P.kind == 0 && yielded && rc == F_OK;
+            const unsigned long long mk = __ballot(lg);
+            if (mk) {
+                const int leader = __ffsll((long long)mk) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(B.rlog_count, (int)__popcll(mk));
+                base = __shfl(base, leader, 64);
+                const int slot = base + (int)__popcll(mk & ((1ull << lane) - 1ull));
+                if (lg && slot < B.rlog_cap) {
+                    float f[kMlpIn], tg[kMlpOut];
+                    surrogate_features(P, dt, ts, C, f);
+                    surrogate_targets(P, ts, f, q, tg);
+                    float* o = B.rlog + (size_t)slot * kLogStride;
+#pragma unroll
+                    for (int i = 0; i < kMlpIn; ++i) o[i] = f[i];
+#pragma unroll
+                    for (int i = 0; i < kMlpOut; ++i) o[kMlpIn + i] = tg[i];
+                    o[19] = ts.qtr;
+                    o[20] = P.ys;
+                    o[21] = 3.0f * P.G * dt;
+                    o[22] = P.K;
+                    o[23] = 1.0f / P.inv_n;
+                    o[24] = q;
+                }
+            }
+        }
+    }
+    if (use_mlp && lane == 0 && mlp_tried) {
+        atomicAdd(B.mlp_stats, mlp_tried);
+        atomicAdd(B.mlp_stats + 1, mlp_taken);
     }
 
     // Phase C: exact tile of the new stencils.

@@ -223,11 +223,16 @@ __device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, 
     return F_RETURN;
 }
 
-// update_paste (constitutive.rs:456-531) on H = F - I:
-//   b - I = H + H^T + H H^T, Hencky e = log1p(beta)/2, radial return on dev(e),
-//   F' = F + U diag(expm1((alpha - 1) dev e)) U^T F, tau = U diag(2G e' + lambda tr e) U^T.
-__device__ inline int paste_update(const Params& P, float dt, const float* Ht, float* Hn, float* tau, float& inc,
-                                   float& dd) {
+// Trial state of update_paste (constitutive.rs:456-500) on H = F - I:
+//   b - I = H + H^T + H H^T, Hencky e = log1p(beta)/2 in the eigenbasis U of b,
+//   q_tr = sqrt(3/2) 2G |dev e|.
+struct PasteTrial {
+    float U[9];
+    float dev[3];
+    float mean, vol, dn, qtr;
+};
+
+__device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& s) {
     float F[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) F[i] = Ht[i] + ((i % 4 == 0) ? 1.0f : 0.0f);
@@ -241,36 +246,42 @@ __device__ inline int paste_update(const Params& P, float dt, const float* Ht, f
         for (int j = 0; j < 3; ++j)
             B[3 * i + j] = Ht[3 * i + j] + Ht[3 * j + i] + Ht[3 * i] * Ht[3 * j] + Ht[3 * i + 1] * Ht[3 * j + 1] +
                            Ht[3 * i + 2] * Ht[3 * j + 2];
-    float U[9];
-    int rc = eigen_sym3(B, U);
+    int rc = eigen_sym3(B, s.U);
     if (rc) return rc;
     float beta[3] = {B[0], B[4], B[8]};
     float e[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float s = sqrtf(fmaxf(1.0f + beta[a], 0.0f));
-        if (!isfinite(s) || s < 1e-6f) return F_STRETCH;
+        float st = sqrtf(fmaxf(1.0f + beta[a], 0.0f));
+        if (!isfinite(st) || st < 1e-6f) return F_STRETCH;
         e[a] = 0.5f * log1pf(beta[a]);
     }
-    float vol = e[0] + e[1] + e[2];
-    float mean = vol / 3.0f;
-    float dev[3] = {e[0] - mean, e[1] - mean, e[2] - mean};
-    float dn = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
-    float qtr = 1.2247448713915890f * 2.0f * P.G * dn;
-    float alpha = 1.0f, q = qtr;
-    inc = 0.0f;
-    if (qtr > P.ys && dn > 0.0f) {
-        rc = hb_return(P, dt, qtr, q, inc);
-        if (rc) return rc;
-        alpha = q / qtr;
-    }
+    s.vol = e[0] + e[1] + e[2];
+    s.mean = s.vol / 3.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s.dev[a] = e[a] - s.mean;
+    s.dn = sqrtf(s.dev[0] * s.dev[0] + s.dev[1] * s.dev[1] + s.dev[2] * s.dev[2]);
+    s.qtr = 1.2247448713915890f * 2.0f * P.G * s.dn;
+    return F_OK;
+}
+
+// Whether the trial state needs the Herschel-Bulkley return (constitutive.rs:500-508).
+__device__ inline bool paste_yields(const Params& P, const PasteTrial& s) { return s.qtr > P.ys && s.dn > 0.0f; }
+
+// Rest of update_paste (constitutive.rs:508-531) for the returned equivalent
+// stress q:  F' = F + U diag(expm1((alpha - 1) dev e)) U^T F,
+// tau = U diag(2G (alpha dev e + mean) + lambda tr e) U^T, alpha = q / q_tr.
+__device__ inline int paste_finish(const Params& P, const PasteTrial& s, const float* Ht, float q, bool yielded,
+                                   float* Hn, float* tau) {
+    float alpha = yielded ? q / s.qtr : 1.0f;
     float rm1[3], pr[3];
-    float lv = P.lam * vol;
+    float lv = P.lam * s.vol;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        rm1[a] = expm1f((alpha - 1.0f) * dev[a]);
-        pr[a] = 2.0f * P.G * (alpha * dev[a] + mean) + lv;
+        rm1[a] = expm1f((alpha - 1.0f) * s.dev[a]);
+        pr[a] = 2.0f * P.G * (alpha * s.dev[a] + s.mean) + lv;
     }
+    const float* U = s.U;
     // M = U diag(rm1) U^T, H' = H + M (I + H)
     float M[9];
 #pragma unroll
@@ -292,8 +303,92 @@ __device__ inline int paste_update(const Params& P, float dt, const float* Ht, f
         tau[t] = U[3 * i] * pr[0] * U[3 * j] + U[3 * i + 1] * pr[1] * U[3 * j + 1] + U[3 * i + 2] * pr[2] * U[3 * j + 2];
         if (!isfinite(tau[t])) return F_NONFINITE;
     }
+    return F_OK;
+}
+
+// update_paste (constitutive.rs:456-531): trial state, exact return, finish.
+__device__ inline int paste_update(const Params& P, float dt, const float* Ht, float* Hn, float* tau, float& inc,
+                                   float& dd) {
+    PasteTrial s;
+    int rc = paste_trial(P, Ht, s);
+    if (rc) return rc;
+    bool y = paste_yields(P, s);
+    float q = s.qtr;
+    inc = 0.0f;
+    if (y) {
+        rc = hb_return(P, dt, s.qtr, q, inc);
+        if (rc) return rc;
+    }
+    rc = paste_finish(P, s, Ht, q, y, Hn, tau);
+    if (rc) return rc;
     dd = q * inc;
     return F_OK;
+}
+
+// ------------------------------------------------------------ surrogate encoding
+
+// Inputs of the return-map surrogate (12, all O(1)):
+//   0-5  trial deviatoric Kirchhoff stress / q_tr (xx, yy, zz, xy, xz, yz, lab frame)
+//   6    log10(r) / 4, r = (q_tr - sigma_y) / (3 G dt): the plastic strain rate a full
+//        return would need, in units of the elastic rate
+//   7    log1p(1e4 dt eps_dot) / 5, eps_dot = sqrt(2/3 dev(D):dev(D)), D = sym(C)
+//   8    asinh(p / sigma_y) / 4, p = -tr(tau_tr) / 3
+//   9    sigma_y / q_tr
+//   10   log10(K / (3 G dt)) / 4
+//   11   n
+// With r, K / (3 G dt) and n the return is fully determined (the dimensionless
+// overstress y solves y + beta y^(1/n) = 1, beta = (1/r) (r / (K / 3G dt))^(1/n)).
+__device__ inline void surrogate_features(const Params& P, float dt, const PasteTrial& s, const float* C, float* f) {
+    const float inv_q = 1.0f / s.qtr;
+    float d2[3] = {2.0f * P.G * s.dev[0] * inv_q, 2.0f * P.G * s.dev[1] * inv_q, 2.0f * P.G * s.dev[2] * inv_q};
+    const float* U = s.U;
+    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        int i = ii[t], j = jj[t];
+        f[t] = U[3 * i] * d2[0] * U[3 * j] + U[3 * i + 1] * d2[1] * U[3 * j + 1] + U[3 * i + 2] * d2[2] * U[3 * j + 2];
+    }
+    const float s3 = 3.0f * P.G * dt;
+    f[6] = log10f(fmaxf((s.qtr - P.ys) / s3, 1e-30f)) * 0.25f;
+    float tr = (C[0] + C[4] + C[8]) / 3.0f;
+    float dxx = C[0] - tr, dyy = C[4] - tr, dzz = C[8] - tr;
+    float dxy = 0.5f * (C[1] + C[3]), dxz = 0.5f * (C[2] + C[6]), dyz = 0.5f * (C[5] + C[7]);
+    float ee = sqrtf((2.0f / 3.0f) * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0f * (dxy * dxy + dxz * dxz + dyz * dyz)));
+    f[7] = log1pf(1e4f * dt * ee) * 0.2f;
+    float p = -(2.0f * P.G / 3.0f + P.lam) * s.vol;
+    f[8] = asinhf(p / P.ys) * 0.25f;
+    f[9] = P.ys * inv_q;
+    f[10] = log10f(P.K / s3) * 0.25f;
+    f[11] = 1.0f / P.inv_n;
+}
+
+// Targets (7): the corrected deviatoric stress / q_tr (alpha times inputs 0-5)
+// and y = (q - sigma_y) / (q_tr - sigma_y) = 1 - 3G dlambda / (q_tr - sigma_y),
+// the plastic multiplier in units of the full return.
+__device__ inline void surrogate_targets(const Params& P, const PasteTrial& s, const float* f, float q, float* t) {
+    float alpha = q / s.qtr;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) t[i] = alpha * f[i];
+    t[6] = (q - P.ys) / (s.qtr - P.ys);
+}
+
+// One safeguarded Newton step of the return equation from the predicted
+// overstress fraction, and the residual |f(q)| / max(q_tr, 1) of the result
+// (constitutive.rs:301-319, the same equation hb_return solves).
+__device__ inline float surrogate_polish(const Params& P, float dt, float qtr, float y, int steps, float& q) {
+    const float s3 = 3.0f * P.G * dt, inv_k = 1.0f / P.K;
+    const float avail = qtr - P.ys;
+    float d = avail * fminf(fmaxf(y, 0.0f), 1.0f);
+    for (int it = 0; it < steps; ++it) {
+        float over = d * inv_k;
+        float r = d - avail + s3 * powf(over, P.inv_n);
+        float deriv = over > 0.0f ? 1.0f + s3 * P.inv_n * powf(over, P.inv_n - 1.0f) * inv_k : INFINITY;
+        float nd = d - r / deriv;
+        d = (isfinite(nd) && nd > 0.0f && nd < avail) ? nd : d;
+    }
+    q = P.ys + d;
+    float r = d - avail + s3 * powf(d * inv_k, P.inv_n);
+    return fabsf(r) / fmaxf(qtr, 1.0f);
 }
 
 // update_liquid (constitutive.rs:557-589) on J - 1, with the Tait pressure

@@ -145,4 +145,5 @@ def test_step_budget_and_callback(engine):
 
     assert b.advance_while(1.0, keep_going) is False
     assert b.history().step_count == 10 == a.history().step_count
-    assert a.history().time == b.history().time
+    # Same steps; the times agree to the round-off of the atomically summed grid.
+    assert abs(a.history().time - b.history().time) <= 1e-9 * a.history().time
