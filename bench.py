@@ -14,6 +14,7 @@ Other workloads (--workload), one JSON line each:
   pellets    1,474,560-particle paste bed + 512 rigid pellets, 96^3 grid, dx 1.5 mm, seed 2
   surrogate  the same bed without pellets, exact return vs the learned return
   sweep      256 paste slumps of 16^3..64^3 grids stepped as one batch, seed 4
+  copy       a plain 256 MiB device copy (calibration of the HBM byte counters)
 """
 import argparse
 import json
@@ -379,12 +380,34 @@ def run_sweep(args):
                  "sequential_sample_value": seq_value, "speedup_vs_sequential": value / seq_value})
 
 
+# ---------------------------------------------------------------- copy calibration
+
+def run_copy(args):
+    """A plain device copy of 256 MiB, 10 times: the reference read and write of
+    known size against which the HBM counters of the kernels are scaled."""
+    import torch
+    n = 64 << 20  # floats
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nbytes = 4 * n
+    return line("device copy bandwidth (calibration)", 2 * nbytes * 10 / el, "B/s", 10, 0, el, "f32",
+                {"workload": f"torch copy of {nbytes} bytes (read {nbytes} B, write {nbytes} B per launch), 10 launches",
+                 "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="headline", choices=["headline", "liquid", "pellets", "surrogate", "sweep"])
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "liquid", "pellets", "surrogate", "sweep", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true")
@@ -396,7 +419,7 @@ def main():
               "(the MLS-MPM substep is measured on a single MI355X).", file=sys.stderr)
         sys.exit(2)
     run = {"headline": run_headline, "liquid": run_liquid, "pellets": run_pellets, "surrogate": run_surrogate,
-           "sweep": run_sweep}[args.workload]
+           "sweep": run_sweep, "copy": run_copy}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

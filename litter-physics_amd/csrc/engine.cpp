@@ -567,11 +567,25 @@ struct mpm_engine {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
+    // Order keys still pending for instances whose last accepted step hit the
+    // 32-step cadence and that took no trial step since (the next trial's G2P2G
+    // writes them otherwise).  The mirror must be current.
+    int flush_refkeys() {
+        bool any = false;
+        for (int i = 0; i < ninst; ++i) any = any || h_is[i].refsort;
+        if (!any) return MPM_OK;
+        launch_refkeys(stream, dev());
+        HIP_OK(hipGetLastError());
+        for (int i = 0; i < ninst; ++i) h_is[i].refsort = 0;
+        return push();
+    }
+
     // Sort every instance's particles by 4^3-cell block (device order, for
     // locality), rebuild the work items and the near-wall lists.  The order keys
     // of the reference order travel with the particles.  The mirror must be
     // current; it is current again on return.
     int sort_all() {
+        if (int rc = flush_refkeys()) return rc;
         BatchDev B = dev();
         launch_sort_keys(stream, B, cell_bits, keys[0], order[0]);
         launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)cap, end_bit);
@@ -684,6 +698,7 @@ struct mpm_engine {
 
     // Copy instance i's committed particles to the host (mirror current).
     int download(int i, HostParticles& h) {
+        if (int rc = flush_refkeys()) return rc;
         const Instance& in = inst[i];
         size_t n = (size_t)h_is[i].n, off = (size_t)in.ip.poff;
         int cur = h_is[i].cur;
@@ -974,6 +989,7 @@ struct mpm_engine {
     int autotune(int steps, mpm_launch* chosen) {
         if (int rc = pull()) return rc;
         if (int rc = prepare()) return rc;
+        if (int rc = flush_refkeys()) return rc;
         Snapshot snap;
         int rc = snapshot(snap);
         bool prof = profiling;
@@ -1123,6 +1139,8 @@ int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pell
     s.n = (int)n;
     s.mode = M_DONE;
     s.max_speed = max_speed;
+    for (const PelletState& p : pel)
+        s.pel_vmax = std::max(s.pel_vmax, std::sqrt(p.vel[0] * p.vel[0] + p.vel[1] * p.vel[1] + p.vel[2] * p.vel[2]));
     if (history) {
         s.time = history->time;
         s.step_count = history->step_count;

@@ -115,6 +115,7 @@ struct InstState {
     double ledger[kLedger];
     double time, target, dt, limit, dt_scale, max_speed, min_dt_used, max_dt_used;
     double trial_dt;  // > 0: one bare trial step of this size (try_step), no time advance
+    double pel_vmax;  // largest pellet speed of the committed state
     unsigned long long step_count, limited_steps, rejected_steps;
     StepStatus st;
     int n;            // particle count
@@ -125,7 +126,7 @@ struct InstState {
     int budget;       // accepted steps left before a pause (host-set)
     int w_off, w_cnt; // work items
     int nw_base;      // inclusive end of the instance's range in the near-wall prefix (traction)
-    int refsort;      // the accepted step hit the 32-step cadence: refresh the order keys
+    int refsort;      // the last accepted step hit the 32-step cadence; its order keys are pending
     int committed;    // a trial step was accepted since the state was set
     int pad;
 };
@@ -270,6 +271,8 @@ int g2p2g_chunk();
 void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, hipEvent_t g2p2g_start,
                  hipEvent_t g2p2g_stop);
 void launch_p2g(hipStream_t s, const BatchDev& B, const Params& P, int poff, int n, int boff, int cur, int gcur);
+// Order keys of the instances whose refresh is pending (the flags are left set).
+void launch_refkeys(hipStream_t s, const BatchDev& B);
 // Near-wall lists of every instance's committed slab (counters cleared by the caller).
 void launch_nearwall(hipStream_t s, const BatchDev& B);
 // Device-order keys of every slab entry: (instance << cell_bits) | 4^3-cell block

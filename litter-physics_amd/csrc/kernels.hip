@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kChunk = 1024;
 constexpr int kGatherCap = 768;
-constexpr int kScatterCap = 1024;
+constexpr int kScatterCap = 1152;  // 9 x 9 x 13 nodes: two z-adjacent cell blocks plus one cell each side
 constexpr double kDtRecovery = 1.25;              // solver.rs:45
 constexpr unsigned long long kSortInterval = 32;  // solver.rs:39
 
@@ -89,19 +89,29 @@ __device__ inline int instance_of(const BatchDev& B, int p) {
     return lo;
 }
 
+// Reference order key (clamped cell index, z fastest, then id; solver.rs:501-517).
+__device__ inline uint64_t ref_cell(const Params& P, float x, float y, float z) {
+    const float pos[3] = {x, y, z};
+    uint64_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float sc = floorf(pos[a] / P.dx);
+        sc = isnan(sc) ? 0.0f : fmaxf(sc, 0.0f);
+        c[a] = sc >= (float)P.cells[a] ? (uint64_t)P.cells[a] : (uint64_t)sc;
+    }
+    return (c[0] * (uint64_t)(P.cells[1] + 1) + c[1]) * (uint64_t)(P.cells[2] + 1) + c[2];
+}
+
 // ------------------------------------------------------------ control
 
 // Simulation::stability_limit, solver.rs:477-498.
-__device__ inline double stability_limit(const BatchDev& B, const InstParams& ip, const InstState& s) {
+__device__ inline double stability_limit(const InstParams& ip, const InstState& s) {
     double limit = ip.cfl * ip.dx / (ip.wave + s.max_speed);
     if (ip.diff > 0.0) limit = fmin(limit, 0.25 * ip.dx * ip.dx / ip.diff);
     if (ip.npel > 0) {
+        // min over pellets of cfl dx / |v_k| is cfl dx / max_k |v_k|
         limit = fmin(limit, ip.pel_contact_dt);
-        for (int k = 0; k < ip.npel; ++k) {
-            const PelletState& p = B.pel[ip.pel_off + k];
-            double sp = sqrt(p.vel[0] * p.vel[0] + p.vel[1] * p.vel[1] + p.vel[2] * p.vel[2]);
-            if (sp > 0.0) limit = fmin(limit, ip.cfl * ip.dx / sp);
-        }
+        if (s.pel_vmax > 0.0) limit = fmin(limit, ip.cfl * ip.dx / s.pel_vmax);
     }
     return limit;
 }
@@ -111,7 +121,6 @@ __global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
         InstState& s = B.is[t];
         const InstParams& ip = B.ip[t];
         s.active = 0;
-        s.refsort = 0;
         if (s.mode == M_RUN) {
             if (s.trial_dt > 0.0) {
                 // A bare trial step of a given size (solver.rs:535-537).
@@ -128,7 +137,7 @@ __global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
             } else {
                 // solver.rs:425-443
                 double remaining = s.target - s.time;
-                double limit = stability_limit(B, ip, s);
+                double limit = stability_limit(ip, s);
                 double scaled = s.dt_scale * limit;
                 int limited = scaled < ip.max_dt;
                 double dt = fmin(fmin(scaled, ip.max_dt), remaining);
@@ -605,18 +614,23 @@ __global__ void pellet_kernel(BatchDev B) {
 // next step's particle-to-grid deposit into the other accumulator set.  A
 // workgroup owns one work item: the particles of one or two z-adjacent cell
 // blocks of one instance.  It stages the node velocities of their stencils in
-// LDS, updates every particle, then deposits the updated particles into an f64
-// LDS tile covering exactly their new stencils and flushes it with one global
-// atomic per touched node and field.  Work items whose tiles exceed the LDS
-// budget take the direct global path.
-template <int CHUNK, int GCAP, int SCAP>
+// LDS and deposits every updated particle straight from registers into an f64
+// LDS tile one cell wider on each side than those stencils (an accepted step
+// moves a particle by at most one cell); the tile is flushed with one global
+// atomic per touched node and field.  Particles that leave the tile, and work
+// items whose tiles exceed the LDS budget, deposit with global atomics.
+// MLP: the return-map surrogate is on (its weights get LDS of their own).
+template <int CHUNK, int GCAP, int SCAP, bool MLP>
 __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
+    constexpr int kMlpWeights = kFwdFrags * kFragElems * 2 + kBiasPacked * 4;  // bytes
+    constexpr int kMlpWave = 64 * 16 * 2 + 64 * 4;                             // staging bytes per wavefront
     __shared__ float4 sg[GCAP];
     __shared__ double ss[7 * SCAP];
+    __shared__ __align__(16) unsigned char smlp[MLP ? kMlpWeights + 4 * kMlpWave : 16];
     __shared__ int sbox[4][6];
     __shared__ Tile tile;
-    __shared__ int sorg[3], sdim[3], stiled;
+    __shared__ int sscat;
     __shared__ double sdiss[4];
     __shared__ float sred[4][2];
     __shared__ unsigned long long sfail[4];
@@ -660,6 +674,8 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
             px[r] = src[PX * cap + p];
             py[r] = src[PY * cap + p];
             pz[r] = src[PZ * cap + p];
+            // The committed step hit the 32-step cadence: order keys of the committed positions.
+            if (is.refsort) B.refkey[p] = (ref_cell(P, px[r], py[r], pz[r]) << 32) | (uint64_t)B.ids[p];
             float xs[3] = {px[r] * P.inv_dx, py[r] * P.inv_dx, pz[r] * P.inv_dx};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -691,18 +707,24 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
             H[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
         }
         bool ok = L[0] <= H[0] && L[1] <= H[1] && L[2] <= H[2];
-        int gsz = 1;
+        int gsz = 1, ssz = 1;
         for (int a = 0; a < 3; ++a) {
             t.org[a] = ok ? L[a] : 0;
             t.gd[a] = ok ? H[a] - L[a] + 3 : 0;
-            t.sd[a] = 0;
+            t.sd[a] = t.gd[a] + 2;  // scatter tile: origin org - 1
             gsz *= t.gd[a];
+            ssz *= t.sd[a];
         }
         t.tiled = ok && gsz <= GCAP;
         tile = t;
+        sscat = ok && ssz <= SCAP;
     }
     __syncthreads();
     const Tile t = tile;
+    const bool stiled = sscat != 0;
+    const int so[3] = {t.org[0] - 1, t.org[1] - 1, t.org[2] - 1};
+    const int sd[3] = {t.sd[0], t.sd[1], t.sd[2]};
+    const int ssz = sd[0] * sd[1] * sd[2];
     if (t.tiled) {
         int gsz = t.gd[0] * t.gd[1] * t.gd[2];
         for (int q = tid; q < gsz; q += kBlock) {
@@ -712,35 +734,31 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
             sg[q] = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
         }
     }
-    // The surrogate's weights and per-wavefront staging live in the scatter
-    // tile's LDS, which is unused until phase C.
-    const bool use_mlp = B.mlp_fwd != nullptr && P.kind == 0;
-    uint16_t* mlp_w = reinterpret_cast<uint16_t*>(ss);
-    float* mlp_b = reinterpret_cast<float*>(mlp_w + kFwdFrags * kFragElems);
-    uint16_t* mlp_in = reinterpret_cast<uint16_t*>(mlp_b + kBiasPacked) + wave * (64 * 16 + 2 * 64);
+    if (stiled)
+        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0.0;
+    uint16_t* mlp_w = reinterpret_cast<uint16_t*>(smlp);
+    float* mlp_b = reinterpret_cast<float*>(smlp + kFwdFrags * kFragElems * 2);
+    uint16_t* mlp_in = reinterpret_cast<uint16_t*>(smlp + kMlpWeights + wave * kMlpWave);
     float* mlp_out = reinterpret_cast<float*>(mlp_in + 64 * 16);
-    if (use_mlp) {
+    if (MLP) {
         for (int q = tid; q < kFwdFrags * kFragElems / 8; q += kBlock)
             reinterpret_cast<uint4*>(mlp_w)[q] = reinterpret_cast<const uint4*>(B.mlp_fwd)[q];
         for (int q = tid; q < kBiasPacked; q += kBlock) mlp_b[q] = B.mlp_bias[q];
     }
     __syncthreads();
 
-    // Phase B: G2P, advection, constitutive update; outputs go straight to the
-    // trial slab.  The new stencil base of each updated particle is kept packed.
+    // Phase B: per particle, G2P, advection, constitutive update, writes to the
+    // trial slab, and the deposit of the updated particle.
     float speed = 0.0f, disp = 0.0f;
     double diss = 0.0;
     unsigned long long fail = ~0ull;
     int escaped = 0;
     unsigned long long mlp_tried = 0, mlp_taken = 0;
-    int nbase[R];
-    lo[0] = lo[1] = lo[2] = INT_MAX;
-    hi[0] = hi[1] = hi[2] = INT_MIN;
+    const int rot = lane & 7;
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int p = chunk0 + r * kBlock + tid;
         bool nw = false;
-        nbase[r] = -1;
         // Trial: gather, advection, trial deformation (or the whole liquid update).
         bool live = false, yielded = false;
         int rc = F_OK;
@@ -789,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
         // this block is uniform across the wavefront, so the MFMAs run with all lanes.
         float q = 0.0f;
         bool settled = false;
-        if (use_mlp) {
+        if (MLP && P.kind == 0) {
             const bool want = yielded && P.K > 0.0f;
             const unsigned long long mask = __ballot(want);
             if (mask) {
@@ -819,7 +837,8 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
                 mlp_taken += __popcll(__ballot(settled));
             }
         }
-        // Finish: exact return where the surrogate was not used or not accepted.
+        // Finish: exact return where the surrogate was not used or not accepted,
+        // trial-slab writes and the deposit.
         if (live) {
             if (P.kind == 0) {
                 float inc = 0.0f, Hn[9];
@@ -856,22 +875,19 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
                 for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
-                float ns[3] = {nx * P.inv_dx, ny * P.inv_dx, nz * P.inv_dx};
-                int nb[3];
-                bool valid = true;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    float bf = floorf(ns[a] - 0.5f) + 1.0f;
-                    valid = valid && bf >= 0.0f && bf + 2.0f < (float)P.nodes[a];
-                    nb[a] = valid ? (int)bf : 0;
-                }
-                if (valid) {
-                    nbase[r] = (nb[0] << 20) | (nb[1] << 10) | nb[2];
+                Stencil s2;
+                if (make_stencil(P, nx, ny, nz, s2)) {
+                    const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
+                    bool in_tile = stiled;
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        lo[a] = min(lo[a], nb[a]);
-                        hi[a] = max(hi[a], nb[a]);
+                        const int li = s2.base[a] - so[a];
+                        in_tile = in_tile && li >= 0 && li + 2 < sd[a];
                     }
+                    if (in_tile)
+                        scatter_lds<SCAP>(ss, so, sd, P, s2, m, v, C, tau, v0, rot);
+                    else
+                        scatter_particle(P, out, s2, m, v[0], v[1], v[2], C, tau, v0);
                     nw = near_wall(P, nx, ny, nz);
                 } else {
                     escaped += 1;
@@ -908,70 +924,13 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
             }
         }
     }
-    if (use_mlp && lane == 0 && mlp_tried) {
+    if (MLP && lane == 0 && mlp_tried) {
         atomicAdd(B.mlp_stats, mlp_tried);
         atomicAdd(B.mlp_stats + 1, mlp_taken);
     }
-
-    // Phase C: exact tile of the new stencils.
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min_i(lo[a]);
-        hi[a] = wave_max_i(hi[a]);
-    }
-    if (lane == 0)
-        for (int a = 0; a < 3; ++a) {
-            sbox[wave][a] = lo[a];
-            sbox[wave][3 + a] = hi[a];
-        }
     __syncthreads();
-    if (tid == 0) {
-        int L[3], H[3];
-        for (int a = 0; a < 3; ++a) {
-            L[a] = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
-            H[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
-        }
-        bool ok = L[0] <= H[0] && L[1] <= H[1] && L[2] <= H[2];
-        int ssz = 1;
-        for (int a = 0; a < 3; ++a) {
-            sorg[a] = ok ? L[a] : 0;
-            sdim[a] = ok ? H[a] - L[a] + 3 : 0;
-            ssz *= sdim[a];
-        }
-        stiled = ok && ssz <= SCAP;
-    }
-    __syncthreads();
-    const int so[3] = {sorg[0], sorg[1], sorg[2]};
-    const int sd[3] = {sdim[0], sdim[1], sdim[2]};
-    const bool tiled2 = stiled != 0;
-    const int ssz = sd[0] * sd[1] * sd[2];
-    if (tiled2)
-        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0.0;
-    __syncthreads();
-
-    // Phase D: deposit the updated particles (each thread re-reads its own writes).
-    const int rot = lane & 7;
-#pragma unroll 1
-    for (int r = 0; r < R; ++r) {
-        if (nbase[r] < 0) continue;
-        int p = chunk0 + r * kBlock + tid;
-        float x = dst[PX * cap + p], y = dst[PY * cap + p], z = dst[PZ * cap + p];
-        Stencil s2;
-        make_stencil(P, x, y, z, s2);
-        float v[3] = {dst[VX * cap + p], dst[VY * cap + p], dst[VZ * cap + p]};
-        float C[9], tau[6];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) C[i] = dst[(C00 + i) * cap + p];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) tau[i] = dst[(TXX + i) * cap + p];
-        float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
-        if (tiled2)
-            scatter_lds<SCAP>(ss, so, sd, P, s2, m, v, C, tau, v0, rot);
-        else
-            scatter_particle(P, out, s2, m, v[0], v[1], v[2], C, tau, v0);
-    }
-    __syncthreads();
-    if (tiled2) {
+    // Flush the tile: one global atomic per touched node and field.
+    if (stiled) {
         for (int q = tid; q < ssz; q += kBlock) {
             double m = ss[q];
             if (m == 0.0) continue;
@@ -1024,90 +983,12 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
 
 // ------------------------------------------------------------ reductions and commit
 
-// Fixed-order reduction of one instance's partials into its step status.
-__global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
-    __shared__ double sd[256];
-    __shared__ float sf[256 * 2];
-    __shared__ unsigned long long su[256];
-    __shared__ int si[256 * 2];
-    const int inst = blockIdx.x, tid = threadIdx.x;
-    InstState& s = B.is[inst];
-    if (!s.active) return;
-    const InstParams& ip = B.ip[inst];
-    const int g0 = ip.boff / 4, g1 = (ip.boff + ip.nblk) / 4;
-    double acc[kGridVals + 1];
-    for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
-    int bad = 0;
-    for (int q = g0 + tid; q < g1; q += 256) {
-#pragma unroll
-        for (int i = 0; i < kGridVals; ++i) acc[i] += B.gpart[(size_t)i * B.ngwg + q];
-        bad += B.gbad[q];
-    }
-    float smax = 0.0f, dmax = 0.0f;
-    unsigned long long f = ~0ull;
-    int esc = 0;
-    for (int q = s.w_off + tid; q < s.w_off + s.w_cnt; q += 256) {
-        smax = fmaxf(smax, B.pmax[2 * q]);
-        dmax = fmaxf(dmax, B.pmax[2 * q + 1]);
-        acc[kGridVals] += B.pdiss[q];
-        f = B.pfail[q] < f ? B.pfail[q] : f;
-        esc += B.pesc[q];
-    }
-    double tot[kGridVals + 1];
-    for (int i = 0; i <= kGridVals; ++i) {
-        double v = wave_sum(acc[i]);
-        if ((tid & 63) == 0) sd[tid >> 6] = v;
-        __syncthreads();
-        tot[i] = sd[0] + sd[1] + sd[2] + sd[3];
-        __syncthreads();
-    }
-    sf[tid * 2] = smax;
-    sf[tid * 2 + 1] = dmax;
-    su[tid] = f;
-    si[tid * 2] = bad;
-    si[tid * 2 + 1] = esc;
-    __syncthreads();
-    for (int stride = 128; stride > 0; stride >>= 1) {
-        if (tid < stride) {
-            sf[tid * 2] = fmaxf(sf[tid * 2], sf[(tid + stride) * 2]);
-            sf[tid * 2 + 1] = fmaxf(sf[tid * 2 + 1], sf[(tid + stride) * 2 + 1]);
-            su[tid] = su[tid + stride] < su[tid] ? su[tid + stride] : su[tid];
-            si[tid * 2] += si[(tid + stride) * 2];
-            si[tid * 2 + 1] += si[(tid + stride) * 2 + 1];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        StepStatus& st = s.st;
-        for (int a = 0; a < 3; ++a) {
-            st.wall_impulse[a] = tot[a];
-            st.traction_impulse[a] = tot[3 + a];
-            st.grid_momentum[a] = tot[9 + a];
-        }
-        st.projection = tot[6];
-        st.friction = tot[7];
-        st.traction_energy = tot[8];
-        st.dissipation = tot[kGridVals];
-        st.max_speed = sf[0];
-        st.max_disp = sf[1];
-        st.fail = su[0];
-        st.grid_nonfinite = si[0];
-        st.escaped = si[1];
-        int pb = 0;
-        for (int k = 0; k < ip.npel; ++k) pb |= B.ptrial[ip.pel_off + k].bad;
-        st.pellet_bad = pb;
-    }
-}
-
-// Accept or reject every running instance's trial (solver.rs:444-467, 593-611,
-// 980-998): failures in the order the reference checks them, then the commit and
-// the ledger sums in the reference's order.
-__global__ void commit_kernel(BatchDev B) {
-    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
-    if (inst >= B.ninst) return;
-    InstState& s = B.is[inst];
-    if (!s.active) return;
-    const InstParams& ip = B.ip[inst];
+// Accept or reject one instance's trial (solver.rs:444-467, 593-611, 980-998):
+// failures in the order the reference checks them, then the commit and the
+// ledger sums.  `pel` holds the instance's pellet sums: total mass, wall impulse
+// (3), coupling impulse (3) and angular impulse (3), wall work, coupling grid
+// energy and the pellets' kinetic-energy change from the coupling.
+__device__ inline void commit_instance(InstState& s, const InstParams& ip, const double* pel, double pel_vmax) {
     const StepStatus& st = s.st;
     double* L = s.ledger;
     int code = 0;
@@ -1122,6 +1003,7 @@ __global__ void commit_kernel(BatchDev B) {
     else if ((double)st.max_disp > 1.0)
         code = F_DISPLACEMENT;
     s.last_failure = code;
+    s.refsort = 0;
     const bool trial_only = s.trial_dt > 0.0;
     if (code) {
         if (trial_only) {
@@ -1148,27 +1030,18 @@ __global__ void commit_kernel(BatchDev B) {
     L[22] += st.friction;
     L[23] += st.traction_energy;
     for (int a = 0; a < 3; ++a) L[6 + a] += ip.gravity[a] * (ip.particle_mass * dt);
-    // pellets, solver.rs:600-609
-    for (int k = 0; k < ip.npel; ++k) {
-        const int pi = ip.pel_off + k;
-        PelletState& p = B.pel[pi];
-        const PelletTrial& t = B.ptrial[pi];
-        const PelletAccum& a = B.pacc[pi];
+    // pellets, solver.rs:600-609 (summed over the pellets first)
+    if (ip.npel > 0) {
         for (int c = 0; c < 3; ++c) {
-            L[6 + c] += ip.gravity[c] * (p.mass * dt);
-            L[12 + c] += t.wall_impulse[c];
-            L[15 + c] += a.impulse[c];
-            L[18 + c] += a.angular[c];
+            L[6 + c] += ip.gravity[c] * (pel[0] * dt);
+            L[12 + c] += pel[1 + c];
+            L[15 + c] += pel[4 + c];
+            L[18 + c] += pel[7 + c];
         }
-        L[24] += t.wall_work;
-        L[25] += a.grid_energy;
-        L[26] += t.energy;
-        for (int c = 0; c < 3; ++c) {
-            p.pos[c] = t.pos[c];
-            p.vel[c] = t.vel[c];
-            p.L[c] = t.L[c];
-        }
-        for (int c = 0; c < 4; ++c) p.q[c] = t.q[c];
+        L[24] += pel[10];
+        L[25] += pel[11];
+        L[26] += pel[12];
+        s.pel_vmax = pel_vmax;
     }
     s.max_speed = (double)st.max_speed;
     if (st.escaped) {
@@ -1188,30 +1061,149 @@ __global__ void commit_kernel(BatchDev B) {
     s.max_dt_used = fmax(s.max_dt_used, dt);
     s.dt_scale = fmin(s.dt_scale * kDtRecovery, 1.0);
     s.budget -= 1;
+    // The next trial's G2P2G (or the host, if no trial follows) writes the order keys.
     if (s.step_count % kSortInterval == 0) s.refsort = 1;
 }
 
-// Reference order key (clamped cell index, z fastest, then id; solver.rs:501-517).
-__device__ inline uint64_t ref_cell(const Params& P, const float* slab, size_t cap, int p) {
-    uint64_t c[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float sc = floorf(slab[(PX + a) * cap + p] / P.dx);
-        sc = isnan(sc) ? 0.0f : fmaxf(sc, 0.0f);
-        c[a] = sc >= (float)P.cells[a] ? (uint64_t)P.cells[a] : (uint64_t)sc;
-    }
-    return (c[0] * (uint64_t)(P.cells[1] + 1) + c[1]) * (uint64_t)(P.cells[2] + 1) + c[2];
+constexpr int kPelSums = 13;
+
+// Block-wide fixed-order sum of one double (every thread gets the total).
+__device__ inline double block_total(double v, double* sd) {
+    const int tid = threadIdx.x;
+    v = wave_sum(v);
+    if ((tid & 63) == 0) sd[tid >> 6] = v;
+    __syncthreads();
+    double t = sd[0] + sd[1] + sd[2] + sd[3];
+    __syncthreads();
+    return t;
 }
 
-// Order keys at the 32-step cadence, for the instances whose accepted step hit it.
+// One workgroup per running instance: fixed-order reduction of its grid and
+// particle partials and of its pellets' step sums, then the accept/reject.
+__global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
+    __shared__ double sd[4];
+    __shared__ float sf[256 * 2];
+    __shared__ unsigned long long su[256];
+    __shared__ int si[256 * 3];
+    const int inst = blockIdx.x, tid = threadIdx.x;
+    InstState& s = B.is[inst];
+    if (!s.active) return;
+    const InstParams& ip = B.ip[inst];
+    const int g0 = ip.boff / 4, g1 = (ip.boff + ip.nblk) / 4;
+    double acc[kGridVals + 1];
+    for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
+    int bad = 0;
+    for (int q = g0 + tid; q < g1; q += 256) {
+#pragma unroll
+        for (int i = 0; i < kGridVals; ++i) acc[i] += B.gpart[(size_t)i * B.ngwg + q];
+        bad += B.gbad[q];
+    }
+    float smax = 0.0f, dmax = 0.0f;
+    unsigned long long f = ~0ull;
+    int esc = 0;
+    for (int q = s.w_off + tid; q < s.w_off + s.w_cnt; q += 256) {
+        smax = fmaxf(smax, B.pmax[2 * q]);
+        dmax = fmaxf(dmax, B.pmax[2 * q + 1]);
+        acc[kGridVals] += B.pdiss[q];
+        f = B.pfail[q] < f ? B.pfail[q] : f;
+        esc += B.pesc[q];
+    }
+    double tot[kGridVals + 1];
+    for (int i = 0; i <= kGridVals; ++i) tot[i] = block_total(acc[i], sd);
+    // Pellet sums (the instance's pellets, in parallel).
+    double pel[kPelSums];
+    for (int i = 0; i < kPelSums; ++i) pel[i] = 0.0;
+    double vmax = 0.0;
+    int pbad = 0;
+    for (int k = tid; k < ip.npel; k += 256) {
+        const int pi = ip.pel_off + k;
+        const PelletTrial& t = B.ptrial[pi];
+        const PelletAccum& a = B.pacc[pi];
+        pel[0] += B.pel[pi].mass;
+        for (int c = 0; c < 3; ++c) {
+            pel[1 + c] += t.wall_impulse[c];
+            pel[4 + c] += a.impulse[c];
+            pel[7 + c] += a.angular[c];
+        }
+        pel[10] += t.wall_work;
+        pel[11] += a.grid_energy;
+        pel[12] += t.energy;
+        pbad |= t.bad;
+        vmax = fmax(vmax, sqrt(t.vel[0] * t.vel[0] + t.vel[1] * t.vel[1] + t.vel[2] * t.vel[2]));
+    }
+    if (ip.npel > 0)
+        for (int i = 0; i < kPelSums; ++i) pel[i] = block_total(pel[i], sd);
+    sf[tid * 2] = smax;
+    sf[tid * 2 + 1] = dmax;
+    su[tid] = f;
+    si[tid * 3] = bad;
+    si[tid * 3 + 1] = esc;
+    si[tid * 3 + 2] = pbad;
+    __shared__ double sv[256];
+    sv[tid] = vmax;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (tid < stride) {
+            sf[tid * 2] = fmaxf(sf[tid * 2], sf[(tid + stride) * 2]);
+            sf[tid * 2 + 1] = fmaxf(sf[tid * 2 + 1], sf[(tid + stride) * 2 + 1]);
+            su[tid] = su[tid + stride] < su[tid] ? su[tid + stride] : su[tid];
+            si[tid * 3] += si[(tid + stride) * 3];
+            si[tid * 3 + 1] += si[(tid + stride) * 3 + 1];
+            si[tid * 3 + 2] |= si[(tid + stride) * 3 + 2];
+            sv[tid] = fmax(sv[tid], sv[tid + stride]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        StepStatus& st = s.st;
+        for (int a = 0; a < 3; ++a) {
+            st.wall_impulse[a] = tot[a];
+            st.traction_impulse[a] = tot[3 + a];
+            st.grid_momentum[a] = tot[9 + a];
+        }
+        st.projection = tot[6];
+        st.friction = tot[7];
+        st.traction_energy = tot[8];
+        st.dissipation = tot[kGridVals];
+        st.max_speed = sf[0];
+        st.max_disp = sf[1];
+        st.fail = su[0];
+        st.grid_nonfinite = si[0];
+        st.escaped = si[1];
+        st.pellet_bad = si[2];
+        commit_instance(s, ip, pel, sv[0]);
+    }
+}
+
+// Commit every accepted instance's trial pellets (rigid.rs:334, solver.rs:600-609).
+__global__ void pellet_commit_kernel(BatchDev B) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B.npel) return;
+    PelletState& p = B.pel[k];
+    if (p.inst < 0) return;
+    const InstState& s = B.is[p.inst];
+    if (!s.active || s.last_failure != 0) return;
+    const PelletTrial& t = B.ptrial[k];
+    for (int c = 0; c < 3; ++c) {
+        p.pos[c] = t.pos[c];
+        p.vel[c] = t.vel[c];
+        p.L[c] = t.L[c];
+    }
+    for (int c = 0; c < 4; ++c) p.q[c] = t.q[c];
+}
+
+// Order keys at the 32-step cadence, for the instances whose accepted step hit
+// it and that took no trial step since.
 __global__ __launch_bounds__(kBlock) void refkey_kernel(BatchDev B) {
     const int4 item = B.work[blockIdx.x];
     const InstState& s = B.is[item.x];
     if (!s.refsort) return;
     const Params& P = B.ip[item.x].P;
     const float* slab = B.slab[s.cur];
+    const size_t cap = B.cap;
     for (int p = item.y + threadIdx.x; p < item.y + item.z; p += kBlock)
-        B.refkey[p] = (ref_cell(P, slab, B.cap, p) << 32) | (uint64_t)B.ids[p];
+        B.refkey[p] = (ref_cell(P, slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]) << 32) |
+                      (uint64_t)B.ids[p];
 }
 
 // ------------------------------------------------------------ setup kernels
@@ -1342,11 +1334,19 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
     grid_update_kernel<<<B.ngwg, 256, 0, s>>>(B);
     if (pellets) couple_kernel<<<B.ngwg * 4, 64, 0, s>>>(B);
     if (g2p2g_start) (void)hipEventRecord(g2p2g_start, s);
-    if (B.nwork > 0) g2p2g_kernel<kChunk, kGatherCap, kScatterCap><<<B.nwork, kBlock, 0, s>>>(B);
+    if (B.nwork > 0) {
+        if (B.mlp_fwd)
+            g2p2g_kernel<kChunk, kGatherCap, kScatterCap, true><<<B.nwork, kBlock, 0, s>>>(B);
+        else
+            g2p2g_kernel<kChunk, kGatherCap, kScatterCap, false><<<B.nwork, kBlock, 0, s>>>(B);
+    }
     if (g2p2g_stop) (void)hipEventRecord(g2p2g_stop, s);
     if (pellets) pellet_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
     finalize_kernel<<<B.ninst, 256, 0, s>>>(B);
-    commit_kernel<<<(B.ninst + 63) / 64, 64, 0, s>>>(B);
+    if (pellets) pellet_commit_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
+}
+
+void launch_refkeys(hipStream_t s, const BatchDev& B) {
     if (B.nwork > 0) refkey_kernel<<<B.nwork, kBlock, 0, s>>>(B);
 }
 
