@@ -108,18 +108,19 @@ def read_traffic():
     return None
 
 
-def kernel_profile(sim, steps):
-    """Average device time of the fused G2P2G kernel from HIP events around every
-    launch on the engine's stream (stream mode for this pass)."""
-    shape = sim.launch()
+def timed_profiled(sim, steps):
+    """Time `steps` accepted steps; HIP events around every fused G2P2G launch of
+    this timed region, on the engine's stream, give the kernel's average duration.
+    Event recording needs stream launches, so the timed region runs in stream mode."""
     sim.set_launch(use_graph=0)
     sim.enable_profiling(True)
-    sim.advance_steps(steps)
+    t0 = time.perf_counter()
+    accepted = sim.advance_steps(steps)
+    elapsed = time.perf_counter() - t0
     avg_ms, launches = sim.g2p2g_profile()
     sim.enable_profiling(False)
-    sim.set_launch(use_graph=shape.use_graph)
-    log(f"g2p2g avg {avg_ms:.4f} ms over {launches} launches")
-    return avg_ms
+    log(f"timed {accepted} steps in {elapsed:.3f} s; g2p2g avg {avg_ms:.4f} ms over {launches} launches")
+    return accepted, elapsed, avg_ms
 
 
 def roofline(bytes_per_particle, n, avg_ms, traffic=None):
@@ -137,14 +138,6 @@ def line(metric, value, unit, steps, warmup, elapsed, dtype, config, rl=None, cp
             "cpu_baseline": cpu}
 
 
-def timed(sim, steps):
-    t0 = time.perf_counter()
-    accepted = sim.advance_steps(steps)
-    elapsed = time.perf_counter() - t0
-    log(f"timed {accepted} steps in {elapsed:.3f} s")
-    return accepted, elapsed
-
-
 # ---------------------------------------------------------------- headline
 
 def run_headline(args):
@@ -160,8 +153,7 @@ def run_headline(args):
     if not args.no_autotune:
         chosen = sim.autotune(steps=16)
         log(f"autotuned launch shape: {chosen.as_dict()}")
-    avg_ms = kernel_profile(sim, args.profile_steps)
-    accepted, elapsed = timed(sim, args.steps)
+    accepted, elapsed, avg_ms = timed_profiled(sim, args.steps)
     h = sim.history()
     cpu = None if args.no_cpu_baseline else cpu_baseline(size, dx, domain, seconds=args.cpu_seconds)
     return line(METRIC, n * accepted / elapsed, "particle-substeps/s", accepted, args.warmup, elapsed, "f32",
@@ -189,8 +181,7 @@ def run_liquid(args):
     sim.advance_steps(args.warmup)
     if not args.no_autotune:
         log(f"autotuned launch shape: {sim.autotune(steps=8).as_dict()}")
-    avg_ms = kernel_profile(sim, args.profile_steps)
-    accepted, elapsed = timed(sim, args.steps)
+    accepted, elapsed, avg_ms = timed_profiled(sim, args.steps)
     h = sim.history()
     return line("particle-substeps/s, dam break 6.3M particles, Tait gamma 7", n * accepted / elapsed,
                 "particle-substeps/s", accepted, args.warmup, elapsed, "f32",
@@ -235,8 +226,7 @@ def run_pellets(args):
     sim.advance_steps(args.warmup)
     if not args.no_autotune:
         log(f"autotuned launch shape: {sim.autotune(steps=8).as_dict()}")
-    avg_ms = kernel_profile(sim, args.profile_steps)
-    accepted, elapsed = timed(sim, args.steps)
+    accepted, elapsed, avg_ms = timed_profiled(sim, args.steps)
     h = sim.history()
     led = sim.ledger()
     return line("particle-substeps/s, paste bed 1.5M particles + 512 coupled pellets", n * accepted / elapsed,
@@ -293,8 +283,7 @@ def run_surrogate(args):
         sim.advance_steps(args.warmup)
         if name == "surrogate":
             learned.set_surrogate(net, tolerance=1e-4, polish=2)  # count the timed run only
-        avg_ms = kernel_profile(sim, args.profile_steps)
-        accepted, elapsed = timed(sim, args.steps)
+        accepted, elapsed, avg_ms = timed_profiled(sim, args.steps)
         out[name] = {"value": n * accepted / elapsed, "ms_per_step": 1e3 * elapsed / accepted, "g2p2g_ms": avg_ms}
     considered, accepted = learned.surrogate_stats()
     fallback = 1.0 - accepted / max(considered, 1)
@@ -411,7 +400,6 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--oracle-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -267,7 +267,7 @@ int mpm_batch_advance(mpm_engine* engine, const double* targets, uint64_t max_st
 typedef struct {
     int steps_per_sync;  /* trial steps enqueued between host synchronisations (1..256) */
     int use_graph;       /* 1: replay the steps as one captured HIP graph */
-    int merge_blocks;    /* cell blocks per G2P2G work item along z (1..4) */
+    int merge_blocks;    /* cell blocks per G2P2G work item along z (1 or 2) */
     int traction_wgs;    /* workgroups of the wall-traction kernel */
 } mpm_launch;
 int mpm_get_launch(const mpm_engine* engine, mpm_launch* out);

@@ -997,7 +997,7 @@ struct mpm_engine {
         std::vector<mpm_launch> cands;
         for (int k : {8, 32})
             for (int g : {0, 1})
-                for (int m : {1, 2, 4}) cands.push_back(mpm_launch{k, g, m, launch.traction_wgs});
+                for (int m : {1, 2}) cands.push_back(mpm_launch{k, g, m, launch.traction_wgs});
         mpm_launch best = launch;
         double best_ms = std::numeric_limits<double>::infinity();
         for (const mpm_launch& c : cands) {
@@ -1786,8 +1786,9 @@ int mpm_get_launch(const mpm_engine* e, mpm_launch* out) {
 
 int mpm_set_launch(mpm_engine* e, const mpm_launch* l) {
     if (!e || !l) return MPM_ERR_INVALID;
+    // More than two cell blocks per work item would not fit the G2P2G scatter tile.
     if (l->steps_per_sync < 1 || l->steps_per_sync > kMaxStepsPerSync || l->merge_blocks < 1 ||
-        l->merge_blocks > 4 || l->traction_wgs < 1 || l->traction_wgs > 65536)
+        l->merge_blocks > 2 || l->traction_wgs < 1 || l->traction_wgs > 65536)
         return e->fail(MPM_ERR_INVALID, "launch shape out of range");
     bool resort = l->merge_blocks != e->launch.merge_blocks;
     e->launch = *l;

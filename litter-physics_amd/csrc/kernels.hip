@@ -619,15 +619,12 @@ __global__ void pellet_kernel(BatchDev B) {
 // moves a particle by at most one cell); the tile is flushed with one global
 // atomic per touched node and field.  Particles that leave the tile, and work
 // items whose tiles exceed the LDS budget, deposit with global atomics.
-// MLP: the return-map surrogate is on (its weights get LDS of their own).
+// MLP: the return-map surrogate is on.
 template <int CHUNK, int GCAP, int SCAP, bool MLP>
-__global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
+__global__ __launch_bounds__(kBlock, MLP ? 1 : 2) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
-    constexpr int kMlpWeights = kFwdFrags * kFragElems * 2 + kBiasPacked * 4;  // bytes
-    constexpr int kMlpWave = 64 * 16 * 2 + 64 * 4;                             // staging bytes per wavefront
     __shared__ float4 sg[GCAP];
     __shared__ double ss[7 * SCAP];
-    __shared__ __align__(16) unsigned char smlp[MLP ? kMlpWeights + 4 * kMlpWave : 16];
     __shared__ int sbox[4][6];
     __shared__ Tile tile;
     __shared__ int sscat;
@@ -736,15 +733,6 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
     }
     if (stiled)
         for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0.0;
-    uint16_t* mlp_w = reinterpret_cast<uint16_t*>(smlp);
-    float* mlp_b = reinterpret_cast<float*>(smlp + kFwdFrags * kFragElems * 2);
-    uint16_t* mlp_in = reinterpret_cast<uint16_t*>(smlp + kMlpWeights + wave * kMlpWave);
-    float* mlp_out = reinterpret_cast<float*>(mlp_in + 64 * 16);
-    if (MLP) {
-        for (int q = tid; q < kFwdFrags * kFragElems / 8; q += kBlock)
-            reinterpret_cast<uint4*>(mlp_w)[q] = reinterpret_cast<const uint4*>(B.mlp_fwd)[q];
-        for (int q = tid; q < kBiasPacked; q += kBlock) mlp_b[q] = B.mlp_bias[q];
-    }
     __syncthreads();
 
     // Phase B: per particle, G2P, advection, constitutive update, writes to the
@@ -804,7 +792,10 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
             }
         }
         // Surrogate return for the wavefront's yielded particles.  Everything in
-        // this block is uniform across the wavefront, so the MFMAs run with all lanes.
+        // this block is uniform across the wavefront, so the MFMAs run with all
+        // lanes; the samples reach the MFMA operand lanes by wave shuffles and the
+        // weights come from global memory (21 KB, cache resident), so the surrogate
+        // needs no LDS.
         float q = 0.0f;
         bool settled = false;
         if (MLP && P.kind == 0) {
@@ -818,21 +809,33 @@ __global__ __launch_bounds__(kBlock) void g2p2g_kernel(BatchDev B) {
 #pragma unroll
                     for (int i = 0; i < kMlpIn; ++i) f[i] = 0.0f;
                 }
-                uint16_t* row = mlp_in + lane * 16;
+                // Features as six words of two bf16 each.
+                int w[6];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) row[i] = i < kMlpIn ? bf16_bits(f[i]) : (uint16_t)0;
-                wave_lds_sync();
+                for (int j = 0; j < 6; ++j) w[j] = (int)bf16_bits(f[2 * j]) | ((int)bf16_bits(f[2 * j + 1]) << 16);
+                const int g = lane >> 4;  // operand lanes of group g take features 4g .. 4g + 3
+                float yhat = 0.0f;
 #pragma unroll 1
                 for (int mt = 0; mt < 4; ++mt) {
-                    const int smp = mt * 16 + (lane & 15);
-                    const bf16x4 xb = *reinterpret_cast<const bf16x4*>(mlp_in + smp * 16 + 4 * (lane >> 4));
+                    const int srcl = mt * 16 + (lane & 15);
+                    int ww[6];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) ww[j] = __shfl(w[j], srcl, 64);
+                    const int lo = g == 0 ? ww[0] : (g == 1 ? ww[2] : (g == 2 ? ww[4] : 0));
+                    const int hi = g == 0 ? ww[1] : (g == 1 ? ww[3] : (g == 2 ? ww[5] : 0));
+                    bf16x4 xb;
+                    xb[0] = (short)(lo & 0xffff);
+                    xb[1] = (short)((unsigned)lo >> 16);
+                    xb[2] = (short)(hi & 0xffff);
+                    xb[3] = (short)((unsigned)hi >> 16);
                     MlpTile tl;
-                    mlp_tile_forward<false>(mlp_w, mlp_b, xb, lane, tl);
-                    // Output 6 (the retained overstress fraction): lane group 1, element 2.
-                    if ((lane >> 4) == 1) mlp_out[smp] = tl.y[2];
+                    mlp_tile_forward<false>(B.mlp_fwd, B.mlp_bias, xb, lane, tl);
+                    // Output 6 (the retained overstress fraction) of sample mt*16 + j
+                    // sits in lane 16 + j, element 2.
+                    const float y6 = __shfl(tl.y[2], 16 + (lane & 15), 64);
+                    if ((lane >> 4) == mt) yhat = y6;
                 }
-                wave_lds_sync();
-                if (want) settled = surrogate_polish(P, dt, ts.qtr, mlp_out[lane], B.mlp_polish, q) <= B.mlp_tol;
+                if (want) settled = surrogate_polish(P, dt, ts.qtr, yhat, B.mlp_polish, q) <= B.mlp_tol;
                 mlp_tried += __popcll(mask);
                 mlp_taken += __popcll(__ballot(settled));
             }

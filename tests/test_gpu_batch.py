@@ -98,13 +98,13 @@ def test_batch_of_one_equals_simulation(engine):
 
 @pytest.mark.parametrize("use_graph", [0, 1])
 def test_launch_shapes_match_oracle(engine, use_graph):
-    # Captured-graph and stream stepping, with one or four blocks per work item,
+    # Captured-graph and stream stepping, with one or two blocks per work item,
     # both follow the oracle.
     spec = cases.slump_cfg0(max_dt=1e-3)
     osim = build_oracle(spec)
     s = spec_from_oracle(engine, osim)
     sim = engine.Simulation(s["material"], s["cfg"], s["domain"], s["spacing"], s["state"])
-    sim.set_launch(use_graph=use_graph, steps_per_sync=8, merge_blocks=4 if use_graph else 1)
+    sim.set_launch(use_graph=use_graph, steps_per_sync=8, merge_blocks=2 if use_graph else 1)
     osim.advance_to(3e-3)
     sim.advance_to(3e-3)
     assert sim.history().step_count == osim.history().step_count
