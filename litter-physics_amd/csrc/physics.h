@@ -224,13 +224,32 @@ __device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, 
 }
 
 // Trial state of update_paste (constitutive.rs:456-500) on H = F - I:
-//   b - I = H + H^T + H H^T, Hencky e = log1p(beta)/2 in the eigenbasis U of b,
-//   q_tr = sqrt(3/2) 2G |dev e|.
+//   X = b - I = H + H^T + H H^T, Hencky e = log(I + X)/2, D = dev e (lab frame,
+//   xx yy zz xy xz yz), q_tr = sqrt(3/2) 2G |D|.
+// For |X| < 0.1 the logarithm is its power series (the matrix function the
+// reference evaluates in the eigenbasis, to f32 round-off, in 3-7 terms); larger
+// strains take the cyclic Jacobi eigenbasis as the reference does.
 struct PasteTrial {
     float U[9];
     float dev[3];
+    float D[6];
     float mean, vol, dn, qtr;
+    int eig;  // 1: eigenbasis path (U, dev valid)
 };
+
+// Product of two symmetric 3x3 matrices that commute (powers of one matrix), as xx yy zz xy xz yz.
+__device__ inline void sym_mul(const float* a, const float* b, float* c) {
+    c[0] = a[0] * b[0] + a[3] * b[3] + a[4] * b[4];
+    c[1] = a[3] * b[3] + a[1] * b[1] + a[5] * b[5];
+    c[2] = a[4] * b[4] + a[5] * b[5] + a[2] * b[2];
+    c[3] = a[0] * b[3] + a[3] * b[1] + a[4] * b[5];
+    c[4] = a[0] * b[4] + a[3] * b[5] + a[4] * b[2];
+    c[5] = a[3] * b[4] + a[1] * b[5] + a[5] * b[2];
+}
+
+__device__ inline float sym_norm2(const float* a) {
+    return a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + 2.0f * (a[3] * a[3] + a[4] * a[4] + a[5] * a[5]);
+}
 
 __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& s) {
     float F[9];
@@ -246,21 +265,65 @@ __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& 
         for (int j = 0; j < 3; ++j)
             B[3 * i + j] = Ht[3 * i + j] + Ht[3 * j + i] + Ht[3 * i] * Ht[3 * j] + Ht[3 * i + 1] * Ht[3 * j + 1] +
                            Ht[3 * i + 2] * Ht[3 * j + 2];
-    int rc = eigen_sym3(B, s.U);
-    if (rc) return rc;
-    float beta[3] = {B[0], B[4], B[8]};
-    float e[3];
+    const float X[6] = {B[0], B[4], B[8], B[1], B[2], B[5]};
+    const float rho2 = sym_norm2(X);
+    if (!isfinite(rho2)) return F_NONFINITE;
+    float e[6];
+    if (rho2 < 0.01f) {
+        // log(I + X) = X - X^2/2 + X^3/3 - ...; every eigenvalue of b lies in (0.9, 1.1),
+        // so no stretch can be degenerate.
+        float pw[6], lg[6];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float st = sqrtf(fmaxf(1.0f + beta[a], 0.0f));
-        if (!isfinite(st) || st < 1e-6f) return F_STRETCH;
-        e[a] = 0.5f * log1pf(beta[a]);
+        for (int i = 0; i < 6; ++i) pw[i] = lg[i] = X[i];
+        const float tol2 = 1e-15f * rho2;
+        float sign = -1.0f;
+        for (int k = 2; k <= 12; ++k) {
+            float nx[6];
+            sym_mul(pw, X, nx);
+            const float c = sign / (float)k;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                pw[i] = nx[i];
+                lg[i] += c * nx[i];
+            }
+            sign = -sign;
+            if (sym_norm2(pw) * c * c <= tol2) break;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) e[i] = 0.5f * lg[i];
+        s.eig = 0;
+    } else {
+        int rc = eigen_sym3(B, s.U);
+        if (rc) return rc;
+        float beta[3] = {B[0], B[4], B[8]};
+        float ev[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float st = sqrtf(fmaxf(1.0f + beta[a], 0.0f));
+            if (!isfinite(st) || st < 1e-6f) return F_STRETCH;
+            ev[a] = 0.5f * log1pf(beta[a]);
+        }
+        const float* U = s.U;
+        const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            int i = ii[t], j = jj[t];
+            e[t] = U[3 * i] * ev[0] * U[3 * j] + U[3 * i + 1] * ev[1] * U[3 * j + 1] + U[3 * i + 2] * ev[2] * U[3 * j + 2];
+        }
+        const float m = (ev[0] + ev[1] + ev[2]) / 3.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) s.dev[a] = ev[a] - m;
+        s.eig = 1;
     }
     s.vol = e[0] + e[1] + e[2];
     s.mean = s.vol / 3.0f;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) s.dev[a] = e[a] - s.mean;
-    s.dn = sqrtf(s.dev[0] * s.dev[0] + s.dev[1] * s.dev[1] + s.dev[2] * s.dev[2]);
+    s.D[0] = e[0] - s.mean;
+    s.D[1] = e[1] - s.mean;
+    s.D[2] = e[2] - s.mean;
+    s.D[3] = e[3];
+    s.D[4] = e[4];
+    s.D[5] = e[5];
+    s.dn = sqrtf(sym_norm2(s.D));
     s.qtr = 1.2247448713915890f * 2.0f * P.G * s.dn;
     return F_OK;
 }
@@ -269,38 +332,63 @@ __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& 
 __device__ inline bool paste_yields(const Params& P, const PasteTrial& s) { return s.qtr > P.ys && s.dn > 0.0f; }
 
 // Rest of update_paste (constitutive.rs:508-531) for the returned equivalent
-// stress q:  F' = F + U diag(expm1((alpha - 1) dev e)) U^T F,
-// tau = U diag(2G (alpha dev e + mean) + lambda tr e) U^T, alpha = q / q_tr.
+// stress q, alpha = q / q_tr:
+//   F' = F + M F,  M = expm1((alpha - 1) D)   (the matrix function),
+//   tau = 2G alpha D + (2G mean + lambda tr e) I.
 __device__ inline int paste_finish(const Params& P, const PasteTrial& s, const float* Ht, float q, bool yielded,
                                    float* Hn, float* tau) {
-    float alpha = yielded ? q / s.qtr : 1.0f;
-    float rm1[3], pr[3];
-    float lv = P.lam * s.vol;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        rm1[a] = expm1f((alpha - 1.0f) * s.dev[a]);
-        pr[a] = 2.0f * P.G * (alpha * s.dev[a] + s.mean) + lv;
-    }
-    const float* U = s.U;
-    // M = U diag(rm1) U^T, H' = H + M (I + H)
+    const float alpha = yielded ? q / s.qtr : 1.0f;
     float M[9];
+    if (s.eig) {
+        float rm1[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int a = 0; a < 3; ++a) rm1[a] = expm1f((alpha - 1.0f) * s.dev[a]);
+        const float* U = s.U;
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            M[3 * i + j] = U[3 * i] * rm1[0] * U[3 * j] + U[3 * i + 1] * rm1[1] * U[3 * j + 1] +
-                           U[3 * i + 2] * rm1[2] * U[3 * j + 2];
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                M[3 * i + j] = U[3 * i] * rm1[0] * U[3 * j] + U[3 * i + 1] * rm1[1] * U[3 * j + 1] +
+                               U[3 * i + 2] * rm1[2] * U[3 * j + 2];
+    } else {
+        // expm1(Y) = Y + Y^2/2! + Y^3/3! + ..., |Y| <= |D| < 0.05
+        float Y[6], pw[6], ex[6];
+        const float am1 = alpha - 1.0f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Y[i] = pw[i] = ex[i] = am1 * s.D[i];
+        const float tol2 = 1e-15f * sym_norm2(Y);
+        float fact = 1.0f;
+        for (int k = 2; k <= 10 && tol2 > 0.0f; ++k) {
+            float nx[6];
+            sym_mul(pw, Y, nx);
+            fact *= (float)k;
+            const float c = 1.0f / fact;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                pw[i] = nx[i];
+                ex[i] += c * nx[i];
+            }
+            if (sym_norm2(pw) * c * c <= tol2) break;
+        }
+        M[0] = ex[0];
+        M[4] = ex[1];
+        M[8] = ex[2];
+        M[1] = M[3] = ex[3];
+        M[2] = M[6] = ex[4];
+        M[5] = M[7] = ex[5];
+    }
+    // H' = H + M (I + H)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             Hn[3 * i + j] = Ht[3 * i + j] + M[3 * i + j] + M[3 * i] * Ht[j] + M[3 * i + 1] * Ht[3 + j] +
                             M[3 * i + 2] * Ht[6 + j];
-    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+    const float g2a = 2.0f * P.G * alpha;
+    const float iso = 2.0f * P.G * s.mean + P.lam * s.vol;
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
-        int i = ii[t], j = jj[t];
-        tau[t] = U[3 * i] * pr[0] * U[3 * j] + U[3 * i + 1] * pr[1] * U[3 * j + 1] + U[3 * i + 2] * pr[2] * U[3 * j + 2];
+        tau[t] = g2a * s.D[t] + (t < 3 ? iso : 0.0f);
         if (!isfinite(tau[t])) return F_NONFINITE;
     }
     return F_OK;
@@ -339,15 +427,9 @@ __device__ inline int paste_update(const Params& P, float dt, const float* Ht, f
 // With r, K / (3 G dt) and n the return is fully determined (the dimensionless
 // overstress y solves y + beta y^(1/n) = 1, beta = (1/r) (r / (K / 3G dt))^(1/n)).
 __device__ inline void surrogate_features(const Params& P, float dt, const PasteTrial& s, const float* C, float* f) {
-    const float inv_q = 1.0f / s.qtr;
-    float d2[3] = {2.0f * P.G * s.dev[0] * inv_q, 2.0f * P.G * s.dev[1] * inv_q, 2.0f * P.G * s.dev[2] * inv_q};
-    const float* U = s.U;
-    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+    const float sc = 2.0f * P.G / s.qtr;
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        int i = ii[t], j = jj[t];
-        f[t] = U[3 * i] * d2[0] * U[3 * j] + U[3 * i + 1] * d2[1] * U[3 * j + 1] + U[3 * i + 2] * d2[2] * U[3 * j + 2];
-    }
+    for (int t = 0; t < 6; ++t) f[t] = sc * s.D[t];
     const float s3 = 3.0f * P.G * dt;
     f[6] = log10f(fmaxf((s.qtr - P.ys) / s3, 1e-30f)) * 0.25f;
     float tr = (C[0] + C[4] + C[8]) / 3.0f;
@@ -357,7 +439,7 @@ __device__ inline void surrogate_features(const Params& P, float dt, const Paste
     f[7] = log1pf(1e4f * dt * ee) * 0.2f;
     float p = -(2.0f * P.G / 3.0f + P.lam) * s.vol;
     f[8] = asinhf(p / P.ys) * 0.25f;
-    f[9] = P.ys * inv_q;
+    f[9] = P.ys / s.qtr;
     f[10] = log10f(P.K / s3) * 0.25f;
     f[11] = 1.0f / P.inv_n;
 }
