@@ -621,7 +621,7 @@ __global__ void pellet_kernel(BatchDev B) {
 // items whose tiles exceed the LDS budget, deposit with global atomics.
 // MLP: the return-map surrogate is on.
 template <int CHUNK, int GCAP, int SCAP, bool MLP>
-__global__ __launch_bounds__(kBlock, MLP ? 1 : 2) void g2p2g_kernel(BatchDev B) {
+__global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
     __shared__ float4 sg[GCAP];
     __shared__ double ss[7 * SCAP];
@@ -814,10 +814,18 @@ __global__ __launch_bounds__(kBlock, MLP ? 1 : 2) void g2p2g_kernel(BatchDev B) 
 #pragma unroll
                 for (int j = 0; j < 6; ++j) w[j] = (int)bf16_bits(f[2 * j]) | ((int)bf16_bits(f[2 * j + 1]) << 16);
                 const int g = lane >> 4;  // operand lanes of group g take features 4g .. 4g + 3
+                // The wanted samples are packed: sample r (the r-th wanted lane)
+                // goes to column r % 16 of tile r / 16, so only ceil(count / 16)
+                // tiles run.  Columns past the count read any lane (finite
+                // features) and are ignored.
+                const int count = __popcll(mask);
+                const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
                 float yhat = 0.0f;
 #pragma unroll 1
-                for (int mt = 0; mt < 4; ++mt) {
-                    const int srcl = mt * 16 + (lane & 15);
+                for (int mt = 0; mt * 16 < count; ++mt) {
+                    const int r = mt * 16 + (lane & 15);
+                    const int srcl = r < count ? nth_set_bit(mask, r) : lane;
                     int ww[6];
 #pragma unroll
                     for (int j = 0; j < 6; ++j) ww[j] = __shfl(w[j], srcl, 64);
@@ -830,10 +838,10 @@ __global__ __launch_bounds__(kBlock, MLP ? 1 : 2) void g2p2g_kernel(BatchDev B) 
                     xb[3] = (short)((unsigned)hi >> 16);
                     MlpTile tl;
                     mlp_tile_forward<false>(B.mlp_fwd, B.mlp_bias, xb, lane, tl);
-                    // Output 6 (the retained overstress fraction) of sample mt*16 + j
+                    // Output 6 (the retained overstress fraction) of column j
                     // sits in lane 16 + j, element 2.
-                    const float y6 = __shfl(tl.y[2], 16 + (lane & 15), 64);
-                    if ((lane >> 4) == mt) yhat = y6;
+                    const float y6 = __shfl(tl.y[2], 16 + (rank & 15), 64);
+                    if (want && (rank >> 4) == mt) yhat = y6;
                 }
                 if (want) settled = surrogate_polish(P, dt, ts.qtr, yhat, B.mlp_polish, q) <= B.mlp_tol;
                 mlp_tried += __popcll(mask);

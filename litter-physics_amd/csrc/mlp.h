@@ -51,20 +51,44 @@ __host__ __device__ inline float bf16_value(uint16_t b) {
     return f;
 }
 
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid(z) = 1 / (1 + e^-z) with the hardware reciprocal (1 ulp): the result
+// is rounded to bf16 (8 bits) before use, so a correctly rounded division buys
+// nothing.
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid(z); }
 // d silu / dz = s (1 + z (1 - s)), s = sigmoid(z)
 __device__ inline float silu_grad(float z) {
-    float s = 1.0f / (1.0f + __expf(-z));
+    float s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// Four f32 to bf16, round to nearest even (v_cvt_pk_bf16_f32, the same rounding as bf16_bits).
 __device__ inline bf16x4 pack4(f32x4 v) {
+    const bf16x2v lo = __builtin_convertvector(f32x2{v[0], v[1]}, bf16x2v);
+    const bf16x2v hi = __builtin_convertvector(f32x2{v[2], v[3]}, bf16x2v);
     bf16x4 r;
-    r[0] = (short)bf16_bits(v[0]);
-    r[1] = (short)bf16_bits(v[1]);
-    r[2] = (short)bf16_bits(v[2]);
-    r[3] = (short)bf16_bits(v[3]);
+    __builtin_memcpy(&r, &lo, 4);
+    __builtin_memcpy(reinterpret_cast<char*>(&r) + 4, &hi, 4);
     return r;
+}
+
+// Position of the n-th (from 0) set bit of m; n < popcount(m).
+__host__ __device__ inline int nth_set_bit(unsigned long long m, int n) {
+    unsigned w = (unsigned)m;
+    int pos = 0, c = __builtin_popcount(w);
+    if (n >= c) n -= c, w = (unsigned)(m >> 32), pos = 32;
+    c = __builtin_popcount(w & 0xffffu);
+    if (n >= c) n -= c, w >>= 16, pos += 16;
+    c = __builtin_popcount(w & 0xffu);
+    if (n >= c) n -= c, w >>= 8, pos += 8;
+    c = __builtin_popcount(w & 0xfu);
+    if (n >= c) n -= c, w >>= 4, pos += 4;
+    c = __builtin_popcount(w & 0x3u);
+    if (n >= c) n -= c, w >>= 2, pos += 2;
+    return pos + (n >= (int)(w & 1u) ? 1 : 0);
 }
 
 // Orders this wavefront's LDS writes before other lanes of the same wavefront read them.
@@ -107,6 +131,9 @@ __device__ inline void mlp_tile_forward(const uint16_t* fwd, const float* bias, 
         if (keep_z) t.z[0][nt] = acc;
         t.h[0][nt] = pack4(f32x4{silu(acc[0]), silu(acc[1]), silu(acc[2]), silu(acc[3])});
     }
+    // The barriers keep the weight loads of later tiles from being hoisted
+    // ahead: 40 fragments in flight would take 80 registers.
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int l = 1; l < 3; ++l) {
         const int fbase = l == 1 ? kFwdFrag2 : kFwdFrag3;
@@ -118,6 +145,7 @@ __device__ inline void mlp_tile_forward(const uint16_t* fwd, const float* bias, 
             for (int kt = 0; kt < 4; ++kt) acc = mfma(frag(fwd, fbase + nt * 4 + kt, lane), t.h[l - 1][kt], acc);
             if (keep_z) t.z[l][nt] = acc;
             t.h[l][nt] = pack4(f32x4{silu(acc[0]), silu(acc[1]), silu(acc[2]), silu(acc[3])});
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     f32x4 acc = bias4(bias, kBias4, lane);

@@ -230,11 +230,8 @@ __device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, 
 // reference evaluates in the eigenbasis, to f32 round-off, in 3-7 terms); larger
 // strains take the cyclic Jacobi eigenbasis as the reference does.
 struct PasteTrial {
-    float U[9];
-    float dev[3];
     float D[6];
     float mean, vol, dn, qtr;
-    int eig;  // 1: eigenbasis path (U, dev valid)
 };
 
 // Product of two symmetric 3x3 matrices that commute (powers of one matrix), as xx yy zz xy xz yz.
@@ -291,9 +288,9 @@ __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& 
         }
 #pragma unroll
         for (int i = 0; i < 6; ++i) e[i] = 0.5f * lg[i];
-        s.eig = 0;
     } else {
-        int rc = eigen_sym3(B, s.U);
+        float U[9];
+        int rc = eigen_sym3(B, U);
         if (rc) return rc;
         float beta[3] = {B[0], B[4], B[8]};
         float ev[3];
@@ -303,17 +300,12 @@ __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& 
             if (!isfinite(st) || st < 1e-6f) return F_STRETCH;
             ev[a] = 0.5f * log1pf(beta[a]);
         }
-        const float* U = s.U;
         const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
         for (int t = 0; t < 6; ++t) {
             int i = ii[t], j = jj[t];
             e[t] = U[3 * i] * ev[0] * U[3 * j] + U[3 * i + 1] * ev[1] * U[3 * j + 1] + U[3 * i + 2] * ev[2] * U[3 * j + 2];
         }
-        const float m = (ev[0] + ev[1] + ev[2]) / 3.0f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) s.dev[a] = ev[a] - m;
-        s.eig = 1;
     }
     s.vol = e[0] + e[1] + e[2];
     s.mean = s.vol / 3.0f;
@@ -335,15 +327,26 @@ __device__ inline bool paste_yields(const Params& P, const PasteTrial& s) { retu
 // stress q, alpha = q / q_tr:
 //   F' = F + M F,  M = expm1((alpha - 1) D)   (the matrix function),
 //   tau = 2G alpha D + (2G mean + lambda tr e) I.
+// The reference applies exp in the trial eigenbasis; D shares that basis, so M
+// is the same matrix.  For |(alpha - 1) D| < 0.1 it is the power series, else
+// D is diagonalised again (cyclic Jacobi), which keeps the eigenvectors out of
+// the trial state that has to stay live across the return.
 __device__ inline int paste_finish(const Params& P, const PasteTrial& s, const float* Ht, float q, bool yielded,
                                    float* Hn, float* tau) {
     const float alpha = yielded ? q / s.qtr : 1.0f;
     float M[9];
-    if (s.eig) {
+    const float am1 = alpha - 1.0f;
+    float Y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Y[i] = am1 * s.D[i];
+    const float y2 = sym_norm2(Y);
+    if (y2 >= 0.01f) {
+        float A[9] = {Y[0], Y[3], Y[4], Y[3], Y[1], Y[5], Y[4], Y[5], Y[2]}, U[9];
+        int rc = eigen_sym3(A, U);
+        if (rc) return rc;
         float rm1[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) rm1[a] = expm1f((alpha - 1.0f) * s.dev[a]);
-        const float* U = s.U;
+        for (int a = 0; a < 3; ++a) rm1[a] = expm1f(A[4 * a]);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -351,12 +354,11 @@ __device__ inline int paste_finish(const Params& P, const PasteTrial& s, const f
                 M[3 * i + j] = U[3 * i] * rm1[0] * U[3 * j] + U[3 * i + 1] * rm1[1] * U[3 * j + 1] +
                                U[3 * i + 2] * rm1[2] * U[3 * j + 2];
     } else {
-        // expm1(Y) = Y + Y^2/2! + Y^3/3! + ..., |Y| <= |D| < 0.05
-        float Y[6], pw[6], ex[6];
-        const float am1 = alpha - 1.0f;
+        // expm1(Y) = Y + Y^2/2! + Y^3/3! + ..., |Y| < 0.1
+        float pw[6], ex[6];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) Y[i] = pw[i] = ex[i] = am1 * s.D[i];
-        const float tol2 = 1e-15f * sym_norm2(Y);
+        for (int i = 0; i < 6; ++i) pw[i] = ex[i] = Y[i];
+        const float tol2 = 1e-15f * y2;
         float fact = 1.0f;
         for (int k = 2; k <= 10 && tol2 > 0.0f; ++k) {
             float nx[6];
