@@ -326,47 +326,101 @@ def sweep_specs(E, count=256, seed=4):
     return specs
 
 
-def run_sweep(args):
-    E = engine()
-    specs = sweep_specs(E)
-    counts = np.array([len(s["state"]) for s in specs])
-    total = int(counts.sum())
-    log(f"{len(specs)} instances, {total} particles ({counts.min()}..{counts.max()} each)")
-    steps = args.steps
+def timed_batch(E, specs, args, net=None):
+    """Warm up, autotune and time `args.steps` trial steps of every instance in one batch."""
     batch = E.Batch(specs)
+    if net is not None:
+        batch.set_surrogate(net, tolerance=1e-4, polish=2)
     batch.advance(1e9, max_steps=args.warmup)
     chosen = batch.autotune(steps=8).as_dict() if not args.no_autotune else batch.launch().as_dict()
     log(f"autotuned launch shape: {chosen}")
+    if net is not None:
+        batch.set_surrogate(net, tolerance=1e-4, polish=2)  # count the timed run only
     t0 = time.perf_counter()
-    status, done_ms = batch.advance(1e9, max_steps=steps)
+    status, done_ms = batch.advance(1e9, max_steps=args.steps)
     elapsed = time.perf_counter() - t0
     stopped = int(np.sum(status == 1))  # instances that took all their steps
-    log(f"{stopped} of {len(specs)} instances took all {steps} steps")
-    value = total * steps / elapsed
-    # The same instances one at a time (a sample of 16, each its own engine),
-    # as the reference's sweep runs its jobs in sequence.
+    log(f"{stopped} of {len(specs)} instances took all {args.steps} steps in {elapsed:.3f} s")
+    return batch, elapsed, stopped, done_ms, chosen
+
+
+def sequential_sample(E, specs, args, count=16):
+    """The same instances one at a time (a sample of `count`, each its own engine,
+    plain implicit return), as the reference's sweep runs its jobs in sequence."""
     rng = np.random.default_rng(5)
-    pick = rng.choice(len(specs), 16, replace=False)
+    pick = rng.choice(len(specs), count, replace=False)
     seq_particles, seq_time = 0, 0.0
     for i in pick:
         s = specs[i]
         sim = E.Simulation(s["material"], s["cfg"], s["domain"], s["spacing"], s["state"])
         sim.advance_steps(args.warmup)
         t1 = time.perf_counter()
-        k = sim.advance_steps(steps)
+        k = sim.advance_steps(args.steps)
         seq_time += time.perf_counter() - t1
         seq_particles += len(s["state"]) * k
         del sim
-    seq_value = seq_particles / seq_time
+    return seq_particles / seq_time
+
+
+SWEEP_WORKLOAD = ("256 paste slumps, grids log-uniform 16^3..64^3 over a 100 mm cube, 40x40 mm columns 20-80 mm "
+                  "tall, yield U[20,300] Pa, K U[0.5,20], n U[0.3,1.0], seed 4, {steps} substeps each with its own "
+                  "adaptive dt, one batch")
+
+
+def run_sweep(args):
+    E = engine()
+    specs = sweep_specs(E)
+    counts = np.array([len(s["state"]) for s in specs])
+    total = int(counts.sum())
+    log(f"{len(specs)} instances, {total} particles ({counts.min()}..{counts.max()} each)")
+    batch, elapsed, stopped, done_ms, chosen = timed_batch(E, specs, args)
+    del batch
+    value = total * args.steps / elapsed
+    seq_value = sequential_sample(E, specs, args)
     return line("aggregate particle-substeps/s, 256-instance batched paste sweep", value, "particle-substeps/s",
-                steps, args.warmup, elapsed, "f32",
-                {"workload": "256 paste slumps, grids log-uniform 16^3..64^3 over a 100 mm cube, 40x40 mm columns "
-                             "20-80 mm tall, yield U[20,300] Pa, K U[0.5,20], n U[0.3,1.0], seed 4, "
-                             f"{steps} substeps each with its own adaptive dt, one batch",
+                args.steps, args.warmup, elapsed, "f32",
+                {"workload": SWEEP_WORKLOAD.format(steps=args.steps),
                  "instances": len(specs), "instances_completed": stopped, "particles": total, "launch": chosen,
                  "instance_wall_ms_p50": float(np.percentile(done_ms, 50)),
                  "instance_wall_ms_p99": float(np.percentile(done_ms, 99)),
                  "sequential_sample_value": seq_value, "speedup_vs_sequential": value / seq_value})
+
+
+def run_surrogate_sweep(args):
+    """Surrogate plus batched mode against the plain GPU mode: the 256-instance
+    sweep stepped in one batch with the learned return (exact fallback), beside
+    the same batch with the exact return and the plain engine run one instance
+    at a time.  The stress of every instance is compared with the exact batch
+    after the same number of steps."""
+    E = engine()
+    import litter_physics_amd.surrogate as S
+    net = S.load_network()
+    specs = sweep_specs(E)
+    counts = np.array([len(s["state"]) for s in specs])
+    total = int(counts.sum())
+    log(f"{len(specs)} instances, {total} particles ({counts.min()}..{counts.max()} each)")
+    plain, el_plain, _, _, _ = timed_batch(E, specs, args)
+    learned, elapsed, stopped, done_ms, chosen = timed_batch(E, specs, args, net)
+    considered, accepted = learned.surrogate_stats()
+    errs = []
+    for i in range(len(specs)):
+        a, b = plain.state(i).by_id(), learned.state(i).by_id()
+        errs.append(float(np.abs(a.kirchhoff - b.kirchhoff).max() / max(np.abs(a.kirchhoff).max(), 1e-30)))
+    del plain, learned
+    value = total * args.steps / elapsed
+    plain_value = total * args.steps / el_plain
+    seq_value = sequential_sample(E, specs, args)
+    return line("aggregate particle-substeps/s, 256-instance sweep, learned HB return in batched mode",
+                value, "particle-substeps/s", args.steps, args.warmup, elapsed, "f32",
+                {"workload": SWEEP_WORKLOAD.format(steps=args.steps) + ", MLP 12-64-64-64-7 bf16 MFMA return + 2 "
+                             "Newton corrections, residual gate 1e-4",
+                 "instances": len(specs), "instances_completed": stopped, "particles": total, "launch": chosen,
+                 "instance_wall_ms_p50": float(np.percentile(done_ms, 50)),
+                 "instance_wall_ms_p99": float(np.percentile(done_ms, 99)),
+                 "fallback_rate": 1.0 - accepted / max(considered, 1), "yielded_returns": considered,
+                 "stress_rel_diff_vs_exact_batch": {"max": max(errs), "median": float(np.median(errs))},
+                 "plain_batched_value": plain_value, "plain_sequential_sample_value": seq_value,
+                 "speedup_vs_plain_sequential": value / seq_value, "speedup_vs_plain_batched": value / plain_value})
 
 
 # ---------------------------------------------------------------- copy calibration
@@ -396,7 +450,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "liquid", "pellets", "surrogate", "sweep", "copy"])
+                    choices=["headline", "liquid", "pellets", "surrogate", "sweep", "surrogate_sweep", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true")
@@ -407,7 +461,7 @@ def main():
               "(the MLS-MPM substep is measured on a single MI355X).", file=sys.stderr)
         sys.exit(2)
     run = {"headline": run_headline, "liquid": run_liquid, "pellets": run_pellets, "surrogate": run_surrogate,
-           "sweep": run_sweep, "copy": run_copy}[args.workload]
+           "sweep": run_sweep, "surrogate_sweep": run_surrogate_sweep, "copy": run_copy}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

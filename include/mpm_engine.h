@@ -327,6 +327,46 @@ int mpm_log_returns(mpm_engine* engine, size_t capacity);
 int mpm_take_returns(mpm_engine* engine, float* inputs, float* targets, float* aux, size_t capacity,
                      size_t* n);
 
+/* ------------------------------------------------------------------ research wire layer
+ * Strings returned through a char** are allocated by the library (also on
+ * failure, where they hold the error message) and released with
+ * mpm_string_free. */
+
+/* protocol_version, core/src/lib.rs:11-16. */
+int mpm_protocol_version(void);
+void mpm_string_free(char* s);
+/* run_json, core/src/lib.rs:24-49, for mode "research" (research::run,
+ * core/src/research.rs:72-132): validate the request, build the fixture on
+ * the GPU, advance record interval by record interval under the request's
+ * wall-clock budget, and return the output JSON (frames, metrics,
+ * observables, diagnostics, checkpoint).  resume_json (may be NULL) is a
+ * checkpoint of an earlier segment (restore, research.rs:324-368).  Returns
+ * 0, or MPM_ERR_INVALID (invalid input; household mode), MPM_ERR_MIN_STEP
+ * (numerical failure) or MPM_ERR_DEVICE, with the message in *output (the
+ * reference's ValueError text). */
+int mpm_run_json(const char* request_json, const char* resume_json, char** output);
+/* specification and FixtureSpec::validate, research.rs:170-232 and
+ * fixtures.rs:152-236, without building anything: 0 or MPM_ERR_INVALID with
+ * the message in *error. */
+int mpm_research_validate(const char* request_json, char** error);
+/* FixtureSpec::observables, fixtures.rs:369-615, of the request's fixture for
+ * a given state (f64, in the reference's particle order): a JSON object. */
+int mpm_research_observables(const char* request_json, const mpm_particles* particles, const mpm_pellet* pellets,
+                             size_t n_pellets, const mpm_ledger* ledger, const mpm_history* history,
+                             char** output);
+/* The checkpoint codec (Checkpoint and Snapshot, research.rs:42-65, 308-321):
+ * encode a state as checkpoint JSON, and decode one back.  Every f64 survives
+ * the round trip bit for bit.  decode fails with MPM_ERR_CAPACITY, setting
+ * particles->n, when the arrays are too small; mpm_checkpoint_particle_count
+ * gives the count first. */
+int mpm_checkpoint_encode(const char* request_json, const mpm_particles* particles, const mpm_pellet* pellets,
+                          size_t n_pellets, const mpm_ledger* ledger, const mpm_history* history,
+                          uint32_t next_record_index, char** output);
+int mpm_checkpoint_particle_count(const char* checkpoint_json, size_t* n);
+int mpm_checkpoint_decode(const char* checkpoint_json, mpm_particles* particles, mpm_pellet* pellets,
+                          size_t pellet_capacity, size_t* n_pellets, mpm_ledger* ledger, mpm_history* history,
+                          uint32_t* next_record_index, char** error);
+
 #ifdef __cplusplus
 }
 #endif

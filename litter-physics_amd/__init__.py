@@ -192,6 +192,19 @@ def lib():
         "mpm_surrogate_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "mpm_log_returns": (C.c_int, [C.c_void_p, C.c_size_t]),
         "mpm_take_returns": (C.c_int, [C.c_void_p, fp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "mpm_protocol_version": (C.c_int, []),
+        "mpm_string_free": (None, [C.c_void_p]),
+        "mpm_run_json": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+        "mpm_research_validate": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+        "mpm_research_observables": (C.c_int, [C.c_char_p, C.POINTER(Particles), C.POINTER(Pellet), C.c_size_t,
+                                               C.POINTER(Ledger), C.POINTER(History), C.POINTER(C.c_void_p)]),
+        "mpm_checkpoint_encode": (C.c_int, [C.c_char_p, C.POINTER(Particles), C.POINTER(Pellet), C.c_size_t,
+                                            C.POINTER(Ledger), C.POINTER(History), C.c_uint32,
+                                            C.POINTER(C.c_void_p)]),
+        "mpm_checkpoint_particle_count": (C.c_int, [C.c_char_p, C.POINTER(C.c_size_t)]),
+        "mpm_checkpoint_decode": (C.c_int, [C.c_char_p, C.POINTER(Particles), C.POINTER(Pellet), C.c_size_t,
+                                            C.POINTER(C.c_size_t), C.POINTER(Ledger), C.POINTER(History),
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -212,6 +225,8 @@ EXPORTED = [
     "mpm_get_launch", "mpm_set_launch", "mpm_autotune", "mpm_mlp_param_count", "mpm_mlp_create", "mpm_mlp_destroy",
     "mpm_mlp_last_error", "mpm_mlp_get_params", "mpm_mlp_set_params", "mpm_mlp_forward", "mpm_mlp_loss_grad",
     "mpm_mlp_train", "mpm_set_surrogate", "mpm_surrogate_stats", "mpm_log_returns", "mpm_take_returns",
+    "mpm_protocol_version", "mpm_string_free", "mpm_run_json", "mpm_research_validate", "mpm_research_observables",
+    "mpm_checkpoint_encode", "mpm_checkpoint_particle_count", "mpm_checkpoint_decode",
 ]
 
 
@@ -774,3 +789,97 @@ def liquid_update_batch(material, dt, volume_ratio, gradient):
     if rc != 0:
         raise MpmError(f"liquid batch failed ({rc})")
     return nj, tau.reshape(n, 3, 3), codes
+
+
+# ------------------------------------------------------------------ research wire layer
+# core/src/lib.rs:11-49 and core/src/research.rs: JSON in, JSON out.  Failures
+# raise ValueError with the message, as the reference's extension does.
+
+def _take_string(ptr):
+    if not ptr.value:
+        return ""
+    try:
+        return C.string_at(ptr.value).decode()
+    finally:
+        lib().mpm_string_free(ptr)
+
+
+def protocol_version() -> int:
+    """protocol_version (core/src/lib.rs:11-16)."""
+    return lib().mpm_protocol_version()
+
+
+def run_json(request_json: str, resume_json: str | None = None) -> str:
+    """run_json (core/src/lib.rs:24-49) for research requests, on the GPU engine."""
+    out = C.c_void_p()
+    rc = lib().mpm_run_json(request_json.encode(), None if resume_json is None else resume_json.encode(),
+                            C.byref(out))
+    text = _take_string(out)
+    if rc != 0:
+        raise ValueError(text)
+    return text
+
+
+def research_validate(request_json: str) -> None:
+    """Request and fixture validation only (research.rs:135-270, fixtures.rs:152-236)."""
+    err = C.c_void_p()
+    rc = lib().mpm_research_validate(request_json.encode(), C.byref(err))
+    text = _take_string(err)
+    if rc != 0:
+        raise ValueError(text)
+
+
+def _pellet_array(pellets):
+    pellets = list(pellets)
+    arr = (Pellet * max(len(pellets), 1))(*pellets)
+    return arr, len(pellets)
+
+
+def research_observables(request_json: str, state: State, pellets, ledger: Ledger, history: History) -> dict:
+    """FixtureSpec::observables (fixtures.rs:369-615) of a given state."""
+    import json
+    parts = state.to_c()
+    arr, n = _pellet_array(pellets)
+    out = C.c_void_p()
+    rc = lib().mpm_research_observables(request_json.encode(), C.byref(parts), arr, n, C.byref(ledger),
+                                        C.byref(history), C.byref(out))
+    text = _take_string(out)
+    if rc != 0:
+        raise ValueError(text)
+    return json.loads(text)
+
+
+def checkpoint_encode(request_json: str, state: State, pellets, ledger: Ledger, history: History,
+                      next_record_index: int) -> str:
+    """Checkpoint JSON of a state (research.rs:42-65, 308-321)."""
+    parts = state.to_c()
+    arr, n = _pellet_array(pellets)
+    out = C.c_void_p()
+    rc = lib().mpm_checkpoint_encode(request_json.encode(), C.byref(parts), arr, n, C.byref(ledger),
+                                     C.byref(history), next_record_index, C.byref(out))
+    text = _take_string(out)
+    if rc != 0:
+        raise ValueError(text)
+    return text
+
+
+def checkpoint_decode(text: str):
+    """(State, pellets, Ledger, History, next_record_index) of checkpoint JSON."""
+    raw = text.encode()
+    n = C.c_size_t(0)
+    if lib().mpm_checkpoint_particle_count(raw, C.byref(n)) != 0:
+        raise ValueError("checkpoint has no particle id array")
+    st = State.empty(n.value)
+    parts = st.to_c()
+    pel = (Pellet * 1)()
+    npel = C.c_size_t(0)
+    led, hist, nri = Ledger(), History(), C.c_uint32(0)
+    err = C.c_void_p()
+    rc = lib().mpm_checkpoint_decode(raw, C.byref(parts), pel, 1, C.byref(npel), C.byref(led), C.byref(hist),
+                                     C.byref(nri), C.byref(err))
+    msg = _take_string(err)
+    if rc != 0:
+        raise ValueError(msg)
+    k = st._keep
+    out = State(st._ids, *k)
+    return out, [pel[i] for i in range(npel.value)], led, hist, nri.value

@@ -7,9 +7,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmpm_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "mlp.hip", "engine.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "mlp.hip", "engine.cpp", "research.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "engine_internal.h"), os.path.join(HERE, "csrc", "physics.h"),
-           os.path.join(HERE, "csrc", "mlp.h"),
+           os.path.join(HERE, "csrc", "mlp.h"), os.path.join(HERE, "csrc", "json.h"),
+           os.path.join(HERE, "csrc", "host_math.h"),
            os.path.join(ROOT, "include", "mpm_engine.h")]
 
 FLAGS = [

@@ -18,16 +18,21 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/mpm_engine.h"
 #include "engine_internal.h"
+#include "host_math.h"
 
 using namespace mpm;
 
 static_assert(sizeof(mpm_ledger) == sizeof(double) * kLedger, "ledger layout");
 
 namespace {
+
+using mpm_host::eigen3;
+using mpm_host::qmatrix;
 
 constexpr int kMaxStepsPerSync = 256;
 constexpr int kLocalitySortInterval = 32;  // host re-sorts for locality every 32 trial steps
@@ -41,16 +46,6 @@ struct V3 {
 };
 inline V3 v3(double a, double b, double c) { return V3{{a, b, c}}; }
 inline double dot(const V3& a, const V3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-
-void qmatrix(const double* q, double r[9]) {
-    double i = q[0], j = q[1], k = q[2], w = q[3];
-    double ww = w * w, ii = i * i, jj = j * j, kk = k * k;
-    double ij = i * j * 2.0, wk = w * k * 2.0, wj = w * j * 2.0, ik = i * k * 2.0, jk = j * k * 2.0,
-           wi = w * i * 2.0;
-    double m[9] = {ww + ii - jj - kk, ij - wk, wj + ik, wk + ij, ww - ii + jj - kk, jk - wi, ik - wj, wi + jk,
-                   ww - ii - jj + kk};
-    std::memcpy(r, m, sizeof(m));
-}
 
 // Pellet::angular_velocity and kinetic energy, rigid.rs:193-210.
 V3 pellet_omega(const PelletState& p) {
@@ -66,45 +61,6 @@ double pellet_kinetic(const PelletState& p) {
     V3 om = pellet_omega(p);
     return 0.5 * p.mass * (p.vel[0] * p.vel[0] + p.vel[1] * p.vel[1] + p.vel[2] * p.vel[2]) +
            0.5 * (om[0] * p.L[0] + om[1] * p.L[1] + om[2] * p.L[2]);
-}
-
-// Symmetric 3x3 cyclic Jacobi in f64 (constitutive.rs:373-425), used for the
-// elastic-energy diagnostic.
-bool eigen3(const double* m, double* vals) {
-    double d[9];
-    for (int i = 0; i < 9; ++i) d[i] = 0.5 * (m[i] + m[3 * (i % 3) + i / 3]);
-    double scale = 0.0;
-    for (int i = 0; i < 9; ++i) scale += d[i] * d[i];
-    scale = std::max(scale, std::numeric_limits<double>::min());
-    for (int sweep = 0; sweep < 64; ++sweep) {
-        double off = d[1] * d[1] + d[2] * d[2] + d[5] * d[5];
-        if (off <= 1e-30 * scale) {
-            vals[0] = d[0];
-            vals[1] = d[4];
-            vals[2] = d[8];
-            return true;
-        }
-        const int pq[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-        for (auto& r : pq) {
-            int p = r[0], q = r[1];
-            double apq = d[3 * p + q];
-            if (apq == 0.0) continue;
-            double th = (d[3 * q + q] - d[3 * p + p]) / (2.0 * apq);
-            double t = std::copysign(1.0, th) / (std::fabs(th) + std::sqrt(th * th + 1.0));
-            double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-            for (int k = 0; k < 3; ++k) {
-                double a = d[3 * k + p], b = d[3 * k + q];
-                d[3 * k + p] = c * a - s * b;
-                d[3 * k + q] = s * a + c * b;
-            }
-            for (int k = 0; k < 3; ++k) {
-                double a = d[3 * p + k], b = d[3 * q + k];
-                d[3 * p + k] = c * a - s * b;
-                d[3 * q + k] = s * a + c * b;
-            }
-        }
-    }
-    return false;
 }
 
 double wave_speed(const mpm_material& m) {
@@ -127,6 +83,46 @@ double damping_ratio2(const mpm_config& c) {
     return 2.0 * zeta;
 }
 
+// Particle mass and reference volume as given, by particle id.  The kernels
+// use f32 copies; state exchange, the ledger seed, outflow and the residuals
+// use these, so a state read back returns the constants bit for bit and the
+// mass ledger closes in f64.  Ids are dense from zero in the reference
+// (particles.rs:13); other id sets use a hash map.
+struct ExactConstants {
+    std::vector<double> dense;  // mass, volume0 per id
+    std::unordered_map<uint32_t, std::pair<double, double>> sparse;
+    bool use_dense = true;
+
+    void assign(size_t n, const uint32_t* id, const double* mass, const double* volume0) {
+        uint32_t top = 0;
+        for (size_t p = 0; p < n; ++p) top = std::max(top, id ? id[p] : (uint32_t)p);
+        use_dense = (size_t)top < 4 * n + 4096;
+        dense.clear();
+        sparse.clear();
+        if (use_dense) dense.assign(2 * ((size_t)top + 1), 0.0);
+        for (size_t p = 0; p < n; ++p) {
+            uint32_t k = id ? id[p] : (uint32_t)p;
+            if (use_dense) {
+                dense[2 * (size_t)k] = mass[p];
+                dense[2 * (size_t)k + 1] = volume0[p];
+            } else {
+                sparse[k] = {mass[p], volume0[p]};
+            }
+        }
+    }
+    // Falls back to the stored f32 values for an id it does not hold.
+    void get(uint32_t id, float m32, float v32, double& m, double& v0) const {
+        m = m32;
+        v0 = v32;
+        if (use_dense) {
+            if (2 * (size_t)id + 1 < dense.size()) m = dense[2 * (size_t)id], v0 = dense[2 * (size_t)id + 1];
+        } else {
+            auto it = sparse.find(id);
+            if (it != sparse.end()) m = it->second.first, v0 = it->second.second;
+        }
+    }
+};
+
 // Host copy of one instance's particles in the f32 slab layout (field-major, n entries).
 struct HostParticles {
     size_t n = 0;
@@ -136,12 +132,13 @@ struct HostParticles {
 };
 
 // Kinetic, potential and elastic energy (solver.rs:1027-1076) from the f32 state.
-void energies_of(const mpm_material& mat, const double* g, const HostParticles& h,
+void energies_of(const mpm_material& mat, const double* g, const HostParticles& h, const ExactConstants& ex,
                  const std::vector<PelletState>& pellets, double* ke_out, double* pe_out, double* el_out) {
     size_t n = h.n;
     double ke = 0.0, pe = 0.0, el = 0.0;
     for (size_t p = 0; p < n; ++p) {
-        double m = h.sh[SMASS * n + p], v0 = h.sh[SVOL0 * n + p];
+        double m, v0;
+        ex.get(h.id[p], h.sh[SMASS * n + p], h.sh[SVOL0 * n + p], m, v0);
         double v2 = 0.0, gx = 0.0;
         for (int a = 0; a < 3; ++a) {
             double v = h.s[(VX + a) * n + p];
@@ -179,6 +176,57 @@ void energies_of(const mpm_material& mat, const double* g, const HostParticles& 
     for (const PelletState& p : pellets) {
         ke += pellet_kinetic(p);
         pe -= p.mass * (g[0] * p.pos[0] + g[1] * p.pos[1] + g[2] * p.pos[2]);
+    }
+    if (ke_out) *ke_out = ke;
+    if (pe_out) *pe_out = pe;
+    if (el_out) *el_out = el;
+}
+
+// Kinetic, potential and elastic energy (solver.rs:1027-1076) of an f64 state
+// in its own order, with the reference's operation order.
+void energies_f64(const mpm_material& mat, const double* g, const mpm_particles& ps, const mpm_pellet* pellets,
+                  size_t n_pellets, double* ke_out, double* pe_out, double* el_out) {
+    double ke = 0.0, pe = 0.0, el = 0.0;
+    for (size_t p = 0; p < ps.n; ++p) {
+        const double m = ps.mass[p];
+        const double* v = ps.velocity ? ps.velocity + 3 * p : nullptr;
+        const double v2 = v ? v[0] * v[0] + v[1] * v[1] + v[2] * v[2] : 0.0;
+        ke += 0.5 * m * v2;
+    }
+    for (size_t p = 0; p < ps.n; ++p) {
+        const double* x = ps.position + 3 * p;
+        pe += -ps.mass[p] * (g[0] * x[0] + g[1] * x[1] + g[2] * x[2]);
+    }
+    for (size_t p = 0; p < ps.n; ++p) {
+        const double v0 = ps.volume0[p];
+        if (mat.kind == 0) {
+            double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, b[9], vals[3];
+            if (ps.deformation)
+                for (int a = 0; a < 9; ++a) F[a] = ps.deformation[9 * p + a];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    b[3 * i + j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
+            if (!eigen3(b, vals)) {
+                el = std::numeric_limits<double>::quiet_NaN();
+                continue;
+            }
+            double hh[3];
+            for (int a = 0; a < 3; ++a) hh[a] = 0.5 * std::log(std::max(vals[a], 1e-300));
+            const double tr = hh[0] + hh[1] + hh[2], h2 = hh[0] * hh[0] + hh[1] * hh[1] + hh[2] * hh[2];
+            el += v0 * (mat.shear_modulus * h2 + 0.5 * mat.lame_lambda * tr * tr);
+        } else {
+            const double j = ps.volume_ratio ? ps.volume_ratio[p] : 1.0;
+            const double gm = mat.tait_gamma;
+            if (gm == 1.0)
+                el += v0 * mat.bulk_modulus * (j - 1.0 - std::log(j));
+            else
+                el += v0 * (mat.bulk_modulus / gm) * (j - 1.0 + (std::pow(j, 1.0 - gm) - 1.0) / (gm - 1.0));
+        }
+    }
+    for (size_t k = 0; k < n_pellets; ++k) {
+        const mpm_pellet& pl = pellets[k];
+        ke += mpm_host::pellet_kinetic(pl);
+        pe -= pl.mass * (g[0] * pl.position[0] + g[1] * pl.position[1] + g[2] * pl.position[2]);
     }
     if (ke_out) *ke_out = ke;
     if (pe_out) *pe_out = pe;
@@ -226,6 +274,7 @@ struct Instance {
     int pel_cap = 0;
     InstParams ip{};
     bool grid_valid = false;
+    ExactConstants exact;
 
     void fill_params() {
         Params& P = ip.P;
@@ -767,7 +816,8 @@ struct mpm_engine {
                 if (std::isnan(padded) || padded < 0.0 || padded + 2.0 >= (double)(in.cells[a] + 3)) inside[p] = 0;
             }
             if (!inside[p]) {
-                double m = h.sh[SMASS * n + p];
+                double m, v0;
+                in.exact.get(h.id[p], h.sh[SMASS * n + p], h.sh[SVOL0 * n + p], m, v0);
                 out_mass += m;
                 for (int a = 0; a < 3; ++a) out_mom[a] += (double)h.s[(VX + a) * n + p] * m;
             } else {
@@ -1044,7 +1094,7 @@ struct mpm_engine {
         if (int rc = download(i, h)) return rc;
         std::vector<PelletState> pel;
         if (int rc = download_pellets(i, pel)) return rc;
-        energies_of(inst[i].mat, inst[i].cfg.gravity, h, pel, ke, pe, el);
+        energies_of(inst[i].mat, inst[i].cfg.gravity, h, inst[i].exact, pel, ke, pe, el);
         return MPM_OK;
     }
 };
@@ -1097,8 +1147,9 @@ int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pell
         h.sh[SEPS * n + p] = eps;
         h.sh[SMASS * n + p] = (float)ps->mass[p];
         h.sh[SVOL0 * n + p] = (float)ps->volume0[p];
-        mass += (double)h.sh[SMASS * n + p];
+        mass += ps->mass[p];
     }
+    in.exact.assign(n, ps->id, ps->mass, ps->volume0);
     // Pellets: the instance's slots, unused ones marked free.
     std::vector<PelletState> pel(in.pel_cap);
     double contact_dt = std::numeric_limits<double>::infinity();
@@ -1158,16 +1209,18 @@ int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pell
     if (ledger) {
         std::memcpy(s.ledger, ledger, sizeof(mpm_ledger));
     } else {
-        // Simulation::new seeds the initial inventory, solver.rs:357-390.
+        // Simulation::new seeds the initial inventory from the state as given
+        // (f64, in its order), solver.rs:357-390.
         double mom[3] = {0.0, 0.0, 0.0};
-        for (size_t p = 0; p < n; ++p)
-            for (int a = 0; a < 3; ++a) mom[a] += (double)h.s[(VX + a) * n + p] * (double)h.sh[SMASS * n + p];
-        for (const PelletState& p : pel)
-            for (int a = 0; a < 3; ++a) mom[a] += p.vel[a] * p.mass;
+        if (ps->velocity)
+            for (size_t p = 0; p < n; ++p)
+                for (int a = 0; a < 3; ++a) mom[a] += ps->velocity[3 * p + a] * ps->mass[p];
+        for (size_t k = 0; k < n_pellets; ++k)
+            for (int a = 0; a < 3; ++a) mom[a] += pellets[k].velocity[a] * pellets[k].mass;
         s.ledger[0] = mass;
         for (int a = 0; a < 3; ++a) s.ledger[2 + a] = mom[a];
         double ke = 0, pe = 0, el = 0;
-        energies_of(in.mat, in.cfg.gravity, h, pel, &ke, &pe, &el);
+        energies_f64(in.mat, in.cfg.gravity, *ps, pellets, n_pellets, &ke, &pe, &el);
         s.ledger[5] = ke + pe + el;
     }
     in.grid_valid = false;
@@ -1222,8 +1275,10 @@ int mpm_engine::get_state(int i, mpm_particles* ps, mpm_pellet* pellets, size_t 
             if (ps->volume_ratio)
                 ps->volume_ratio[k] = (in.mat.kind == 0 && !fresh) ? 1.0 : 1.0 + (double)h.s[JM1 * n + p];
             if (ps->plastic_strain) ps->plastic_strain[k] = in.mat.kind == 0 ? h.s[EPS * n + p] : h.sh[SEPS * n + p];
-            if (ps->mass) ps->mass[k] = h.sh[SMASS * n + p];
-            if (ps->volume0) ps->volume0[k] = h.sh[SVOL0 * n + p];
+            double m, v0;
+            in.exact.get(h.id[p], h.sh[SMASS * n + p], h.sh[SVOL0 * n + p], m, v0);
+            if (ps->mass) ps->mass[k] = m;
+            if (ps->volume0) ps->volume0[k] = v0;
         }
     }
     if (pellets) {
@@ -1588,7 +1643,11 @@ double mpm_mass_residual(mpm_engine* e) {
     HostParticles h;
     if (e->download(0, h)) return std::numeric_limits<double>::quiet_NaN();
     double m = 0.0;
-    for (size_t p = 0; p < h.n; ++p) m += h.sh[SMASS * h.n + p];
+    for (size_t p = 0; p < h.n; ++p) {
+        double mp, v0;
+        e->inst[0].exact.get(h.id[p], h.sh[SMASS * h.n + p], h.sh[SVOL0 * h.n + p], mp, v0);
+        m += mp;
+    }
     const double* L = e->h_is[0].ledger;
     return m + L[1] - L[0];
 }
@@ -1602,8 +1661,11 @@ int mpm_momentum_residual(mpm_engine* e, double* out) {
     if (int rc = e->download_pellets(0, pel)) return rc;
     size_t n = h.n;
     double p[3] = {0.0, 0.0, 0.0};
-    for (size_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) p[a] += (double)h.s[(VX + a) * n + i] * (double)h.sh[SMASS * n + i];
+    for (size_t i = 0; i < n; ++i) {
+        double m, v0;
+        e->inst[0].exact.get(h.id[i], h.sh[SMASS * n + i], h.sh[SVOL0 * n + i], m, v0);
+        for (int a = 0; a < 3; ++a) p[a] += (double)h.s[(VX + a) * n + i] * m;
+    }
     for (const PelletState& pl : pel)
         for (int a = 0; a < 3; ++a) p[a] += pl.vel[a] * pl.mass;
     const double* L = e->h_is[0].ledger;
