@@ -1,10 +1,12 @@
 """Training data, training and evaluation of the Herschel-Bulkley return-map surrogate.
 
 Data are exact returns logged by the engine (mpm_log_returns) from batches of
-paste slumps like the small-slump configuration (32^3 grids, dx 2.5 or 1.5 mm,
-column heights 20-80 mm) with per-instance rheology: shear yield stress
-U[10, 300] Pa, consistency U[0.5, 20] Pa s^n, flow index U[0.3, 1.0].  The
-network trains on the GPU through mpm_mlp_train.
+paste slumps with per-instance rheology (shear yield stress U[10, 300] Pa,
+consistency U[0.5, 20] Pa s^n, flow index U[0.3, 1.0]) in two shapes: like the
+small-slump configuration (32^3 grids, dx 2.5 or 1.5 mm, column heights
+20-80 mm), and like the batched calibration sweep (grids log-uniform 16^3 to
+64^3 cells over a 100 mm cube, so dx 1.6-6.3 mm, 40 x 40 mm columns 20-80 mm
+tall).  The network trains on the GPU through mpm_mlp_train.
 
     python litter-physics_amd/surrogate.py train --samples 20000000 --epochs 6
 
@@ -64,12 +66,35 @@ def slump_specs(E, count, seed, cells=32, yield_range=(10.0, 300.0), k_range=(0.
     return specs
 
 
-def collect(samples, seed=3, instances=64, windows=40, gap=48, log=print):
+def sweep_like_specs(E, count, seed, yield_range=(10.0, 300.0), k_range=(0.5, 20.0), n_range=(0.3, 1.0)):
+    """Paste slumps shaped like the batched calibration sweep: grids log-uniform
+    16^3..64^3 cells over a 100 mm cube (dx = 100 mm / cells), 40 x 40 mm
+    columns 20-80 mm tall, random rheology (shear units) from `seed`."""
+    rng = np.random.default_rng(seed)
+    bulk, shear = 2e5, 5e4
+    young = 9.0 * bulk * shear / (3.0 * bulk + shear)
+    poisson = (3.0 * bulk - 2.0 * shear) / (2.0 * (3.0 * bulk + shear))
+    face = [0.0, 0.0, 0.0, 0.0, math.inf, 0.0]
+    specs = []
+    for _ in range(count):
+        cells = int(round(math.exp(rng.uniform(math.log(16), math.log(64)))))
+        dx = 0.1 / cells
+        domain = [cells * dx] * 3
+        mat = E.paste(1200.0, young, poisson, rng.uniform(*yield_range), rng.uniform(*k_range),
+                      rng.uniform(*n_range), shear_units=True)
+        cfg = E.config(wall_friction=0.0, cfl=0.4, max_dt=1e-4, min_dt=1e-12, face_friction=face)
+        st = E.place_particles("slump", mat, dx, domain, [0.04, 0.04, rng.uniform(0.02, 0.08)],
+                               seed=int(rng.integers(1 << 31)), jitter=0.5)
+        specs.append({"material": mat, "cfg": cfg, "domain": domain, "spacing": dx, "state": st})
+    return specs
+
+
+def collect(samples, seed=3, instances=64, sweep_instances=0, windows=40, gap=48, log=print):
     """Log exact returns: `windows` single logged steps, the first ones after 1, 2,
     4, ... steps (the loading transient) and then `gap` steps apart, randomly
     subsampled to `samples` in total.  Returns (inputs, targets, aux)."""
     E = engine()
-    specs = slump_specs(E, instances, seed)
+    specs = slump_specs(E, instances, seed) + sweep_like_specs(E, sweep_instances, seed + 1000)
     total = sum(len(s["state"]) for s in specs)
     batch = E.Batch(specs)
     rng = np.random.default_rng(seed + 1)
@@ -143,6 +168,7 @@ def main():
     ap.add_argument("cmd", choices=["train"])
     ap.add_argument("--samples", type=int, default=20_000_000)
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--sweep-instances", type=int, default=64)
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -156,7 +182,8 @@ def main():
         print(m, file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    x, t, aux = collect(args.samples, seed=args.seed, instances=args.instances, log=log)
+    x, t, aux = collect(args.samples, seed=args.seed, instances=args.instances,
+                        sweep_instances=args.sweep_instances, log=log)
     log(f"collected {len(x)} samples in {time.perf_counter() - t0:.1f} s")
     rng = np.random.default_rng(args.seed + 2)
     order = rng.permutation(len(x))
