@@ -1076,10 +1076,37 @@ int grid_to_particle(const oracle_sim& s, double dt, std::vector<Trial>& trials)
 }
 
 // solver.rs:540-611 (pellets coupled one after another in index order).
-int try_step(oracle_sim& s, double dt) {
+// Kinetic snapshots of one trial (audit.rs:70-88), filled when try_step is
+// given somewhere to put them (solver.rs:540-611 with `audit: Some`).
+struct Stages {
+    double transport = 0.0, after_stress = 0.0, after_traction = 0.0, after_gravity = 0.0, after_walls = 0.0,
+           after_coupling = 0.0, pellet_after_coupling = 0.0, plastic = 0.0;
+};
+
+// grid_kinetic, audit.rs:360-374: active nodes with mass, momentum or velocity storage.
+double grid_kinetic(const Grid& g, bool velocities, const V3& kick) {
+    double e = 0.0;
+    for (size_t i : g.active) {
+        double m = g.mass[i];
+        if (!(m > 0.0)) continue;
+        V3 v = g.momentum[i];
+        if (!velocities) v = V3{v[0] / m, v[1] / m, v[2] / m};
+        e += 0.5 * m * norm2(add(v, kick));
+    }
+    return e;
+}
+
+int try_step(oracle_sim& s, double dt, Stages* audit = nullptr) {
     sweep_outflow(s);
     deposit_particles(s, dt);
+    const V3 kick0{0.0, 0.0, 0.0};
+    if (audit) audit->after_stress = grid_kinetic(s.grid, false, kick0);
     Transfer traction = wall_traction(s, dt);
+    if (audit) {
+        audit->after_traction = grid_kinetic(s.grid, false, kick0);
+        audit->after_gravity =
+            grid_kinetic(s.grid, false, V3{s.config.gravity[0] * dt, s.config.gravity[1] * dt, s.config.gravity[2] * dt});
+    }
     if (!std::isfinite(traction.energy) || !std::isfinite(s.ledger.wall_normal_traction_energy + traction.energy))
         return ORACLE_FAIL_TRACTION;
     std::vector<NodeTrial> nodes;
@@ -1093,6 +1120,7 @@ int try_step(oracle_sim& s, double dt) {
         projection += nodes[q].projection;
         friction += nodes[q].friction;
     }
+    if (audit) audit->after_walls = grid_kinetic(s.grid, true, kick0);
     std::vector<Pellet> trial_pellets = s.pellets;
     std::vector<CouplingResult> couplings;
     std::vector<double> pellet_energies;
@@ -1103,15 +1131,22 @@ int try_step(oracle_sim& s, double dt) {
         double before = pellet.kinetic_energy();
         pellet.apply_impulse(res.impulse, res.angular_impulse);
         double pe = pellet.kinetic_energy() - before;
+        if (audit) audit->pellet_after_coupling += pellet.kinetic_energy();
         WallContact wc = pellet_integrate(pellet, dt, g, s.grid.layout.extents(), s.config);
         if (!pellet.is_finite() || !std::isfinite(pe)) return ORACLE_FAIL_PELLET;
         couplings.push_back(res);
         pellet_energies.push_back(pe);
         walls.push_back(wc);
     }
+    if (audit) audit->after_coupling = grid_kinetic(s.grid, true, kick0);
     std::vector<Trial> trials;
     rc = grid_to_particle(s, dt, trials);
     if (rc) return rc;
+    if (audit) {
+        double sum = 0.0;
+        for (const Trial& t : trials) sum += t.dissipation;
+        audit->plastic = sum;
+    }
     double max_cells = 0.0;
     for (const Trial& t : trials) max_cells = std::max(max_cells, t.displacement);
     if (max_cells > 1.0) return ORACLE_FAIL_DISPLACEMENT;
@@ -1150,6 +1185,152 @@ int try_step(oracle_sim& s, double dt) {
         L.coupling_pellet_energy += pellet_energies[k];
     }
     s.pellets = trial_pellets;
+    return 0;
+}
+
+// ---------------------------------------------------------------- energy audit, audit.rs
+
+// Simulation::validate (solver.rs:1106-1140) with ParticleSet::validate
+// (particles.rs:159-211) and Ledger::validate (solver.rs:182-243).
+bool validate_sim(const oracle_sim& s) {
+    const Particles& ps = s.particles;
+    std::vector<uint32_t> ids(ps.id);
+    std::sort(ids.begin(), ids.end());
+    for (size_t k = 1; k < ids.size(); ++k)
+        if (ids[k] == ids[k - 1]) return false;
+    for (size_t p = 0; p < ps.size(); ++p) {
+        bool ok = finite3(ps.pos[p]) && finite3(ps.vel[p]) && std::isfinite(ps.volume_ratio[p]) &&
+                  std::isfinite(ps.plastic[p]) && std::isfinite(ps.mass[p]) && std::isfinite(ps.volume0[p]);
+        for (int a = 0; a < 9; ++a)
+            ok = ok && std::isfinite(ps.affine[p][a]) && std::isfinite(ps.deformation[p][a]) && std::isfinite(ps.kirchhoff[p][a]);
+        if (!ok || ps.mass[p] <= 0.0 || ps.volume0[p] <= 0.0 || ps.volume_ratio[p] <= 0.0 || ps.plastic[p] < 0.0)
+            return false;
+    }
+    const oracle_ledger& L = s.ledger;
+    const double scalars[] = {L.initial_mass, L.outflow_mass, L.initial_mechanical_energy, L.wall_normal_projection_energy,
+                              L.wall_friction_dissipation, L.wall_normal_traction_energy, L.pellet_wall_work,
+                              L.coupling_grid_energy, L.coupling_pellet_energy, L.plastic_dissipation};
+    for (double v : scalars)
+        if (!std::isfinite(v)) return false;
+    const double* vectors[] = {L.initial_momentum, L.gravity_impulse, L.wall_impulse, L.pellet_wall_impulse,
+                               L.coupling_impulse, L.coupling_angular_impulse, L.outflow_momentum};
+    for (const double* v : vectors)
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(v[a])) return false;
+    if (L.initial_mass < 0.0 || L.outflow_mass < 0.0 || L.plastic_dissipation < 0.0 || L.wall_friction_dissipation < 0.0 ||
+        L.wall_normal_projection_energy > 0.0)
+        return false;
+    if (!std::isfinite(s.time) || s.time < 0.0) return false;
+    if (!(std::isfinite(s.dt_scale) && s.dt_scale > 0.0 && s.dt_scale <= 1.0)) return false;
+    for (const Pellet& pl : s.pellets)
+        if (!pl.is_finite()) return false;
+    Stencil st;
+    for (size_t p = 0; p < ps.size(); ++p)
+        if (!stencil(s.grid.layout, ps.pos[p], st)) return false;
+    return true;
+}
+
+// EnergyState::measure, audit.rs:37-54.
+void measure_energy(const oracle_sim& s, double* e) {
+    const Particles& ps = s.particles;
+    double sum = 0.0;
+    for (size_t p = 0; p < ps.size(); ++p) {
+        double n2 = 0.0;  // norm_squared over the column-major storage
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) n2 += at(ps.affine[p], i, j) * at(ps.affine[p], i, j);
+        sum += ps.mass[p] * n2;
+    }
+    double h = s.grid.layout.spacing;
+    double pellet = 0.0;
+    for (const Pellet& pl : s.pellets) pellet += pl.kinetic_energy();
+    e[0] = ps.kinetic_energy();
+    e[1] = h * h / 8.0 * sum;
+    e[2] = elastic_energy(s);
+    e[3] = potential_energy(s);
+    e[4] = pellet;
+}
+
+// transport_energy, audit.rs:376-398: APIC-only deposit on a fresh grid.
+double transport_energy(const oracle_sim& s) {
+    const Layout& l = s.grid.layout;
+    Grid shadow(l);
+    Stencil st;
+    for (size_t p = 0; p < s.particles.size(); ++p) {
+        if (!stencil(l, s.particles.pos[p], st)) return std::numeric_limits<double>::quiet_NaN();
+        double mass = s.particles.mass[p];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c) {
+                    double w = st.weight(a, b, c);
+                    V3 velocity = add(s.particles.vel[p], mv(s.particles.affine[p], st.distance(a, b, c)));
+                    size_t index = l.index(st.base[0] + a, st.base[1] + b, st.base[2] + c);
+                    shadow.deposit(index, w * mass, scale(velocity, w * mass));
+                }
+    }
+    return grid_kinetic(shadow, false, V3{0.0, 0.0, 0.0});
+}
+
+// Simulation::audit_step, audit.rs:275-345, and EnergyTerms::measure (193-232).
+int audit_step(const oracle_sim& s, double dt, oracle_audit* out) {
+    if (!validate_sim(s)) return ORACLE_ERR_INVALID;
+    const oracle_config& c = s.config;
+    double stability = stability_limit(s);
+    if (!std::isfinite(c.gravity[0]) || !std::isfinite(c.gravity[1]) || !std::isfinite(c.gravity[2]) ||
+        !std::isfinite(c.cfl) || c.cfl <= 0.0 || !std::isfinite(c.wall_friction) || c.wall_friction < 0.0 ||
+        !std::isfinite(c.min_dt) || c.min_dt <= 0.0 || !std::isfinite(c.max_dt) || c.max_dt < c.min_dt ||
+        !std::isfinite(stability) || stability <= 0.0)
+        return ORACLE_ERR_INVALID;
+    double limit = std::min(s.dt_scale * stability, c.max_dt);
+    if (!std::isfinite(dt) || dt <= 0.0 || dt < c.min_dt || !std::isfinite(limit) || dt > limit) return ORACLE_ERR_INVALID;
+    oracle_audit a{};
+    a.time = s.time;
+    a.dt = dt;
+    measure_energy(s, a.before);
+    for (double v : a.before)
+        if (!std::isfinite(v)) return ORACLE_ERR_INVALID;
+    Stages st;
+    st.transport = transport_energy(s);
+    oracle_sim trial(s);
+    if (try_step(trial, dt, &st) != 0) return ORACLE_ERR_INVALID;
+    if (!validate_sim(trial)) return ORACLE_ERR_INVALID;
+    measure_energy(trial, a.after);
+    const double stages[8] = {st.transport, st.after_stress, st.after_traction, st.after_gravity,
+                              st.after_walls, st.after_coupling, st.pellet_after_coupling, st.plastic};
+    std::copy(stages, stages + 8, a.stages);
+    const oracle_ledger &B = s.ledger, &A = trial.ledger;
+    a.increments[0] = A.plastic_dissipation - B.plastic_dissipation;
+    a.increments[1] = A.wall_friction_dissipation - B.wall_friction_dissipation;
+    a.increments[2] = A.wall_normal_projection_energy - B.wall_normal_projection_energy;
+    a.increments[3] = A.wall_normal_traction_energy - B.wall_normal_traction_energy;
+    a.increments[4] = A.coupling_grid_energy - B.coupling_grid_energy;
+    a.increments[5] = A.coupling_pellet_energy - B.coupling_pellet_energy;
+    a.increments[6] = A.pellet_wall_work - B.pellet_wall_work;
+    const double* be = a.before;
+    const double* af = a.after;
+    double p2g = be[0] + be[1] - st.transport;
+    double g2p = st.after_coupling - af[0] - af[1];
+    double stress = st.after_stress - st.transport + af[2] - be[2] + st.plastic;
+    double gravity = st.after_gravity - st.after_traction + af[3] - be[3];
+    double integrator = af[4] - st.pellet_after_coupling;
+    double traction = st.after_traction - st.after_stress;
+    double walls = st.after_walls - st.after_gravity;
+    double coupling = st.after_coupling - st.after_walls + st.pellet_after_coupling - be[4];
+    double total_after = af[0] + af[1] + af[2] + af[3] + af[4];
+    double total_before = be[0] + be[1] + be[2] + be[3] + be[4];
+    double change = total_after - total_before;
+    double sum = -p2g - g2p + stress - st.plastic + gravity + integrator + traction + walls + coupling;
+    const double terms[11] = {p2g, g2p, stress, gravity, integrator, traction, walls, coupling, change, sum, change - sum};
+    std::copy(terms, terms + 11, a.terms);
+    a.coupled = s.pellets.empty() ? 0 : 1;
+    for (double v : a.after)
+        if (!std::isfinite(v)) return ORACLE_ERR_INVALID;
+    for (double v : a.stages)
+        if (!std::isfinite(v)) return ORACLE_ERR_INVALID;
+    for (double v : a.increments)
+        if (!std::isfinite(v)) return ORACLE_ERR_INVALID;
+    for (double v : a.terms)
+        if (!std::isfinite(v)) return ORACLE_ERR_INVALID;
+    *out = a;
     return 0;
 }
 
@@ -1635,6 +1816,8 @@ double oracle_sim_kinetic_energy(const oracle_sim* s) { return kinetic_energy(*s
 double oracle_sim_potential_energy(const oracle_sim* s) { return potential_energy(*s); }
 double oracle_sim_elastic_energy(const oracle_sim* s) { return elastic_energy(*s); }
 double oracle_sim_mechanical_energy(const oracle_sim* s) { return mechanical_energy(*s); }
+int oracle_sim_audit_step(const oracle_sim* s, double dt, oracle_audit* out) { return audit_step(*s, dt, out); }
+
 double oracle_sim_energy_residual(const oracle_sim* s) {
     return mechanical_energy(*s) - s->ledger.initial_mechanical_energy - ledgered_energy(s->ledger);
 }

@@ -118,6 +118,26 @@ typedef struct {
 
 typedef struct oracle_sim oracle_sim;
 
+/* EnergyAudit, audit.rs:17-250, flattened.
+ *   before/after: translation, affine (h^2/8 sum m |C|^2), elastic, potential, pellet;
+ *   stages: grid_transport, grid_after_stress, grid_after_traction, grid_after_gravity,
+ *           grid_after_walls, grid_after_coupling, pellet_after_coupling, plastic_dissipation;
+ *   increments: plastic_dissipation, wall_friction_dissipation, wall_normal_projection_energy,
+ *           wall_normal_traction_energy, coupling_grid_energy, coupling_pellet_energy, pellet_wall_work;
+ *   terms: p2g_transfer_loss, g2p_transfer_loss, stress_storage_plastic_mismatch, gravity
+ *           (uncoupled_gravity_potential_mismatch when coupled == 0, else
+ *           coupled_grid_gravity_total_potential_change), pellet_integrator_combined_change,
+ *           traction_change, walls_change, coupling_change, augmented_energy_change,
+ *           telescoping_sum, telescoping_error. */
+typedef struct {
+    double time, dt;
+    double before[5], after[5];
+    double stages[8];
+    double increments[7];
+    double terms[11];
+    int coupled;
+} oracle_audit;
+
 /* Step failure codes (solver.rs:249-259, constitutive.rs:41-66). */
 enum {
     ORACLE_OK = 0,
@@ -217,6 +237,8 @@ double oracle_sim_kinetic_energy(const oracle_sim* s);
 double oracle_sim_potential_energy(const oracle_sim* s);
 double oracle_sim_elastic_energy(const oracle_sim* s);
 double oracle_sim_mechanical_energy(const oracle_sim* s);
+/* Simulation::audit_step, audit.rs:275-345: 0, or ORACLE_ERR_INVALID without touching the simulation. */
+int oracle_sim_audit_step(const oracle_sim* s, double dt, oracle_audit* out);
 double oracle_sim_energy_residual(const oracle_sim* s);
 /* Thread count used by the node- and particle-parallel phases. */
 void oracle_set_threads(int threads);

@@ -80,6 +80,32 @@ def build():
 _lib = None
 
 
+class Audit(C.Structure):
+    """EnergyAudit (audit.rs:17-250), flattened as oracle_audit in mpm_oracle.h."""
+    _fields_ = [("time", C.c_double), ("dt", C.c_double), ("before", C.c_double * 5), ("after", C.c_double * 5),
+                ("stages", C.c_double * 8), ("increments", C.c_double * 7), ("terms", C.c_double * 11),
+                ("coupled", C.c_int)]
+
+    STATE = ("translation", "affine", "elastic", "potential", "pellet")
+    STAGES = ("grid_transport", "grid_after_stress", "grid_after_traction", "grid_after_gravity", "grid_after_walls",
+              "grid_after_coupling", "pellet_after_coupling", "plastic_dissipation")
+    INCREMENTS = ("plastic_dissipation", "wall_friction_dissipation", "wall_normal_projection_energy",
+                  "wall_normal_traction_energy", "coupling_grid_energy", "coupling_pellet_energy", "pellet_wall_work")
+    TERMS = ("p2g_transfer_loss", "g2p_transfer_loss", "stress_storage_plastic_mismatch", "gravity",
+             "pellet_integrator_combined_change", "traction_change", "walls_change", "coupling_change",
+             "augmented_energy_change", "telescoping_sum", "telescoping_error")
+
+    def as_dict(self):
+        """The reference's serialised layout (Option terms as None)."""
+        terms = dict(zip(self.TERMS, self.terms))
+        g = terms.pop("gravity")
+        terms["uncoupled_gravity_potential_mismatch"] = None if self.coupled else g
+        terms["coupled_grid_gravity_total_potential_change"] = g if self.coupled else None
+        return {"time": self.time, "dt": self.dt, "before": dict(zip(self.STATE, self.before)),
+                "after": dict(zip(self.STATE, self.after)), "stages": dict(zip(self.STAGES, self.stages)),
+                "increments": dict(zip(self.INCREMENTS, self.increments)), "terms": terms}
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -137,6 +163,7 @@ def lib():
             "oracle_sim_mechanical_energy": (C.c_double, [C.c_void_p]),
             "oracle_sim_energy_residual": (C.c_double, [C.c_void_p]),
             "oracle_set_threads": (None, [C.c_int]),
+            "oracle_sim_audit_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(Audit)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -374,6 +401,13 @@ class Sim:
 
     def energy_residual(self):
         return lib().oracle_sim_energy_residual(self.h)
+
+    def audit_step(self, dt) -> dict:
+        """Simulation::audit_step (audit.rs:275-345); raises ValueError when the reference would err."""
+        a = Audit()
+        if lib().oracle_sim_audit_step(self.h, dt, C.byref(a)) != 0:
+            raise ValueError("audit rejected")
+        return a.as_dict()
 
 
 def hb_return(trial, ys, k, n, mu, dt):
