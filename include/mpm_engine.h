@@ -327,6 +327,41 @@ int mpm_log_returns(mpm_engine* engine, size_t capacity);
 int mpm_take_returns(mpm_engine* engine, float* inputs, float* targets, float* aux, size_t capacity,
                      size_t* n);
 
+/* ------------------------------------------------------------------ validation and energy audit */
+
+/* Simulation::validate, solver.rs:1106-1140 (with ParticleSet::validate,
+ * particles.rs:159-211, and Ledger::validate, solver.rs:182-243): 0, or
+ * MPM_ERR_INVALID with the reason in mpm_last_error.  Single-instance engines. */
+int mpm_validate(mpm_engine* engine);
+
+/* EnergyAudit, audit.rs:17-250.
+ *   before/after: translation, affine (h^2/8 sum m |C|^2), elastic, potential, pellet;
+ *   stages: grid_transport, grid_after_stress, grid_after_traction, grid_after_gravity,
+ *     grid_after_walls, grid_after_coupling, pellet_after_coupling, plastic_dissipation;
+ *   increments: plastic_dissipation, wall_friction_dissipation, wall_normal_projection_energy,
+ *     wall_normal_traction_energy, coupling_grid_energy, coupling_pellet_energy, pellet_wall_work;
+ *   terms: p2g_transfer_loss, g2p_transfer_loss, stress_storage_plastic_mismatch, gravity
+ *     (uncoupled_gravity_potential_mismatch when coupled == 0, else
+ *     coupled_grid_gravity_total_potential_change), pellet_integrator_combined_change,
+ *     traction_change, walls_change, coupling_change, augmented_energy_change,
+ *     telescoping_sum, telescoping_error. */
+typedef struct {
+    double time, dt;
+    double before[5], after[5];
+    double stages[8];
+    double increments[7];
+    double terms[11];
+    int coupled;
+} mpm_energy_audit;
+/* Simulation::audit_step, audit.rs:275-345: measure one trial step of size dt
+ * without changing the simulation (the device state is snapshotted and
+ * restored).  The grid stage energies are reduced on the device in f64 from
+ * the f32 node values.  Errors (invalid state, dt outside
+ * [min_dt, min(max_dt, dt_scale * stability_limit)], a rejected trial,
+ * nonfinite energies) return MPM_ERR_INVALID with the reason in mpm_last_error.
+ * Single-instance engines. */
+int mpm_audit_step(mpm_engine* engine, double dt, mpm_energy_audit* out);
+
 /* ------------------------------------------------------------------ research wire layer
  * Strings returned through a char** are allocated by the library (also on
  * failure, where they hold the error message) and released with

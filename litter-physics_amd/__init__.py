@@ -107,6 +107,32 @@ class Launch(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class EnergyAudit(C.Structure):
+    """EnergyAudit (audit.rs:17-250) as mpm_energy_audit in include/mpm_engine.h."""
+    _fields_ = [("time", C.c_double), ("dt", C.c_double), ("before", C.c_double * 5), ("after", C.c_double * 5),
+                ("stages", C.c_double * 8), ("increments", C.c_double * 7), ("terms", C.c_double * 11),
+                ("coupled", C.c_int)]
+
+    STATE = ("translation", "affine", "elastic", "potential", "pellet")
+    STAGES = ("grid_transport", "grid_after_stress", "grid_after_traction", "grid_after_gravity", "grid_after_walls",
+              "grid_after_coupling", "pellet_after_coupling", "plastic_dissipation")
+    INCREMENTS = ("plastic_dissipation", "wall_friction_dissipation", "wall_normal_projection_energy",
+                  "wall_normal_traction_energy", "coupling_grid_energy", "coupling_pellet_energy", "pellet_wall_work")
+    TERMS = ("p2g_transfer_loss", "g2p_transfer_loss", "stress_storage_plastic_mismatch", "gravity",
+             "pellet_integrator_combined_change", "traction_change", "walls_change", "coupling_change",
+             "augmented_energy_change", "telescoping_sum", "telescoping_error")
+
+    def as_dict(self):
+        """The reference's serialised layout (the Option terms as None)."""
+        terms = dict(zip(self.TERMS, self.terms))
+        g = terms.pop("gravity")
+        terms["uncoupled_gravity_potential_mismatch"] = None if self.coupled else g
+        terms["coupled_grid_gravity_total_potential_change"] = g if self.coupled else None
+        return {"time": self.time, "dt": self.dt, "before": dict(zip(self.STATE, self.before)),
+                "after": dict(zip(self.STATE, self.after)), "stages": dict(zip(self.STAGES, self.stages)),
+                "increments": dict(zip(self.INCREMENTS, self.increments)), "terms": terms}
+
+
 class FixtureSpec(C.Structure):
     _fields_ = [("fixture", C.c_int), ("material", Material), ("grid_spacing", C.c_double),
                 ("domain", C.c_double * 3), ("initial_size", C.c_double * 3),
@@ -192,6 +218,8 @@ def lib():
         "mpm_surrogate_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "mpm_log_returns": (C.c_int, [C.c_void_p, C.c_size_t]),
         "mpm_take_returns": (C.c_int, [C.c_void_p, fp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "mpm_validate": (C.c_int, [C.c_void_p]),
+        "mpm_audit_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(EnergyAudit)]),
         "mpm_protocol_version": (C.c_int, []),
         "mpm_string_free": (None, [C.c_void_p]),
         "mpm_run_json": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -225,7 +253,7 @@ EXPORTED = [
     "mpm_get_launch", "mpm_set_launch", "mpm_autotune", "mpm_mlp_param_count", "mpm_mlp_create", "mpm_mlp_destroy",
     "mpm_mlp_last_error", "mpm_mlp_get_params", "mpm_mlp_set_params", "mpm_mlp_forward", "mpm_mlp_loss_grad",
     "mpm_mlp_train", "mpm_set_surrogate", "mpm_surrogate_stats", "mpm_log_returns", "mpm_take_returns",
-    "mpm_protocol_version", "mpm_string_free", "mpm_run_json", "mpm_research_validate", "mpm_research_observables",
+    "mpm_validate", "mpm_audit_step", "mpm_protocol_version", "mpm_string_free", "mpm_run_json", "mpm_research_validate", "mpm_research_observables",
     "mpm_checkpoint_encode", "mpm_checkpoint_particle_count", "mpm_checkpoint_decode",
 ]
 
@@ -495,6 +523,17 @@ class Simulation:
 
     def stability_limit(self):
         return lib().mpm_stability_limit(self.h)
+
+    def validate(self):
+        """Simulation::validate (solver.rs:1106-1140); raises MpmError with the reason."""
+        self._check(lib().mpm_validate(self.h))
+
+    def audit_step(self, dt) -> dict:
+        """Simulation::audit_step (audit.rs:275-345): one measured trial of size dt that
+        leaves the simulation unchanged; raises MpmError where the reference errs."""
+        a = EnergyAudit()
+        self._check(lib().mpm_audit_step(self.h, dt, C.byref(a)))
+        return a.as_dict()
 
     def mass_residual(self):
         return lib().mpm_mass_residual(self.h)

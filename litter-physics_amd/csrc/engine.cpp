@@ -334,6 +334,10 @@ struct mpm_engine {
     int* blk_inst = nullptr;
     double* gpart = nullptr;
     int* gbad = nullptr;
+    // energy audit (allocated on first use)
+    double* apart = nullptr;
+    double* d_audit = nullptr;
+    bool auditing = false;
     // instances
     InstParams* d_ip = nullptr;
     InstState* d_is = nullptr;
@@ -425,6 +429,8 @@ struct mpm_engine {
         B.counters = counters;
         B.gpart = gpart;
         B.gbad = gbad;
+        B.apart = auditing ? apart : nullptr;
+        B.audit = auditing ? d_audit : nullptr;
         B.pmax = pmax;
         B.pdiss = pdiss;
         B.pfail = pfail;
@@ -588,6 +594,8 @@ struct mpm_engine {
         (void)hipFree(blk_inst);
         (void)hipFree(gpart);
         (void)hipFree(gbad);
+        (void)hipFree(apart);
+        (void)hipFree(d_audit);
         (void)hipFree(d_ip);
         (void)hipFree(d_is);
         if (h_is) (void)hipHostFree(h_is);
@@ -850,7 +858,7 @@ struct mpm_engine {
     int enqueue(int k) {
         BatchDev B = dev();
         bool pel = any_pellets();
-        if (launch.use_graph && !profiling) {
+        if (launch.use_graph && !profiling && !auditing) {
             if (graph_stale || graph_steps != k || !graph) {
                 if (graph) {
                     HIP_OK(hipGraphExecDestroy(graph));
@@ -1558,6 +1566,241 @@ int mpm_try_step(mpm_engine* e, double dt, mpm_step_report* report) {
     if (e->dirty)
         if (int rc = e->sort_all()) return rc;
     return code;
+}
+
+// ------------------------------------------------------------ validation and the energy audit
+
+namespace {
+
+// A state read back in the reference's particle order (f64), for host-side checks.
+struct HostState {
+    std::vector<uint32_t> id;
+    std::vector<double> pos, vel, aff, def, ratio, tau, eps, mass, vol0;
+    std::vector<mpm_pellet> pellets;
+    mpm_ledger ledger{};
+    mpm_history history{};
+    size_t n() const { return id.size(); }
+    mpm_particles view() {
+        return mpm_particles{id.size(), id.data(), pos.data(), vel.data(), aff.data(), def.data(), ratio.data(),
+                             tau.data(), eps.data(), mass.data(), vol0.data()};
+    }
+};
+
+int read_state(mpm_engine* e, HostState& h) {
+    size_t n = mpm_particle_count(e);
+    h.id.resize(n);
+    for (auto* v : {&h.pos, &h.vel}) v->resize(3 * n);
+    for (auto* v : {&h.aff, &h.def, &h.tau}) v->resize(9 * n);
+    for (auto* v : {&h.ratio, &h.eps, &h.mass, &h.vol0}) v->resize(n);
+    h.pellets.resize(mpm_pellet_count(e));
+    mpm_particles ps = h.view();
+    return mpm_get_state(e, &ps, h.pellets.data(), h.pellets.size(), &h.ledger, &h.history);
+}
+
+// Simulation::validate, solver.rs:1106-1140, with ParticleSet::validate
+// (particles.rs:159-211) and Ledger::validate (solver.rs:182-243).  Returns
+// the reason, or an empty string.
+std::string validate_state(const HostState& h, double dx, const int* cells) {
+    std::vector<uint32_t> ids(h.id);
+    std::sort(ids.begin(), ids.end());
+    for (size_t k = 1; k < ids.size(); ++k)
+        if (ids[k] == ids[k - 1]) return "particle ids are not unique";
+    for (size_t p = 0; p < h.n(); ++p) {
+        bool ok = std::isfinite(h.ratio[p]) && std::isfinite(h.eps[p]) && std::isfinite(h.mass[p]) &&
+                  std::isfinite(h.vol0[p]);
+        for (int a = 0; a < 3; ++a) ok = ok && std::isfinite(h.pos[3 * p + a]) && std::isfinite(h.vel[3 * p + a]);
+        for (int a = 0; a < 9; ++a)
+            ok = ok && std::isfinite(h.aff[9 * p + a]) && std::isfinite(h.def[9 * p + a]) && std::isfinite(h.tau[9 * p + a]);
+        std::string who = "particle id '" + std::to_string(h.id[p]) + "'";
+        if (!ok) return who + " has non-finite state";
+        if (h.mass[p] <= 0.0 || h.vol0[p] <= 0.0) return who + " has non-positive mass or volume";
+        if (h.ratio[p] <= 0.0 || h.eps[p] < 0.0) return who + " has invalid ratio/plastic state";
+    }
+    const mpm_ledger& L = h.ledger;
+    const double* flat = reinterpret_cast<const double*>(&L);
+    for (int k = 0; k < kLedger; ++k)
+        if (!std::isfinite(flat[k])) return "ledger is nonfinite";
+    if (L.initial_mass < 0.0 || L.outflow_mass < 0.0 || L.plastic_dissipation < 0.0 || L.wall_friction_dissipation < 0.0)
+        return "ledger has a negative mass or dissipation";
+    if (L.wall_normal_projection_energy > 0.0) return "ledger `wall_normal_projection_energy` must be non-positive";
+    if (!std::isfinite(h.history.time) || h.history.time < 0.0) return "time invalid";
+    if (!(std::isfinite(h.history.dt_scale) && h.history.dt_scale > 0.0 && h.history.dt_scale <= 1.0))
+        return "dt_scale invalid";
+    for (const mpm_pellet& pl : h.pellets) {
+        bool ok = true;
+        for (int a = 0; a < 3; ++a)
+            ok = ok && std::isfinite(pl.position[a]) && std::isfinite(pl.velocity[a]) && std::isfinite(pl.angular_momentum[a]);
+        for (int a = 0; a < 4; ++a) ok = ok && std::isfinite(pl.orientation[a]);
+        if (!ok) return "pellet state non-finite";
+    }
+    for (size_t p = 0; p < h.n(); ++p)
+        for (int a = 0; a < 3; ++a) {
+            double padded = std::floor(h.pos[3 * p + a] / dx - 0.5) + 1.0;
+            if (std::isnan(padded) || padded < 0.0 || padded + 2.0 >= (double)(cells[a] + 3))
+                return "particle id '" + std::to_string(h.id[p]) + "' lies outside the grid";
+        }
+    return "";
+}
+
+// EnergyState::measure, audit.rs:37-54: translation, affine norm h^2/8 sum m |C|^2,
+// elastic, potential (material and pellets), pellet kinetic energy.
+void measure_energy(HostState& h, const mpm_material& mat, const double* g, double dx, double* e) {
+    double translation = 0.0, affine = 0.0;
+    for (size_t p = 0; p < h.n(); ++p) {
+        const double* v = &h.vel[3 * p];
+        translation += 0.5 * h.mass[p] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    for (size_t p = 0; p < h.n(); ++p) {
+        const double* c = &h.aff[9 * p];
+        double n2 = 0.0;  // column by column, as nalgebra stores the matrix
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) n2 += c[3 * i + j] * c[3 * i + j];
+        affine += h.mass[p] * n2;
+    }
+    mpm_particles ps = h.view();
+    double ke = 0.0, pe = 0.0, el = 0.0;
+    energies_f64(mat, g, ps, h.pellets.data(), h.pellets.size(), &ke, &pe, &el);
+    double pellet = 0.0;
+    for (const mpm_pellet& pl : h.pellets) pellet += mpm_host::pellet_kinetic(pl);
+    e[0] = translation;
+    e[1] = dx * dx / 8.0 * affine;
+    e[2] = el;
+    e[3] = pe;
+    e[4] = pellet;
+}
+
+const char* failure_name(int code) {
+    switch (code) {
+        case MPM_FAIL_INVERTED: return "constitutive update failed: inverted deformation";
+        case MPM_FAIL_EIGEN: return "constitutive update failed: eigen-decomposition did not converge";
+        case MPM_FAIL_STRETCH: return "constitutive update failed: degenerate stretch";
+        case MPM_FAIL_RETURN: return "constitutive update failed: return map did not converge";
+        case MPM_FAIL_NONFINITE: return "constitutive update failed: non-finite quantity";
+        case MPM_FAIL_NONFINITE_GRID: return "non-finite grid velocity";
+        case MPM_FAIL_DISPLACEMENT: return "particle moved more than one cell in one step";
+        case MPM_FAIL_PELLET: return "pellet state became non-finite";
+        case MPM_FAIL_TRACTION: return "wall traction kinetic-energy jump became non-finite";
+        default: return "step failed";
+    }
+}
+
+bool all_finite(const double* v, int n) {
+    for (int k = 0; k < n; ++k)
+        if (!std::isfinite(v[k])) return false;
+    return true;
+}
+
+}  // namespace
+
+int mpm_validate(mpm_engine* e) {
+    if (!e) return MPM_ERR_INVALID;
+    if (e->ninst != 1) return e->fail(MPM_ERR_INVALID, "validation covers a single-instance engine");
+    HostState h;
+    if (int rc = read_state(e, h)) return rc;
+    std::string bad = validate_state(h, e->inst[0].dx, e->inst[0].cells);
+    if (!bad.empty()) return e->fail(MPM_ERR_INVALID, "invalid solver state: " + bad);
+    return MPM_OK;
+}
+
+// Simulation::audit_step, audit.rs:275-345: one trial of the production
+// kernels from a device snapshot, with the grid stage energies reduced on the
+// device, and the snapshot restored afterwards.  The two coupling stages follow
+// from the trial's coupling histories (the kinetic-energy changes of the
+// constrained nodes and of the pellets, solver.rs:568-585).
+int mpm_audit_step(mpm_engine* e, double dt, mpm_energy_audit* out) {
+    if (!e || !out) return MPM_ERR_INVALID;
+    if (e->ninst != 1) return e->fail(MPM_ERR_INVALID, "the energy audit covers a single-instance engine");
+    HostState before;
+    if (int rc = read_state(e, before)) return rc;
+    const Instance& in = e->inst[0];
+    std::string bad = validate_state(before, in.dx, in.cells);
+    if (!bad.empty()) return e->fail(MPM_ERR_INVALID, "invalid solver state: " + bad);
+    const mpm_config& c = in.cfg;
+    const double stability = e->stability_limit(0);
+    if (!all_finite(c.gravity, 3) || !std::isfinite(c.cfl) || c.cfl <= 0.0 || !std::isfinite(c.wall_friction) ||
+        c.wall_friction < 0.0 || !std::isfinite(c.min_dt) || c.min_dt <= 0.0 || !std::isfinite(c.max_dt) ||
+        c.max_dt < c.min_dt || !std::isfinite(stability) || stability <= 0.0)
+        return e->fail(MPM_ERR_INVALID, "invalid solver state: invalid audit step configuration or stability limit");
+    const double limit = std::min(before.history.dt_scale * stability, c.max_dt);
+    if (!std::isfinite(dt) || dt <= 0.0 || dt < c.min_dt || !std::isfinite(limit) || dt > limit) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "invalid solver state: audit `dt`='%.17g' outside finite accepted range ending at '%.17g'",
+                      dt, limit);
+        return e->fail(MPM_ERR_INVALID, buf);
+    }
+    mpm_energy_audit a{};
+    a.time = before.history.time;
+    a.dt = dt;
+    measure_energy(before, in.mat, c.gravity, in.dx, a.before);
+    if (!all_finite(a.before, 5)) return e->fail(MPM_ERR_INVALID, "invalid solver state: nonfinite initial audit energy");
+    if (!e->apart) {
+        if (int rc = e->dalloc(&e->apart, (size_t)kAuditVals * e->ngwg)) return rc;
+        if (int rc = e->dalloc(&e->d_audit, (size_t)kAuditOut * e->ninst)) return rc;
+    }
+    mpm_engine::Snapshot snap;
+    if (int rc = e->snapshot(snap)) {
+        mpm_engine::release(snap);
+        return rc;
+    }
+    const InstParams saved = e->inst[0].ip;
+    e->auditing = true;
+    mpm_step_report report{};
+    const int code = mpm_try_step(e, dt, &report);
+    e->auditing = false;
+    double dev[kAuditOut] = {0, 0, 0, 0, 0, 0};
+    HostState after;
+    int rc_read = 0;
+    if (code == 0) {
+        if (hipMemcpy(dev, e->d_audit, sizeof(dev), hipMemcpyDeviceToHost) != hipSuccess) rc_read = MPM_ERR_DEVICE;
+        if (!rc_read) rc_read = read_state(e, after);
+    }
+    e->inst[0].ip = saved;
+    int rc_restore = e->upload_params(0);
+    if (!rc_restore) rc_restore = e->restore(snap);
+    mpm_engine::release(snap);
+    if (rc_restore) return rc_restore;
+    if (code < 0) return code;
+    if (code > 0) return e->fail(MPM_ERR_INVALID, std::string("invalid solver state: audit trial rejected: ") + failure_name(code));
+    if (rc_read) return e->fail(rc_read, "device failure reading the audit trial");
+    if (after.n() != before.n())
+        return e->fail(MPM_ERR_INVALID, "invalid solver state: a particle lies outside the grid after the audit trial");
+    bad = validate_state(after, in.dx, in.cells);
+    if (!bad.empty()) return e->fail(MPM_ERR_INVALID, "invalid solver state: " + bad);
+    measure_energy(after, in.mat, c.gravity, in.dx, a.after);
+    const mpm_ledger &B = before.ledger, &A = after.ledger;
+    a.increments[0] = A.plastic_dissipation - B.plastic_dissipation;
+    a.increments[1] = A.wall_friction_dissipation - B.wall_friction_dissipation;
+    a.increments[2] = A.wall_normal_projection_energy - B.wall_normal_projection_energy;
+    a.increments[3] = A.wall_normal_traction_energy - B.wall_normal_traction_energy;
+    a.increments[4] = A.coupling_grid_energy - B.coupling_grid_energy;
+    a.increments[5] = A.coupling_pellet_energy - B.coupling_pellet_energy;
+    a.increments[6] = A.pellet_wall_work - B.pellet_wall_work;
+    const bool coupled = !before.pellets.empty();
+    double* st = a.stages;
+    for (int k = 0; k < kAuditVals; ++k) st[k] = dev[k];
+    st[5] = st[4] + a.increments[4];
+    st[6] = coupled ? a.before[4] + a.increments[5] : 0.0;
+    st[7] = dev[kAuditVals];
+    // EnergyTerms::measure, audit.rs:193-232.
+    const double* be = a.before;
+    const double* af = a.after;
+    const double p2g = be[0] + be[1] - st[0];
+    const double g2p = st[5] - af[0] - af[1];
+    const double stress = st[1] - st[0] + af[2] - be[2] + st[7];
+    const double gravity = st[3] - st[2] + af[3] - be[3];
+    const double integrator = af[4] - st[6];
+    const double traction = st[2] - st[1];
+    const double walls = st[4] - st[3];
+    const double coupling = st[5] - st[4] + st[6] - be[4];
+    const double change = (af[0] + af[1] + af[2] + af[3] + af[4]) - (be[0] + be[1] + be[2] + be[3] + be[4]);
+    const double sum = -p2g - g2p + stress - st[7] + gravity + integrator + traction + walls + coupling;
+    const double terms[11] = {p2g, g2p, stress, gravity, integrator, traction, walls, coupling, change, sum, change - sum};
+    std::copy(terms, terms + 11, a.terms);
+    a.coupled = coupled ? 1 : 0;
+    if (!all_finite(a.after, 5) || !all_finite(a.stages, 8) || !all_finite(a.increments, 7) || !all_finite(a.terms, 11))
+        return e->fail(MPM_ERR_INVALID, "invalid solver state: audit energy became nonfinite");
+    *out = a;
+    return MPM_OK;
 }
 
 // Simulation::advance_while, solver.rs:413-472, for instance 0.  With a

@@ -65,6 +65,10 @@ enum { E_MIN_STEP = 1, E_MIN_AFTER_REJECT = 2 };
 constexpr int kMaxSpheres = 64;
 constexpr int kLedger = 31;  // doubles of mpm_ledger
 constexpr int kGridVals = 12;
+// Energy audit (audit.rs): grid kinetic energies at five stages of one trial
+// (transport, after stress, after traction, after gravity, after walls), and
+// per instance those five plus the trial's plastic-work sum.
+constexpr int kAuditVals = 5, kAuditOut = 6;
 
 // Launch-constant parameters of one instance (f32 copies of the f64 host values).
 struct Params {
@@ -202,6 +206,8 @@ struct BatchDev {
     int ngwg;         // grid-update workgroups (all instances)
     int* counters;    // [0] unused, [1] near-wall entries of the running instances
     double* gpart;    // grid partials, field-major: gpart[v * ngwg + workgroup]
+    double* apart;    // audit partials, field-major like gpart (NULL: no audit)
+    double* audit;    // audit sums per instance, kAuditOut each (NULL: no audit)
     int* gbad;
     float* pmax;      // g2p2g partials per work item
     double* pdiss;

@@ -283,8 +283,15 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
 // workgroup, one per wavefront; ledger sums go to per-workgroup partials (no
 // same-address atomics).  The other accumulator set is cleared on the way for
 // the trial deposit.
+// AUDIT: also the grid kinetic energy sum 1/2 m |v|^2 over nodes with mass at
+// the stages where the reference's audit takes it (solver.rs:540-625,
+// audit.rs:360-398): the APIC transport momentum alone (the accumulated
+// sum w (m v + m C d) is exactly the audit's shadow deposit), after the stress
+// impulse, after the wall traction, with gravity's kick, and after the walls.
+template <bool AUDIT>
 __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
     __shared__ double scratch[4 * kGridVals];
+    __shared__ double ascratch[AUDIT ? 4 * kAuditVals : 1];
     __shared__ int sbad[4];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int b = blockIdx.x * 4 + wave;
@@ -297,6 +304,7 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
     const float dt = (float)s.dt;
     const size_t na = B.na;
     double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
     const size_t n = ((size_t)b << 6) | l;
     if (C.flag[b]) {
@@ -327,6 +335,14 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
                 }
                 if (tb[a] != 0.0f) B.trT[a * na + n] = 0.0f;
                 v[a] = (p[a] + D) / m + P.gravity[a] * dt;
+                if (AUDIT) {
+                    const double md = (double)m;
+                    const double pt = a == 0 ? (double)A.px[n] : (a == 1 ? (double)A.py[n] : (double)A.pz[n]);
+                    avals[0] += 0.5 * pt * pt / md;
+                    avals[1] += 0.5 * (double)p[a] * (double)p[a] / md;
+                    avals[2] += 0.5 * ((double)p[a] + (double)D) * ((double)p[a] + (double)D) / md;
+                    avals[3] += 0.5 * md * (double)v[a] * (double)v[a];
+                }
             }
             if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
                 bad = 1;
@@ -367,6 +383,7 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
                 for (int a = 0; a < 3; ++a) {
                     vals[a] = ((double)v[a] - (double)before[a]) * (double)m;
                     vals[9 + a] = (double)v[a] * (double)m;
+                    if (AUDIT) avals[4] += 0.5 * (double)m * (double)v[a] * (double)v[a];
                 }
             }
             B.gv[n] = v[0];
@@ -381,10 +398,14 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
     int nbad = wave_sum_i(bad);
     if (l == 0) sbad[wave] = nbad;
     block_sum<kGridVals>(vals, scratch);
+    if (AUDIT) block_sum<kAuditVals>(avals, ascratch);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 0; i < kGridVals; ++i) B.gpart[(size_t)i * B.ngwg + blockIdx.x] = vals[i];
         B.gbad[blockIdx.x] = sbad[0] + sbad[1] + sbad[2] + sbad[3];
+        if (AUDIT)
+#pragma unroll
+            for (int i = 0; i < kAuditVals; ++i) B.apart[(size_t)i * B.ngwg + blockIdx.x] = avals[i];
     }
 }
 
@@ -1121,6 +1142,16 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     }
     double tot[kGridVals + 1];
     for (int i = 0; i <= kGridVals; ++i) tot[i] = block_total(acc[i], sd);
+    if (B.audit) {
+        // Audit stage sums of this instance, in the same fixed order.
+        for (int i = 0; i < kAuditVals; ++i) {
+            double a = 0.0;
+            for (int q = g0 + tid; q < g1; q += 256) a += B.apart[(size_t)i * B.ngwg + q];
+            a = block_total(a, sd);
+            if (tid == 0) B.audit[(size_t)inst * kAuditOut + i] = a;
+        }
+        if (tid == 0) B.audit[(size_t)inst * kAuditOut + kAuditVals] = tot[kGridVals];
+    }
     // Pellet sums (the instance's pellets, in parallel).
     double pel[kPelSums];
     for (int i = 0; i < kPelSums; ++i) pel[i] = 0.0;
@@ -1342,7 +1373,10 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
                  hipEvent_t g2p2g_stop) {
     control_kernel<<<1, 256, 0, s>>>(B);
     traction_kernel<<<traction_wgs, kBlock, 0, s>>>(B);
-    grid_update_kernel<<<B.ngwg, 256, 0, s>>>(B);
+    if (B.apart)
+        grid_update_kernel<true><<<B.ngwg, 256, 0, s>>>(B);
+    else
+        grid_update_kernel<false><<<B.ngwg, 256, 0, s>>>(B);
     if (pellets) couple_kernel<<<B.ngwg * 4, 64, 0, s>>>(B);
     if (g2p2g_start) (void)hipEventRecord(g2p2g_start, s);
     if (B.nwork > 0) {

@@ -1818,6 +1818,8 @@ double oracle_sim_elastic_energy(const oracle_sim* s) { return elastic_energy(*s
 double oracle_sim_mechanical_energy(const oracle_sim* s) { return mechanical_energy(*s); }
 int oracle_sim_audit_step(const oracle_sim* s, double dt, oracle_audit* out) { return audit_step(*s, dt, out); }
 
+void oracle_sim_get_material(const oracle_sim* s, oracle_material* out) { *out = s->material; }
+
 double oracle_sim_energy_residual(const oracle_sim* s) {
     return mechanical_energy(*s) - s->ledger.initial_mechanical_energy - ledgered_energy(s->ledger);
 }

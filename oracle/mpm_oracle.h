@@ -239,6 +239,7 @@ double oracle_sim_elastic_energy(const oracle_sim* s);
 double oracle_sim_mechanical_energy(const oracle_sim* s);
 /* Simulation::audit_step, audit.rs:275-345: 0, or ORACLE_ERR_INVALID without touching the simulation. */
 int oracle_sim_audit_step(const oracle_sim* s, double dt, oracle_audit* out);
+void oracle_sim_get_material(const oracle_sim* s, oracle_material* out);
 double oracle_sim_energy_residual(const oracle_sim* s);
 /* Thread count used by the node- and particle-parallel phases. */
 void oracle_set_threads(int threads);

@@ -164,6 +164,7 @@ def lib():
             "oracle_sim_energy_residual": (C.c_double, [C.c_void_p]),
             "oracle_set_threads": (None, [C.c_int]),
             "oracle_sim_audit_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(Audit)]),
+            "oracle_sim_get_material": (None, [C.c_void_p, C.POINTER(Material)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
