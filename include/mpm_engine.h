@@ -380,6 +380,17 @@ void mpm_string_free(char* s);
  * (numerical failure) or MPM_ERR_DEVICE, with the message in *output (the
  * reference's ValueError text). */
 int mpm_run_json(const char* request_json, const char* resume_json, char** output);
+/* run_json over many research requests at once.  The reference's sweep and
+ * refinement-study runners call run_json once per job, in sequence
+ * (python/litter_physics/sweep.py:206-261, verification.py:645-680); here all
+ * valid jobs step together as the instances of one batch engine, each to its
+ * own record boundaries under its own wall budget, and each ends with the
+ * output run_json would give it.  requests_json: a JSON array of requests;
+ * resumes_json: NULL, or an array of the same length of checkpoints or nulls.
+ * *output: a JSON array with one element per request, {"ok": true, "output":
+ * <run_json output>} or {"ok": false, "error": "<message>"}.  Returns 0 unless
+ * the arrays are malformed or the device fails. */
+int mpm_run_json_batch(const char* requests_json, const char* resumes_json, char** output);
 /* specification and FixtureSpec::validate, research.rs:170-232 and
  * fixtures.rs:152-236, without building anything: 0 or MPM_ERR_INVALID with
  * the message in *error. */

@@ -223,6 +223,7 @@ def lib():
         "mpm_protocol_version": (C.c_int, []),
         "mpm_string_free": (None, [C.c_void_p]),
         "mpm_run_json": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+        "mpm_run_json_batch": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
         "mpm_research_validate": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
         "mpm_research_observables": (C.c_int, [C.c_char_p, C.POINTER(Particles), C.POINTER(Pellet), C.c_size_t,
                                                C.POINTER(Ledger), C.POINTER(History), C.POINTER(C.c_void_p)]),
@@ -253,7 +254,7 @@ EXPORTED = [
     "mpm_get_launch", "mpm_set_launch", "mpm_autotune", "mpm_mlp_param_count", "mpm_mlp_create", "mpm_mlp_destroy",
     "mpm_mlp_last_error", "mpm_mlp_get_params", "mpm_mlp_set_params", "mpm_mlp_forward", "mpm_mlp_loss_grad",
     "mpm_mlp_train", "mpm_set_surrogate", "mpm_surrogate_stats", "mpm_log_returns", "mpm_take_returns",
-    "mpm_validate", "mpm_audit_step", "mpm_protocol_version", "mpm_string_free", "mpm_run_json", "mpm_research_validate", "mpm_research_observables",
+    "mpm_validate", "mpm_audit_step", "mpm_protocol_version", "mpm_string_free", "mpm_run_json", "mpm_run_json_batch", "mpm_research_validate", "mpm_research_observables",
     "mpm_checkpoint_encode", "mpm_checkpoint_particle_count", "mpm_checkpoint_decode",
 ]
 
@@ -857,6 +858,21 @@ def run_json(request_json: str, resume_json: str | None = None) -> str:
     if rc != 0:
         raise ValueError(text)
     return text
+
+
+def run_json_batch(requests_json, resumes_json=None) -> list:
+    """run_json for many research requests stepped together on one batch engine.
+    Returns one dict per request: {"ok": True, "output": <output dict>} or
+    {"ok": False, "error": <message>}."""
+    import json
+    reqs = "[" + ",".join(requests_json) + "]"
+    res = None if resumes_json is None else "[" + ",".join("null" if r is None else r for r in resumes_json) + "]"
+    out = C.c_void_p()
+    rc = lib().mpm_run_json_batch(reqs.encode(), None if res is None else res.encode(), C.byref(out))
+    text = _take_string(out)
+    if rc != 0:
+        raise ValueError(text)
+    return json.loads(text)
 
 
 def research_validate(request_json: str) -> None:

@@ -1323,8 +1323,10 @@ void validate_snapshot(const Checkpoint& cp, const Request& r, const State& init
     if (!pel_ok) bail("research checkpoint pellet geometry is incompatible");
 }
 
-// restore, research.rs:324-368.
-uint32_t restore(Sim& sim, const Spec& spec, const Request& r, const Value& value) {
+// restore, research.rs:324-368, up to the assignment: the checkpoint checked
+// against the request and against the freshly built initial state.
+Checkpoint checked_checkpoint(const Spec& spec, const Request& r, const Value& value, const State& initial, double dx,
+                              const size_t* cells) {
     Checkpoint cp = parse_checkpoint(value);
     if (cp.schema_version != 1 || cp.mode != "research" || !std::isfinite(cp.time_s) || cp.time_s < 0.0 ||
         cp.time_s > r.duration_s)
@@ -1337,13 +1339,18 @@ uint32_t restore(Sim& sim, const Spec& spec, const Request& r, const Value& valu
     if (cp.state.n() > count) bail("research checkpoint particle identities are incompatible");
     for (uint32_t id : cp.state.id)
         if ((size_t)id >= count) bail("research checkpoint particle identities are incompatible");
-    validate_snapshot(cp, r, sim.st);
+    validate_snapshot(cp, r, initial);
     // check_closed_state would refuse a particle outside the grid after the
     // assignment; refusing it before the state reaches the device is the same
     // outcome.
     for (size_t p = 0; p < cp.state.n(); ++p)
-        if (!inside_grid(&cp.state.pos[3 * p], sim.dx, sim.cells))
+        if (!inside_grid(&cp.state.pos[3 * p], dx, cells))
             bail("invalid solver state: particle id '" + std::to_string(cp.state.id[p]) + "' lies outside the grid");
+    return cp;
+}
+
+uint32_t restore(Sim& sim, const Spec& spec, const Request& r, const Value& value) {
+    Checkpoint cp = checked_checkpoint(spec, r, value, sim.st, sim.dx, sim.cells);
     sim.st = cp.state;
     sim.push();
     sim.pull();
@@ -1405,6 +1412,9 @@ int keep_going(void* ctx) {
     return b->elapsed() < b->limit ? 1 : 0;
 }
 
+Value output_json(const Spec& spec, const Value& request, const State& st, uint32_t next_record, bool completed,
+                  Value frames, Value metrics);
+
 // research::run, research.rs:72-132.
 Value run(const Value& request, const Value* resume) {
     Budget budget{std::chrono::steady_clock::now(), 0.0};
@@ -1436,8 +1446,16 @@ Value run(const Value& request, const Value* resume) {
     }
     sim.pull();
     check_closed_state(sim.st, sim.dx, sim.cells);
-    Value checkpoint = checkpoint_json(request, sim.st, next_record);
-    std::map<std::string, double> obs = observables(spec, sim.st);
+    Value out = output_json(spec, request, sim.st, next_record, completed, std::move(frames), std::move(metrics));
+    if (budget.elapsed() >= typed.max_wall_time_s) out["status"] = Value::string("budget_exhausted");
+    return out;
+}
+
+// The output document of a segment, research.rs:102-131.
+Value output_json(const Spec& spec, const Value& request, const State& st, uint32_t next_record, bool completed,
+                  Value frames, Value metrics) {
+    Value checkpoint = checkpoint_json(request, st, next_record);
+    std::map<std::string, double> obs = observables(spec, st);
     Value observ = Value::object();
     for (const auto& kv : obs) {
         if (!std::isfinite(kv.second)) bail("research observables became nonfinite");
@@ -1448,7 +1466,7 @@ Value run(const Value& request, const Value* resume) {
     out["mode"] = Value::string("research");
     out["status"] = Value::string(completed ? "completed" : "budget_exhausted");
     out["fidelity"] = Value::string("research_unvalidated");
-    out["time_s"] = Value::number(sim.st.history.time);
+    out["time_s"] = Value::number(st.history.time);
     out["frames"] = std::move(frames);
     out["metrics"] = std::move(metrics);
     out["observables"] = std::move(observ);
@@ -1464,7 +1482,212 @@ Value run(const Value& request, const Value* resume) {
                           "signed grid kinetic-energy diagnostic, not physical wall work; energy closure remains open"})
         diag.a.push_back(Value::string(d));
     out["diagnostics"] = std::move(diag);
-    if (budget.elapsed() >= typed.max_wall_time_s) out["status"] = Value::string("budget_exhausted");
+    return out;
+}
+
+// ------------------------------------------------------------ batched segments
+// The reference's sweep and refinement-study runners call run_json once per
+// job, one after another (python/litter_physics/sweep.py:206-261,
+// verification.py:645-680).  Here every valid job is one instance of a single
+// batch engine; all of them advance together, each to its own record
+// boundaries, and each job ends with the output run_json would give it.
+
+struct Job {
+    Value request;
+    Request typed;
+    Spec spec;
+    int inst = -1;  // batch instance; -1 when the request failed validation
+    size_t cells[3] = {0, 0, 0};
+    State st;
+    uint32_t next_record = 1;
+    Value frames = Value::array(), metrics = Value::array();
+    bool finished = false;
+    Value result;
+};
+
+void job_fail(Job& j, const std::string& msg) {
+    j.finished = true;
+    j.result = Value::object();
+    j.result["ok"] = Value::boolean(false);
+    j.result["error"] = Value::string(msg);
+}
+
+// Final checks and the output document; `elapsed` past the job's budget marks
+// it budget_exhausted even when complete, as research::run does.
+void job_finish(Job& j, bool completed, double elapsed) {
+    try {
+        check_closed_state(j.st, j.spec.dx, j.cells);
+        Value out = output_json(j.spec, j.request, j.st, j.next_record, completed, std::move(j.frames),
+                                std::move(j.metrics));
+        if (elapsed >= j.typed.max_wall_time_s) out["status"] = Value::string("budget_exhausted");
+        j.finished = true;
+        j.result = Value::object();
+        j.result["ok"] = Value::boolean(true);
+        j.result["output"] = std::move(out);
+    } catch (const ResearchError& e) {
+        job_fail(j, e.what());
+    }
+}
+
+void pull_instance(mpm_engine* eng, Job& j) {
+    size_t n = mpm_instance_particle_count(eng, (size_t)j.inst);
+    j.st.resize(n);
+    mpm_particles v = j.st.view();
+    j.st.pellets.resize(j.spec.has_pellet && j.spec.fixture == 3 ? 1 : 0);
+    int rc = mpm_instance_get_state(eng, (size_t)j.inst, &v, j.st.pellets.data(), j.st.pellets.size(), &j.st.ledger,
+                                    &j.st.history);
+    if (rc) bail(mpm_last_error(eng), rc);
+}
+
+void push_instance(mpm_engine* eng, Job& j, bool seed_ledger) {
+    mpm_particles v = j.st.view();
+    int rc = mpm_instance_set_state(eng, (size_t)j.inst, &v, j.st.pellets.data(), j.st.pellets.size(),
+                                    seed_ledger ? nullptr : &j.st.ledger, seed_ledger ? nullptr : &j.st.history);
+    if (rc) bail(mpm_last_error(eng), rc);
+}
+
+// Fixture placement and pellet of a job (FixtureSpec::build, fixtures.rs:238-270).
+void place(Job& j) {
+    const Spec& s = j.spec;
+    mpm_fixture f{};
+    f.fixture = s.fixture;
+    f.material = s.material;
+    f.grid_spacing = s.dx;
+    for (int a = 0; a < 3; ++a) {
+        f.domain[a] = s.domain[a];
+        f.initial_size[a] = s.initial_size[a];
+        f.initial_velocity[a] = s.initial_velocity[a];
+    }
+    f.seed = s.seed;
+    f.gravity_norm = s.gravity_norm();
+    f.jitter = kJitterFraction;
+    j.st.resize(s.particle_count());
+    mpm_particles view = j.st.view();
+    if (mpm_fixture_place(&f, &view) != MPM_OK || view.n != s.particle_count()) bail("fixture placement failed");
+    j.st.pellets.clear();
+    if (s.fixture == 3 && s.has_pellet) {
+        double center[3] = {0.5 * s.domain[0], 0.5 * s.domain[1], s.initial_size[2] + kPelletGapCells * s.dx + s.pel_radius};
+        j.st.pellets.push_back(make_pellet(s.pel_radius, s.pel_length, s.pel_density, center));
+    }
+    grid_layout(s.domain, s.dx, s.max_nodes, j.cells);
+}
+
+struct BatchEngine {
+    mpm_engine* eng = nullptr;
+    ~BatchEngine() {
+        if (eng) mpm_destroy(eng);
+    }
+};
+
+Value run_batch(const Value& requests, const Value* resumes) {
+    Budget clock{std::chrono::steady_clock::now(), 0.0};
+    if (requests.kind != Value::Array) bail("the batch must be a JSON array of requests");
+    if (resumes && (resumes->kind != Value::Array || resumes->a.size() != requests.a.size()))
+        bail("resumes must be an array with one checkpoint or null per request");
+    std::vector<Job> jobs(requests.a.size());
+    std::vector<Job*> live;
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        Job& j = jobs[k];
+        j.request = requests.a[k];
+        try {
+            j.typed = parse_request(j.request);
+            j.spec = specification(j.typed);
+            place(j);
+            j.inst = (int)live.size();
+            live.push_back(&j);
+        } catch (const ResearchError& e) {
+            job_fail(j, e.what());
+        }
+    }
+    BatchEngine be;
+    if (!live.empty()) {
+        std::vector<mpm_material> mats;
+        std::vector<mpm_config> cfgs;
+        std::vector<mpm_layout> lays;
+        std::vector<size_t> caps, pels;
+        for (Job* j : live) {
+            mats.push_back(j->spec.material);
+            cfgs.push_back(j->spec.config);
+            mpm_layout l{};
+            for (int a = 0; a < 3; ++a) l.domain[a] = j->spec.domain[a];
+            l.spacing = j->spec.dx;
+            l.max_nodes = j->spec.max_nodes;
+            lays.push_back(l);
+            caps.push_back(j->st.n());
+            pels.push_back(j->st.pellets.size());
+        }
+        int rc = mpm_batch_create(live.size(), mats.data(), cfgs.data(), lays.data(), caps.data(), pels.data(), &be.eng);
+        if (rc) bail(std::string("engine creation failed: ") + (be.eng ? mpm_last_error(be.eng) : "no device"), rc);
+    }
+    // Seed every instance (Simulation::new), then restore or record the first frame.
+    for (Job* j : live) {
+        try {
+            push_instance(be.eng, *j, true);
+            pull_instance(be.eng, *j);
+            const Value* res = resumes ? &resumes->a[(size_t)(j - jobs.data())] : nullptr;
+            if (res && res->kind != Value::Null) {
+                Checkpoint cp = checked_checkpoint(j->spec, j->typed, *res, j->st, j->spec.dx, j->cells);
+                j->st = cp.state;
+                j->next_record = cp.next_record_index;
+                push_instance(be.eng, *j, false);
+                pull_instance(be.eng, *j);
+                check_closed_state(j->st, j->spec.dx, j->cells);
+            } else {
+                record(j->spec, j->st, j->frames, j->metrics);
+            }
+            if (j->st.history.time >= j->typed.duration_s) job_finish(*j, true, clock.elapsed());
+        } catch (const ResearchError& e) {
+            job_fail(*j, e.what());
+        }
+    }
+    // Advance all unfinished jobs towards their next record boundaries, at most
+    // 256 trial steps per round so that every job's budget is checked often.
+    std::vector<double> targets(live.size());
+    std::vector<int> status(live.size());
+    for (;;) {
+        const double elapsed = clock.elapsed();
+        bool any = false;
+        for (Job* j : live) {
+            if (!j->finished && elapsed >= j->typed.max_wall_time_s) job_finish(*j, false, elapsed);
+            any = any || !j->finished;
+        }
+        if (!any) break;
+        for (Job* j : live) {
+            mpm_history h{};
+            int rc = mpm_instance_get_state(be.eng, (size_t)j->inst, nullptr, nullptr, 0, nullptr, &h);
+            if (rc) bail(mpm_last_error(be.eng), rc);
+            targets[(size_t)j->inst] =
+                j->finished ? h.time
+                            : std::max(h.time, std::min((double)j->next_record * j->typed.record_interval_s, j->typed.duration_s));
+        }
+        int rc = mpm_batch_advance(be.eng, targets.data(), 256, status.data(), nullptr);
+        if (rc) bail(mpm_last_error(be.eng), rc);
+        const double after = clock.elapsed();
+        for (Job* j : live) {
+            if (j->finished) continue;
+            const int st = status[(size_t)j->inst];
+            try {
+                if (st < 0) {
+                    pull_instance(be.eng, *j);
+                    job_fail(*j, "solver aborted at t='" + show(j->st.history.time) +
+                                     "' s: step below minimum after '" + std::to_string(j->st.history.rejected_steps) +
+                                     "' rejections");
+                    continue;
+                }
+                if (st == 0) {
+                    pull_instance(be.eng, *j);
+                    check_closed_state(j->st, j->spec.dx, j->cells);
+                    record(j->spec, j->st, j->frames, j->metrics);
+                    j->next_record += 1;
+                    if (j->st.history.time >= j->typed.duration_s) job_finish(*j, true, after);
+                }
+            } catch (const ResearchError& e) {
+                job_fail(*j, e.what());
+            }
+        }
+    }
+    Value out = Value::array();
+    for (Job& j : jobs) out.a.push_back(std::move(j.result));
     return out;
 }
 
@@ -1520,6 +1743,25 @@ int mpm_run_json(const char* request_json, const char* resume_json, char** outpu
     try {
         Value out = run(request, resume_json ? &resume : nullptr);
         return finish(MPM_OK, mpm_json::dump(out), output);
+    } catch (const ResearchError& e) {
+        return finish(e.code, e.what(), output);
+    } catch (const std::exception& e) {
+        return finish(MPM_ERR_INVALID, e.what(), output);
+    }
+}
+
+int mpm_run_json_batch(const char* requests_json, const char* resumes_json, char** output) {
+    if (output) *output = nullptr;
+    if (!requests_json) return finish(MPM_ERR_INVALID, "requests are NULL", output);
+    Value requests, resumes;
+    try {
+        requests = mpm_json::parse(requests_json);
+        if (resumes_json) resumes = mpm_json::parse(resumes_json);
+    } catch (const std::exception& e) {
+        return finish(MPM_ERR_INVALID, std::string("invalid batch JSON: ") + e.what(), output);
+    }
+    try {
+        return finish(MPM_OK, mpm_json::dump(run_batch(requests, resumes_json ? &resumes : nullptr)), output);
     } catch (const ResearchError& e) {
         return finish(e.code, e.what(), output);
     } catch (const std::exception& e) {
