@@ -1568,6 +1568,8 @@ int mpm_try_step(mpm_engine* e, double dt, mpm_step_report* report) {
     return code;
 }
 
+}  // extern "C"
+
 // ------------------------------------------------------------ validation and the energy audit
 
 namespace {
@@ -1691,6 +1693,8 @@ bool all_finite(const double* v, int n) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int mpm_validate(mpm_engine* e) {
     if (!e) return MPM_ERR_INVALID;

@@ -199,25 +199,27 @@ __device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, 
         return F_OK;
     }
     float vs = G3 * dt;
-    float inv_k = 1.0f / P.K;
+    float inv_k = __builtin_amdgcn_rcpf(P.K);
     float lo = 0.0f, hi = avail, d = 0.5f * avail;
     float tol = 2.5e-7f * fmaxf(qtr, 1.0f);
-    float r = d - avail + vs * powf(d * inv_k, P.inv_n);
+    // pw = (d/K)^(1/n); its derivative in d is (1/n) pw / d, so one power per iteration.
+    float pw = powf(d * inv_k, P.inv_n);
+    float r = d - avail + vs * pw;
     for (int it = 0; it < 100; ++it) {
         if (fabsf(r) <= tol || (hi - lo) <= tol) {
             q = P.ys + d;
-            inc = (avail - d) / G3;
+            inc = (avail - d) * __builtin_amdgcn_rcpf(G3);
             return F_OK;
         }
         if (r > 0.0f)
             hi = d;
         else
             lo = d;
-        float over = d * inv_k;
-        float deriv = over > 0.0f ? 1.0f + vs * P.inv_n * powf(over, P.inv_n - 1.0f) * inv_k : INFINITY;
-        float nd = d - r / deriv;
+        float deriv = d > 0.0f ? 1.0f + vs * P.inv_n * pw * __builtin_amdgcn_rcpf(d) : INFINITY;
+        float nd = d - r * __builtin_amdgcn_rcpf(deriv);
         d = (isfinite(nd) && nd > lo && nd < hi) ? nd : 0.5f * (lo + hi);
-        r = d - avail + vs * powf(d * inv_k, P.inv_n);
+        pw = powf(d * inv_k, P.inv_n);
+        r = d - avail + vs * pw;
         if (!isfinite(r)) return F_NONFINITE;
     }
     return F_RETURN;
@@ -273,17 +275,18 @@ __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& 
 #pragma unroll
         for (int i = 0; i < 6; ++i) pw[i] = lg[i] = X[i];
         const float tol2 = 1e-15f * rho2;
-        float sign = -1.0f;
+        // (-1)^(k+1) / k for k = 2 .. 12
+        constexpr float kLogC[11] = {-1.0f / 2, 1.0f / 3, -1.0f / 4, 1.0f / 5, -1.0f / 6, 1.0f / 7,
+                                     -1.0f / 8, 1.0f / 9, -1.0f / 10, 1.0f / 11, -1.0f / 12};
         for (int k = 2; k <= 12; ++k) {
             float nx[6];
             sym_mul(pw, X, nx);
-            const float c = sign / (float)k;
+            const float c = kLogC[k - 2];
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
                 pw[i] = nx[i];
                 lg[i] += c * nx[i];
             }
-            sign = -sign;
             if (sym_norm2(pw) * c * c <= tol2) break;
         }
 #pragma unroll
@@ -308,7 +311,7 @@ __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& 
         }
     }
     s.vol = e[0] + e[1] + e[2];
-    s.mean = s.vol / 3.0f;
+    s.mean = s.vol * (1.0f / 3.0f);
     s.D[0] = e[0] - s.mean;
     s.D[1] = e[1] - s.mean;
     s.D[2] = e[2] - s.mean;
@@ -333,7 +336,7 @@ __device__ inline bool paste_yields(const Params& P, const PasteTrial& s) { retu
 // the trial state that has to stay live across the return.
 __device__ inline int paste_finish(const Params& P, const PasteTrial& s, const float* Ht, float q, bool yielded,
                                    float* Hn, float* tau) {
-    const float alpha = yielded ? q / s.qtr : 1.0f;
+    const float alpha = yielded ? q * __builtin_amdgcn_rcpf(s.qtr) : 1.0f;
     float M[9];
     const float am1 = alpha - 1.0f;
     float Y[6];
@@ -359,12 +362,13 @@ __device__ inline int paste_finish(const Params& P, const PasteTrial& s, const f
 #pragma unroll
         for (int i = 0; i < 6; ++i) pw[i] = ex[i] = Y[i];
         const float tol2 = 1e-15f * y2;
-        float fact = 1.0f;
+        // 1 / k! for k = 2 .. 10
+        constexpr float kExpC[9] = {1.0f / 2, 1.0f / 6, 1.0f / 24, 1.0f / 120, 1.0f / 720, 1.0f / 5040,
+                                    1.0f / 40320, 1.0f / 362880, 1.0f / 3628800};
         for (int k = 2; k <= 10 && tol2 > 0.0f; ++k) {
             float nx[6];
             sym_mul(pw, Y, nx);
-            fact *= (float)k;
-            const float c = 1.0f / fact;
+            const float c = kExpC[k - 2];
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
                 pw[i] = nx[i];

@@ -104,7 +104,7 @@ def test_paste_batch_matches_oracle(engine):
 
 def test_manuscript_return_on_device(engine):
     # core/tests/formal_examples.rs:12-26 through the f32 kernel: q = 78.0059568837,
-    # plastic increment 0.001595234493, each within 1e-6 relative.
+    # plastic increment 0.001595234493, each within 1e-5 relative.
     m = engine.paste(1000.0, 1000.0, 0.2, 30.0, 10.0, 0.6, shear_units=True)
     strain = 80.0 / (2.0 * math.sqrt(3.0) * m.shear_modulus)
     trial = np.diag([math.exp(strain), math.exp(-strain), 1.0])[None]
