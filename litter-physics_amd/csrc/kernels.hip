@@ -635,17 +635,19 @@ __global__ void pellet_kernel(BatchDev B) {
 // next step's particle-to-grid deposit into the other accumulator set.  A
 // workgroup owns one work item: the particles of one or two z-adjacent cell
 // blocks of one instance.  It stages the node velocities of their stencils in
-// LDS and deposits every updated particle straight from registers into an f64
-// LDS tile one cell wider on each side than those stencils (an accepted step
-// moves a particle by at most one cell); the tile is flushed with one global
-// atomic per touched node and field.  Particles that leave the tile, and work
-// items whose tiles exceed the LDS budget, deposit with global atomics.
+// LDS and deposits every updated particle straight from registers into a
+// fixed-point LDS tile (exact 64-bit integer sums, TileScale) one cell wider on
+// each side than those stencils (an accepted step moves a particle by at most
+// one cell); the tile is flushed with one global atomic per touched node and
+// field.  Particles that leave the tile or exceed its scales, and work items
+// whose tiles exceed the LDS budget, deposit with global atomics.
 // MLP: the return-map surrogate is on.
 template <int CHUNK, int GCAP, int SCAP, bool MLP>
 __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
     __shared__ float4 sg[GCAP];
-    __shared__ double ss[7 * SCAP];
+    __shared__ unsigned long long ss[7 * SCAP];
+    __shared__ float sbound[4][5];
     __shared__ int sbox[4][6];
     __shared__ Tile tile;
     __shared__ int sscat;
@@ -684,6 +686,9 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     // Phase A: stencil bases of the current positions and their bounding box.
     float px[R], py[R], pz[R];
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    // Deposit-scale inputs: the largest mass and the largest v0 max_a sum_b |tau_ab|
+    // of the committed stresses.
+    float bnd[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int p = chunk0 + r * kBlock + tid;
@@ -694,6 +699,12 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             pz[r] = src[PZ * cap + p];
             // The committed step hit the 32-step cadence: order keys of the committed positions.
             if (is.refsort) B.refkey[p] = (ref_cell(P, px[r], py[r], pz[r]) << 32) | (uint64_t)B.ids[p];
+            float t6[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) t6[i] = fabsf(src[(TXX + i) * cap + p]);
+            bnd[0] = fmaxf(bnd[0], shared[SMASS * cap + p]);
+            bnd[4] = fmaxf(bnd[4], shared[SVOL0 * cap + p] * fmaxf(fmaxf(t6[0] + t6[3] + t6[4], t6[3] + t6[1] + t6[5]),
+                                                                  t6[4] + t6[5] + t6[2]));
             float xs[3] = {px[r] * P.inv_dx, py[r] * P.inv_dx, pz[r] * P.inv_dx};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -735,7 +746,8 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         }
         t.tiled = ok && gsz <= GCAP;
         tile = t;
-        sscat = ok && ssz <= SCAP;
+        // The deposit scales need the gather tile's velocity bound.
+        sscat = t.tiled && ssz <= SCAP;
     }
     __syncthreads();
     const Tile t = tile;
@@ -749,12 +761,38 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             int k = q % t.gd[2], rest = q / t.gd[2];
             int j = rest % t.gd[1], i = rest / t.gd[1];
             int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
-            sg[q] = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
+            const float4 g = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
+            sg[q] = g;
+            bnd[1] = fmaxf(bnd[1], fabsf(g.x));
+            bnd[2] = fmaxf(bnd[2], fabsf(g.y));
+            bnd[3] = fmaxf(bnd[3], fabsf(g.z));
         }
     }
     if (stiled)
-        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0.0;
+        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0ull;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        bnd[i] = wave_max(bnd[i]);
+        if (lane == 0) sbound[wave][i] = bnd[i];
+    }
     __syncthreads();
+    // Deposit scales.  The new velocity is a convex combination of the tile's
+    // node velocities and |C_ab| dx <= 2 max |v_a| (sum_i w_i |d_i| <= 1/2 cell),
+    // so a momentum bound is at most 10 m max|v_a|; the stress bound allows the
+    // committed stresses to grow eightfold in the step.
+    TileScale tsc;
+    {
+        float b[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            b[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                                                 int, fmaxf(fmaxf(sbound[0][i], sbound[1][i]),
+                                                            fmaxf(sbound[2][i], sbound[3][i])))));
+        tile_scale(b[0], tsc.sm, tsc.lm);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tile_scale(10.25f * b[0] * b[1 + a], tsc.sp[a], tsc.lp[a]);
+        tile_scale(12.0f * P.dx * b[4], tsc.sf, tsc.lf);
+    }
 
     // Phase B: per particle, G2P, advection, constitutive update, writes to the
     // trial slab, and the deposit of the updated particle.
@@ -916,8 +954,8 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                         const int li = s2.base[a] - so[a];
                         in_tile = in_tile && li >= 0 && li + 2 < sd[a];
                     }
-                    if (in_tile)
-                        scatter_lds<SCAP>(ss, so, sd, P, s2, m, v, C, tau, v0, rot);
+                    if (in_tile && tile_fits(tsc, P, m, v, C, tau, v0))
+                        scatter_lds<SCAP>(ss, so, sd, P, s2, tsc, m, v, C, tau, v0, rot);
                     else
                         scatter_particle(P, out, s2, m, v[0], v[1], v[2], C, tau, v0);
                     nw = near_wall(P, nx, ny, nz);
@@ -961,21 +999,24 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         atomicAdd(B.mlp_stats + 1, mlp_taken);
     }
     __syncthreads();
-    // Flush the tile: one global atomic per touched node and field.
+    // Flush the tile: one global atomic per touched node and field, each sum
+    // converted exactly to f64 (|sum| < 2^40) and scaled back by 2^-k.
     if (stiled) {
+        const double um = 1.0 / tsc.sm, ux = 1.0 / tsc.sp[0], uy = 1.0 / tsc.sp[1], uz = 1.0 / tsc.sp[2],
+                     uf = 1.0 / tsc.sf;
         for (int q = tid; q < ssz; q += kBlock) {
-            double m = ss[q];
-            if (m == 0.0) continue;
+            const long long qm = (long long)ss[q];
+            if (qm == 0) continue;
             int k = q % sd[2], rest = q / sd[2];
             int j = rest % sd[1], i = rest / sd[1];
             int ni = node_index(P, so[0] + i, so[1] + j, so[2] + k);
-            atomicAdd(out.m + ni, (float)m);
-            atomicAdd(out.px + ni, (float)ss[1 * SCAP + q]);
-            atomicAdd(out.py + ni, (float)ss[2 * SCAP + q]);
-            atomicAdd(out.pz + ni, (float)ss[3 * SCAP + q]);
-            atomicAdd(out.fx + ni, (float)ss[4 * SCAP + q]);
-            atomicAdd(out.fy + ni, (float)ss[5 * SCAP + q]);
-            atomicAdd(out.fz + ni, (float)ss[6 * SCAP + q]);
+            atomicAdd(out.m + ni, (float)((double)qm * um));
+            atomicAdd(out.px + ni, (float)((double)(long long)ss[1 * SCAP + q] * ux));
+            atomicAdd(out.py + ni, (float)((double)(long long)ss[2 * SCAP + q] * uy));
+            atomicAdd(out.pz + ni, (float)((double)(long long)ss[3 * SCAP + q] * uz));
+            atomicAdd(out.fx + ni, (float)((double)(long long)ss[4 * SCAP + q] * uf));
+            atomicAdd(out.fy + ni, (float)((double)(long long)ss[5 * SCAP + q] * uf));
+            atomicAdd(out.fz + ni, (float)((double)(long long)ss[6 * SCAP + q] * uf));
             out.flag[ni >> 6] = 1;
         }
     }

@@ -2,7 +2,7 @@
 // stencils (grid.rs:151-183), the constitutive updates (constitutive.rs:280-589),
 // the wall velocity law (solver.rs:1145-1155) and the gather/scatter helpers
 // of the fused grid-to-particle-to-grid kernel.  f32 throughout, f64 where a
-// reduction or LDS accumulation needs it.
+// reduction needs it, exact fixed point in the LDS deposit tile.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -607,21 +607,82 @@ __device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil
 // would place the stencil in scratch memory).
 __device__ inline float sel3(int i, float x0, float x1, float x2) { return i == 0 ? x0 : (i == 1 ? x1 : x2); }
 
-// deposit_particles for one particle into the f64 LDS accumulator tile (LDS
-// f64 atomic adds run at close to full LDS rate on CDNA4; f32 ones do not).
-// Lanes walk the nine (a, b) stencil columns starting at a lane-dependent
-// column so that particles of one cell, which share most nodes, hit different
-// LDS words.
-template <int SCAP>
-__device__ inline void scatter_lds(double* ss, const int* so, const int* sd, const Params& P, const Stencil& s,
-                                   float m, const float* v, const float* C, const float* tau, float v0, int rot) {
-    float mc[9], vt[6];
-    float mdx = m * P.dx, vdx = v0 * P.dx;
+// Fixed-point deposit tile.  Each of the seven fields (m, the three APIC
+// momentum sums and the three stress sums) accumulates in LDS as a 64-bit
+// integer count of 2^-k units, with k chosen per workgroup and field so that
+// no admitted contribution exceeds 2^30 units: the at most 1024 contributions a
+// node receives from one workgroup sum below 2^40, so the tile sums are exact
+// and independent of the order of the atomics.  CDNA4's LDS integer atomics
+// also run well ahead of its f64 ones under the address conflicts of a deposit.
+// A particle whose bound exceeds what the scales admit (or is not finite)
+// deposits with global atomics instead.
+struct TileScale {
+    float sm, sp[3], sf;  // 2^k of the mass, the momentum components and the three stress sums
+    float lm, lp[3], lf;  // largest particle bound (tile_bounds) each scale admits; 0 admits only zero
+};
+
+// Largest weight of the quadratic B-spline stencil: 0.75^3.
+constexpr float kMaxWeight = 0.421875f;
+
+// 2^k with k such that bound * 2^k < 2^30, and the bound it admits.
+__device__ inline void tile_scale(float bound, float& s, float& lim) {
+    if (!(bound > 0.0f) || !(bound <= FLT_MAX)) {
+        s = 1.0f;
+        lim = 0.0f;
+        return;
+    }
+    int e;
+    frexpf(kMaxWeight * bound, &e);  // kMaxWeight * bound < 2^e
+    s = ldexpf(1.0f, min(30 - e, 126));
+    lim = bound;
+}
+
+// The quantities tile_fits compares with TileScale's limits: m; per momentum
+// component m |v_a| + 1.5 sum_b |m dx C_ab| (|node - particle| <= 1.5 cells per
+// axis); 1.5 v0 dx max_a sum_b |tau_ab| for the stress sums.  Each stencil
+// contribution is at most kMaxWeight times its bound.
+__device__ inline bool tile_fits(const TileScale& ts, const Params& P, float m, const float* v, const float* C,
+                                 const float* tau, float v0) {
+    const float mdx = m * P.dx;
+    bool ok = m <= ts.lm;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx;
+    for (int a = 0; a < 3; ++a)
+        ok = ok && fabsf(m * v[a]) + 1.5f * (fabsf(C[3 * a] * mdx) + fabsf(C[3 * a + 1] * mdx) +
+                                             fabsf(C[3 * a + 2] * mdx)) <= ts.lp[a];
+    const float vdx = v0 * P.dx;
+    const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]), r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
+                r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
+    return ok && 1.5f * (vdx * fmaxf(fmaxf(r0, r1), r2)) <= ts.lf;
+}
+
+// Round to the nearest integer (halves up) in one conversion.
+__device__ inline int round_int(float x) {
+    int q;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(q) : "v"(x));
+    return q;
+}
+
+__device__ inline void tile_add(unsigned long long* t, int q) {
+    atomicAdd(t, (unsigned long long)(long long)q);
+}
+
+// deposit_particles for one particle into the fixed-point LDS tile.  Mass
+// contributions round up, so every node the particle reaches with a nonzero
+// weight is marked occupied, as its f64 mass would be.  Lanes walk the nine
+// (a, b) stencil columns starting at a lane-dependent column so that particles
+// of one cell, which share most nodes, hit different LDS words.
+template <int SCAP>
+__device__ inline void scatter_lds(unsigned long long* ss, const int* so, const int* sd, const Params& P,
+                                   const Stencil& s, const TileScale& ts, float m, const float* v, const float* C,
+                                   const float* tau, float v0, int rot) {
+    float mc[9], vt[6];
+    const float mdx = m * P.dx, vdx = v0 * P.dx * ts.sf;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx * ts.sp[i / 3];
 #pragma unroll
     for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
-    float mv0 = m * v[0], mv1 = m * v[1], mv2 = m * v[2];
+    const float ms = m * ts.sm;
+    float mv0 = m * v[0] * ts.sp[0], mv1 = m * v[1] * ts.sp[1], mv2 = m * v[2] * ts.sp[2];
     int li = s.base[0] - so[0], lj = s.base[1] - so[1], lk = s.base[2] - so[2];
     for (int kk = 0; kk < 9; ++kk) {
         int ab = kk + rot;
@@ -639,13 +700,13 @@ __device__ inline void scatter_lds(double* ss, const int* so, const int* sd, con
             float w = wab * s.w[2][c];
             float dz = s.d[2][c];
             int idx = row + c;
-            atomicAdd(ss + 0 * SCAP + idx, (double)(w * m));
-            atomicAdd(ss + 1 * SCAP + idx, (double)(w * (px0 + mc[2] * dz)));
-            atomicAdd(ss + 2 * SCAP + idx, (double)(w * (py0 + mc[5] * dz)));
-            atomicAdd(ss + 3 * SCAP + idx, (double)(w * (pz0 + mc[8] * dz)));
-            atomicAdd(ss + 4 * SCAP + idx, (double)(w * (fx0 + vt[4] * dz)));
-            atomicAdd(ss + 5 * SCAP + idx, (double)(w * (fy0 + vt[5] * dz)));
-            atomicAdd(ss + 6 * SCAP + idx, (double)(w * (fz0 + vt[2] * dz)));
+            atomicAdd(ss + 0 * SCAP + idx, (unsigned long long)(unsigned)(int)ceilf(w * ms));
+            tile_add(ss + 1 * SCAP + idx, round_int(w * (px0 + mc[2] * dz)));
+            tile_add(ss + 2 * SCAP + idx, round_int(w * (py0 + mc[5] * dz)));
+            tile_add(ss + 3 * SCAP + idx, round_int(w * (pz0 + mc[8] * dz)));
+            tile_add(ss + 4 * SCAP + idx, round_int(w * (fx0 + vt[4] * dz)));
+            tile_add(ss + 5 * SCAP + idx, round_int(w * (fy0 + vt[5] * dz)));
+            tile_add(ss + 6 * SCAP + idx, round_int(w * (fz0 + vt[2] * dz)));
         }
     }
 }
