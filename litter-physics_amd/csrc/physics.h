@@ -652,7 +652,14 @@ __device__ inline bool tile_fits(const TileScale& ts, const Params& P, float m, 
     const float vdx = v0 * P.dx;
     const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]), r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
                 r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
-    return ok && 1.5f * (vdx * fmaxf(fmaxf(r0, r1), r2)) <= ts.lf;
+    const float rf = 1.5f * (vdx * fmaxf(fmaxf(r0, r1), r2));
+    // The stress scale derives from the committed stresses of the workgroup,
+    // which the new ones need not follow (a state may store any Kirchhoff
+    // stress, and one workgroup may span stresses of very different size): a
+    // particle more than 2^12 below it keeps full f32 resolution through the
+    // global path, so every stressed particle in the tile keeps at least 2^17
+    // units in its largest stress contribution.
+    return ok && rf <= ts.lf && (rf == 0.0f || rf >= ts.lf * 0x1p-12f);
 }
 
 // Round to the nearest integer (halves up) in one conversion.
