@@ -660,7 +660,10 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     InstState& is = B.is[inst];
     if (!is.active) return;
     const InstParams& ip = B.ip[inst];
-    const Params& P = ip.P;
+    // A private copy: read through the reference, the compiler reloads every
+    // field after each store to the slabs (it cannot rule out aliasing), one
+    // memory round trip per use inside the particle loop.
+    const Params P = ip.P;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int chunk0 = item.y;
     const int chunk_end = item.y + item.z;
@@ -813,6 +816,20 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         float v[3] = {0.0f, 0.0f, 0.0f}, C[9], Ht[9], tau[6];
         float jn = 0.0f;
         PasteTrial ts;
+        // The particle's remaining slab reads, issued before the gather so that
+        // their memory latency overlaps it (the barrier keeps the scheduler from
+        // sinking them to their first use).
+        float Hp[9], pm = 0.0f, pv0 = 0.0f, pe = 0.0f;
+        if (p < chunk_end) {
+            pm = shared[SMASS * cap + p];
+            pv0 = shared[SVOL0 * cap + p];
+            pe = src[(P.kind == 0 ? EPS : JM1) * cap + p];
+            if (P.kind == 0) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Hp[i] = src[(H00 + i) * cap + p];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
         if (p < chunk_end) {
             Stencil s;
             if (!make_stencil(P, x, y, z, s)) {
@@ -833,9 +850,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 speed = fmaxf(speed, vn);
                 disp = fmaxf(disp, vn * dt * P.inv_dx);
                 if (P.kind == 0) {
-                    float H[9];
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) H[i] = src[(H00 + i) * cap + p];
+                    const float* H = Hp;
                     // (I + dt C)(I + H) - I = H + dt C + dt C H
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
@@ -846,7 +861,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                     rc = paste_trial(P, Ht, ts);
                     yielded = rc == F_OK && paste_yields(P, ts);
                 } else {
-                    rc = liquid_update(P, dt, src[JM1 * cap + p], C, jn, tau);
+                    rc = liquid_update(P, dt, pe, C, jn, tau);
                 }
             }
         }
@@ -924,8 +939,8 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 if (rc == F_OK) {
 #pragma unroll
                     for (int i = 0; i < 9; ++i) dst[(H00 + i) * cap + p] = Hn[i];
-                    diss += (double)(q * inc) * (double)shared[SVOL0 * cap + p];
-                    dst[EPS * cap + p] = src[EPS * cap + p] + inc;
+                    diss += (double)(q * inc) * (double)pv0;
+                    dst[EPS * cap + p] = pe + inc;
                 }
             } else if (rc == F_OK) {
                 dst[JM1 * cap + p] = jn;
@@ -947,7 +962,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
                 Stencil s2;
                 if (make_stencil(P, nx, ny, nz, s2)) {
-                    const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
+                    const float m = pm, v0 = pv0;
                     bool in_tile = stiled;
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
