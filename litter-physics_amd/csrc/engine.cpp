@@ -489,7 +489,7 @@ struct mpm_engine {
             poffs[i] = (int)total;
             total += pc;
             int nb = in.node_blocks();
-            int nblk = (nb + 3) / 4 * 4;
+            int nblk = (nb + kGridWgBlocks - 1) / kGridWgBlocks * kGridWgBlocks;
             in.ip.boff = total_blocks;
             in.ip.nblk = nblk;
             for (int b = 0; b < nblk; ++b) binst.push_back(i);
@@ -505,7 +505,7 @@ struct mpm_engine {
         poffs[ninst] = (int)total;
         cap = total;
         na = (size_t)total_blocks * 64;
-        ngwg = total_blocks / 4;
+        ngwg = total_blocks / kGridWgBlocks;
         work_cap = wcap;
         // Device-order key: (instance, cell block, cell within block); the all-ones
         // block value marks the unused entries of an instance's range.

@@ -65,6 +65,8 @@ enum { E_MIN_STEP = 1, E_MIN_AFTER_REJECT = 2 };
 constexpr int kMaxSpheres = 64;
 constexpr int kLedger = 31;  // doubles of mpm_ledger
 constexpr int kGridVals = 12;
+// Node blocks per grid-update workgroup; instance block ranges are multiples of it.
+constexpr int kGridWgBlocks = 16;
 // Energy audit (audit.rs): grid kinetic energies at five stages of one trial
 // (transport, after stress, after traction, after gravity, after walls), and
 // per instance those five plus the trial's plastic-work sum.
@@ -92,7 +94,7 @@ struct InstParams {
     double particle_mass;   // sum of the stored particle masses (f64)
     double pel_contact_dt;  // min over the instance's pellets of 0.2 sqrt(m / k)
     int poff, pcap;         // particle range of the slabs
-    int boff, nblk;         // node-block range (nblk a multiple of 4)
+    int boff, nblk;         // node-block range (nblk a multiple of kGridWgBlocks)
     int pel_off, npel;      // pellet range
 };
 

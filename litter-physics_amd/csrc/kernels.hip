@@ -207,7 +207,15 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
     const int total = B.counters[1];
     const float* shared = B.shared[B.sh];
     const size_t cap = B.cap, na = B.na;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    // One thread per (near-wall particle, stencil node): a particle's 27 node
+    // visits are independent (each reads its own wall-plane node and deposits on
+    // its own node), so spreading them over threads overlaps their memory
+    // round trips instead of chaining 27 of them in one thread.
+    const long long items = 27LL * total;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < items;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(t / 27), node = (int)(t - 27LL * q);
+        const int sa = node / 9, sb = (node / 3) % 3, sc = node % 3;
         int lo = 0, hi = B.ninst - 1;
         while (lo < hi) {
             int mid = (lo + hi) >> 1;
@@ -218,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
         }
         const InstState& s = B.is[lo];
         const InstParams& ip = B.ip[lo];
-        const Params& P = ip.P;
+        const Params P = ip.P;
         int local = q - (s.nw_base - s.nw_count[s.cur]);
         int p = B.nw_list[s.cur][ip.poff + local];
         const float* slab = B.slab[s.cur];
@@ -242,6 +250,10 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
             img[axis] = 2.0f * face - pos[axis];
             Stencil st;
             if (!make_stencil(P, img[0], img[1], img[2], st)) continue;
+            int co[3] = {st.base[0] + sa, st.base[1] + sb, st.base[2] + sc};
+            if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
+            float w = st.w[0][sa] * st.w[1][sb] * st.w[2][sc];
+            if (w == 0.0f) continue;
             int plane = face == 0.0f ? 1 : P.cells[axis] + 1;
             float inward = face == 0.0f ? 1.0f : -1.0f;
             float g_a = P.gravity[axis];
@@ -250,39 +262,32 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
             float weight_momentum = mass * g_a * dt;
             const float* mom = axis == 0 ? g.px : (axis == 1 ? g.py : g.pz);
             const float* frc = axis == 0 ? g.fx : (axis == 1 ? g.fy : g.fz);
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b)
-                    for (int c = 0; c < 3; ++c) {
-                        int co[3] = {st.base[0] + a, st.base[1] + b, st.base[2] + c};
-                        if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
-                        float w = st.w[0][a] * st.w[1][b] * st.w[2][c];
-                        if (w == 0.0f) continue;
-                        int pc[3] = {co[0], co[1], co[2]};
-                        pc[axis] = plane;
-                        size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
-                        float pm = g.m[pi];
-                        if (pm <= 0.0f) continue;
-                        float trial = (mom[pi] - dt * k4 * frc[pi]) / pm + g_a * dt;
-                        if (trial * inward >= 0.0f) continue;
-                        int off = axis == 0 ? a : (axis == 1 ? b : c);
-                        float distance = st.d[axis][off] * P.dx;
-                        float reaction = (normal_stress * distance * force_scale + weight_momentum) * w * inward;
-                        if (reaction <= 0.0f) continue;
-                        size_t ni = noff + node_index(P, co[0], co[1], co[2]);
-                        if (g.m[ni] <= 0.0f) continue;
-                        atomicAdd(B.trS + axis * na + ni, reaction * inward);
-                        atomicAdd(B.trT + axis * na + ni, reaction);
-                    }
+            int pc[3] = {co[0], co[1], co[2]};
+            pc[axis] = plane;
+            size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
+            float pm = g.m[pi];
+            if (pm <= 0.0f) continue;
+            float trial = (mom[pi] - dt * k4 * frc[pi]) / pm + g_a * dt;
+            if (trial * inward >= 0.0f) continue;
+            int off = axis == 0 ? sa : (axis == 1 ? sb : sc);
+            float distance = st.d[axis][off] * P.dx;
+            float reaction = (normal_stress * distance * force_scale + weight_momentum) * w * inward;
+            if (reaction <= 0.0f) continue;
+            size_t ni = noff + node_index(P, co[0], co[1], co[2]);
+            if (g.m[ni] <= 0.0f) continue;
+            atomicAdd(B.trS + axis * na + ni, reaction * inward);
+            atomicAdd(B.trT + axis * na + ni, reaction);
         }
     }
 }
 
 // ------------------------------------------------------------ grid update
 
-// grid_update (solver.rs:822-892).  Four 64-node blocks of one instance per
-// workgroup, one per wavefront; ledger sums go to per-workgroup partials (no
-// same-address atomics).  The other accumulator set is cleared on the way for
-// the trial deposit.
+// grid_update (solver.rs:822-892).  kGridWgBlocks 64-node blocks of one
+// instance per workgroup, kGridWgBlocks / 4 per wavefront; ledger sums go to
+// per-workgroup partials (no same-address atomics), reduced once per
+// workgroup.  The other accumulator set is cleared on the way for the trial
+// deposit.
 // AUDIT: also the grid kinetic energy sum 1/2 m |v|^2 over nodes with mass at
 // the stages where the reference's audit takes it (solver.rs:540-625,
 // audit.rs:360-398): the APIC transport momentum alone (the accumulated
@@ -294,106 +299,116 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
     __shared__ double ascratch[AUDIT ? 4 * kAuditVals : 1];
     __shared__ int sbad[4];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int b = blockIdx.x * 4 + wave;
-    const int inst = B.blk_inst[blockIdx.x * 4];
+    const int inst = B.blk_inst[blockIdx.x * kGridWgBlocks];
     const InstState& s = B.is[inst];
     if (!s.active) return;
     const InstParams& ip = B.ip[inst];
-    const Params& P = ip.P;
+    const Params P = ip.P;  // private copy: no reloads behind the grid stores
     GridPtrs A = grid_set(B, s.gcur), C = grid_set(B, s.gcur ^ 1);
     const float dt = (float)s.dt;
     const size_t na = B.na;
+    const float k4 = 4.0f * P.inv_dx * P.inv_dx;
     double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
-    const size_t n = ((size_t)b << 6) | l;
-    if (C.flag[b]) {
-        C.m[n] = 0.0f;
-        C.px[n] = 0.0f;
-        C.py[n] = 0.0f;
-        C.pz[n] = 0.0f;
-        C.fx[n] = 0.0f;
-        C.fy[n] = 0.0f;
-        C.fz[n] = 0.0f;
-        if (l == 0) C.flag[b] = 0;
-    }
-    if (A.flag[b]) {
-        float m = A.m[n];
-        float k4 = 4.0f * P.inv_dx * P.inv_dx;
-        if (m > 0.0f) {
-            float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
-            float v[3], tb[3];
+    for (int j = 0; j < kGridWgBlocks / 4; ++j) {
+        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
+        const size_t n = ((size_t)b << 6) | l;
+        if (C.flag[b]) {
+            C.m[n] = 0.0f;
+            C.px[n] = 0.0f;
+            C.py[n] = 0.0f;
+            C.pz[n] = 0.0f;
+            C.fx[n] = 0.0f;
+            C.fy[n] = 0.0f;
+            C.fz[n] = 0.0f;
+            if (l == 0) C.flag[b] = 0;
+        }
+        if (!A.flag[b]) continue;
+        // The traction buffers can be nonzero only where wall_traction_to_grid
+        // deposits: a particle's wall image lands its deposits on node 2 or node
+        // `cells` of the wall axis, in the first block or the block holding node
+        // `cells` along that axis.
+        int bc[3];
+        node_coords(P, (b - ip.boff) << 6, bc[0], bc[1], bc[2]);
+        bool wall_block = false;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                float D = B.trS[a * na + n];
-                tb[a] = B.trT[a * na + n];
-                if (D != 0.0f) {
-                    // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
-                    vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
-                    vals[3 + a] += D;
-                    B.trS[a * na + n] = 0.0f;
-                }
-                if (tb[a] != 0.0f) B.trT[a * na + n] = 0.0f;
-                v[a] = (p[a] + D) / m + P.gravity[a] * dt;
-                if (AUDIT) {
-                    const double md = (double)m;
-                    const double pt = a == 0 ? (double)A.px[n] : (a == 1 ? (double)A.py[n] : (double)A.pz[n]);
-                    avals[0] += 0.5 * pt * pt / md;
-                    avals[1] += 0.5 * (double)p[a] * (double)p[a] / md;
-                    avals[2] += 0.5 * ((double)p[a] + (double)D) * ((double)p[a] + (double)D) / md;
-                    avals[3] += 0.5 * md * (double)v[a] * (double)v[a];
-                }
-            }
-            if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
-                bad = 1;
-                v[0] = v[1] = v[2] = 0.0f;
-            } else {
-                float before[3] = {v[0], v[1], v[2]};
-                int c[3];
-                node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
-                for (int axis = 0; axis < 3; ++axis) {
-                    int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
-                    if (sign == 0) continue;
-                    float ns = v[axis] * (float)sign;
-                    if (ns >= 0.0f) continue;
-                    vals[6] -= 0.5 * (double)m * (double)ns * (double)ns;
-                    v[axis] = 0.0f;
-                    double tn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                    reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
-                    double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                    vals[7] += 0.5 * (double)m * (tn - vn);
-                }
-                for (int axis = 0; axis < 3; ++axis) {
-                    float reaction = tb[axis];
-                    if (reaction <= 0.0f) continue;
-                    float normal = v[axis];
-                    float t[3] = {v[0], v[1], v[2]};
-                    t[axis] = 0.0f;
-                    double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                    int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
-                    reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
-                    double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                    vals[7] += 0.5 * (double)m * (tn - rn);
-                    v[0] = t[0];
-                    v[1] = t[1];
-                    v[2] = t[2];
-                    v[axis] = normal;
-                }
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    vals[a] = ((double)v[a] - (double)before[a]) * (double)m;
-                    vals[9 + a] = (double)v[a] * (double)m;
-                    if (AUDIT) avals[4] += 0.5 * (double)m * (double)v[a] * (double)v[a];
-                }
-            }
-            B.gv[n] = v[0];
-            B.gv[na + n] = v[1];
-            B.gv[2 * na + n] = v[2];
-        } else {
+        for (int a = 0; a < 3; ++a) wall_block = wall_block || bc[a] == 0 || (bc[a] >> 2) == (P.cells[a] >> 2);
+        const float m = A.m[n];
+        if (!(m > 0.0f)) {
             B.gv[n] = 0.0f;
             B.gv[na + n] = 0.0f;
             B.gv[2 * na + n] = 0.0f;
+            continue;
         }
+        float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
+        float v[3], tb[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float D = wall_block ? B.trS[a * na + n] : 0.0f;
+            tb[a] = wall_block ? B.trT[a * na + n] : 0.0f;
+            if (D != 0.0f) {
+                // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
+                vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
+                vals[3 + a] += D;
+                B.trS[a * na + n] = 0.0f;
+            }
+            if (tb[a] != 0.0f) B.trT[a * na + n] = 0.0f;
+            v[a] = (p[a] + D) / m + P.gravity[a] * dt;
+            if (AUDIT) {
+                const double md = (double)m;
+                const double pt = a == 0 ? (double)A.px[n] : (a == 1 ? (double)A.py[n] : (double)A.pz[n]);
+                avals[0] += 0.5 * pt * pt / md;
+                avals[1] += 0.5 * (double)p[a] * (double)p[a] / md;
+                avals[2] += 0.5 * ((double)p[a] + (double)D) * ((double)p[a] + (double)D) / md;
+                avals[3] += 0.5 * md * (double)v[a] * (double)v[a];
+            }
+        }
+        if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
+            bad += 1;
+            v[0] = v[1] = v[2] = 0.0f;
+        } else {
+            float before[3] = {v[0], v[1], v[2]};
+            int c[3];
+            node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
+            for (int axis = 0; axis < 3; ++axis) {
+                int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
+                if (sign == 0) continue;
+                float ns = v[axis] * (float)sign;
+                if (ns >= 0.0f) continue;
+                vals[6] -= 0.5 * (double)m * (double)ns * (double)ns;
+                v[axis] = 0.0f;
+                double tn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
+                double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                vals[7] += 0.5 * (double)m * (tn - vn);
+            }
+            for (int axis = 0; axis < 3; ++axis) {
+                float reaction = tb[axis];
+                if (reaction <= 0.0f) continue;
+                float normal = v[axis];
+                float t[3] = {v[0], v[1], v[2]};
+                t[axis] = 0.0f;
+                double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
+                reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
+                double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                vals[7] += 0.5 * (double)m * (tn - rn);
+                v[0] = t[0];
+                v[1] = t[1];
+                v[2] = t[2];
+                v[axis] = normal;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                vals[a] += ((double)v[a] - (double)before[a]) * (double)m;
+                vals[9 + a] += (double)v[a] * (double)m;
+                if (AUDIT) avals[4] += 0.5 * (double)m * (double)v[a] * (double)v[a];
+            }
+        }
+        B.gv[n] = v[0];
+        B.gv[na + n] = v[1];
+        B.gv[2 * na + n] = v[2];
     }
     int nbad = wave_sum_i(bad);
     if (l == 0) sbad[wave] = nbad;
@@ -1177,7 +1192,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     InstState& s = B.is[inst];
     if (!s.active) return;
     const InstParams& ip = B.ip[inst];
-    const int g0 = ip.boff / 4, g1 = (ip.boff + ip.nblk) / 4;
+    const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
     double acc[kGridVals + 1];
     for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
     int bad = 0;
@@ -1433,7 +1448,7 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
         grid_update_kernel<true><<<B.ngwg, 256, 0, s>>>(B);
     else
         grid_update_kernel<false><<<B.ngwg, 256, 0, s>>>(B);
-    if (pellets) couple_kernel<<<B.ngwg * 4, 64, 0, s>>>(B);
+    if (pellets) couple_kernel<<<B.ngwg * kGridWgBlocks, 64, 0, s>>>(B);
     if (g2p2g_start) (void)hipEventRecord(g2p2g_start, s);
     if (B.nwork > 0) {
         if (B.mlp_fwd)

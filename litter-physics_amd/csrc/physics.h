@@ -524,13 +524,35 @@ __device__ inline void reduce_tangential(float* v, float reduction) {
     }
 }
 
+// v from the lane CTRL names (a DPP row shift or row broadcast) in the rows
+// ROWS enables, 0 elsewhere and where the source lies outside the row.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_take(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the wavefront in a fixed order, left in lane 63: row prefix sums by
+// DPP row shifts, then the row totals by row broadcasts (VALU moves only, no
+// LDS permutes).
+__device__ inline double wave_sum_last(double v) {
+    v += dpp_take<0x111, 0xF>(v);  // row_shr:1
+    v += dpp_take<0x112, 0xF>(v);  // row_shr:2
+    v += dpp_take<0x114, 0xF>(v);  // row_shr:4
+    v += dpp_take<0x118, 0xF>(v);  // row_shr:8: lane 15 of each row holds the row's sum
+    v += dpp_take<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_take<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // Block-wide sum of NV doubles held by every thread; thread 0 gets the totals.
 template <int NV>
 __device__ inline void block_sum(double* vals, double* scratch) {
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) vals[i] = wave_sum(vals[i]);
-    if (lane == 0)
+    for (int i = 0; i < NV; ++i) vals[i] = wave_sum_last(vals[i]);
+    if (lane == 63)
 #pragma unroll
         for (int i = 0; i < NV; ++i) scratch[wave * NV + i] = vals[i];
     __syncthreads();
