@@ -368,7 +368,7 @@ struct mpm_engine {
     int cell_bits = 0, end_bit = 0;
     bool dirty = true;  // work items and near-wall lists must be rebuilt
     // launch
-    mpm_launch launch{32, 0, 2, 512};
+    mpm_launch launch{32, 0, 2, 2048};
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;

@@ -200,9 +200,10 @@ __global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
 // ------------------------------------------------------------ traction
 
 // wall_traction_to_grid (solver.rs:725-819) over the near-wall lists of the
-// running instances, concatenated.  Each deposit adds to the signed traction
-// impulse (trS) and the Coulomb budget (trT) of its node; the grid update books
-// the telescoped kinetic-energy jump per node.
+// running instances, concatenated.  Each deposit adds its reaction to the
+// Coulomb budget (trT) of its node, which also gives the signed traction
+// impulse (trS, kept separately only on grids two cells deep); the grid update
+// books the telescoped kinetic-energy jump per node.
 __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
     const int total = B.counters[1];
     const float* shared = B.shared[B.sh];
@@ -275,8 +276,8 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
             if (reaction <= 0.0f) continue;
             size_t ni = noff + node_index(P, co[0], co[1], co[2]);
             if (g.m[ni] <= 0.0f) continue;
-            atomicAdd(B.trS + axis * na + ni, reaction * inward);
             atomicAdd(B.trT + axis * na + ni, reaction);
+            if (P.cells[axis] <= 2) atomicAdd(B.trS + axis * na + ni, reaction * inward);
         }
     }
 }
@@ -343,15 +344,26 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
         }
         float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
         float v[3], tb[3];
+        int c[3];
+        node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            float D = wall_block ? B.trS[a * na + n] : 0.0f;
+            // trT holds a node's summed reactions; the signed impulse follows from
+            // the face the node serves (node 2 the low face, inward +1; node
+            // `cells` the high face, -1).  A grid two cells deep puts both faces
+            // on node 2 and keeps the signed sum in trS.
             tb[a] = wall_block ? B.trT[a * na + n] : 0.0f;
+            float D;
+            if (P.cells[a] <= 2) {
+                D = wall_block ? B.trS[a * na + n] : 0.0f;
+                if (D != 0.0f) B.trS[a * na + n] = 0.0f;
+            } else {
+                D = c[a] == 2 ? tb[a] : -tb[a];
+            }
             if (D != 0.0f) {
                 // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
                 vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
                 vals[3 + a] += D;
-                B.trS[a * na + n] = 0.0f;
             }
             if (tb[a] != 0.0f) B.trT[a * na + n] = 0.0f;
             v[a] = (p[a] + D) / m + P.gravity[a] * dt;
@@ -369,8 +381,6 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
             v[0] = v[1] = v[2] = 0.0f;
         } else {
             float before[3] = {v[0], v[1], v[2]};
-            int c[3];
-            node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
             for (int axis = 0; axis < 3; ++axis) {
                 int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
                 if (sign == 0) continue;
@@ -1185,9 +1195,6 @@ __device__ inline double block_total(double v, double* sd) {
 // particle partials and of its pellets' step sums, then the accept/reject.
 __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     __shared__ double sd[4];
-    __shared__ float sf[256 * 2];
-    __shared__ unsigned long long su[256];
-    __shared__ int si[256 * 3];
     const int inst = blockIdx.x, tid = threadIdx.x;
     InstState& s = B.is[inst];
     if (!s.active) return;
@@ -1211,18 +1218,6 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         f = B.pfail[q] < f ? B.pfail[q] : f;
         esc += B.pesc[q];
     }
-    double tot[kGridVals + 1];
-    for (int i = 0; i <= kGridVals; ++i) tot[i] = block_total(acc[i], sd);
-    if (B.audit) {
-        // Audit stage sums of this instance, in the same fixed order.
-        for (int i = 0; i < kAuditVals; ++i) {
-            double a = 0.0;
-            for (int q = g0 + tid; q < g1; q += 256) a += B.apart[(size_t)i * B.ngwg + q];
-            a = block_total(a, sd);
-            if (tid == 0) B.audit[(size_t)inst * kAuditOut + i] = a;
-        }
-        if (tid == 0) B.audit[(size_t)inst * kAuditOut + kAuditVals] = tot[kGridVals];
-    }
     // Pellet sums (the instance's pellets, in parallel).
     double pel[kPelSums];
     for (int i = 0; i < kPelSums; ++i) pel[i] = 0.0;
@@ -1244,30 +1239,60 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         pbad |= t.bad;
         vmax = fmax(vmax, sqrt(t.vel[0] * t.vel[0] + t.vel[1] * t.vel[1] + t.vel[2] * t.vel[2]));
     }
-    if (ip.npel > 0)
-        for (int i = 0; i < kPelSums; ++i) pel[i] = block_total(pel[i], sd);
-    sf[tid * 2] = smax;
-    sf[tid * 2 + 1] = dmax;
-    su[tid] = f;
-    si[tid * 3] = bad;
-    si[tid * 3 + 1] = esc;
-    si[tid * 3 + 2] = pbad;
-    __shared__ double sv[256];
-    sv[tid] = vmax;
+    // One fixed-order block reduction of everything: each wavefront reduces its
+    // lanes (sums by DPP into lane 63, extrema by shuffles), then thread 0
+    // combines the four wavefronts in order.
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int i = 0; i <= kGridVals; ++i) acc[i] = wave_sum_last(acc[i]);
+#pragma unroll
+    for (int i = 0; i < kPelSums; ++i) pel[i] = wave_sum_last(pel[i]);
+    smax = wave_max(smax);
+    dmax = wave_max(dmax);
+    f = wave_min_u64(f);
+    bad = wave_sum_i(bad);
+    esc = wave_sum_i(esc);
+    pbad = wave_sum_i(pbad != 0 ? 1 : 0);
+    vmax = wave_max_d(vmax);
+    __shared__ double ssum[4][kGridVals + 1 + kPelSums];
+    __shared__ double svm[4];
+    __shared__ float sfm[4][2];
+    __shared__ unsigned long long sfl[4];
+    __shared__ int sint[4][3];
+    if (lane == 63) {
+#pragma unroll
+        for (int i = 0; i <= kGridVals; ++i) ssum[wave][i] = acc[i];
+#pragma unroll
+        for (int i = 0; i < kPelSums; ++i) ssum[wave][kGridVals + 1 + i] = pel[i];
+    }
+    if (lane == 0) {
+        svm[wave] = vmax;
+        sfm[wave][0] = smax;
+        sfm[wave][1] = dmax;
+        sfl[wave] = f;
+        sint[wave][0] = bad;
+        sint[wave][1] = esc;
+        sint[wave][2] = pbad;
+    }
     __syncthreads();
-    for (int stride = 128; stride > 0; stride >>= 1) {
-        if (tid < stride) {
-            sf[tid * 2] = fmaxf(sf[tid * 2], sf[(tid + stride) * 2]);
-            sf[tid * 2 + 1] = fmaxf(sf[tid * 2 + 1], sf[(tid + stride) * 2 + 1]);
-            su[tid] = su[tid + stride] < su[tid] ? su[tid + stride] : su[tid];
-            si[tid * 3] += si[(tid + stride) * 3];
-            si[tid * 3 + 1] += si[(tid + stride) * 3 + 1];
-            si[tid * 3 + 2] |= si[(tid + stride) * 3 + 2];
-            sv[tid] = fmax(sv[tid], sv[tid + stride]);
+    double tot[kGridVals + 1];
+    if (tid == 0)
+        for (int i = 0; i <= kGridVals; ++i) tot[i] = ((ssum[0][i] + ssum[1][i]) + ssum[2][i]) + ssum[3][i];
+    if (B.audit) {
+        // Audit stage sums of this instance, in the same fixed order.
+        for (int i = 0; i < kAuditVals; ++i) {
+            double a = 0.0;
+            for (int q = g0 + tid; q < g1; q += 256) a += B.apart[(size_t)i * B.ngwg + q];
+            a = block_total(a, sd);
+            if (tid == 0) B.audit[(size_t)inst * kAuditOut + i] = a;
         }
-        __syncthreads();
+        if (tid == 0) B.audit[(size_t)inst * kAuditOut + kAuditVals] = tot[kGridVals];
     }
     if (tid == 0) {
+        for (int i = 0; i < kPelSums; ++i) {
+            const int j = kGridVals + 1 + i;
+            pel[i] = ((ssum[0][j] + ssum[1][j]) + ssum[2][j]) + ssum[3][j];
+        }
         StepStatus& st = s.st;
         for (int a = 0; a < 3; ++a) {
             st.wall_impulse[a] = tot[a];
@@ -1278,13 +1303,16 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         st.friction = tot[7];
         st.traction_energy = tot[8];
         st.dissipation = tot[kGridVals];
-        st.max_speed = sf[0];
-        st.max_disp = sf[1];
-        st.fail = su[0];
-        st.grid_nonfinite = si[0];
-        st.escaped = si[1];
-        st.pellet_bad = si[2];
-        commit_instance(s, ip, pel, sv[0]);
+        st.max_speed = fmaxf(fmaxf(sfm[0][0], sfm[1][0]), fmaxf(sfm[2][0], sfm[3][0]));
+        st.max_disp = fmaxf(fmaxf(sfm[0][1], sfm[1][1]), fmaxf(sfm[2][1], sfm[3][1]));
+        unsigned long long fl = sfl[0];
+        for (int w = 1; w < 4; ++w) fl = sfl[w] < fl ? sfl[w] : fl;
+        st.fail = fl;
+        st.grid_nonfinite = sint[0][0] + sint[1][0] + sint[2][0] + sint[3][0];
+        st.escaped = sint[0][1] + sint[1][1] + sint[2][1] + sint[3][1];
+        st.pellet_bad = (sint[0][2] + sint[1][2] + sint[2][2] + sint[3][2]) != 0;
+        const double pv = fmax(fmax(svm[0], svm[1]), fmax(svm[2], svm[3]));
+        commit_instance(s, ip, pel, pv);
     }
 }
 
