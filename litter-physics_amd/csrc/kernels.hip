@@ -687,8 +687,10 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     const InstParams& ip = B.ip[inst];
     // A private copy: read through the reference, the compiler reloads every
     // field after each store to the slabs (it cannot rule out aliasing), one
-    // memory round trip per use inside the particle loop.
-    const Params P = ip.P;
+    // memory round trip per use inside the particle loop.  The surrogate's
+    // instance keeps the reference: its tiles need the registers the copy holds.
+    const Params Pv = ip.P;
+    const Params& P = MLP ? ip.P : Pv;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int chunk0 = item.y;
     const int chunk_end = item.y + item.z;
