@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
     __shared__ float4 sg[GCAP];
     __shared__ unsigned long long ss[7 * SCAP];
-    __shared__ float sbound[4][5];
+    __shared__ float sbound[4][6];
     __shared__ int sbox[4][6];
     __shared__ Tile tile;
     __shared__ int sscat;
@@ -716,9 +716,10 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     // Phase A: stencil bases of the current positions and their bounding box.
     float px[R], py[R], pz[R];
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    // Deposit-scale inputs: the largest mass and the largest v0 max_a sum_b |tau_ab|
-    // of the committed stresses.
-    float bnd[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // Deposit-scale inputs: the largest mass, the largest v0 max_a sum_b |tau_ab|
+    // of the committed stresses and the largest v0 (the tile's node speeds
+    // join them below).
+    float bnd[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int p = chunk0 + r * kBlock + tid;
@@ -732,9 +733,10 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             float t6[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) t6[i] = fabsf(src[(TXX + i) * cap + p]);
+            const float v0 = shared[SVOL0 * cap + p];
             bnd[0] = fmaxf(bnd[0], shared[SMASS * cap + p]);
-            bnd[4] = fmaxf(bnd[4], shared[SVOL0 * cap + p] * fmaxf(fmaxf(t6[0] + t6[3] + t6[4], t6[3] + t6[1] + t6[5]),
-                                                                  t6[4] + t6[5] + t6[2]));
+            bnd[4] = fmaxf(bnd[4], v0 * fmaxf(fmaxf(t6[0] + t6[3] + t6[4], t6[3] + t6[1] + t6[5]), t6[4] + t6[5] + t6[2]));
+            bnd[5] = fmaxf(bnd[5], v0);
             float xs[3] = {px[r] * P.inv_dx, py[r] * P.inv_dx, pz[r] * P.inv_dx};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -801,7 +803,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     if (stiled)
         for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0ull;
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < 6; ++i) {
         bnd[i] = wave_max(bnd[i]);
         if (lane == 0) sbound[wave][i] = bnd[i];
     }
@@ -809,19 +811,29 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     // Deposit scales.  The new velocity is a convex combination of the tile's
     // node velocities and |C_ab| dx <= 2 max |v_a| (sum_i w_i |d_i| <= 1/2 cell),
     // so a momentum bound is at most 10 m max|v_a|; the stress bound allows the
-    // committed stresses to grow eightfold in the step.
+    // committed stresses to grow eightfold in the step.  A workgroup whose
+    // committed stresses are all zero (a state at rest) takes an estimate of
+    // what the step's velocity gradient can build instead: elastic moduli times
+    // dt |C| for the paste, the pressure and viscous response for the liquid.
     TileScale tsc;
     {
-        float b[5];
+        float b[6];
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
+        for (int i = 0; i < 6; ++i)
             b[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
                                                  int, fmaxf(fmaxf(sbound[0][i], sbound[1][i]),
                                                             fmaxf(sbound[2][i], sbound[3][i])))));
         tile_scale(b[0], tsc.sm, tsc.lm);
 #pragma unroll
         for (int a = 0; a < 3; ++a) tile_scale(10.25f * b[0] * b[1 + a], tsc.sp[a], tsc.lp[a]);
-        tile_scale(12.0f * P.dx * b[4], tsc.sf, tsc.lf);
+        float rows = b[4];
+        if (!(rows > 0.0f)) {
+            const float cmax = 2.0f * fmaxf(fmaxf(b[1], b[2]), b[3]) * P.inv_dx;
+            const float rate = P.kind == 0 ? 3.0f * (2.0f * P.G + 3.0f * fabsf(P.lam)) * dt
+                                           : 9.0f * P.bulk * dt + 15.0f * P.visc;
+            rows = b[5] * rate * cmax;
+        }
+        tile_scale(12.0f * P.dx * rows, tsc.sf, tsc.lf);
     }
 
     // Phase B: per particle, G2P, advection, constitutive update, writes to the
@@ -841,6 +853,8 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         int rc = F_OK;
         const float x = px[r], y = py[r], z = pz[r];
         float v[3] = {0.0f, 0.0f, 0.0f}, C[9], Ht[9], tau[6];
+        bool dep = false;  // the updated particle deposits (stencil s2)
+        Stencil s2;
         float jn = 0.0f;
         PasteTrial ts;
         // The particle's remaining slab reads, issued before the gather so that
@@ -987,24 +1001,34 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
-                Stencil s2;
                 if (make_stencil(P, nx, ny, nz, s2)) {
-                    const float m = pm, v0 = pv0;
-                    bool in_tile = stiled;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const int li = s2.base[a] - so[a];
-                        in_tile = in_tile && li >= 0 && li + 2 < sd[a];
-                    }
-                    if (in_tile && tile_fits(tsc, P, m, v, C, tau, v0))
-                        scatter_lds<SCAP>(ss, so, sd, P, s2, tsc, m, v, C, tau, v0, rot);
-                    else
-                        scatter_particle(P, out, s2, m, v[0], v[1], v[2], C, tau, v0);
+                    dep = true;
                     nw = near_wall(P, nx, ny, nz);
                 } else {
                     escaped += 1;
                 }
             }
+        }
+        // Deposit.  The stress scale derives from the committed stresses of the
+        // workgroup, which the new ones need not follow (a state may store any
+        // Kirchhoff stress): a wavefront whose new stresses all lie more than
+        // 2^20 below the scale deposits through the global path, at full f32
+        // resolution.
+        {
+            bool in_tile = dep && stiled;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int li = s2.base[a] - so[a];
+                in_tile = in_tile && li >= 0 && li + 2 < sd[a];
+            }
+            float rf = 0.0f;
+            const bool fits = in_tile && tile_fits(tsc, P, pm, v, C, tau, pv0, rf);
+            const float rfw = wave_max(fits ? rf : 0.0f);
+            const bool coarse = rfw > 0.0f && rfw < tsc.lf * 0x1p-20f;
+            if (fits && !coarse)
+                scatter_lds<SCAP>(ss, so, sd, P, s2, tsc, pm, v, C, tau, pv0, rot);
+            else if (dep)
+                scatter_particle(P, out, s2, pm, v[0], v[1], v[2], C, tau, pv0);
         }
         wave_append(nw, p, nw_list, nw_count);
         // Log of exact returns (training data of the surrogate).

@@ -666,10 +666,10 @@ __device__ inline void tile_scale(float bound, float& s, float& lim) {
 
 // The quantities tile_fits compares with TileScale's limits: m; per momentum
 // component m |v_a| + 1.5 sum_b |m dx C_ab| (|node - particle| <= 1.5 cells per
-// axis); 1.5 v0 dx max_a sum_b |tau_ab| for the stress sums.  Each stencil
+// axis); rf = 1.5 v0 dx max_a sum_b |tau_ab| for the stress sums.  Each stencil
 // contribution is at most kMaxWeight times its bound.
 __device__ inline bool tile_fits(const TileScale& ts, const Params& P, float m, const float* v, const float* C,
-                                 const float* tau, float v0) {
+                                 const float* tau, float v0, float& rf) {
     const float mdx = m * P.dx;
     bool ok = m <= ts.lm;
 #pragma unroll
@@ -679,14 +679,8 @@ __device__ inline bool tile_fits(const TileScale& ts, const Params& P, float m, 
     const float vdx = v0 * P.dx;
     const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]), r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
                 r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
-    const float rf = 1.5f * (vdx * fmaxf(fmaxf(r0, r1), r2));
-    // The stress scale derives from the committed stresses of the workgroup,
-    // which the new ones need not follow (a state may store any Kirchhoff
-    // stress, and one workgroup may span stresses of very different size): a
-    // particle more than 2^12 below it keeps full f32 resolution through the
-    // global path, so every stressed particle in the tile keeps at least 2^17
-    // units in its largest stress contribution.
-    return ok && rf <= ts.lf && (rf == 0.0f || rf >= ts.lf * 0x1p-12f);
+    rf = 1.5f * (vdx * fmaxf(fmaxf(r0, r1), r2));
+    return ok && rf <= ts.lf;
 }
 
 // Round to the nearest integer (halves up) in one conversion.
