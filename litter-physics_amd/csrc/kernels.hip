@@ -807,6 +807,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         bnd[i] = wave_max(bnd[i]);
         if (lane == 0) sbound[wave][i] = bnd[i];
     }
+    const float wave_rows = bnd[4];  // this wavefront's committed stress bound
     __syncthreads();
     // Deposit scales.  The new velocity is a convex combination of the tile's
     // node velocities and |C_ab| dx <= 2 max |v_a| (sum_i w_i |d_i| <= 1/2 cell),
@@ -1012,8 +1013,8 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         // Deposit.  The stress scale derives from the committed stresses of the
         // workgroup, which the new ones need not follow (a state may store any
         // Kirchhoff stress): a wavefront whose new stresses all lie more than
-        // 2^20 below the scale deposits through the global path, at full f32
-        // resolution.
+        // 2^20 below its own committed ones deposits through the global path, at
+        // full f32 resolution.
         {
             bool in_tile = dep && stiled;
 #pragma unroll
@@ -1024,7 +1025,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             float rf = 0.0f;
             const bool fits = in_tile && tile_fits(tsc, P, pm, v, C, tau, pv0, rf);
             const float rfw = wave_max(fits ? rf : 0.0f);
-            const bool coarse = rfw > 0.0f && rfw < tsc.lf * 0x1p-20f;
+            const bool coarse = rfw > 0.0f && rfw < wave_rows * (12.0f * 0x1p-20f) * P.dx;
             if (fits && !coarse)
                 scatter_lds<SCAP>(ss, so, sd, P, s2, tsc, pm, v, C, tau, pv0, rot);
             else if (dep)
