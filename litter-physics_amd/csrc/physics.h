@@ -645,13 +645,14 @@ __device__ inline float sel3(int i, float x0, float x1, float x2) { return i == 
 // deposits with global atomics instead.
 struct TileScale {
     float sm, sp[3], sf;  // 2^k of the mass, the momentum components and the three stress sums
-    float lm, lp[3], lf;  // largest particle bound (tile_bounds) each scale admits; 0 admits only zero
+    float lm, lp[3], lf;  // largest particle bound (tile_fits) each scale admits; 0 admits only zero
 };
 
 // Largest weight of the quadratic B-spline stencil: 0.75^3.
 constexpr float kMaxWeight = 0.421875f;
 
-// 2^k with k such that bound * 2^k < 2^30, and the bound it admits.
+// 2^k with k such that kMaxWeight * bound * 2^k < 2^30 (so no contribution of a
+// particle within the bound reaches 2^30 units), and the bound it admits.
 __device__ inline void tile_scale(float bound, float& s, float& lim) {
     if (!(bound > 0.0f) || !(bound <= FLT_MAX)) {
         s = 1.0f;

@@ -215,6 +215,43 @@ def test_rejected_step_leaves_state_untouched(engine):
     gsim.advance_to(1.5e-3)
 
 
+@pytest.mark.parametrize("depth", [2, 3])
+def test_thin_grid_wall_traction_on_both_faces(engine, depth):
+    # A paste block filling a grid `depth` cells deep, compressed along z, so that
+    # the wall-image deposits of the floor and of the ceiling both reach the grid
+    # (solver.rs:725-819).  Two cells deep, both faces deposit on node 2.
+    m = O.paste(1000.0, 1e5, 0.2, 1e6, 1.0, 1.0)
+    cfg = O.config(wall_friction=0.4, cfl=0.3, max_dt=5e-5, min_dt=1e-9)
+    dx = 0.002
+    spec = O.fixture("slump", m, dx, [10 * dx, 10 * dx, depth * dx], [6 * dx, 6 * dx, depth * dx], cfg, seed=7)
+    osim = build_oracle(spec)
+    st = osim.state()
+    st.kirchhoff[:] = 0.0
+    st.kirchhoff[:, 2, 2] = -2000.0
+    st.deformation[:, 2, 2] = np.exp(-2000.0 / 1e5)
+    osim.set_state(st)
+    gsim = engine_from_oracle(engine, osim)
+    dt = min(osim.stability_limit(), cfg.max_dt)
+    for k in range(30):
+        assert osim.try_step(dt) == 0, k
+        rep = gsim.try_step(dt)
+        assert rep.accepted == 1, (k, rep.failure)
+    # The block sits near equilibrium, so its velocities (about 1.5e-4 m/s) are the
+    # small residual of cancelling stress, traction and gravity forces: they are
+    # compared against the velocity scale of the prestress, sigma / (rho c) with
+    # c = sqrt(E / rho) (0.2 m/s), not against their own size.
+    g, r, o = compare_states(gsim, osim, vel_tol=1.0)
+    vscale = 2000.0 / (1000.0 * np.sqrt(1e5 / 1000.0))
+    assert np.abs(g.vel - r.vel[o]).max() < 1e-4 * vscale
+    lo, lg = osim.ledger(), gsim.ledger()
+    assert lo.wall_normal_traction_energy != 0.0
+    assert abs(lg.wall_normal_traction_energy - lo.wall_normal_traction_energy) <= \
+        1e-3 * abs(lo.wall_normal_traction_energy)
+    scale = np.linalg.norm(lo.gravity_impulse)
+    for a in range(3):
+        assert abs(lg.wall_impulse[a] - lo.wall_impulse[a]) < 1e-3 * scale
+
+
 @pytest.mark.parametrize("gamma", [1.0, 7.0])
 def test_dam_break_liquid_parity(engine, gamma):
     spec = cases.dam_break_cfg1(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=gamma)
