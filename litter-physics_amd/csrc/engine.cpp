@@ -737,7 +737,7 @@ struct mpm_engine {
         size_t noff = (size_t)in.ip.boff * 64, nn = (size_t)in.ip.nblk * 64;
         for (int f = 0; f < 7; ++f) HIP_OK(hipMemsetAsync(gmem[g] + f * na + noff, 0, sizeof(float) * nn, stream));
         HIP_OK(hipMemsetAsync(gflag[g] + in.ip.boff, 0, sizeof(int) * in.ip.nblk, stream));
-        launch_p2g(stream, dev(), in.ip.P, in.ip.poff, s.n, in.ip.boff, s.cur, g);
+        launch_p2g(stream, dev(), i, g);
         HIP_OK(hipGetLastError());
         HIP_OK(hipStreamSynchronize(stream));
         in.grid_valid = true;

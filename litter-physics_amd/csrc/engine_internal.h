@@ -278,7 +278,7 @@ int g2p2g_chunk();
 // 32-step order keys.
 void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, hipEvent_t g2p2g_start,
                  hipEvent_t g2p2g_stop);
-void launch_p2g(hipStream_t s, const BatchDev& B, const Params& P, int poff, int n, int boff, int cur, int gcur);
+void launch_p2g(hipStream_t s, const BatchDev& B, int inst, int gcur);
 // Order keys of the instances whose refresh is pending (the flags are left set).
 void launch_refkeys(hipStream_t s, const BatchDev& B);
 // Near-wall lists of every instance's committed slab (counters cleared by the caller).

@@ -1376,20 +1376,159 @@ __global__ __launch_bounds__(kBlock) void refkey_kernel(BatchDev B) {
 
 // ------------------------------------------------------------ setup kernels
 
-__global__ __launch_bounds__(kBlock) void p2g_kernel(Params P, int n, const float* __restrict__ slab, size_t cap,
-                                                     const float* __restrict__ shared, GridPtrs g) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float x = slab[PX * cap + p], y = slab[PY * cap + p], z = slab[PZ * cap + p];
-    Stencil s;
-    if (!make_stencil(P, x, y, z, s)) return;
-    float C[9], tau[6];
+// deposit_particles (solver.rs:632-661) of one instance's committed state into
+// accumulator set `gset`, for a state that was set from outside (the reference
+// rebuilds the grid from the particles at the start of a step).  One workgroup
+// per work item, as in the fused kernel: the particles' own values give exact
+// deposit scales, so every particle whose stencil stays in the tile deposits
+// there; the tile is flushed with one global atomic per touched node and field.
+template <int CHUNK, int SCAP>
+__global__ __launch_bounds__(kBlock, 2) void setup_deposit_kernel(BatchDev B, int inst, int gset) {
+    constexpr int R = CHUNK / kBlock;
+    __shared__ unsigned long long ss[7 * SCAP];
+    __shared__ int sbox[4][6];
+    __shared__ float sbound[4][5];
+    __shared__ int sgeo[7];
+    const int4 item = B.work[blockIdx.x];
+    if (item.x != inst) return;
+    const InstState& is = B.is[inst];
+    const InstParams& ip = B.ip[inst];
+    const Params P = ip.P;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t cap = B.cap;
+    const float* __restrict__ src = B.slab[is.cur];
+    const float* __restrict__ shared = B.shared[B.sh];
+    const size_t noff = (size_t)ip.boff * 64;
+    GridPtrs out = grid_set(B, gset);
+    out.m += noff;
+    out.px += noff;
+    out.py += noff;
+    out.pz += noff;
+    out.fx += noff;
+    out.fy += noff;
+    out.fz += noff;
+    out.flag += ip.boff;
+    const int end = item.y + item.z;
+
+    // Stencil-base box and the exact bounds tile_fits compares.
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    float bnd[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int i = 0; i < 9; ++i) C[i] = slab[(C00 + i) * cap + p];
+    for (int r = 0; r < R; ++r) {
+        const int p = item.y + r * kBlock + tid;
+        if (p >= end) continue;
+        Stencil s;
+        if (!make_stencil(P, src[PX * cap + p], src[PY * cap + p], src[PZ * cap + p], s)) continue;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) tau[i] = slab[(TXX + i) * cap + p];
-    scatter_particle(P, g, s, shared[SMASS * cap + p], slab[VX * cap + p], slab[VY * cap + p], slab[VZ * cap + p],
-                     C, tau, shared[SVOL0 * cap + p]);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], s.base[a]);
+            hi[a] = max(hi[a], s.base[a]);
+        }
+        const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
+        const float v[3] = {src[VX * cap + p], src[VY * cap + p], src[VZ * cap + p]};
+        float C[9], tau[6];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) C[i] = src[(C00 + i) * cap + p];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) tau[i] = src[(TXX + i) * cap + p];
+        const float mdx = m * P.dx;
+        bnd[0] = fmaxf(bnd[0], m);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            bnd[1 + a] = fmaxf(bnd[1 + a], fabsf(m * v[a]) + 1.5f * (fabsf(C[3 * a] * mdx) + fabsf(C[3 * a + 1] * mdx) +
+                                                                   fabsf(C[3 * a + 2] * mdx)));
+        const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]),
+                    r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
+                    r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
+        bnd[4] = fmaxf(bnd[4], 1.5f * (v0 * P.dx * fmaxf(fmaxf(r0, r1), r2)));
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min_i(lo[a]);
+        hi[a] = wave_max_i(hi[a]);
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) bnd[i] = wave_max(bnd[i]);
+    if (lane == 0) {
+        for (int a = 0; a < 3; ++a) {
+            sbox[wave][a] = lo[a];
+            sbox[wave][3 + a] = hi[a];
+        }
+        for (int i = 0; i < 5; ++i) sbound[wave][i] = bnd[i];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int ssz = 1;
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            const int L = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
+            const int H = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
+            ok = ok && L <= H;
+            sgeo[a] = ok ? L : 0;
+            sgeo[3 + a] = ok ? H - L + 3 : 0;
+            ssz *= ok ? H - L + 3 : 0;
+        }
+        sgeo[6] = ok && ssz <= SCAP;
+    }
+    __syncthreads();
+    const int so[3] = {sgeo[0], sgeo[1], sgeo[2]};
+    const int sd[3] = {sgeo[3], sgeo[4], sgeo[5]};
+    const bool stiled = sgeo[6] != 0;
+    const int ssz = sd[0] * sd[1] * sd[2];
+    if (stiled)
+        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0ull;
+    TileScale tsc;
+    {
+        float b[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            b[i] = fmaxf(fmaxf(sbound[0][i], sbound[1][i]), fmaxf(sbound[2][i], sbound[3][i]));
+        tile_scale(b[0], tsc.sm, tsc.lm);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tile_scale(b[1 + a], tsc.sp[a], tsc.lp[a]);
+        tile_scale(b[4], tsc.sf, tsc.lf);
+    }
+    __syncthreads();
+    const int rot = lane & 7;
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        const int p = item.y + r * kBlock + tid;
+        if (p >= end) continue;
+        Stencil s;
+        if (!make_stencil(P, src[PX * cap + p], src[PY * cap + p], src[PZ * cap + p], s)) continue;
+        const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
+        const float v[3] = {src[VX * cap + p], src[VY * cap + p], src[VZ * cap + p]};
+        float C[9], tau[6];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) C[i] = src[(C00 + i) * cap + p];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) tau[i] = src[(TXX + i) * cap + p];
+        float rf;
+        if (stiled && tile_fits(tsc, P, m, v, C, tau, v0, rf))
+            scatter_lds<SCAP>(ss, so, sd, P, s, tsc, m, v, C, tau, v0, rot);
+        else
+            scatter_particle(P, out, s, m, v[0], v[1], v[2], C, tau, v0);
+    }
+    __syncthreads();
+    if (stiled) {
+        const double um = 1.0 / tsc.sm, ux = 1.0 / tsc.sp[0], uy = 1.0 / tsc.sp[1], uz = 1.0 / tsc.sp[2],
+                     uf = 1.0 / tsc.sf;
+        for (int q = tid; q < ssz; q += kBlock) {
+            const long long qm = (long long)ss[q];
+            if (qm == 0) continue;
+            int k = q % sd[2], rest = q / sd[2];
+            int j = rest % sd[1], i = rest / sd[1];
+            int ni = node_index(P, so[0] + i, so[1] + j, so[2] + k);
+            atomicAdd(out.m + ni, (float)((double)qm * um));
+            atomicAdd(out.px + ni, (float)((double)(long long)ss[1 * SCAP + q] * ux));
+            atomicAdd(out.py + ni, (float)((double)(long long)ss[2 * SCAP + q] * uy));
+            atomicAdd(out.pz + ni, (float)((double)(long long)ss[3 * SCAP + q] * uz));
+            atomicAdd(out.fx + ni, (float)((double)(long long)ss[4 * SCAP + q] * uf));
+            atomicAdd(out.fy + ni, (float)((double)(long long)ss[5 * SCAP + q] * uf));
+            atomicAdd(out.fz + ni, (float)((double)(long long)ss[6 * SCAP + q] * uf));
+            out.flag[ni >> 6] = 1;
+        }
+    }
 }
 
 // Near-wall lists of the committed slabs, one workgroup per work item.
@@ -1521,19 +1660,8 @@ void launch_refkeys(hipStream_t s, const BatchDev& B) {
     if (B.nwork > 0) refkey_kernel<<<B.nwork, kBlock, 0, s>>>(B);
 }
 
-void launch_p2g(hipStream_t s, const BatchDev& B, const Params& P, int poff, int n, int boff, int cur, int gcur) {
-    if (n <= 0) return;
-    GridPtrs g = grid_set(B, gcur);
-    const size_t noff = (size_t)boff * 64;
-    g.m += noff;
-    g.px += noff;
-    g.py += noff;
-    g.pz += noff;
-    g.fx += noff;
-    g.fy += noff;
-    g.fz += noff;
-    g.flag += boff;
-    p2g_kernel<<<blocks_for(n), kBlock, 0, s>>>(P, n, B.slab[cur] + poff, B.cap, B.shared[B.sh] + poff, g);
+void launch_p2g(hipStream_t s, const BatchDev& B, int inst, int gcur) {
+    if (B.nwork > 0) setup_deposit_kernel<kChunk, kScatterCap><<<B.nwork, kBlock, 0, s>>>(B, inst, gcur);
 }
 
 void launch_nearwall(hipStream_t s, const BatchDev& B) {
