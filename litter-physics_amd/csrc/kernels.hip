@@ -208,15 +208,16 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
     const int total = B.counters[1];
     const float* shared = B.shared[B.sh];
     const size_t cap = B.cap, na = B.na;
-    // One thread per (near-wall particle, stencil node): a particle's 27 node
-    // visits are independent (each reads its own wall-plane node and deposits on
-    // its own node), so spreading them over threads overlaps their memory
-    // round trips instead of chaining 27 of them in one thread.
-    const long long items = 27LL * total;
+    // One thread per (near-wall particle, stencil column across the wall): a
+    // particle's node visits are independent (each reads its own wall-plane node
+    // and deposits on its own node), so spreading them over threads overlaps
+    // their memory round trips.  A thread takes the three nodes of its column
+    // along the wall axis; one of them is past the wall-side nodes.
+    const long long items = 9LL * total;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < items;
          t += (long long)gridDim.x * blockDim.x) {
-        const int q = (int)(t / 27), node = (int)(t - 27LL * q);
-        const int sa = node / 9, sb = (node / 3) % 3, sc = node % 3;
+        const int q = (int)(t / 9), col = (int)(t - 9LL * q);
+        const int cu = col / 3, cv = col % 3;
         int lo = 0, hi = B.ninst - 1;
         while (lo < hi) {
             int mid = (lo + hi) >> 1;
@@ -251,33 +252,36 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
             img[axis] = 2.0f * face - pos[axis];
             Stencil st;
             if (!make_stencil(P, img[0], img[1], img[2], st)) continue;
-            int co[3] = {st.base[0] + sa, st.base[1] + sb, st.base[2] + sc};
-            if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
-            float w = st.w[0][sa] * st.w[1][sb] * st.w[2][sc];
-            if (w == 0.0f) continue;
-            int plane = face == 0.0f ? 1 : P.cells[axis] + 1;
-            float inward = face == 0.0f ? 1.0f : -1.0f;
             float g_a = P.gravity[axis];
             float normal_stress = slab[(TXX + axis) * cap + p] * v0 - mass * g_a * (img[axis] - pos[axis]);
             if (normal_stress >= 0.0f) continue;
+            int plane = face == 0.0f ? 1 : P.cells[axis] + 1;
+            float inward = face == 0.0f ? 1.0f : -1.0f;
             float weight_momentum = mass * g_a * dt;
             const float* mom = axis == 0 ? g.px : (axis == 1 ? g.py : g.pz);
             const float* frc = axis == 0 ? g.fx : (axis == 1 ? g.fy : g.fz);
-            int pc[3] = {co[0], co[1], co[2]};
-            pc[axis] = plane;
-            size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
-            float pm = g.m[pi];
-            if (pm <= 0.0f) continue;
-            float trial = (mom[pi] - dt * k4 * frc[pi]) / pm + g_a * dt;
-            if (trial * inward >= 0.0f) continue;
-            int off = axis == 0 ? sa : (axis == 1 ? sb : sc);
-            float distance = st.d[axis][off] * P.dx;
-            float reaction = (normal_stress * distance * force_scale + weight_momentum) * w * inward;
-            if (reaction <= 0.0f) continue;
-            size_t ni = noff + node_index(P, co[0], co[1], co[2]);
-            if (g.m[ni] <= 0.0f) continue;
-            atomicAdd(B.trT + axis * na + ni, reaction);
-            if (P.cells[axis] <= 2) atomicAdd(B.trS + axis * na + ni, reaction * inward);
+            for (int off = 0; off < 3; ++off) {
+                const int sa = axis == 0 ? off : cu, sb = axis == 0 ? cu : (axis == 1 ? off : cv),
+                          sc = axis == 2 ? off : cv;
+                int co[3] = {st.base[0] + sa, st.base[1] + sb, st.base[2] + sc};
+                if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
+                float w = st.w[0][sa] * st.w[1][sb] * st.w[2][sc];
+                if (w == 0.0f) continue;
+                int pc[3] = {co[0], co[1], co[2]};
+                pc[axis] = plane;
+                size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
+                float pm = g.m[pi];
+                if (pm <= 0.0f) continue;
+                float trial = (mom[pi] - dt * k4 * frc[pi]) / pm + g_a * dt;
+                if (trial * inward >= 0.0f) continue;
+                float distance = st.d[axis][off] * P.dx;
+                float reaction = (normal_stress * distance * force_scale + weight_momentum) * w * inward;
+                if (reaction <= 0.0f) continue;
+                size_t ni = noff + node_index(P, co[0], co[1], co[2]);
+                if (g.m[ni] <= 0.0f) continue;
+                atomicAdd(B.trT + axis * na + ni, reaction);
+                if (P.cells[axis] <= 2) atomicAdd(B.trS + axis * na + ni, reaction * inward);
+            }
         }
     }
 }
