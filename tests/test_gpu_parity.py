@@ -215,6 +215,26 @@ def test_rejected_step_leaves_state_untouched(engine):
     gsim.advance_to(1.5e-3)
 
 
+def test_sliding_block_reaction_on_the_coulomb_cone(engine):
+    # core/tests/wall_friction.rs:55-70: a uniaxially prestressed paste block sliding at
+    # 1 m/s on a floor with mu = 0.4.  The total wall reaction lies on the Coulomb cone,
+    # |I_t| = mu I_n (1e-12 in f64; the f32 node sums hold it to about 1e-6), with
+    # I = (-7.850e-7, 0, 1.9624e-6) N s (docs/hydrostatic-balance.md).
+    spec, osim = cases.prestressed_block(0.4, (1.0, 0.0, 0.0))
+    _materials[id(osim)] = spec.material
+    gsim = engine_from_oracle(engine, osim)
+    gsim.advance_to(5e-5)
+    osim.advance_to(5e-5)
+    imp = np.array(gsim.ledger().wall_impulse[:])
+    assert imp[2] > 0.0
+    ratio = np.linalg.norm(imp[:2]) / (0.4 * imp[2])
+    print(f"coulomb ratio {ratio!r}, impulse {imp}", flush=True)
+    assert abs(ratio - 1.0) < 1e-5
+    assert abs(imp[0] - -7.850e-7) < 1e-3 * 7.850e-7 and abs(imp[2] - 1.9624e-6) < 1e-3 * 1.9624e-6
+    assert gsim.history().step_count == osim.history().step_count
+    compare_states(gsim, osim)
+
+
 @pytest.mark.parametrize("depth", [2, 3])
 def test_thin_grid_wall_traction_on_both_faces(engine, depth):
     # A paste block filling a grid `depth` cells deep, compressed along z, so that
