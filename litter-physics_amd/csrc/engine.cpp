@@ -351,6 +351,7 @@ struct mpm_engine {
     PelletTrial* d_ptrial = nullptr;
     // work items and partials
     int4* d_work = nullptr;
+    ItemPre* wpre = nullptr;  // set-up records, 2 x work_cap
     int nwork = 0;
     size_t work_cap = 0;
     float* pmax = nullptr;
@@ -424,6 +425,8 @@ struct mpm_engine {
         B.refkey = refkey[sh];
         B.blk_inst = blk_inst;
         B.work = d_work;
+        B.wpre[0] = wpre;
+        B.wpre[1] = wpre + work_cap;
         B.nwork = nwork;
         B.ngwg = ngwg;
         B.counters = counters;
@@ -555,6 +558,7 @@ struct mpm_engine {
             HIP_OK(hipMemcpy(d_pel, empty.data(), sizeof(PelletState) * total_pel, hipMemcpyHostToDevice));
         }
         if (int rc = dalloc(&d_work, work_cap)) return rc;
+        if (int rc = dalloc(&wpre, 2 * work_cap)) return rc;
         if (int rc = dalloc(&pmax, 2 * work_cap)) return rc;
         if (int rc = dalloc(&pdiss, work_cap)) return rc;
         if (int rc = dalloc(&pfail, work_cap)) return rc;
@@ -606,6 +610,7 @@ struct mpm_engine {
         (void)hipFree(d_pacc);
         (void)hipFree(d_ptrial);
         (void)hipFree(d_work);
+        (void)hipFree(wpre);
         (void)hipFree(pmax);
         (void)hipFree(pdiss);
         (void)hipFree(pfail);

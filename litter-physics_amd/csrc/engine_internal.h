@@ -183,6 +183,18 @@ struct GridPtrs {
     int* flag;
 };
 
+// Set-up record of one work item for one slab, written by the step that
+// produced the slab (or by nearwall_kernel after a sort): the box of the
+// stencil bases of the item's particles, and the bounds the deposit scales
+// start from.  G2P2G reads it to lay out its tiles without a pass over the
+// particles first; a particle outside the box takes the untiled path.
+struct ItemPre {
+    int lo[3], hi[3];  // stencil-base box of the particles inside the padded grid (lo > hi: none)
+    float m;           // largest particle mass
+    float v0;          // largest V0
+    float rows[4];     // per wavefront: largest V0 max_a sum_b |tau_ab| of the slab's stresses
+};
+
 // Device pointers of the whole batch.
 struct BatchDev {
     int ninst;
@@ -204,6 +216,7 @@ struct BatchDev {
     int* nw_list[2];  // per slab, instance ranges like the particles
     int* blk_inst;    // node block -> instance
     int4* work;       // (instance, start, count, 0)
+    ItemPre* wpre[2];  // per slab, one per work item
     int nwork;        // number of work items
     int ngwg;         // grid-update workgroups (all instances)
     int* counters;    // [0] unused, [1] near-wall entries of the running instances

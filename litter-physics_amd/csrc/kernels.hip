@@ -676,10 +676,9 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
     __shared__ float4 sg[GCAP];
     __shared__ unsigned long long ss[7 * SCAP];
-    __shared__ float sbound[4][6];
-    __shared__ int sbox[4][6];
-    __shared__ Tile tile;
-    __shared__ int sscat;
+    __shared__ float sbound[4][3];
+    __shared__ int snbox[4][6];
+    __shared__ float snrow[4];
     __shared__ double sdiss[4];
     __shared__ float sred[4][2];
     __shared__ unsigned long long sfail[4];
@@ -717,13 +716,35 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     int* nw_list = B.nw_list[is.cur ^ 1] + ip.poff;
     int* nw_count = &is.nw_count[is.cur ^ 1];
 
-    // Phase A: stencil bases of the current positions and their bounding box.
+    // The committed slab's set-up record gives the tiles: the gather tile spans
+    // the stencil bases of the item's particles, the scatter tile one cell more
+    // on each side (an accepted step moves a particle by at most one cell).
+    const ItemPre* __restrict__ pre = B.wpre[is.cur] + blockIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    Tile t;
+    bool stiled;
+    {
+        bool ok = true;
+        int gsz = 1, ssz = 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ok = ok && pre->lo[a] <= pre->hi[a];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            t.org[a] = ok ? pre->lo[a] : 0;
+            t.gd[a] = ok ? pre->hi[a] - pre->lo[a] + 3 : 0;
+            t.sd[a] = t.gd[a] + 2;
+            gsz *= t.gd[a];
+            ssz *= t.sd[a];
+        }
+        t.tiled = ok && gsz <= GCAP;
+        stiled = t.tiled && ssz <= SCAP;
+    }
+    const int so[3] = {t.org[0] - 1, t.org[1] - 1, t.org[2] - 1};
+    const int sd[3] = {t.sd[0], t.sd[1], t.sd[2]};
+    const int ssz = sd[0] * sd[1] * sd[2];
+
+    // Phase A: the current positions (and the order keys at the sort cadence).
     float px[R], py[R], pz[R];
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    // Deposit-scale inputs: the largest mass, the largest v0 max_a sum_b |tau_ab|
-    // of the committed stresses and the largest v0 (the tile's node speeds
-    // join them below).
-    float bnd[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int p = chunk0 + r * kBlock + tid;
@@ -734,63 +755,10 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             pz[r] = src[PZ * cap + p];
             // The committed step hit the 32-step cadence: order keys of the committed positions.
             if (is.refsort) B.refkey[p] = (ref_cell(P, px[r], py[r], pz[r]) << 32) | (uint64_t)B.ids[p];
-            float t6[6];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) t6[i] = fabsf(src[(TXX + i) * cap + p]);
-            const float v0 = shared[SVOL0 * cap + p];
-            bnd[0] = fmaxf(bnd[0], shared[SMASS * cap + p]);
-            bnd[4] = fmaxf(bnd[4], v0 * fmaxf(fmaxf(t6[0] + t6[3] + t6[4], t6[3] + t6[1] + t6[5]), t6[4] + t6[5] + t6[2]));
-            bnd[5] = fmaxf(bnd[5], v0);
-            float xs[3] = {px[r] * P.inv_dx, py[r] * P.inv_dx, pz[r] * P.inv_dx};
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                float bf = floorf(xs[a] - 0.5f) + 1.0f;
-                if (bf >= 0.0f && bf + 2.0f < (float)P.nodes[a]) {
-                    int bi = (int)bf;
-                    lo[a] = min(lo[a], bi);
-                    hi[a] = max(hi[a], bi);
-                }
-            }
         }
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min_i(lo[a]);
-        hi[a] = wave_max_i(hi[a]);
-    }
-    if (lane == 0)
-        for (int a = 0; a < 3; ++a) {
-            sbox[wave][a] = lo[a];
-            sbox[wave][3 + a] = hi[a];
-        }
-    __syncthreads();
-    if (tid == 0) {
-        Tile t;
-        int L[3], H[3];
-        for (int a = 0; a < 3; ++a) {
-            L[a] = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
-            H[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
-        }
-        bool ok = L[0] <= H[0] && L[1] <= H[1] && L[2] <= H[2];
-        int gsz = 1, ssz = 1;
-        for (int a = 0; a < 3; ++a) {
-            t.org[a] = ok ? L[a] : 0;
-            t.gd[a] = ok ? H[a] - L[a] + 3 : 0;
-            t.sd[a] = t.gd[a] + 2;  // scatter tile: origin org - 1
-            gsz *= t.gd[a];
-            ssz *= t.sd[a];
-        }
-        t.tiled = ok && gsz <= GCAP;
-        tile = t;
-        // The deposit scales need the gather tile's velocity bound.
-        sscat = t.tiled && ssz <= SCAP;
-    }
-    __syncthreads();
-    const Tile t = tile;
-    const bool stiled = sscat != 0;
-    const int so[3] = {t.org[0] - 1, t.org[1] - 1, t.org[2] - 1};
-    const int sd[3] = {t.sd[0], t.sd[1], t.sd[2]};
-    const int ssz = sd[0] * sd[1] * sd[2];
+    // Gather tile, and the largest node speed per axis for the momentum scales.
+    float bnd[3] = {0.0f, 0.0f, 0.0f};
     if (t.tiled) {
         int gsz = t.gd[0] * t.gd[1] * t.gd[2];
         for (int q = tid; q < gsz; q += kBlock) {
@@ -799,19 +767,19 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
             const float4 g = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
             sg[q] = g;
-            bnd[1] = fmaxf(bnd[1], fabsf(g.x));
-            bnd[2] = fmaxf(bnd[2], fabsf(g.y));
-            bnd[3] = fmaxf(bnd[3], fabsf(g.z));
+            bnd[0] = fmaxf(bnd[0], fabsf(g.x));
+            bnd[1] = fmaxf(bnd[1], fabsf(g.y));
+            bnd[2] = fmaxf(bnd[2], fabsf(g.z));
         }
     }
     if (stiled)
         for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0ull;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < 3; ++i) {
         bnd[i] = wave_max(bnd[i]);
         if (lane == 0) sbound[wave][i] = bnd[i];
     }
-    const float wave_rows = bnd[4];  // this wavefront's committed stress bound
+    const float wave_rows = pre->rows[wv];  // this wavefront's committed stress bound
     __syncthreads();
     // Deposit scales.  The new velocity is a convex combination of the tile's
     // node velocities and |C_ab| dx <= 2 max |v_a| (sum_i w_i |d_i| <= 1/2 cell),
@@ -823,11 +791,14 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     TileScale tsc;
     {
         float b[6];
+        b[0] = pre->m;
 #pragma unroll
-        for (int i = 0; i < 6; ++i)
-            b[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
-                                                 int, fmaxf(fmaxf(sbound[0][i], sbound[1][i]),
-                                                            fmaxf(sbound[2][i], sbound[3][i])))));
+        for (int i = 0; i < 3; ++i)
+            b[1 + i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                                                     int, fmaxf(fmaxf(sbound[0][i], sbound[1][i]),
+                                                                fmaxf(sbound[2][i], sbound[3][i])))));
+        b[4] = fmaxf(fmaxf(pre->rows[0], pre->rows[1]), fmaxf(pre->rows[2], pre->rows[3]));
+        b[5] = pre->v0;
         tile_scale(b[0], tsc.sm, tsc.lm);
 #pragma unroll
         for (int a = 0; a < 3; ++a) tile_scale(10.25f * b[0] * b[1 + a], tsc.sp[a], tsc.lp[a]);
@@ -849,6 +820,9 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     int escaped = 0;
     unsigned long long mlp_tried = 0, mlp_taken = 0;
     const int rot = lane & 7;
+    // The trial slab's set-up record, gathered round by round as uniform values.
+    int nlo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    float nrows = 0.0f;
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int p = chunk0 + r * kBlock + tid;
@@ -865,7 +839,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         // The particle's remaining slab reads, issued before the gather so that
         // their memory latency overlaps it (the barrier keeps the scheduler from
         // sinking them to their first use).
-        float Hp[9], pm = 0.0f, pv0 = 0.0f, pe = 0.0f;
+        float Hp[9], pm = 0.0f, pv0 = 0.0f, pe = 0.0f, prow = 0.0f;
         if (p < chunk_end) {
             pm = shared[SMASS * cap + p];
             pv0 = shared[SVOL0 * cap + p];
@@ -881,11 +855,20 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             if (!make_stencil(P, x, y, z, s)) {
                 // Outside the padded grid: frozen for this trial (solver.rs:905-917).
                 for (int f = 0; f < NPF; ++f) dst[f * cap + p] = src[f * cap + p];
+                float tf[6];
+                for (int i = 0; i < 6; ++i) tf[i] = src[(TXX + i) * cap + p];
+                prow = stress_rows(pv0, tf);
                 escaped += 1;
             } else {
                 live = true;
                 float Bm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                if (t.tiled)
+                bool in_box = t.tiled;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int li = s.base[a] - t.org[a];
+                    in_box = in_box && li >= 0 && li + 2 < t.gd[a];
+                }
+                if (in_box)
                     gather_lds(sg, t, s, v, Bm);
                 else
                     gather_global(P, gv, na, s, v, Bm);
@@ -1006,6 +989,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
+                prow = stress_rows(pv0, tau);
                 if (make_stencil(P, nx, ny, nz, s2)) {
                     dep = true;
                     nw = near_wall(P, nx, ny, nz);
@@ -1036,6 +1020,12 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 scatter_particle(P, out, s2, pm, v[0], v[1], v[2], C, tau, pv0);
         }
         wave_append(nw, p, nw_list, nw_count);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            nlo[a] = min(nlo[a], wave_min_uniform(dep ? s2.base[a] : INT_MAX));
+            nhi[a] = max(nhi[a], wave_max_uniform(dep ? s2.base[a] : INT_MIN));
+        }
+        nrows = fmaxf(nrows, wave_fmax_uniform(prow));
         // Log of exact returns (training data of the surrogate).
         if (B.rlog) {
             const bool lg = live && P.kind == 0 && yielded && rc == F_OK;
@@ -1103,9 +1093,25 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         sdiss[wave] = diss;
         sfail[wave] = fail;
         sesc[wave] = escaped;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            snbox[wave][a] = nlo[a];
+            snbox[wave][3 + a] = nhi[a];
+        }
+        snrow[wave] = nrows;
     }
     __syncthreads();
     if (tid == 0) {
+        // Set-up record of the trial slab for the next step (used if it commits).
+        ItemPre rec;
+        for (int a = 0; a < 3; ++a) {
+            rec.lo[a] = min(min(snbox[0][a], snbox[1][a]), min(snbox[2][a], snbox[3][a]));
+            rec.hi[a] = max(max(snbox[0][3 + a], snbox[1][3 + a]), max(snbox[2][3 + a], snbox[3][3 + a]));
+        }
+        rec.m = pre->m;
+        rec.v0 = pre->v0;
+        for (int w = 0; w < 4; ++w) rec.rows[w] = snrow[w];
+        B.wpre[is.cur ^ 1][blockIdx.x] = rec;
         float s0 = 0.0f, s1 = 0.0f;
         double d = 0.0;
         unsigned long long f = ~0ull;
@@ -1535,20 +1541,70 @@ __global__ __launch_bounds__(kBlock, 2) void setup_deposit_kernel(BatchDev B, in
     }
 }
 
-// Near-wall lists of the committed slabs, one workgroup per work item.
+// Near-wall lists of the committed slabs and the work items' set-up records
+// (ItemPre) for them, one workgroup per work item.
 __global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
+    __shared__ int sbox[4][6];
+    __shared__ float sbnd[4][3];
     const int4 item = B.work[blockIdx.x];
     InstState& s = B.is[item.x];
     const InstParams& ip = B.ip[item.x];
     const float* slab = B.slab[s.cur];
+    const float* shared = B.shared[B.sh];
     int* list = B.nw_list[s.cur] + ip.poff;
     int* count = &s.nw_count[s.cur];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    float m = 0.0f, v0 = 0.0f, rows = 0.0f;
     // Every lane of a wavefront takes part in the aggregated append.
     for (int base = item.y; base < item.y + item.z; base += kBlock) {
-        int p = base + threadIdx.x;
-        bool nw = p < item.y + item.z &&
-                  near_wall(ip.P, slab[PX * B.cap + p], slab[PY * B.cap + p], slab[PZ * B.cap + p]);
+        const int p = base + threadIdx.x;
+        bool nw = false;
+        if (p < item.y + item.z) {
+            const float x = slab[PX * B.cap + p], y = slab[PY * B.cap + p], z = slab[PZ * B.cap + p];
+            nw = near_wall(ip.P, x, y, z);
+            Stencil st;
+            if (make_stencil(ip.P, x, y, z, st))
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = min(lo[a], st.base[a]);
+                    hi[a] = max(hi[a], st.base[a]);
+                }
+            float tau[6];
+            for (int i = 0; i < 6; ++i) tau[i] = slab[(TXX + i) * B.cap + p];
+            const float pv0 = shared[SVOL0 * B.cap + p];
+            m = fmaxf(m, shared[SMASS * B.cap + p]);
+            v0 = fmaxf(v0, pv0);
+            rows = fmaxf(rows, stress_rows(pv0, tau));
+        }
         wave_append(nw, p, list, count);
+    }
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min_uniform(lo[a]);
+        hi[a] = wave_max_uniform(hi[a]);
+    }
+    m = wave_fmax_uniform(m);
+    v0 = wave_fmax_uniform(v0);
+    rows = wave_fmax_uniform(rows);
+    if (lane == 0) {
+        for (int a = 0; a < 3; ++a) {
+            sbox[wave][a] = lo[a];
+            sbox[wave][3 + a] = hi[a];
+        }
+        sbnd[wave][0] = m;
+        sbnd[wave][1] = v0;
+        sbnd[wave][2] = rows;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ItemPre r;
+        for (int a = 0; a < 3; ++a) {
+            r.lo[a] = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
+            r.hi[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
+        }
+        r.m = fmaxf(fmaxf(sbnd[0][0], sbnd[1][0]), fmaxf(sbnd[2][0], sbnd[3][0]));
+        r.v0 = fmaxf(fmaxf(sbnd[0][1], sbnd[1][1]), fmaxf(sbnd[2][1], sbnd[3][1]));
+        for (int w = 0; w < 4; ++w) r.rows[w] = sbnd[w][2];
+        B.wpre[s.cur][blockIdx.x] = r;
     }
 }
 

@@ -551,6 +551,53 @@ __device__ inline double wave_sum_last(double v) {
     return v;
 }
 
+// Wavefront minimum and maximum by the same DPP pattern, returned as a uniform
+// (scalar) value.  Lanes without a DPP source keep their own value.  Every lane
+// of the wavefront must be active.
+template <int CTRL, int ROWS>
+__device__ inline int dpp_self(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false);
+}
+
+__device__ inline int wave_min_uniform(int v) {
+    v = min(v, dpp_self<0x111, 0xF>(v));
+    v = min(v, dpp_self<0x112, 0xF>(v));
+    v = min(v, dpp_self<0x114, 0xF>(v));
+    v = min(v, dpp_self<0x118, 0xF>(v));
+    v = min(v, dpp_self<0x142, 0xA>(v));
+    v = min(v, dpp_self<0x143, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ inline int wave_max_uniform(int v) {
+    v = max(v, dpp_self<0x111, 0xF>(v));
+    v = max(v, dpp_self<0x112, 0xF>(v));
+    v = max(v, dpp_self<0x114, 0xF>(v));
+    v = max(v, dpp_self<0x118, 0xF>(v));
+    v = max(v, dpp_self<0x142, 0xA>(v));
+    v = max(v, dpp_self<0x143, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// fmaxf over the wavefront (NaN lanes ignored, as fmaxf does).
+__device__ inline float wave_fmax_uniform(float v) {
+    v = fmaxf(v, __int_as_float(dpp_self<0x111, 0xF>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_self<0x112, 0xF>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_self<0x114, 0xF>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_self<0x118, 0xF>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_self<0x142, 0xA>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_self<0x143, 0xC>(__float_as_int(v))));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// V0 max_a sum_b |tau_ab| of a symmetric stress (xx, yy, zz, xy, xz, yz): the
+// per-particle quantity the deposit's stress scale bounds.
+__device__ inline float stress_rows(float v0, const float* tau) {
+    const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]), r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
+                r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
+    return v0 * fmaxf(fmaxf(r0, r1), r2);
+}
+
 // Block-wide sum of NV doubles held by every thread; thread 0 gets the totals.
 template <int NV>
 __device__ inline void block_sum(double* vals, double* scratch) {
