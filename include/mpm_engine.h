@@ -230,7 +230,10 @@ int mpm_liquid_update_batch(const mpm_material* material, double dt, size_t n,
 
 /* Per-kernel timing since profiling was last enabled, for the benchmark: the
  * average device time of the fused grid-to-particle-to-grid kernel in ms and
- * its launch count. */
+ * the number of launches timed.  mpm_enable_profiling with on = N > 0 times
+ * every N-th launch with a pair of events around it (each event leaves the GPU
+ * idle for a few microseconds between two kernels); on = 0 stops.  Either call
+ * resets the sums.  Timing turns the graph replay off. */
 int mpm_profile_g2p2g(mpm_engine* engine, double* avg_ms, uint64_t* launches);
 void mpm_enable_profiling(mpm_engine* engine, int on);
 

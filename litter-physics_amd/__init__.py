@@ -557,8 +557,9 @@ class Simulation:
         led = self.ledger()
         return self.mechanical_energy() - led.initial_mechanical_energy - led.ledgered_energy()
 
-    def enable_profiling(self, on=True):
-        lib().mpm_enable_profiling(self.h, 1 if on else 0)
+    def enable_profiling(self, on=True, every=1):
+        """Time every `every`-th fused G2P2G launch with HIP events (on=False stops)."""
+        lib().mpm_enable_profiling(self.h, max(int(every), 1) if on else 0)
 
     def g2p2g_profile(self):
         ms = C.c_double()
@@ -790,8 +791,9 @@ class Batch:
         """(inputs n x 12, targets n x 7, aux n x 6) of the logged exact returns."""
         return _take_returns(self, capacity)
 
-    def enable_profiling(self, on=True):
-        lib().mpm_enable_profiling(self.h, 1 if on else 0)
+    def enable_profiling(self, on=True, every=1):
+        """Time every `every`-th fused G2P2G launch with HIP events (on=False stops)."""
+        lib().mpm_enable_profiling(self.h, max(int(every), 1) if on else 0)
 
     def g2p2g_profile(self):
         ms = C.c_double()

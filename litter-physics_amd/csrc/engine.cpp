@@ -376,6 +376,9 @@ struct mpm_engine {
     bool graph_stale = true;
     hipEvent_t ev0[kMaxStepsPerSync] = {}, ev1[kMaxStepsPerSync] = {};
     bool profiling = false;
+    int profile_every = 1;        // time every profile_every-th G2P2G launch
+    uint64_t profile_seq = 0;     // G2P2G launches since profiling was enabled
+    bool profiled[kMaxStepsPerSync] = {};  // the launches of the enqueued steps that carry events
     double g2p2g_ms = 0.0;
     uint64_t g2p2g_launches = 0;
     // return-map surrogate and the log of exact returns
@@ -569,9 +572,13 @@ struct mpm_engine {
         if (int rc = dalloc(&mlp_stats, 2)) return rc;
         sort_bytes = std::max(sort_temp_bytes((int)cap, end_bit), rle_temp_bytes((int)cap));
         HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
+        // Timing-only events: without the system-scope fence, recording one does
+        // not write back and invalidate the caches between the kernels it separates
+        // (about 5.7 us per record on a 1M-particle step).  They are read only
+        // after the stream has been synchronised.
         for (int k = 0; k < kMaxStepsPerSync; ++k) {
-            HIP_OK(hipEventCreate(&ev0[k]));
-            HIP_OK(hipEventCreate(&ev1[k]));
+            HIP_OK(hipEventCreateWithFlags(&ev0[k], hipEventDisableSystemFence));
+            HIP_OK(hipEventCreateWithFlags(&ev1[k], hipEventDisableSystemFence));
         }
         for (int i = 0; i < ninst; ++i)
             if (int rc = upload_params(i)) return rc;
@@ -892,9 +899,11 @@ struct mpm_engine {
             }
             HIP_OK(hipGraphLaunch(graph, stream));
         } else {
-            for (int j = 0; j < k; ++j)
-                launch_step(stream, B, launch.traction_wgs, pel, profiling ? ev0[j] : nullptr,
-                            profiling ? ev1[j] : nullptr);
+            for (int j = 0; j < k; ++j) {
+                profiled[j] = profiling && profile_seq++ % (uint64_t)profile_every == 0;
+                launch_step(stream, B, launch.traction_wgs, pel, profiled[j] ? ev0[j] : nullptr,
+                            profiled[j] ? ev1[j] : nullptr);
+            }
         }
         HIP_OK(hipGetLastError());
         return MPM_OK;
@@ -902,6 +911,8 @@ struct mpm_engine {
     int collect_profile(int k) {
         if (!profiling) return MPM_OK;
         for (int j = 0; j < k; ++j) {
+            if (!profiled[j]) continue;
+            profiled[j] = false;
             float ms = 0.0f;
             HIP_OK(hipEventElapsedTime(&ms, ev0[j], ev1[j]));
             g2p2g_ms += ms;
@@ -2024,7 +2035,9 @@ int mpm_liquid_update_batch(const mpm_material* material, double dt, size_t n, c
 
 void mpm_enable_profiling(mpm_engine* e, int on) {
     if (!e) return;
-    e->profiling = on != 0;
+    e->profiling = on > 0;
+    e->profile_every = on > 0 ? on : 1;
+    e->profile_seq = 0;
     e->g2p2g_ms = 0.0;
     e->g2p2g_launches = 0;
 }
