@@ -236,10 +236,12 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
         GridPtrs g = grid_set(B, s.gcur);
         const size_t noff = (size_t)ip.boff * 64;
         float pos[3] = {slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]};
+        const float tnn[3] = {slab[TXX * cap + p], slab[TYY * cap + p], slab[TZZ * cap + p]};
         float half = 0.5f * P.dx;
         float mass = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
         float k4 = 4.0f * P.inv_dx * P.inv_dx;
         float force_scale = -dt * k4;
+#pragma unroll
         for (int axis = 0; axis < 3; ++axis) {
             float face;
             if (pos[axis] < half)
@@ -253,34 +255,51 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
             Stencil st;
             if (!make_stencil(P, img[0], img[1], img[2], st)) continue;
             float g_a = P.gravity[axis];
-            float normal_stress = slab[(TXX + axis) * cap + p] * v0 - mass * g_a * (img[axis] - pos[axis]);
+            float normal_stress = tnn[axis] * v0 - mass * g_a * (img[axis] - pos[axis]);
             if (normal_stress >= 0.0f) continue;
             int plane = face == 0.0f ? 1 : P.cells[axis] + 1;
             float inward = face == 0.0f ? 1.0f : -1.0f;
             float weight_momentum = mass * g_a * dt;
             const float* mom = axis == 0 ? g.px : (axis == 1 ? g.py : g.pz);
             const float* frc = axis == 0 ? g.fx : (axis == 1 ? g.fy : g.fz);
+            // The column's three nodes: all their grid reads first (wall-plane node
+            // mass, momentum and force, deposit node mass), then the deposits, so a
+            // thread waits for one memory round trip rather than two per node.
+            float w[3], pm[3], pmo[3], pfr[3], dm[3];
+            size_t nis[3];
+#pragma unroll
             for (int off = 0; off < 3; ++off) {
                 const int sa = axis == 0 ? off : cu, sb = axis == 0 ? cu : (axis == 1 ? off : cv),
                           sc = axis == 2 ? off : cv;
                 int co[3] = {st.base[0] + sa, st.base[1] + sb, st.base[2] + sc};
+                w[off] = 0.0f;
+                pm[off] = pmo[off] = pfr[off] = dm[off] = 0.0f;
+                nis[off] = 0;
                 if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
-                float w = st.w[0][sa] * st.w[1][sb] * st.w[2][sc];
-                if (w == 0.0f) continue;
+                // sel3: the stencil stays in registers (a runtime index would place it in scratch).
+                w[off] = sel3(sa, st.w[0][0], st.w[0][1], st.w[0][2]) * sel3(sb, st.w[1][0], st.w[1][1], st.w[1][2]) *
+                         sel3(sc, st.w[2][0], st.w[2][1], st.w[2][2]);
+                if (w[off] == 0.0f) continue;
                 int pc[3] = {co[0], co[1], co[2]};
                 pc[axis] = plane;
-                size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
-                float pm = g.m[pi];
-                if (pm <= 0.0f) continue;
-                float trial = (mom[pi] - dt * k4 * frc[pi]) / pm + g_a * dt;
+                const size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
+                nis[off] = noff + node_index(P, co[0], co[1], co[2]);
+                pm[off] = g.m[pi];
+                pmo[off] = mom[pi];
+                pfr[off] = frc[pi];
+                dm[off] = g.m[nis[off]];
+            }
+#pragma unroll
+            for (int off = 0; off < 3; ++off) {
+                if (w[off] == 0.0f || pm[off] <= 0.0f) continue;
+                float trial = (pmo[off] - dt * k4 * pfr[off]) / pm[off] + g_a * dt;
                 if (trial * inward >= 0.0f) continue;
                 float distance = st.d[axis][off] * P.dx;
-                float reaction = (normal_stress * distance * force_scale + weight_momentum) * w * inward;
+                float reaction = (normal_stress * distance * force_scale + weight_momentum) * w[off] * inward;
                 if (reaction <= 0.0f) continue;
-                size_t ni = noff + node_index(P, co[0], co[1], co[2]);
-                if (g.m[ni] <= 0.0f) continue;
-                atomicAdd(B.trT + axis * na + ni, reaction);
-                if (P.cells[axis] <= 2) atomicAdd(B.trS + axis * na + ni, reaction * inward);
+                if (dm[off] <= 0.0f) continue;
+                atomicAdd(B.trT + axis * na + nis[off], reaction);
+                if (P.cells[axis] <= 2) atomicAdd(B.trS + axis * na + nis[off], reaction * inward);
             }
         }
     }
@@ -299,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
 // sum w (m v + m C d) is exactly the audit's shadow deposit), after the stress
 // impulse, after the wall traction, with gravity's kick, and after the walls.
 template <bool AUDIT>
-__global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
+__global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     __shared__ double scratch[4 * kGridVals];
     __shared__ double ascratch[AUDIT ? 4 * kAuditVals : 1];
     __shared__ int sbad[4];
@@ -316,10 +335,52 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
     double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
-    for (int j = 0; j < kGridWgBlocks / 4; ++j) {
+    constexpr int J = kGridWgBlocks / 4;  // node blocks per wavefront
+    // All loads of the wavefront's blocks are issued before any result is used:
+    // the flags of both sets, then the sums and traction of the touched blocks,
+    // so a wavefront waits for two memory round trips instead of two per block.
+    int cf[J], af[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
+        cf[j] = C.flag[b];
+        af[j] = A.flag[b];
+    }
+    float fm[J], fpx[J], fpy[J], fpz[J], ffx[J], ffy[J], ffz[J], ftb[J][3];
+    bool wall[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
         const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
         const size_t n = ((size_t)b << 6) | l;
-        if (C.flag[b]) {
+        fm[j] = fpx[j] = fpy[j] = fpz[j] = ffx[j] = ffy[j] = ffz[j] = 0.0f;
+        wall[j] = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ftb[j][a] = 0.0f;
+        if (!af[j]) continue;
+        // The traction buffers can be nonzero only where wall_traction_to_grid
+        // deposits: a particle's wall image lands its deposits on node 2 or node
+        // `cells` of the wall axis, in the first block or the block holding node
+        // `cells` along that axis.
+        int bc[3];
+        node_coords(P, (b - ip.boff) << 6, bc[0], bc[1], bc[2]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) wall[j] = wall[j] || bc[a] == 0 || (bc[a] >> 2) == (P.cells[a] >> 2);
+        fm[j] = A.m[n];
+        fpx[j] = A.px[n];
+        fpy[j] = A.py[n];
+        fpz[j] = A.pz[n];
+        ffx[j] = A.fx[n];
+        ffy[j] = A.fy[n];
+        ffz[j] = A.fz[n];
+        if (wall[j])
+#pragma unroll
+            for (int a = 0; a < 3; ++a) ftb[j][a] = B.trT[a * na + n];
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
+        const size_t n = ((size_t)b << 6) | l;
+        if (cf[j]) {
             C.m[n] = 0.0f;
             C.px[n] = 0.0f;
             C.py[n] = 0.0f;
@@ -329,24 +390,21 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
             C.fz[n] = 0.0f;
             if (l == 0) C.flag[b] = 0;
         }
-        if (!A.flag[b]) continue;
-        // The traction buffers can be nonzero only where wall_traction_to_grid
-        // deposits: a particle's wall image lands its deposits on node 2 or node
-        // `cells` of the wall axis, in the first block or the block holding node
-        // `cells` along that axis.
-        int bc[3];
-        node_coords(P, (b - ip.boff) << 6, bc[0], bc[1], bc[2]);
-        bool wall_block = false;
+    }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) wall_block = wall_block || bc[a] == 0 || (bc[a] >> 2) == (P.cells[a] >> 2);
-        const float m = A.m[n];
+    for (int j = 0; j < J; ++j) {
+        if (!af[j]) continue;
+        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
+        const size_t n = ((size_t)b << 6) | l;
+        const bool wall_block = wall[j];
+        const float m = fm[j];
         if (!(m > 0.0f)) {
             B.gv[n] = 0.0f;
             B.gv[na + n] = 0.0f;
             B.gv[2 * na + n] = 0.0f;
             continue;
         }
-        float p[3] = {A.px[n] - dt * k4 * A.fx[n], A.py[n] - dt * k4 * A.fy[n], A.pz[n] - dt * k4 * A.fz[n]};
+        float p[3] = {fpx[j] - dt * k4 * ffx[j], fpy[j] - dt * k4 * ffy[j], fpz[j] - dt * k4 * ffz[j]};
         float v[3], tb[3];
         int c[3];
         node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
@@ -356,7 +414,7 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
             // the face the node serves (node 2 the low face, inward +1; node
             // `cells` the high face, -1).  A grid two cells deep puts both faces
             // on node 2 and keeps the signed sum in trS.
-            tb[a] = wall_block ? B.trT[a * na + n] : 0.0f;
+            tb[a] = ftb[j][a];
             float D;
             if (P.cells[a] <= 2) {
                 D = wall_block ? B.trS[a * na + n] : 0.0f;
@@ -373,7 +431,7 @@ __global__ __launch_bounds__(256) void grid_update_kernel(BatchDev B) {
             v[a] = (p[a] + D) / m + P.gravity[a] * dt;
             if (AUDIT) {
                 const double md = (double)m;
-                const double pt = a == 0 ? (double)A.px[n] : (a == 1 ? (double)A.py[n] : (double)A.pz[n]);
+                const double pt = a == 0 ? (double)fpx[j] : (a == 1 ? (double)fpy[j] : (double)fpz[j]);
                 avals[0] += 0.5 * pt * pt / md;
                 avals[1] += 0.5 * (double)p[a] * (double)p[a] / md;
                 avals[2] += 0.5 * ((double)p[a] + (double)D) * ((double)p[a] + (double)D) / md;
@@ -1239,21 +1297,54 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
     double acc[kGridVals + 1];
     for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
+    // Each thread sums its partials in index order.  The loads of several
+    // partials are issued together (one memory round trip per batch, not per
+    // partial); a load past the end contributes 0, which leaves a sum unchanged.
     int bad = 0;
-    for (int q = g0 + tid; q < g1; q += 256) {
+    for (int q0 = g0 + tid; q0 < g1; q0 += 4 * 256) {
+        double x[4][kGridVals];
+        int xb[4];
 #pragma unroll
-        for (int i = 0; i < kGridVals; ++i) acc[i] += B.gpart[(size_t)i * B.ngwg + q];
-        bad += B.gbad[q];
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * 256;
+#pragma unroll
+            for (int i = 0; i < kGridVals; ++i) x[u][i] = q < g1 ? B.gpart[(size_t)i * B.ngwg + q] : 0.0;
+            xb[u] = q < g1 ? B.gbad[q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int i = 0; i < kGridVals; ++i) acc[i] += x[u][i];
+            bad += xb[u];
+        }
     }
     float smax = 0.0f, dmax = 0.0f;
     unsigned long long f = ~0ull;
     int esc = 0;
-    for (int q = s.w_off + tid; q < s.w_off + s.w_cnt; q += 256) {
-        smax = fmaxf(smax, B.pmax[2 * q]);
-        dmax = fmaxf(dmax, B.pmax[2 * q + 1]);
-        acc[kGridVals] += B.pdiss[q];
-        f = B.pfail[q] < f ? B.pfail[q] : f;
-        esc += B.pesc[q];
+    const int w1 = s.w_off + s.w_cnt;
+    for (int q0 = s.w_off + tid; q0 < w1; q0 += 8 * 256) {
+        float xs[8], xd[8];
+        double xe[8];
+        unsigned long long xf[8];
+        int xc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u * 256;
+            const bool in = q < w1;
+            xs[u] = in ? B.pmax[2 * q] : 0.0f;
+            xd[u] = in ? B.pmax[2 * q + 1] : 0.0f;
+            xe[u] = in ? B.pdiss[q] : 0.0;
+            xf[u] = in ? B.pfail[q] : ~0ull;
+            xc[u] = in ? B.pesc[q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            smax = fmaxf(smax, xs[u]);
+            dmax = fmaxf(dmax, xd[u]);
+            acc[kGridVals] += xe[u];
+            f = xf[u] < f ? xf[u] : f;
+            esc += xc[u];
+        }
     }
     // Pellet sums (the instance's pellets, in parallel).
     double pel[kPelSums];
