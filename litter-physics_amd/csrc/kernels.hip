@@ -877,7 +877,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     unsigned long long fail = ~0ull;
     int escaped = 0;
     unsigned long long mlp_tried = 0, mlp_taken = 0;
-    const int rot = lane & 7;
+    const int rot = (lane * 5) % 9;
     // The trial slab's set-up record, gathered round by round as uniform values.
     int nlo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MIN, INT_MIN, INT_MIN};
     float nrows = 0.0f;
@@ -1590,7 +1590,7 @@ __global__ __launch_bounds__(kBlock, 2) void setup_deposit_kernel(BatchDev B, in
         tile_scale(b[4], tsc.sf, tsc.lf);
     }
     __syncthreads();
-    const int rot = lane & 7;
+    const int rot = (lane * 5) % 9;
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int p = item.y + r * kBlock + tid;
