@@ -1,0 +1,62 @@
+"""The 1M-particle paste column at full size, through size-independent checks.
+
+The oracle cannot step a million particles in a test's time, so this case is
+checked through what holds at any size: the particle set is preserved, the
+mass ledger closes, the momentum ledger matches the state to f32 round-off,
+no step is rejected across two locality sorts, and the state stays inside the
+domain and finite.  The parity against the oracle itself runs on the smaller
+cases of test_gpu_parity.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def column(E):
+    # 100^3 particles (2 per cell and axis) of a Herschel-Bulkley paste on a
+    # 96^3 grid, dx 2.5 mm: free-slip walls, sticky floor, CFL 0.4.
+    bulk, shear = 2e5, 5e4
+    young = 9.0 * bulk * shear / (3.0 * bulk + shear)
+    poisson = (3.0 * bulk - 2.0 * shear) / (2.0 * (3.0 * bulk + shear))
+    mat = E.paste(1200.0, young, poisson, 50.0, 5.0, 0.5, shear_units=True)
+    dx = 0.0025
+    domain = [96 * dx] * 3
+    cfg = E.config(wall_friction=0.0, cfl=0.4, max_dt=1e-4, min_dt=1e-12,
+                   face_friction=[0.0, 0.0, 0.0, 0.0, math.inf, 0.0])
+    st = E.place_particles("slump", mat, dx, domain, [0.125] * 3, seed=0, jitter=0.5)
+    return mat, cfg, dx, domain, st
+
+
+def test_million_particle_column_invariants(engine):
+    mat, cfg, dx, domain, st = column(engine)
+    n = len(st)
+    assert n == 1_000_000
+    sim = engine.Simulation(mat, cfg, domain, dx, st)
+    print("column placed and uploaded", flush=True)
+    steps = 70  # crosses the locality sorts after steps 32 and 64
+    assert sim.advance_steps(steps) == steps
+    h, led = sim.history(), sim.ledger()
+    assert h.step_count == steps and h.rejected_steps == 0
+    out = sim.state().by_id()
+    assert np.array_equal(out.id, np.sort(np.asarray(st.id)))
+    # Mass closes in f64 (the exact particle masses are kept by id; two f64 sums
+    # of a million masses in different orders differ by at most n eps); momentum
+    # matches the state to f32 round-off of the gravity impulse.
+    assert abs(sim.mass_residual()) <= 1e-10 * led.initial_mass
+    g = np.linalg.norm(led.gravity_impulse)
+    assert g > 0.0
+    assert np.linalg.norm(sim.momentum_residual()) < 1e-5 * g
+    # Every particle is still inside the padded grid (a stencil base in range:
+    # -dx/2 <= x < L + dx/2; the walls act on the grid, so a particle may sit
+    # slightly past a face), every field is finite, and the plastic strain never
+    # goes below zero.
+    assert np.isfinite(out.pos).all() and np.isfinite(out.vel).all() and np.isfinite(out.kirchhoff).all()
+    assert (out.pos >= -0.5 * dx).all() and (out.pos < domain[0] + 0.5 * dx).all()
+    print("lowest particle", out.pos.min(axis=0), flush=True)
+    assert (out.plastic >= 0.0).all()
+    # Gravity has started the slump: the column moves down, not up.
+    assert float(np.mean(out.vel[:, 2])) < 0.0
+    print(f"{steps} steps, t = {h.time:.3e} s, max speed {np.abs(out.vel).max():.3e} m/s", flush=True)
