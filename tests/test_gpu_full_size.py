@@ -60,3 +60,66 @@ def test_million_particle_column_invariants(engine):
     # Gravity has started the slump: the column moves down, not up.
     assert float(np.mean(out.vel[:, 2])) < 0.0
     print(f"{steps} steps, t = {h.time:.3e} s, max speed {np.abs(out.vel).max():.3e} m/s", flush=True)
+
+
+def check_invariants(sim, st, steps, dx, domain):
+    h, led = sim.history(), sim.ledger()
+    assert h.step_count == steps and h.rejected_steps == 0
+    out = sim.state().by_id()
+    assert np.array_equal(out.id, np.sort(np.asarray(st.id)))
+    assert abs(sim.mass_residual()) <= 1e-10 * led.initial_mass
+    g = np.linalg.norm(led.gravity_impulse)
+    assert g > 0.0
+    assert np.linalg.norm(sim.momentum_residual()) < 1e-5 * g
+    assert np.isfinite(out.pos).all() and np.isfinite(out.vel).all() and np.isfinite(out.kirchhoff).all()
+    for a in range(3):
+        assert (out.pos[:, a] >= -0.5 * dx).all() and (out.pos[:, a] < domain[a] + 0.5 * dx).all()
+    return out
+
+
+def test_six_million_particle_dam_break_invariants(engine):
+    # Water dam break, 64 x 128 x 96 cells x 8 particles (6,291,456) on a 128^3
+    # grid, dx 1 mm, Tait gamma 7, free-slip walls: 40 substeps.
+    E = engine
+    dx = 0.001
+    domain = [128 * dx] * 3
+    mat = E.liquid(1000.0, 3.6e5, 0.2, 1e-3, tait_gamma=7.0)
+    cfg = E.config(wall_friction=0.0, cfl=0.4, max_dt=1e-4, min_dt=1e-12, face_friction=[0.0] * 6)
+    st = E.place_particles("dam_break", mat, dx, domain, [64 * dx, 128 * dx, 96 * dx], seed=1, jitter=0.5)
+    assert len(st) == 6_291_456
+    sim = E.Simulation(mat, cfg, domain, dx, st)
+    print("dam break placed and uploaded", flush=True)
+    assert sim.advance_steps(40) == 40
+    out = check_invariants(sim, st, 40, dx, domain)
+    # The volume ratio J of every particle stays finite and positive.
+    assert np.isfinite(out.volume_ratio).all() and (out.volume_ratio > 0.0).all()
+    print(f"dam break: t = {sim.history().time:.3e} s", flush=True)
+
+
+def test_pellet_bed_full_size_invariants(engine):
+    # 1.5M-particle paste bed (96 x 96 x 20 cells x 8) on a 96^3 grid, dx 1.5 mm,
+    # with 512 rigid pellets above it: 40 substeps.
+    E = engine
+    rng = np.random.default_rng(2)
+    dx = 0.0015
+    domain = [96 * dx] * 3
+    bulk, shear = 2e5, 5e4
+    young = 9.0 * bulk * shear / (3.0 * bulk + shear)
+    poisson = (3.0 * bulk - 2.0 * shear) / (2.0 * (3.0 * bulk + shear))
+    mat = E.paste(1200.0, young, poisson, float(rng.uniform(20.0, 200.0)), 5.0, 0.5, shear_units=True)
+    cfg = E.config(wall_friction=0.4, cfl=0.4, max_dt=1e-4, min_dt=1e-12,
+                   face_friction=[0.0, 0.0, 0.0, 0.0, math.inf, 0.0])
+    st = E.place_particles("slump", mat, dx, domain, [96 * dx, 96 * dx, 20 * dx], seed=2, jitter=0.5)
+    pellets = []
+    for _ in range(512):
+        pos = [rng.uniform(0.006, domain[0] - 0.006), rng.uniform(0.006, domain[1] - 0.006),
+               rng.uniform(20 * dx + 0.006, domain[2] - 0.006)]
+        q = rng.normal(size=4)
+        pellets.append(E.new_pellet(0.0015, 0.008, 650.0, pos, orientation=list(q / np.linalg.norm(q))))
+    assert len(st) == 1_474_560
+    sim = E.Simulation(mat, cfg, domain, dx, st, pellets=pellets)
+    print("pellet bed placed and uploaded", flush=True)
+    assert sim.advance_steps(40) == 40
+    out = check_invariants(sim, st, 40, dx, domain)
+    assert (out.plastic >= 0.0).all()
+    print(f"pellet bed: t = {sim.history().time:.3e} s", flush=True)
