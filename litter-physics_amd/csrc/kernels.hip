@@ -1070,7 +1070,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             }
             float rf = 0.0f;
             const bool fits = in_tile && tile_fits(tsc, P, pm, v, C, tau, pv0, rf);
-            const float rfw = wave_max(fits ? rf : 0.0f);
+            const float rfw = wave_fmax_uniform(fits ? rf : 0.0f);  // DPP: no LDS permutes next to the deposit
             const bool coarse = rfw > 0.0f && rfw < wave_rows * (12.0f * 0x1p-20f) * P.dx;
             if (fits && !coarse)
                 scatter_lds<SCAP>(ss, so, sd, P, s2, tsc, pm, v, C, tau, pv0, rot);
@@ -1140,15 +1140,16 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         }
     }
     // Per-work-item partials of the step reductions.
-    speed = wave_max(speed);
-    disp = wave_max(disp);
-    diss = wave_sum(diss);
+    // Maxima and the dissipation sum by DPP (VALU), not LDS permutes.
+    speed = wave_fmax_uniform(speed);
+    disp = wave_fmax_uniform(disp);
+    diss = wave_sum_last(diss);
     fail = wave_min_u64(fail);
     escaped = wave_sum_i(escaped);
+    if (lane == 63) sdiss[wave] = diss;
     if (lane == 0) {
         sred[wave][0] = speed;
         sred[wave][1] = disp;
-        sdiss[wave] = diss;
         sfail[wave] = fail;
         sesc[wave] = escaped;
 #pragma unroll

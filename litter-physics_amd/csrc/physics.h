@@ -815,7 +815,7 @@ __device__ inline void wave_append(bool pred, int value, int* list, int* count) 
     int leader = __ffsll((long long)mask) - 1;
     int base = 0;
     if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader, 64);
+    base = __builtin_amdgcn_readlane(base, leader);  // leader is uniform: no LDS permute
     if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
