@@ -733,7 +733,7 @@ template <int CHUNK, int GCAP, int SCAP, bool MLP>
 __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     constexpr int R = CHUNK / kBlock;
     __shared__ float4 sg[GCAP];
-    __shared__ unsigned long long ss[7 * SCAP];
+    __shared__ __align__(16) unsigned long long ss[7 * SCAP];
     __shared__ float sbound[4][3];
     __shared__ int snbox[4][6];
     __shared__ float snrow[4];
@@ -831,7 +831,8 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         }
     }
     if (stiled)
-        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0ull;
+        // The whole tile region, 16 bytes per store: no division by the tile size per word.
+        for (int q = tid; q < 7 * SCAP / 2; q += kBlock) reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         bnd[i] = wave_max(bnd[i]);
@@ -1487,7 +1488,7 @@ __global__ __launch_bounds__(kBlock) void refkey_kernel(BatchDev B) {
 template <int CHUNK, int SCAP>
 __global__ __launch_bounds__(kBlock, 2) void setup_deposit_kernel(BatchDev B, int inst, int gset) {
     constexpr int R = CHUNK / kBlock;
-    __shared__ unsigned long long ss[7 * SCAP];
+    __shared__ __align__(16) unsigned long long ss[7 * SCAP];
     __shared__ int sbox[4][6];
     __shared__ float sbound[4][5];
     __shared__ int sgeo[7];
@@ -1578,7 +1579,8 @@ __global__ __launch_bounds__(kBlock, 2) void setup_deposit_kernel(BatchDev B, in
     const bool stiled = sgeo[6] != 0;
     const int ssz = sd[0] * sd[1] * sd[2];
     if (stiled)
-        for (int q = tid; q < 7 * ssz; q += kBlock) ss[(q / ssz) * SCAP + (q % ssz)] = 0ull;
+        // The whole tile region, 16 bytes per store: no division by the tile size per word.
+        for (int q = tid; q < 7 * SCAP / 2; q += kBlock) reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
     TileScale tsc;
     {
         float b[5];
