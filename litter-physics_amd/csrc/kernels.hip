@@ -819,9 +819,10 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     float bnd[3] = {0.0f, 0.0f, 0.0f};
     if (t.tiled) {
         int gsz = t.gd[0] * t.gd[1] * t.gd[2];
+        const float ig2 = 1.0f / (float)t.gd[2], ig1 = 1.0f / (float)t.gd[1];
         for (int q = tid; q < gsz; q += kBlock) {
-            int k = q % t.gd[2], rest = q / t.gd[2];
-            int j = rest % t.gd[1], i = rest / t.gd[1];
+            const int rest = div_small(q, ig2), k = q - rest * t.gd[2];
+            const int i = div_small(rest, ig1), j = rest - i * t.gd[1];
             int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
             const float4 g = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
             sg[q] = g;
@@ -835,7 +836,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         for (int q = tid; q < 7 * SCAP / 2; q += kBlock) reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        bnd[i] = wave_max(bnd[i]);
+        bnd[i] = wave_fmax_uniform(bnd[i]);
         if (lane == 0) sbound[wave][i] = bnd[i];
     }
     const float wave_rows = pre->rows[wv];  // this wavefront's committed stress bound
@@ -1124,11 +1125,12 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     if (stiled) {
         const double um = 1.0 / tsc.sm, ux = 1.0 / tsc.sp[0], uy = 1.0 / tsc.sp[1], uz = 1.0 / tsc.sp[2],
                      uf = 1.0 / tsc.sf;
+        const float is2 = 1.0f / (float)sd[2], is1 = 1.0f / (float)sd[1];
         for (int q = tid; q < ssz; q += kBlock) {
             const long long qm = (long long)ss[q];
             if (qm == 0) continue;
-            int k = q % sd[2], rest = q / sd[2];
-            int j = rest % sd[1], i = rest / sd[1];
+            const int rest = div_small(q, is2), k = q - rest * sd[2];
+            const int i = div_small(rest, is1), j = rest - i * sd[1];
             int ni = node_index(P, so[0] + i, so[1] + j, so[2] + k);
             atomicAdd(out.m + ni, (float)((double)qm * um));
             atomicAdd(out.px + ni, (float)((double)(long long)ss[1 * SCAP + q] * ux));
@@ -1617,11 +1619,12 @@ __global__ __launch_bounds__(kBlock, 2) void setup_deposit_kernel(BatchDev B, in
     if (stiled) {
         const double um = 1.0 / tsc.sm, ux = 1.0 / tsc.sp[0], uy = 1.0 / tsc.sp[1], uz = 1.0 / tsc.sp[2],
                      uf = 1.0 / tsc.sf;
+        const float is2 = 1.0f / (float)sd[2], is1 = 1.0f / (float)sd[1];
         for (int q = tid; q < ssz; q += kBlock) {
             const long long qm = (long long)ss[q];
             if (qm == 0) continue;
-            int k = q % sd[2], rest = q / sd[2];
-            int j = rest % sd[1], i = rest / sd[1];
+            const int rest = div_small(q, is2), k = q - rest * sd[2];
+            const int i = div_small(rest, is1), j = rest - i * sd[1];
             int ni = node_index(P, so[0] + i, so[1] + j, so[2] + k);
             atomicAdd(out.m + ni, (float)((double)qm * um));
             atomicAdd(out.px + ni, (float)((double)(long long)ss[1 * SCAP + q] * ux));

@@ -590,6 +590,12 @@ __device__ inline float wave_fmax_uniform(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// q / d for 0 <= q < 2^20 and a divisor d > 0 given as inv = 1/d in f32: the
+// quotient plus 1/2d lies at least 1/2d from an integer, far beyond the few-ulp
+// error of the product, so the conversion truncates to the exact quotient.
+// Three VALU instructions instead of an integer division sequence.
+__device__ inline int div_small(int q, float inv) { return (int)(((float)q + 0.5f) * inv); }
+
 // V0 max_a sum_b |tau_ab| of a symmetric stress (xx, yy, zz, xy, xz, yz): the
 // per-particle quantity the deposit's stress scale bounds.
 __device__ inline float stress_rows(float v0, const float* tau) {
