@@ -243,7 +243,7 @@ void mpm_enable_profiling(mpm_engine* engine, int on);
  * config, grid, particles, pellets, adaptive dt, accept/reject state and
  * ledgers; one launch sequence steps all of them.  An engine from mpm_create
  * is a batch of one, and the single-instance calls above act on instance 0. */
-int mpm_batch_create(size_t n_instances, const mpm_material* materials, const mpm_config* configs,
+int mpm_batch_create(size_t n_instances, const mpm_material* materials, const mpm_config* solver_configs,
                      const mpm_layout* layouts, const size_t* capacities, const size_t* max_pellets,
                      mpm_engine** out);
 size_t mpm_instance_count(const mpm_engine* engine);
@@ -282,7 +282,7 @@ int mpm_autotune(mpm_engine* engine, int steps, mpm_launch* chosen);
 
 /* ---- Learned surrogate of the Herschel-Bulkley return map.  No counterpart
  * in the reference; the exact return (solve_herschel_bulkley_return,
- * constitutive.rs:280-362) stays the fallback and the judge of every accepted
+ * constitutive.rs:280-362) stays the fallback and the acceptance test of every
  * prediction.  The network is 12 -> 64 -> 64 -> 64 -> 7 with SiLU, bf16
  * weights on MFMA with f32 accumulation; its f32 master parameters are, in
  * order, W1[12][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][7] b4[7]

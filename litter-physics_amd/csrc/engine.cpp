@@ -1478,19 +1478,19 @@ int mpm_fixture_place(const mpm_fixture* f, mpm_particles* out) {
     return MPM_OK;
 }
 
-int mpm_batch_create(size_t n_instances, const mpm_material* materials, const mpm_config* configs,
+int mpm_batch_create(size_t n_instances, const mpm_material* materials, const mpm_config* solver_configs,
                      const mpm_layout* layouts, const size_t* capacities, const size_t* max_pellets,
                      mpm_engine** out) {
     if (!out) return MPM_ERR_INVALID;
     *out = nullptr;
-    if (n_instances == 0 || n_instances > 65536 || !materials || !configs || !layouts) return MPM_ERR_INVALID;
+    if (n_instances == 0 || n_instances > 65536 || !materials || !solver_configs || !layouts) return MPM_ERR_INVALID;
     std::vector<Instance> inst(n_instances);
     for (size_t i = 0; i < n_instances; ++i) {
         if (max_pellets && max_pellets[i] > 4096) return MPM_ERR_CAPACITY;
         int rc = grid_cells(layouts[i], inst[i].cells);
         if (rc) return rc;
         inst[i].mat = materials[i];
-        inst[i].cfg = configs[i];
+        inst[i].cfg = solver_configs[i];
         inst[i].dx = layouts[i].spacing;
         inst[i].fill_params();
     }

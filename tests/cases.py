@@ -81,8 +81,8 @@ def prestressed_block(friction, velocity):
     return f, sim
 
 
-def slump_cfg0(particles_scale=1.0, seed=0, jitter=0.5, cfl=0.4, max_dt=1e-4, friction=0.4):
-    """HB paste column 40x40x60 mm on a 32^3 grid, dx = 2.5 mm.
+def paste_column_32(particles_scale=1.0, seed=0, jitter=0.5, cfl=0.4, max_dt=1e-4, friction=0.4):
+    """Herschel-Bulkley paste column 40x40x60 mm on a 32^3 grid, dx = 2.5 mm (49,152 particles).
 
     rho 1200, K 2e5, G 5e4 (E = 9KG/(3K+G), nu = (3K-2G)/(2(3K+G))), shear
     yield 50 Pa, K_s 5 Pa s^n, n 0.5; jitter +-0.25 dx = 0.5 of the sub-cell.
@@ -98,9 +98,9 @@ def slump_cfg0(particles_scale=1.0, seed=0, jitter=0.5, cfl=0.4, max_dt=1e-4, fr
     return O.fixture("slump", m, dx, [32 * dx] * 3, size, cfg, seed=seed, jitter=jitter)
 
 
-def dam_break_cfg1(cells=(128, 128, 128), block=(64, 128, 96), dx=0.001, seed=1, gamma=1.0, jitter=0.5,
+def dam_break_block(cells=(128, 128, 128), block=(64, 128, 96), dx=0.001, seed=1, gamma=1.0, jitter=0.5,
                    cfl=0.4, max_dt=1e-4, friction=0.0):
-    """BASELINE config 1 (scalable): water block in the x=0,y=0 corner, Tait EOS."""
+    """Water dam break: a block in the x=0, y=0 corner of the grid, Tait EOS (any size)."""
     # K = 2e5 from the wire pair E = 3.6e5, nu = 0.2 (constitutive.rs:211-223).
     m = O.liquid(1000.0, 3.6e5, 0.2, 1e-3, tait_gamma=gamma)
     cfg = O.config(wall_friction=friction, cfl=cfl, max_dt=max_dt, min_dt=1e-12)

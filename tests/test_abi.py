@@ -28,9 +28,9 @@ def test_library_exports_every_declared_symbol(engine):
 
 def test_fixture_placement_is_bit_identical_to_oracle(engine):
     # fixtures.rs:272-319 with SplitMix64: both builders must agree bit for bit.
-    for spec, kind in ((cases.slump_cfg0(), "slump"), (cases.water_column([0.02] * 3, 0.01, 0.002, 5e-5),
+    for spec, kind in ((cases.paste_column_32(), "slump"), (cases.water_column([0.02] * 3, 0.01, 0.002, 5e-5),
                                                        "hydrostatic"),
-                       (cases.dam_break_cfg1(cells=(16, 16, 16), block=(8, 16, 12), dx=0.001), "dam_break")):
+                       (cases.dam_break_block(cells=(16, 16, 16), block=(8, 16, 12), dx=0.001), "dam_break")):
         ref = O.Sim.from_fixture(spec).state()
         mat = engine.Material(*[getattr(spec.material, n) for n, _ in spec.material._fields_])
         got = engine.place_particles(kind, mat, spec.grid_spacing, list(spec.domain), list(spec.initial_size),

@@ -62,7 +62,7 @@ def test_batch_instances_match_their_oracles(engine):
     # Three unlike instances: a paste slump (16^3 grid), a gamma = 7 dam break
     # (24 x 16 x 16 grid) and a pellet pressed into a paste patch.
     oracles = [build_oracle(small_slump()),
-               build_oracle(cases.dam_break_cfg1(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=7.0)),
+               build_oracle(cases.dam_break_block(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=7.0)),
                pressed_patch()]
     specs = [spec_from_oracle(engine, o) for o in oracles]
     batch = engine.Batch(specs)
@@ -100,7 +100,7 @@ def test_batch_of_one_equals_simulation(engine):
 def test_launch_shapes_match_oracle(engine, use_graph):
     # Captured-graph and stream stepping, with one or two blocks per work item,
     # both follow the oracle.
-    spec = cases.slump_cfg0(max_dt=1e-3)
+    spec = cases.paste_column_32(max_dt=1e-3)
     osim = build_oracle(spec)
     s = spec_from_oracle(engine, osim)
     sim = engine.Simulation(s["material"], s["cfg"], s["domain"], s["spacing"], s["state"])

@@ -45,8 +45,8 @@ def engine_twin(E, osim):
 
 def spec_of(name):
     if name == "slump":
-        return cases.slump_cfg0()
-    return cases.dam_break_cfg1(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=7.0)
+        return cases.paste_column_32()
+    return cases.dam_break_block(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=7.0)
 
 
 @pytest.mark.parametrize("edit", ["stored", "zero", "scaled_down", "outliers"])

@@ -148,9 +148,9 @@ def fixed_dt_run(E, spec, steps, dt=None):
     return osim, gsim, dt
 
 
-def test_slump_cfg0_parity_100_substeps(engine):
+def test_paste_column_parity_100_substeps(engine):
     # 49,152-particle Herschel-Bulkley column (32^3 grid, dx 2.5 mm), 100 substeps.
-    spec = cases.slump_cfg0()
+    spec = cases.paste_column_32()
     osim, gsim, dt = fixed_dt_run(engine, spec, 100)
     g, r, o = compare_states(gsim, osim)
     lo, lg = osim.ledger(), gsim.ledger()
@@ -188,7 +188,7 @@ def test_hydrostatic_column_matches_golden(engine):
 
 
 def test_adaptive_advance_matches_oracle(engine):
-    spec = cases.slump_cfg0(max_dt=1e-3)
+    spec = cases.paste_column_32(max_dt=1e-3)
     osim = build_oracle(spec)
     gsim = engine_from_oracle(engine, osim)
     osim.advance_to(4e-3)
@@ -200,7 +200,7 @@ def test_adaptive_advance_matches_oracle(engine):
 
 
 def test_rejected_step_leaves_state_untouched(engine):
-    spec = cases.slump_cfg0()
+    spec = cases.paste_column_32()
     osim = build_oracle(spec)
     gsim = engine_from_oracle(engine, osim)
     gsim.advance_to(1e-3)
@@ -274,7 +274,7 @@ def test_thin_grid_wall_traction_on_both_faces(engine, depth):
 
 @pytest.mark.parametrize("gamma", [1.0, 7.0])
 def test_dam_break_liquid_parity(engine, gamma):
-    spec = cases.dam_break_cfg1(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=gamma)
+    spec = cases.dam_break_block(cells=(24, 16, 16), block=(12, 16, 12), dx=0.001, gamma=gamma)
     osim, gsim, dt = fixed_dt_run(engine, spec, 100)
     g, r, o = compare_states(gsim, osim)
     assert np.abs(g.volume_ratio - r.volume_ratio[o]).max() < 1e-5
@@ -311,7 +311,7 @@ def test_coupled_patch_pellet_parity(engine):
 
 
 def test_sorted_order_follows_reference_key(engine):
-    spec = cases.slump_cfg0()
+    spec = cases.paste_column_32()
     osim = build_oracle(spec)
     gsim = engine_from_oracle(engine, osim)
     dt = 0.5 * osim.stability_limit()

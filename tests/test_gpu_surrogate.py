@@ -115,7 +115,7 @@ def weights_for_deployment(engine, trained):
 
 def test_substep_with_surrogate_matches_oracle(engine, trained):
     net, committed = weights_for_deployment(engine, trained)
-    spec = cases.slump_cfg0()
+    spec = cases.paste_column_32()
     osim = build_oracle(spec)
     gsim = engine_from_oracle(engine, osim)
     gsim.set_surrogate(net, tolerance=1e-4, polish=2)
@@ -138,7 +138,7 @@ def test_substep_with_surrogate_matches_oracle(engine, trained):
 def test_exact_path_unchanged_without_surrogate(engine, trained):
     # Switching the surrogate off restores the exact return (no offers counted).
     net, _ = weights_for_deployment(engine, trained)
-    spec = cases.slump_cfg0()
+    spec = cases.paste_column_32()
     osim = build_oracle(spec)
     gsim = engine_from_oracle(engine, osim)
     gsim.set_surrogate(net)

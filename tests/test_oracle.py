@@ -291,7 +291,7 @@ def test_pellet_geometry_and_signed_distance():
     phi, normal = np.zeros(1), np.zeros(3)
     O.lib().oracle_pellet_signed_distance(p, O.ptr(np.array([0.0, 0.0, 0.004])), O.ptr(phi), O.ptr(normal))
     assert abs(phi[0] - 0.001) < 1e-12 and np.linalg.norm(normal - [0, 0, 1]) < 1e-12
-    # BASELINE config 2 pellet: r = 1.5 mm, L = 8 mm gives five spheres.
+    # The pellets of the 1.5M-particle bed: r = 1.5 mm, L = 8 mm gives five spheres.
     q = O.new_pellet(0.0015, 0.008, 650.0, [0.0, 0.0, 0.0])
     assert q.sphere_count == 5
 
