@@ -718,6 +718,38 @@ __global__ void pellet_kernel(BatchDev B) {
 
 // ------------------------------------------------------------ fused G2P2G
 
+// Appends the near-wall particles of one work item to the instance's list:
+// bit r of a thread's `bits` flags particle first + r * kBlock + threadIdx.x.
+// One atomic on the list counter per workgroup (wavefront appends on a single
+// counter serialise); the entries follow round, wavefront and lane order.
+// Every thread of the workgroup must call it.
+template <int R>
+__device__ inline void block_append_rounds(unsigned bits, int first, int* list, int* count) {
+    __shared__ int swc[kBlock / 64], sbase;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int wcount = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) wcount += __popcll(__ballot((bits >> r) & 1u));
+    if (lane == 0) swc[wave] = wcount;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int w = 0; w < kBlock / 64; ++w) total += swc[w];
+        sbase = total ? atomicAdd(count, total) : 0;
+    }
+    __syncthreads();
+    int at = sbase;
+    for (int w = 0; w < wave; ++w) at += swc[w];
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool on = (bits >> r) & 1u;
+        const unsigned long long mask = __ballot(on);
+        if (on) list[at + __popcll(mask & below)] = first + r * kBlock + (int)threadIdx.x;
+        at += __popcll(mask);
+    }
+}
+
 // grid_to_particle + constitutive update (solver.rs:895-977) fused with the
 // next step's particle-to-grid deposit into the other accumulator set.  A
 // workgroup owns one work item: the particles of one or two z-adjacent cell
@@ -883,6 +915,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     // The trial slab's set-up record, gathered round by round as uniform values.
     int nlo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MIN, INT_MIN, INT_MIN};
     float nrows = 0.0f;
+    unsigned nwbits = 0u;  // near-wall flags of the thread's updated particles, bit r: round r
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int p = chunk0 + r * kBlock + tid;
@@ -1079,7 +1112,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             else if (dep)
                 scatter_particle(P, out, s2, pm, v[0], v[1], v[2], C, tau, pv0);
         }
-        wave_append(nw, p, nw_list, nw_count);
+        nwbits |= nw ? (1u << r) : 0u;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             nlo[a] = min(nlo[a], wave_min_uniform(dep ? s2.base[a] : INT_MAX));
@@ -1120,6 +1153,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         atomicAdd(B.mlp_stats + 1, mlp_taken);
     }
     __syncthreads();
+    block_append_rounds<R>(nwbits, chunk0, nw_list, nw_count);
     // Flush the tile: one global atomic per touched node and field, each sum
     // converted exactly to f64 (|sum| < 2^40) and scaled back by 2^-k.
     if (stiled) {
@@ -1653,8 +1687,11 @@ __global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
     float m = 0.0f, v0 = 0.0f, rows = 0.0f;
-    // Every lane of a wavefront takes part in the aggregated append.
-    for (int base = item.y; base < item.y + item.z; base += kBlock) {
+    // Near-wall flags of the thread's particles (bit r: round r), appended once
+    // per workgroup below: one atomic on the instance's counter per work item.
+    unsigned nwbits = 0u;
+    constexpr int R = kChunk / kBlock;
+    for (int base = item.y, r = 0; base < item.y + item.z; base += kBlock, ++r) {
         const int p = base + threadIdx.x;
         bool nw = false;
         if (p < item.y + item.z) {
@@ -1673,8 +1710,9 @@ __global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
             v0 = fmaxf(v0, pv0);
             rows = fmaxf(rows, stress_rows(pv0, tau));
         }
-        wave_append(nw, p, list, count);
+        nwbits |= nw ? (1u << r) : 0u;
     }
+    block_append_rounds<R>(nwbits, item.y, list, count);
     for (int a = 0; a < 3; ++a) {
         lo[a] = wave_min_uniform(lo[a]);
         hi[a] = wave_max_uniform(hi[a]);
