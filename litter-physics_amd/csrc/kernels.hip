@@ -1148,11 +1148,21 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             }
         }
     }
-    if (MLP && lane == 0 && mlp_tried) {
-        atomicAdd(B.mlp_stats, mlp_tried);
-        atomicAdd(B.mlp_stats + 1, mlp_taken);
+    // Surrogate counters: summed over the workgroup first, so one pair of
+    // atomics per workgroup reaches the two global counters.
+    __shared__ unsigned long long smlp[4][2];
+    if (MLP && lane == 0) {
+        smlp[wave][0] = mlp_tried;
+        smlp[wave][1] = mlp_taken;
     }
     __syncthreads();
+    if (MLP && tid == 0) {
+        const unsigned long long tried = smlp[0][0] + smlp[1][0] + smlp[2][0] + smlp[3][0];
+        if (tried) {
+            atomicAdd(B.mlp_stats, tried);
+            atomicAdd(B.mlp_stats + 1, smlp[0][1] + smlp[1][1] + smlp[2][1] + smlp[3][1]);
+        }
+    }
     block_append_rounds<R>(nwbits, chunk0, nw_list, nw_count);
     // Flush the tile: one global atomic per touched node and field, each sum
     // converted exactly to f64 (|sum| < 2^40) and scaled back by 2^-k.
