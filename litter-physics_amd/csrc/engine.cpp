@@ -570,7 +570,10 @@ struct mpm_engine {
         if (int rc = dalloc(&run_count, cap, false)) return rc;
         if (int rc = dalloc(&d_nruns, 1)) return rc;
         if (int rc = dalloc(&mlp_stats, 2)) return rc;
-        sort_bytes = std::max(sort_temp_bytes((int)cap, end_bit), rle_temp_bytes((int)cap));
+        size_t sort_need = 0, rle_need = 0;
+        HIP_OK(sort_temp_bytes((int)cap, end_bit, &sort_need));
+        HIP_OK(rle_temp_bytes((int)cap, &rle_need));
+        sort_bytes = std::max(sort_need, rle_need);
         HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
         // Timing-only events: without the system-scope fence, recording one does
         // not write back and invalidate the caches between the kernels it separates
@@ -657,11 +660,14 @@ struct mpm_engine {
         if (int rc = flush_refkeys()) return rc;
         BatchDev B = dev();
         launch_sort_keys(stream, B, cell_bits, keys[0], order[0]);
-        launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)cap, end_bit);
+        HIP_OK(hipGetLastError());
+        HIP_OK(launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)cap,
+                                 end_bit));
         launch_permute(stream, B, order[1], shared[sh], shared[sh ^ 1], ids[sh], ids[sh ^ 1], refkey[sh],
                        refkey[sh ^ 1]);
-        launch_block_runs(stream, sort_temp, sort_bytes, (int)cap, keys[1], keys[0], run_unique, run_count, d_nruns);
         HIP_OK(hipGetLastError());
+        HIP_OK(launch_block_runs(stream, sort_temp, sort_bytes, (int)cap, keys[1], keys[0], run_unique, run_count,
+                                 d_nruns));
         int nruns = 0;
         HIP_OK(hipMemcpyAsync(&nruns, d_nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));

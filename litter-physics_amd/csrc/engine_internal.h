@@ -304,13 +304,15 @@ void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t*
 void launch_permute(hipStream_t s, const BatchDev& B, const uint32_t* order, const float* shared_src,
                     float* shared_dst, const uint32_t* id_src, uint32_t* id_dst, const uint64_t* ref_src,
                     uint64_t* ref_dst);
-size_t sort_temp_bytes(int n, int end_bit);
-size_t rle_temp_bytes(int n);
-void launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
-                       const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit);
+// Temporary storage of the radix sort and of the run-length encoding (the
+// size is written to *bytes); the sort and encode calls return hipcub's status.
+hipError_t sort_temp_bytes(int n, int end_bit, size_t* bytes);
+hipError_t rle_temp_bytes(int n, size_t* bytes);
+hipError_t launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in,
+                             uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit);
 // Runs of equal (instance, cell block) in the sorted keys.
-void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
-                       uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns);
+hipError_t launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
+                             uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns);
 void launch_paste_batch(hipStream_t s, const Params& P, float dt, int n, const float* h_in, float* h_out,
                         float* tau, float* inc, float* dd, int* codes);
 void launch_liquid_batch(hipStream_t s, const Params& P, float dt, int n, const float* jm1, const float* c,

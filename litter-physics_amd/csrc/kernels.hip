@@ -89,17 +89,21 @@ __device__ inline int instance_of(const BatchDev& B, int p) {
     return lo;
 }
 
-// Reference order key (clamped cell index, z fastest, then id; solver.rs:501-517).
-__device__ inline uint64_t ref_cell(const Params& P, float x, float y, float z) {
-    const float pos[3] = {x, y, z};
+// Reference order key (clamped cell index, z fastest, then id; solver.rs:501-517),
+// in f64 as the reference computes it: the f32 position promoted exactly and
+// divided by the f64 spacing, so a particle within an ulp of a cell face floors
+// to the same cell as in the reference (an f32 quotient by the rounded f32
+// spacing need not).  Only evaluated at the 32-step cadence.
+__device__ inline uint64_t ref_cell(const InstParams& ip, float x, float y, float z) {
+    const double pos[3] = {(double)x, (double)y, (double)z};
     uint64_t c[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float sc = floorf(pos[a] / P.dx);
-        sc = isnan(sc) ? 0.0f : fmaxf(sc, 0.0f);
-        c[a] = sc >= (float)P.cells[a] ? (uint64_t)P.cells[a] : (uint64_t)sc;
+        double sc = floor(pos[a] / ip.dx);
+        sc = isnan(sc) ? 0.0 : fmax(sc, 0.0);
+        c[a] = sc >= (double)ip.P.cells[a] ? (uint64_t)ip.P.cells[a] : (uint64_t)sc;
     }
-    return (c[0] * (uint64_t)(P.cells[1] + 1) + c[1]) * (uint64_t)(P.cells[2] + 1) + c[2];
+    return (c[0] * (uint64_t)(ip.P.cells[1] + 1) + c[1]) * (uint64_t)(ip.P.cells[2] + 1) + c[2];
 }
 
 // ------------------------------------------------------------ control
@@ -844,7 +848,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             py[r] = src[PY * cap + p];
             pz[r] = src[PZ * cap + p];
             // The committed step hit the 32-step cadence: order keys of the committed positions.
-            if (is.refsort) B.refkey[p] = (ref_cell(P, px[r], py[r], pz[r]) << 32) | (uint64_t)B.ids[p];
+            if (is.refsort) B.refkey[p] = (ref_cell(ip, px[r], py[r], pz[r]) << 32) | (uint64_t)B.ids[p];
         }
     }
     // Gather tile, and the largest node speed per axis for the momentum scales.
@@ -1515,11 +1519,11 @@ __global__ __launch_bounds__(kBlock) void refkey_kernel(BatchDev B) {
     const int4 item = B.work[blockIdx.x];
     const InstState& s = B.is[item.x];
     if (!s.refsort) return;
-    const Params& P = B.ip[item.x].P;
+    const InstParams& ip = B.ip[item.x];
     const float* slab = B.slab[s.cur];
     const size_t cap = B.cap;
     for (int p = item.y + threadIdx.x; p < item.y + item.z; p += kBlock)
-        B.refkey[p] = (ref_cell(P, slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]) << 32) |
+        B.refkey[p] = (ref_cell(ip, slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]) << 32) |
                       (uint64_t)B.ids[p];
 }
 
@@ -1884,31 +1888,31 @@ void launch_permute(hipStream_t s, const BatchDev& B, const uint32_t* order, con
                                                         ref_dst);
 }
 
-size_t sort_temp_bytes(int n, int end_bit) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
-    return bytes;
+hipError_t sort_temp_bytes(int n, int end_bit, size_t* bytes) {
+    *bytes = 0;
+    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
 }
 
-size_t rle_temp_bytes(int n) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                (uint32_t*)nullptr, (int*)nullptr, n);
-    return bytes;
+hipError_t rle_temp_bytes(int n, size_t* bytes) {
+    *bytes = 0;
+    return hipcub::DeviceRunLengthEncode::Encode(nullptr, *bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                 (uint32_t*)nullptr, (int*)nullptr, n);
 }
 
-void launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
-                       const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit) {
-    (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, end_bit,
-                                             s);
+hipError_t launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in,
+                             uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit) {
+    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, end_bit,
+                                              s);
 }
 
-void launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
-                       uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns) {
-    if (n <= 0) return;
+hipError_t launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
+                             uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns) {
+    if (n <= 0) return hipSuccess;
     block_ids_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, sorted_keys, ids_tmp);
-    (void)hipcub::DeviceRunLengthEncode::Encode(temp, temp_bytes, ids_tmp, unique, counts, nruns, n, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipcub::DeviceRunLengthEncode::Encode(temp, temp_bytes, ids_tmp, unique, counts, nruns, n, s);
 }
 
 void launch_paste_batch(hipStream_t s, const Params& P, float dt, int n, const float* h_in, float* h_out, float* tau,

@@ -1802,6 +1802,69 @@ size_t oracle_sim_probe_grid(const oracle_sim* s, double dt, int with_traction, 
     return copy.grid.active.size();
 }
 
+size_t oracle_sim_probe_update(const oracle_sim* s, double dt, double* mass, double* before, double* after,
+                               double* budget, int* code) {
+    oracle_sim copy(*s);
+    sweep_outflow(copy);
+    deposit_particles(copy, dt);
+    wall_traction(copy, dt);
+    std::vector<NodeTrial> trials;
+    int rc = grid_update(copy, dt, trials);
+    if (code) *code = rc;
+    const Grid& g = copy.grid;
+    size_t n = g.layout.count();
+    for (size_t i = 0; i < n; ++i) {
+        if (mass) mass[i] = g.mass[i];
+        for (int a = 0; a < 3; ++a) {
+            if (before) before[3 * i + a] = 0.0;
+            if (after) after[3 * i + a] = 0.0;
+            if (budget) budget[3 * i + a] = g.traction[i][a];
+        }
+    }
+    if (rc) return 0;
+    for (size_t q = 0; q < g.active.size(); ++q) {
+        size_t i = g.active[q];
+        double m = g.mass[i];
+        if (!(m > 0.0)) continue;
+        for (int a = 0; a < 3; ++a) {
+            // solver.rs:833: the velocity the walls act on.
+            if (before) before[3 * i + a] = g.momentum[i][a] / m + copy.config.gravity[a] * dt;
+            if (after) after[3 * i + a] = trials[q].velocity[a];
+        }
+    }
+    return g.active.size();
+}
+
+int oracle_couple_nodes(const double* domain, double spacing, size_t n_nodes, const size_t* index, const double* mass,
+                        double* velocity, const oracle_pellet* pellets, int n_pellets, double friction,
+                        double* results) {
+    Layout l;
+    std::string err;
+    if (!make_layout(domain, spacing, SIZE_MAX, l, err)) return ORACLE_ERR_INVALID;
+    Grid g(l);
+    for (size_t q = 0; q < n_nodes; ++q) {
+        if (index[q] >= l.count()) return ORACLE_ERR_INVALID;
+        g.active.push_back(index[q]);
+        g.mass[index[q]] = mass[q];
+        g.momentum[index[q]] = v3(velocity[3 * q], velocity[3 * q + 1], velocity[3 * q + 2]);
+    }
+    for (int k = 0; k < n_pellets; ++k) {
+        Pellet p;
+        from_c(pellets + k, p);
+        CouplingResult r = couple_grid(g, p, friction);
+        double* o = results + 8 * k;
+        for (int a = 0; a < 3; ++a) {
+            o[a] = r.impulse[a];
+            o[3 + a] = r.angular_impulse[a];
+        }
+        o[6] = r.grid_energy;
+        o[7] = (double)r.constrained;
+    }
+    for (size_t q = 0; q < n_nodes; ++q)
+        for (int a = 0; a < 3; ++a) velocity[3 * q + a] = g.momentum[index[q]][a];
+    return 0;
+}
+
 double oracle_sim_mass_residual(const oracle_sim* s) {
     return s->particles.total_mass() + s->ledger.outflow_mass - s->ledger.initial_mass;
 }

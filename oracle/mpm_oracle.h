@@ -230,6 +230,21 @@ size_t oracle_sim_probe_transfer(const oracle_sim* s, double dt, double* tractio
 /* Dense node arrays of the scratch grid after a P2G probe (sized nodes^3). */
 size_t oracle_sim_probe_grid(const oracle_sim* s, double dt, int with_traction, double* mass,
                              double* momentum);
+/* P2G, wall traction and grid_update (solver.rs:632-892) on a scratch copy.
+ * Dense node arrays (sized nodes^3; vectors 3 per node): mass, the velocity
+ * the walls act on (p/m + g dt, solver.rs:833), the velocity after the walls
+ * and the traction budget.  *code is 0 or the grid failure code.  Returns the
+ * number of active nodes (0 on failure). */
+size_t oracle_sim_probe_update(const oracle_sim* s, double dt, double* mass, double* before, double* after,
+                               double* budget, int* code);
+/* couple_grid (rigid.rs:385-438) of n_pellets pellets, one after another in
+ * index order (one pellet is the reference's step), on a grid given by its
+ * active nodes in order: dense node index, mass and velocity (3 per node,
+ * updated in place).  results: 8 per pellet -- impulse (3), angular impulse
+ * about the pellet centre (3), grid energy, constrained node count. */
+int oracle_couple_nodes(const double* domain, double spacing, size_t n_nodes, const size_t* index, const double* mass,
+                        double* velocity, const oracle_pellet* pellets, int n_pellets, double friction,
+                        double* results);
 /* Diagnostics (solver.rs:1002-1099). */
 double oracle_sim_mass_residual(const oracle_sim* s);
 void oracle_sim_momentum_residual(const oracle_sim* s, double* out);

@@ -155,6 +155,9 @@ def lib():
             "oracle_sim_sort": (None, [C.c_void_p]),
             "oracle_sim_probe_transfer": (C.c_size_t, [C.c_void_p, C.c_double, dp, dp, dp]),
             "oracle_sim_probe_grid": (C.c_size_t, [C.c_void_p, C.c_double, C.c_int, dp, dp]),
+            "oracle_sim_probe_update": (C.c_size_t, [C.c_void_p, C.c_double, dp, dp, dp, dp, C.POINTER(C.c_int)]),
+            "oracle_couple_nodes": (C.c_int, [dp, C.c_double, C.c_size_t, C.POINTER(C.c_size_t), dp, dp,
+                                              C.POINTER(Pellet), C.c_int, C.c_double, dp]),
             "oracle_sim_mass_residual": (C.c_double, [C.c_void_p]),
             "oracle_sim_momentum_residual": (None, [C.c_void_p, dp]),
             "oracle_sim_kinetic_energy": (C.c_double, [C.c_void_p]),
@@ -380,6 +383,18 @@ class Sim:
         lib().oracle_sim_probe_grid(self.h, dt, int(with_traction), ptr(m), ptr(p))
         return m, p
 
+    def probe_update(self, dt):
+        """P2G, traction and grid_update on a scratch copy: dense node mass, velocity
+        before and after the walls, traction budget (solver.rs:632-892)."""
+        sp, cells = self.layout()
+        n = (cells[0] + 3) * (cells[1] + 3) * (cells[2] + 3)
+        m, vb, va, tb = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        code = C.c_int()
+        lib().oracle_sim_probe_update(self.h, dt, ptr(m), ptr(vb), ptr(va), ptr(tb), C.byref(code))
+        if code.value != 0:
+            raise RuntimeError(f"grid update failed: {FAILURES.get(code.value, code.value)}")
+        return m, vb, va, tb
+
     def mass_residual(self):
         return lib().oracle_sim_mass_residual(self.h)
 
@@ -465,6 +480,24 @@ def new_pellet(radius, length, density, position, orientation=(0.0, 0.0, 0.0, 1.
     if rc != 0:
         raise ValueError("invalid pellet geometry")
     return p
+
+
+def couple_nodes(domain, spacing, index, mass, velocity, pellets, friction):
+    """couple_grid (rigid.rs:385-438) of `pellets` in order on the given active
+    nodes (dense index, mass, velocity).  Returns the new velocities and, per
+    pellet, impulse (3), angular impulse about its centre (3), grid energy and
+    constrained node count."""
+    dom = np.asarray(domain, dtype=np.float64)
+    idx = np.ascontiguousarray(index, dtype=np.uintp)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1, 3).copy()
+    arr = (Pellet * max(len(pellets), 1))(*pellets)
+    res = np.zeros((max(len(pellets), 1), 8))
+    rc = lib().oracle_couple_nodes(ptr(dom), spacing, len(idx), idx.ctypes.data_as(C.POINTER(C.c_size_t)), ptr(m),
+                                   ptr(v), arr, len(pellets), friction, ptr(res))
+    if rc != 0:
+        raise ValueError("invalid coupling layout")
+    return v, res[:len(pellets)]
 
 
 def set_threads(n):

@@ -310,17 +310,148 @@ def test_coupled_patch_pellet_parity(engine):
     assert np.linalg.norm(gsim.momentum_residual()) < 1e-5 * np.linalg.norm(lo.gravity_impulse)
 
 
-def test_sorted_order_follows_reference_key(engine):
+def _pellet_box(p):
+    i, j, k, w = list(p.orientation)
+    axis = np.array([1 - 2 * (j * j + k * k), 2 * (i * j + w * k), 2 * (i * k - w * j)])
+    half = max(abs(o) for o in p.sphere_offsets[:p.sphere_count])
+    reach = np.abs(axis) * half + p.radius
+    c = np.array(p.position[:])
+    return c - reach, c + reach
+
+
+def test_overlapping_pellets_parity(engine):
+    # Four pellets (r = 1 dx, L = 8 mm, rho 650: the 512-pellet bed's geometry at
+    # this grid) pressed into the top of the 49,152-particle paste column with
+    # overlapping boxes, so grid nodes see two or three pellets one after another
+    # in index order (couple_grid per pellet, rigid.rs:385-438).  100 substeps.
     spec = cases.paste_column_32()
     osim = build_oracle(spec)
+    r = 0.0025
+    top = 0.06 + r - 0.001
+    layout = [((0.0385, 0.0400, top), None, (0.02, 0.0, -0.2)),
+              ((0.0410, 0.0405, top + 0.0005), ((0, 0, 1), 0.5 * math.pi), (-0.02, 0.01, -0.15)),
+              ((0.0400, 0.0370, top), ((0, 0, 1), 0.25 * math.pi), (0.0, 0.03, -0.2)),
+              ((0.0405, 0.0430, top + 0.001), ((1, 0, 1), 0.6), (0.01, -0.03, -0.25))]
+    mom = np.zeros(3)
+    for pos, rot, vel in layout:
+        q = (0.0, 0.0, 0.0, 1.0)
+        if rot is not None:
+            ax = np.asarray(rot[0], float) / np.linalg.norm(rot[0])
+            q = tuple(list(ax * math.sin(0.5 * rot[1])) + [math.cos(0.5 * rot[1])])
+        p = O.new_pellet(r, 0.008, 650.0, list(pos), orientation=q)
+        p.velocity[:] = list(vel)
+        osim.add_pellet(p)
+        mom += p.mass * np.array(vel)
+    pels = osim.pellets()
+    boxes = [_pellet_box(p) for p in pels]
+    overlaps = sum(bool(np.all(boxes[a][0] <= boxes[b][1]) and np.all(boxes[b][0] <= boxes[a][1]))
+                   for a in range(4) for b in range(a + 1, 4))
+    assert overlaps >= 3, overlaps
+    led = osim.ledger()
+    for a in range(3):
+        led.initial_momentum[a] += mom[a]
+    osim.set_ledger(led)
     gsim = engine_from_oracle(engine, osim)
-    dt = 0.5 * osim.stability_limit()
-    cfg = osim.config()
-    cfg.max_dt = dt
-    osim.set_config(cfg)
-    gsim.set_config(as_engine(engine, cfg, engine.Config))
-    osim.advance_to(32 * dt)
-    gsim.advance_to(32 * dt)
-    assert gsim.history().step_count == 32
-    same = np.mean(gsim.state().id == osim.state().id)
-    assert same > 0.99, same
+    dt = min(osim.stability_limit(), spec.config.max_dt)
+    for k in range(100):
+        assert osim.try_step(dt) == 0, k
+        rep = gsim.try_step(dt)
+        assert rep.accepted == 1, (k, rep.failure)
+    compare_states(gsim, osim)
+    po, pg = osim.pellets(), gsim.pellets()
+    for a, b in zip(pg, po):
+        assert rel(np.array(a.position[:]), np.array(b.position[:])) < 1e-5
+        vo = np.array(b.velocity[:])
+        assert np.abs(np.array(a.velocity[:]) - vo).max() < 1e-3 * np.abs(vo).max()
+        lo_ = np.array(b.angular_momentum[:])
+        assert np.abs(np.array(a.angular_momentum[:]) - lo_).max() <= 1e-3 * max(np.abs(lo_).max(), 1e-12)
+    lo, lg = osim.ledger(), gsim.ledger()
+    for f in ("coupling_impulse", "coupling_angular_impulse"):
+        ci = np.array(getattr(lo, f)[:])
+        assert np.abs(ci).max() > 0.0
+        assert np.abs(np.array(getattr(lg, f)[:]) - ci).max() < 2e-3 * np.abs(ci).max(), f
+    assert abs(lg.coupling_grid_energy - lo.coupling_grid_energy) < 2e-3 * abs(lo.coupling_grid_energy)
+    assert np.linalg.norm(gsim.momentum_residual()) < 1e-5 * np.linalg.norm(lo.gravity_impulse)
+
+
+@pytest.mark.parametrize("depth", [2, 3])
+def test_thin_grid_traction_out_of_equilibrium(engine, depth):
+    # The thin block of test_thin_grid_wall_traction_on_both_faces out of
+    # equilibrium: a lateral compressive stress that its deformation does not
+    # hold makes it spread across both faces at a few cm/s, under the faces'
+    # wall-image traction (solver.rs:725-819) and a low Coulomb coefficient whose
+    # traction budget (solver.rs:866-886) acts on the sliding.  The velocities are
+    # their own scale here, so the plain 1e-4 bar applies.
+    m = O.paste(1000.0, 1e5, 0.2, 1e6, 1.0, 1.0)
+    cfg = O.config(wall_friction=0.05, cfl=0.3, max_dt=5e-5, min_dt=1e-9)
+    dx = 0.002
+    spec = O.fixture("slump", m, dx, [10 * dx, 10 * dx, depth * dx], [6 * dx, 6 * dx, depth * dx], cfg, seed=7)
+    osim = build_oracle(spec)
+    st = osim.state()
+    st.kirchhoff[:] = 0.0
+    for a in range(3):
+        st.kirchhoff[:, a, a] = -2000.0
+    st.deformation[:, 2, 2] = np.exp(-2000.0 / 1e5)
+    osim.set_state(st)
+    gsim = engine_from_oracle(engine, osim)
+    dt = min(osim.stability_limit(), cfg.max_dt)
+    for k in range(30):
+        assert osim.try_step(dt) == 0, k
+        rep = gsim.try_step(dt)
+        assert rep.accepted == 1, (k, rep.failure)
+    g, r, o = compare_states(gsim, osim)
+    assert np.abs(r.vel).max() > 1e-2
+    lo, lg = osim.ledger(), gsim.ledger()
+    for f in ("wall_normal_traction_energy", "wall_friction_dissipation"):
+        ref = getattr(lo, f)
+        assert ref != 0.0
+        assert abs(getattr(lg, f) - ref) <= 1e-3 * abs(ref), f
+
+
+def _f32_key_cells(pos, dx):
+    """The cell of each coordinate as an f32 quotient by the f32 spacing (the
+    rounding the f64 key avoids)."""
+    return np.floor(pos.astype(np.float32) / np.float32(dx)).astype(np.int64)
+
+
+def test_reference_order_is_the_reference_key_exactly(engine):
+    # solver.rs:501-517: every 32 accepted steps the particles are ordered by
+    # (clamped cell, id), the cell from the f64 position over the f64 spacing.
+    # A paste column at rest without gravity keeps every position bit for bit
+    # through a step, so the engine and the oracle sort the same positions; a
+    # third of the coordinates sit on, or one f32 ulp either side of, a cell
+    # face, where a quotient rounded to f32 can floor into the neighbouring cell.
+    spec = cases.paste_column_32()
+    spec.config.gravity[:] = [0.0, 0.0, 0.0]
+    osim = build_oracle(spec)
+    st = osim.state()
+    n = len(st.id)
+    dx = spec.grid_spacing
+    rng = np.random.default_rng(11)
+    pos = st.pos.astype(np.float32)
+    face = (np.round(pos.astype(np.float64) / dx) * dx).astype(np.float32)
+    way = rng.integers(0, 3, pos.shape)
+    moved = np.where(way == 0, face, np.where(way == 1, np.nextafter(face, np.float32(np.inf)),
+                                              np.nextafter(face, np.float32(-np.inf))))
+    pos = np.where(rng.random(pos.shape) < 0.3, moved, pos).astype(np.float64)
+    assert np.any(_f32_key_cells(pos, dx) != np.floor(pos / dx).astype(np.int64))
+    perm = rng.permutation(n)
+    for f in ("id", "pos", "vel", "affine", "deformation", "volume_ratio", "kirchhoff", "plastic", "mass", "volume0"):
+        setattr(st, f, np.ascontiguousarray(getattr(st, f)[perm]))
+    st.pos = np.ascontiguousarray(pos[perm])
+    osim.set_state(st)
+    h = osim.history()
+    h.step_count = 31
+    osim.set_history(h)
+    gsim = engine_from_oracle(engine, osim)
+    gh = engine.History()
+    for name, _ in h._fields_:
+        setattr(gh, name, getattr(h, name))
+    gsim.set_state(gsim.state(), history=gh)
+    osim.advance_to(1e-5)
+    gsim.advance_to(1e-5)
+    assert osim.history().step_count == 32 and gsim.history().step_count == 32
+    g, r = gsim.state(), osim.state()
+    o = np.argsort(st.id)
+    assert np.array_equal(g.by_id().pos, st.pos[o]), "positions changed during the step at rest"
+    assert np.array_equal(g.id, r.id)
