@@ -29,13 +29,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s at 1M particles; surrogate vs implicit HB at ≤1e-3 stress err"
-# Algorithmic HBM bytes per particle-substep of the fused grid-to-particle-to-grid
-# kernel, f32 state read once and written once: paste reads x3 v3 C9 F9 tau6 eps1
-# m1 V0 1 = 33 words and writes x3 v3 C9 F9 tau6 eps1 = 31 words (256 B); liquid
-# reads x3 v3 C9 J1 tau6 m1 V0 1 = 24 words and writes 22 words (184 B).
+# Algorithmic HBM bytes per particle-substep of the substep's two particle
+# kernels together, the f32 state read once and written once: paste reads x3 v3
+# C9 F9 tau6 eps1 m1 V0 1 = 33 words and writes x3 v3 C9 F9 tau6 eps1 = 31 words
+# (256 B); liquid reads x3 v3 C9 J1 tau6 m1 V0 1 = 24 words and writes 22 (184 B).
 PASTE_BYTES_PER_PARTICLE = 4 * (33 + 31)
 LIQUID_BYTES_PER_PARTICLE = 4 * (24 + 22)
+# Per kernel: the deposit reads x3 v3 C9 tau6 m1 V0 1 = 23 words; G2P reads
+# x3 F9 eps1 m1 V0 1 = 15 words (liquid: x3 J1 m1 V0 1 = 6) and writes the state.
+P2G_BYTES_PER_PARTICLE = 4 * 23
+G2P_PASTE_BYTES_PER_PARTICLE = 4 * (15 + 31)
+G2P_LIQUID_BYTES_PER_PARTICLE = 4 * (6 + 22)
 PEAK_HBM = 8.0e12
+# The reference's largest published CPU rate of this path (slump
+# refinement at dx 2.5 mm, 4096 particles x 10000 substeps on an Apple M4, f64).
+REFERENCE_PUBLISHED = 5.16e6
+REFERENCE_BASIS = ("reference's largest published CPU rate: slump refinement dx 2.5 mm, 4096 particles x 10000 "
+                   "substeps, Apple M4, f64 (5.16e6 particle-substeps/s)")
 
 
 def log(msg):
@@ -67,19 +77,26 @@ def workload():
     return E, mat, cfg, dx, domain, size
 
 
-def cpu_baseline(size, dx, domain, seconds=10.0, max_steps=20):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """All cores this process may use: its CPU affinity, within the OpenMP
+    thread count the host sets for it (its CPU share), if any."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(avail, int(omp)) if omp.isdigit() and int(omp) > 0 else avail
+
+
+def _oracle_steps(O, sim, threads, seconds, max_steps):
     O.set_threads(threads)
-    bulk, shear = 2e5, 5e4
-    young = 9.0 * bulk * shear / (3.0 * bulk + shear)
-    poisson = (3.0 * bulk - 2.0 * shear) / (2.0 * (3.0 * bulk + shear))
-    m = O.paste(1200.0, young, poisson, 50.0, 5.0, 0.5, shear_units=True)
-    cfg = O.config(wall_friction=0.0, cfl=0.4, max_dt=1e-4, min_dt=1e-12, face_friction=FACES)
-    spec = O.fixture("slump", m, dx, domain, size, cfg, seed=0, jitter=0.5)
-    sim = O.Sim.from_fixture(spec)
-    n = len(sim)
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps:
@@ -89,17 +106,39 @@ def cpu_baseline(size, dx, domain, seconds=10.0, max_steps=20):
             raise RuntimeError(f"oracle step rejected ({rc})")
         steps += 1
         el = time.perf_counter() - t0
-        log(f"cpu baseline: step {steps} at {el:.1f} s")
+        log(f"cpu baseline ({threads} threads): step {steps} at {el:.1f} s")
         if el >= seconds:
             break
-    el = time.perf_counter() - t0
+    return steps, time.perf_counter() - t0
+
+
+def cpu_baseline(size, dx, domain, seconds=10.0, max_steps=20):
+    """The f64 oracle on the same workload, timed on this host's cores: all the
+    cores the process may use, then one thread, each on a bounded sample of
+    steps from the same placed state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    bulk, shear = 2e5, 5e4
+    young = 9.0 * bulk * shear / (3.0 * bulk + shear)
+    poisson = (3.0 * bulk - 2.0 * shear) / (2.0 * (3.0 * bulk + shear))
+    m = O.paste(1200.0, young, poisson, 50.0, 5.0, 0.5, shear_units=True)
+    cfg = O.config(wall_friction=0.0, cfl=0.4, max_dt=1e-4, min_dt=1e-12, face_friction=FACES)
+    spec = O.fixture("slump", m, dx, domain, size, cfg, seed=0, jitter=0.5)
+    base = O.Sim.from_fixture(spec)
+    n = len(base)
+    threads = cpu_threads()
+    steps, el = _oracle_steps(O, base.clone(), threads, seconds, max_steps)
+    steps1, el1 = _oracle_steps(O, base.clone(), 1, seconds, max_steps)
     return {"value": n * steps / el, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} particles x {steps} substeps of the same workload, f64 oracle, {threads} threads "
-                      f"(serial P2G and traction, threaded grid update and G2P), {el:.1f} s"}
+            "sample": f"{n} particles x {steps} substeps of the same workload from the same placed state, f64 "
+                      f"oracle, {threads} threads (serial P2G and traction, threaded grid update and G2P), {el:.1f} s",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "single_thread": {"value": n * steps1 / el1, "unit": "particle-substeps/s", "cores": 1,
+                              "sample": f"{n} particles x {steps1} substeps, 1 thread, {el1:.1f} s"}}
 
 
 def read_traffic():
-    path = os.path.join(ROOT, "profiles", "g2p2g_paste_1M_pmc.md")
+    path = os.path.join(ROOT, "profiles", "substep_paste_1M_pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -108,42 +147,50 @@ def read_traffic():
     return None
 
 
-# Every PROFILE_EVERY-th fused G2P2G launch of a timed region carries HIP events.
-# Each event record leaves the GPU idle for about 4.5 us between two kernels, so
-# timing all launches would slow the step it measures by about 4 %.
-PROFILE_EVERY = 8
-
-
 def timed_profiled(sim, steps):
-    """Time `steps` accepted steps; HIP events around every eighth fused G2P2G
-    launch of this timed region, on the engine's stream, give the kernel's average
-    duration.  Event recording needs stream launches, so the timed region runs in
-    stream mode."""
+    """Time `steps` accepted steps, with the G2P and deposit kernels of every
+    trial step of this timed region timed on the device (each kernel stamps its
+    first workgroup start and last workgroup end; no idle time is added).
+    Timing needs stream launches, so the timed region runs in stream mode."""
     sim.set_launch(use_graph=0)
-    sim.enable_profiling(True, every=PROFILE_EVERY)
+    sim.enable_profiling(True, every=1)
     t0 = time.perf_counter()
     accepted = sim.advance_steps(steps)
     elapsed = time.perf_counter() - t0
-    avg_ms, launches = sim.g2p2g_profile()
+    g2p_ms, p2g_ms, launches = sim.kernel_profile()
     sim.enable_profiling(False)
-    log(f"timed {accepted} steps in {elapsed:.3f} s; g2p2g avg {avg_ms:.4f} ms over {launches} timed launches")
-    return accepted, elapsed, avg_ms, launches
+    log(f"timed {accepted} steps in {elapsed:.3f} s; g2p avg {g2p_ms:.4f} ms, p2g avg {p2g_ms:.4f} ms over "
+        f"{launches} timed steps")
+    return accepted, elapsed, (g2p_ms, p2g_ms), launches
 
 
-def roofline(bytes_per_particle, n, avg_ms, launches, traffic=None):
+def roofline(bytes_per_particle, g2p_bytes, n, ms, launches, traffic=None):
+    """HBM roofline of the substep's particle kernels (deposit + G2P): the
+    algorithmic bytes per particle-substep over their summed average time, and
+    each kernel on its own bytes."""
+    g2p_ms, p2g_ms = ms
+    both = g2p_ms + p2g_ms
     per_launch = bytes_per_particle * n
-    achieved = per_launch / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
+    achieved = per_launch / (both * 1e-3) if both > 0 else 0.0
+
+    def one(b, t):
+        return {"ms": t, "algorithmic_bytes_per_launch": b * n,
+                "achieved_gbs": b * n / (t * 1e-3) / 1e9 if t > 0 else 0.0}
+
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM, "traffic": traffic, "kernel": "g2p2g_kernel", "kernel_ms": avg_ms,
-            "kernel_launches_timed": launches,
-            "algorithmic_bytes_per_launch": per_launch}
+            "frac": achieved / PEAK_HBM, "traffic": traffic, "kernel": "p2g_kernel + g2p_kernel",
+            "kernel_ms": both, "kernel_launches_timed": launches, "algorithmic_bytes_per_launch": per_launch,
+            "kernels": {"g2p_kernel": one(g2p_bytes, g2p_ms), "p2g_kernel": one(P2G_BYTES_PER_PARTICLE, p2g_ms)}}
 
 
-def line(metric, value, unit, steps, warmup, elapsed, dtype, config, rl=None, cpu=None):
-    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
-            "ms_per_step": 1e3 * elapsed / max(steps, 1), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config, "roofline": rl,
-            "cpu_baseline": cpu}
+def line(metric, value, unit, steps, warmup, elapsed, dtype, config, rl=None, cpu=None, vs=None):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": 1e3 * elapsed / max(steps, 1), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": vs, "dtype": dtype, "data": "synthetic", "config": config, "roofline": rl,
+           "cpu_baseline": cpu}
+    if vs is not None:
+        out["vs_baseline_basis"] = REFERENCE_BASIS
+    return out
 
 
 # ---------------------------------------------------------------- headline
@@ -161,15 +208,17 @@ def run_headline(args):
     if not args.no_autotune:
         chosen = sim.autotune(steps=16)
         log(f"autotuned launch shape: {chosen.as_dict()}")
-    accepted, elapsed, avg_ms, launches = timed_profiled(sim, args.steps)
+    accepted, elapsed, ms, launches = timed_profiled(sim, args.steps)
     h = sim.history()
     cpu = None if args.no_cpu_baseline else cpu_baseline(size, dx, domain, seconds=args.cpu_seconds)
-    return line(METRIC, n * accepted / elapsed, "particle-substeps/s", accepted, args.warmup, elapsed, "f32",
+    value = n * accepted / elapsed
+    return line(METRIC, value, "particle-substeps/s", accepted, args.warmup, elapsed, "f32",
                 {"workload": "HB paste column 100^3 particles (1,000,000) on a 96^3 grid, dx 2.5 mm, plain "
                              "implicit HB, CFL 0.4, sticky floor, free-slip walls, seed 0",
                  "particles": n, "grid_cells": [96, 96, 96], "rejected_steps": h.rejected_steps,
                  "dt_last_s": h.max_dt_used, "launch": sim.launch().as_dict()},
-                roofline(PASTE_BYTES_PER_PARTICLE, n, avg_ms, launches, read_traffic()), cpu)
+                roofline(PASTE_BYTES_PER_PARTICLE, G2P_PASTE_BYTES_PER_PARTICLE, n, ms, launches, read_traffic()),
+                cpu, vs=value / REFERENCE_PUBLISHED)
 
 
 # ---------------------------------------------------------------- liquid (dam break)
@@ -189,7 +238,7 @@ def run_liquid(args):
     sim.advance_steps(args.warmup)
     if not args.no_autotune:
         log(f"autotuned launch shape: {sim.autotune(steps=8).as_dict()}")
-    accepted, elapsed, avg_ms, launches = timed_profiled(sim, args.steps)
+    accepted, elapsed, ms, launches = timed_profiled(sim, args.steps)
     h = sim.history()
     return line("particle-substeps/s, dam break 6.3M particles, Tait gamma 7", n * accepted / elapsed,
                 "particle-substeps/s", accepted, args.warmup, elapsed, "f32",
@@ -197,7 +246,7 @@ def run_liquid(args):
                              "Tait gamma 7, K 2e5 Pa, viscosity 1e-3 Pa s, free-slip walls, CFL 0.4, seed 1",
                  "particles": n, "grid_cells": cells, "rejected_steps": h.rejected_steps,
                  "dt_last_s": h.max_dt_used, "launch": sim.launch().as_dict()},
-                roofline(LIQUID_BYTES_PER_PARTICLE, n, avg_ms, launches))
+                roofline(LIQUID_BYTES_PER_PARTICLE, G2P_LIQUID_BYTES_PER_PARTICLE, n, ms, launches))
 
 
 # ---------------------------------------------------------------- paste bed (+ pellets)
@@ -234,7 +283,7 @@ def run_pellets(args):
     sim.advance_steps(args.warmup)
     if not args.no_autotune:
         log(f"autotuned launch shape: {sim.autotune(steps=8).as_dict()}")
-    accepted, elapsed, avg_ms, launches = timed_profiled(sim, args.steps)
+    accepted, elapsed, ms, launches = timed_profiled(sim, args.steps)
     h = sim.history()
     led = sim.ledger()
     return line("particle-substeps/s, paste bed 1.5M particles + 512 coupled pellets", n * accepted / elapsed,
@@ -245,7 +294,7 @@ def run_pellets(args):
                  "particles": n, "pellets": len(pellets), "grid_cells": cells, "rejected_steps": h.rejected_steps,
                  "dt_last_s": h.max_dt_used, "coupling_impulse": list(led.coupling_impulse),
                  "launch": sim.launch().as_dict()},
-                roofline(PASTE_BYTES_PER_PARTICLE, n, avg_ms, launches))
+                roofline(PASTE_BYTES_PER_PARTICLE, G2P_PASTE_BYTES_PER_PARTICLE, n, ms, launches))
 
 
 # ---------------------------------------------------------------- surrogate (cfg 2 bed, no pellets)
@@ -291,8 +340,9 @@ def run_surrogate(args):
         sim.advance_steps(args.warmup)
         if name == "surrogate":
             learned.set_surrogate(net, tolerance=1e-4, polish=2)  # count the timed run only
-        accepted, elapsed, avg_ms, launches = timed_profiled(sim, args.steps)
-        out[name] = {"value": n * accepted / elapsed, "ms_per_step": 1e3 * elapsed / accepted, "g2p2g_ms": avg_ms}
+        accepted, elapsed, ms, launches = timed_profiled(sim, args.steps)
+        out[name] = {"value": n * accepted / elapsed, "ms_per_step": 1e3 * elapsed / accepted, "g2p_ms": ms[0],
+                     "p2g_ms": ms[1]}
     considered, accepted = learned.surrogate_stats()
     fallback = 1.0 - accepted / max(considered, 1)
     return line("particle-substeps/s, 1.5M paste bed with the learned HB return (exact fallback)",

@@ -229,12 +229,14 @@ int mpm_liquid_update_batch(const mpm_material* material, double dt, size_t n,
                             double* kirchhoff, int* codes);
 
 /* Per-kernel timing since profiling was last enabled, for the benchmark: the
- * average device time of the fused grid-to-particle-to-grid kernel in ms and
- * the number of launches timed.  mpm_enable_profiling with on = N > 0 times
- * every N-th launch with a pair of events around it (each event leaves the GPU
- * idle for a few microseconds between two kernels); on = 0 stops.  Either call
- * resets the sums.  Timing turns the graph replay off. */
-int mpm_profile_g2p2g(mpm_engine* engine, double* avg_ms, uint64_t* launches);
+ * average device time in ms of the grid-to-particle kernel (G2P with the
+ * constitutive update) and of the particle-to-grid deposit kernel, and the
+ * number of steps timed.  mpm_enable_profiling with on = N > 0 times the
+ * kernels of every N-th trial step; on = 0 stops.  The kernels stamp their
+ * first workgroup start and last workgroup end on the device clock, which
+ * costs two atomics per workgroup and no idle time between kernels.  Either
+ * call resets the sums.  Timing turns the graph replay off. */
+int mpm_profile_kernels(mpm_engine* engine, double* g2p_avg_ms, double* p2g_avg_ms, uint64_t* launches);
 void mpm_enable_profiling(mpm_engine* engine, int on);
 
 /* ---- Batched instances.  No counterpart in the reference, whose sweeps run
@@ -270,8 +272,9 @@ int mpm_batch_advance(mpm_engine* engine, const double* targets, uint64_t max_st
 typedef struct {
     int steps_per_sync;  /* trial steps enqueued between host synchronisations (1..256) */
     int use_graph;       /* 1: replay the steps as one captured HIP graph */
-    int merge_blocks;    /* cell blocks per G2P2G work item along z (1 or 2) */
+    int merge_blocks;    /* cell blocks per work item along z (1 or 2) */
     int traction_wgs;    /* workgroups of the wall-traction kernel */
+    int g2p_occupancy;   /* G2P workgroups per CU its register budget allows (2 or 3) */
 } mpm_launch;
 int mpm_get_launch(const mpm_engine* engine, mpm_launch* out);
 int mpm_set_launch(mpm_engine* engine, const mpm_launch* launch);

@@ -101,7 +101,7 @@ class StepReport(C.Structure):
 class Launch(C.Structure):
     """mpm_launch: launch shape of the stepping loop (the autotuner's search space)."""
     _fields_ = [("steps_per_sync", C.c_int), ("use_graph", C.c_int), ("merge_blocks", C.c_int),
-                ("traction_wgs", C.c_int)]
+                ("traction_wgs", C.c_int), ("g2p_occupancy", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -190,7 +190,7 @@ def lib():
                                              C.POINTER(C.c_int)]),
         "mpm_liquid_update_batch": (C.c_int, [C.POINTER(Material), C.c_double, C.c_size_t, dp, dp, dp, dp,
                                               C.POINTER(C.c_int)]),
-        "mpm_profile_g2p2g": (C.c_int, [C.c_void_p, dp, C.POINTER(C.c_uint64)]),
+        "mpm_profile_kernels": (C.c_int, [C.c_void_p, dp, dp, C.POINTER(C.c_uint64)]),
         "mpm_enable_profiling": (None, [C.c_void_p, C.c_int]),
         "mpm_batch_create": (C.c_int, [C.c_size_t, C.POINTER(Material), C.POINTER(Config), C.POINTER(Layout),
                                        C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]),
@@ -249,7 +249,7 @@ EXPORTED = [
     "mpm_set_state", "mpm_get_state", "mpm_particle_count", "mpm_pellet_count", "mpm_set_config",
     "mpm_try_step", "mpm_advance_while", "mpm_advance_to", "mpm_stability_limit", "mpm_mass_residual",
     "mpm_momentum_residual", "mpm_energies", "mpm_paste_update_batch", "mpm_liquid_update_batch",
-    "mpm_profile_g2p2g", "mpm_enable_profiling", "mpm_batch_create", "mpm_instance_count",
+    "mpm_profile_kernels", "mpm_enable_profiling", "mpm_batch_create", "mpm_instance_count",
     "mpm_instance_particle_count", "mpm_instance_set_state", "mpm_instance_get_state", "mpm_batch_advance",
     "mpm_get_launch", "mpm_set_launch", "mpm_autotune", "mpm_mlp_param_count", "mpm_mlp_create", "mpm_mlp_destroy",
     "mpm_mlp_last_error", "mpm_mlp_get_params", "mpm_mlp_set_params", "mpm_mlp_forward", "mpm_mlp_loss_grad",
@@ -558,14 +558,16 @@ class Simulation:
         return self.mechanical_energy() - led.initial_mechanical_energy - led.ledgered_energy()
 
     def enable_profiling(self, on=True, every=1):
-        """Time every `every`-th fused G2P2G launch with HIP events (on=False stops)."""
+        """Time the G2P and deposit kernels of every `every`-th trial step (on=False stops)."""
         lib().mpm_enable_profiling(self.h, max(int(every), 1) if on else 0)
 
-    def g2p2g_profile(self):
-        ms = C.c_double()
+    def kernel_profile(self):
+        """(G2P average ms, deposit average ms, steps timed) since profiling was enabled."""
+        g = C.c_double()
+        d = C.c_double()
         n = C.c_uint64()
-        self._check(lib().mpm_profile_g2p2g(self.h, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
+        self._check(lib().mpm_profile_kernels(self.h, C.byref(g), C.byref(d), C.byref(n)))
+        return g.value, d.value, n.value
 
 
 def _f32(a, cols=None):
@@ -792,14 +794,16 @@ class Batch:
         return _take_returns(self, capacity)
 
     def enable_profiling(self, on=True, every=1):
-        """Time every `every`-th fused G2P2G launch with HIP events (on=False stops)."""
+        """Time the G2P and deposit kernels of every `every`-th trial step (on=False stops)."""
         lib().mpm_enable_profiling(self.h, max(int(every), 1) if on else 0)
 
-    def g2p2g_profile(self):
-        ms = C.c_double()
+    def kernel_profile(self):
+        """(G2P average ms, deposit average ms, steps timed) since profiling was enabled."""
+        g = C.c_double()
+        d = C.c_double()
         n = C.c_uint64()
-        self._check(lib().mpm_profile_g2p2g(self.h, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
+        self._check(lib().mpm_profile_kernels(self.h, C.byref(g), C.byref(d), C.byref(n)))
+        return g.value, d.value, n.value
 
 
 def paste_update_batch(material, dt, trial):

@@ -273,7 +273,7 @@ struct Instance {
     int cb[3] = {0, 0, 0};  // 4^3-cell blocks per axis (cells 0..cells inclusive after clamping)
     int pel_cap = 0;
     InstParams ip{};
-    bool grid_valid = false;
+    bool grid_clear = false;  // both accumulator sets of the instance are zero
     ExactConstants exact;
 
     void fill_params() {
@@ -326,11 +326,11 @@ struct mpm_engine {
     int* nw_list[2] = {nullptr, nullptr};
     int sh = 0;
     // grid
-    float* gmem[2] = {nullptr, nullptr};
+    long long* gmem[2] = {nullptr, nullptr};
     int* gflag[2] = {nullptr, nullptr};
     float* gv = nullptr;
-    float* trS = nullptr;
-    float* trT = nullptr;
+    long long* trS = nullptr;
+    long long* trT = nullptr;
     int* blk_inst = nullptr;
     double* gpart = nullptr;
     int* gbad = nullptr;
@@ -355,7 +355,7 @@ struct mpm_engine {
     int nwork = 0;
     size_t work_cap = 0;
     float* pmax = nullptr;
-    double* pdiss = nullptr;
+    Fix128* pdiss = nullptr;
     unsigned long long* pfail = nullptr;
     int* pesc = nullptr;
     // sort
@@ -367,20 +367,23 @@ struct mpm_engine {
     void* sort_temp = nullptr;
     size_t sort_bytes = 0;
     int cell_bits = 0, end_bit = 0;
-    bool dirty = true;  // work items and near-wall lists must be rebuilt
+    bool dirty = true;  // work items, set-up records and near-wall lists must be rebuilt
     // launch
-    mpm_launch launch{32, 0, 2, 2048};
+    mpm_launch launch{32, 0, 2, 2048, 2};
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     bool graph_stale = true;
-    hipEvent_t ev0[kMaxStepsPerSync] = {}, ev1[kMaxStepsPerSync] = {};
+    // Kernel timing: four device-clock stamps per enqueued step (BatchDev::ktime).
+    unsigned long long* d_ktime = nullptr;
+    unsigned long long* h_ktime = nullptr;  // pinned
+    double clock_khz = 0.0;
     bool profiling = false;
-    int profile_every = 1;        // time every profile_every-th G2P2G launch
-    uint64_t profile_seq = 0;     // G2P2G launches since profiling was enabled
-    bool profiled[kMaxStepsPerSync] = {};  // the launches of the enqueued steps that carry events
-    double g2p2g_ms = 0.0;
-    uint64_t g2p2g_launches = 0;
+    int profile_every = 1;        // time every profile_every-th step
+    uint64_t profile_seq = 0;     // steps since profiling was enabled
+    bool profiled[kMaxStepsPerSync] = {};  // the enqueued steps whose kernels carry events
+    double g2p_ms = 0.0, p2g_ms = 0.0;
+    uint64_t profiled_launches = 0;
     // return-map surrogate and the log of exact returns
     const mpm_mlp* mlp = nullptr;
     float mlp_tol = 1e-4f;
@@ -421,6 +424,7 @@ struct mpm_engine {
             B.nw_list[b] = nw_list[b];
         }
         B.sh = sh;
+        B.g2p_occupancy = launch.g2p_occupancy;
         B.gv = gv;
         B.trS = trS;
         B.trT = trT;
@@ -506,7 +510,7 @@ struct mpm_engine {
             total_pel += in.pel_cap;
             int cbs = in.cb[0] * in.cb[1] * in.cb[2];
             max_cb = std::max(max_cb, cbs);
-            wcap += (size_t)cbs + pc / (size_t)g2p2g_chunk() + 16;
+            wcap += (size_t)cbs + pc / (size_t)g2p_chunk() + 16;
         }
         poffs[ninst] = (int)total;
         cap = total;
@@ -526,7 +530,7 @@ struct mpm_engine {
             if (int rc = dalloc(&nw_list[b], cap)) return rc;
             if (int rc = dalloc(&keys[b], cap, false)) return rc;
             if (int rc = dalloc(&order[b], cap, false)) return rc;
-            if (int rc = dalloc(&gmem[b], 7 * na)) return rc;
+            if (int rc = dalloc(&gmem[b], (size_t)kAuditFields * na)) return rc;
             if (int rc = dalloc(&gflag[b], (size_t)total_blocks)) return rc;
         }
         if (int rc = dalloc(&gv, 3 * na)) return rc;
@@ -562,7 +566,7 @@ struct mpm_engine {
         }
         if (int rc = dalloc(&d_work, work_cap)) return rc;
         if (int rc = dalloc(&wpre, 2 * work_cap)) return rc;
-        if (int rc = dalloc(&pmax, 2 * work_cap)) return rc;
+        if (int rc = dalloc(&pmax, 4 * work_cap)) return rc;
         if (int rc = dalloc(&pdiss, work_cap)) return rc;
         if (int rc = dalloc(&pfail, work_cap)) return rc;
         if (int rc = dalloc(&pesc, work_cap)) return rc;
@@ -575,13 +579,16 @@ struct mpm_engine {
         HIP_OK(rle_temp_bytes((int)cap, &rle_need));
         sort_bytes = std::max(sort_need, rle_need);
         HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
-        // Timing-only events: without the system-scope fence, recording one does
-        // not write back and invalidate the caches between the kernels it separates
-        // (about 5.7 us per record on a 1M-particle step).  They are read only
-        // after the stream has been synchronised.
-        for (int k = 0; k < kMaxStepsPerSync; ++k) {
-            HIP_OK(hipEventCreateWithFlags(&ev0[k], hipEventDisableSystemFence));
-            HIP_OK(hipEventCreateWithFlags(&ev1[k], hipEventDisableSystemFence));
+        // Kernel timing stamps (the kernels write them; no event records, which
+        // would leave the GPU idle for microseconds between the kernels they separate).
+        if (int rc = dalloc(&d_ktime, (size_t)4 * kMaxStepsPerSync)) return rc;
+        HIP_OK(hipHostMalloc((void**)&h_ktime, sizeof(unsigned long long) * 4 * kMaxStepsPerSync,
+                             hipHostMallocDefault));
+        {
+            int dev = 0, khz = 0;
+            HIP_OK(hipGetDevice(&dev));
+            HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+            clock_khz = (double)khz;
         }
         for (int i = 0; i < ninst; ++i)
             if (int rc = upload_params(i)) return rc;
@@ -632,10 +639,8 @@ struct mpm_engine {
         (void)hipFree(rlog);
         (void)hipFree(rlog_count);
         (void)hipFree(sort_temp);
-        for (int k = 0; k < kMaxStepsPerSync; ++k) {
-            if (ev0[k]) (void)hipEventDestroy(ev0[k]);
-            if (ev1[k]) (void)hipEventDestroy(ev1[k]);
-        }
+        (void)hipFree(d_ktime);
+        if (h_ktime) (void)hipHostFree(h_ktime);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -682,7 +687,7 @@ struct mpm_engine {
         for (int i = 0; i < ninst; ++i) h_is[i].cur ^= 1;
         // Work items: runs of one cell block, merged along z up to launch.merge_blocks
         // blocks of one instance and one chunk of particles; longer runs split.
-        const int chunk = g2p2g_chunk();
+        const int chunk = g2p_chunk();
         const int shift = cell_bits - 6;
         const uint64_t bmask = (1ull << shift) - 1;
         std::vector<int4> work;
@@ -746,30 +751,40 @@ struct mpm_engine {
             s.w_off = w;
             s.w_cnt += 1;
         }
-        for (int i = 0; i < ninst; ++i) h_is[i].nw_count[h_is[i].cur] = 0;
+        for (int i = 0; i < ninst; ++i) {
+            InstState& s = h_is[i];
+            s.nw_count[s.cur] = 0;
+            s.bound[0] = s.bound[1] = 0.0f;
+            s.setup_speed = 0.0f;
+        }
         if (nwork) HIP_OK(hipMemcpyAsync(d_work, work.data(), sizeof(int4) * nwork, hipMemcpyHostToDevice, stream));
         if (int rc = push()) return rc;
-        launch_nearwall(stream, dev());
+        launch_setup(stream, dev());
         HIP_OK(hipGetLastError());
         if (int rc = pull()) return rc;
+        // The committed speed as the trial steps compute it (particle_speed in
+        // f32), so that a state set from outside takes the same dt as the same
+        // state reached by stepping.
+        for (int i = 0; i < ninst; ++i) h_is[i].max_speed = (double)h_is[i].setup_speed;
+        if (int rc = push()) return rc;
         dirty = false;
         graph_stale = true;
         return MPM_OK;
     }
 
-    // Deposit instance i's committed state into its current accumulator set
-    // (the reference rebuilds the grid at the start of a step from the particles).
-    int rebuild_grid(int i) {
+    // Zero both accumulator sets of instance i: each trial deposits into the
+    // set its predecessor left cleared, and a state set from outside starts the
+    // alternation afresh.
+    int clear_grid(int i) {
         Instance& in = inst[i];
-        const InstState& s = h_is[i];
-        int g = s.gcur;
         size_t noff = (size_t)in.ip.boff * 64, nn = (size_t)in.ip.nblk * 64;
-        for (int f = 0; f < 7; ++f) HIP_OK(hipMemsetAsync(gmem[g] + f * na + noff, 0, sizeof(float) * nn, stream));
-        HIP_OK(hipMemsetAsync(gflag[g] + in.ip.boff, 0, sizeof(int) * in.ip.nblk, stream));
-        launch_p2g(stream, dev(), i, g);
-        HIP_OK(hipGetLastError());
+        for (int g = 0; g < 2; ++g) {
+            for (int f = 0; f < kAuditFields; ++f)
+                HIP_OK(hipMemsetAsync(gmem[g] + f * na + noff, 0, sizeof(long long) * nn, stream));
+            HIP_OK(hipMemsetAsync(gflag[g] + in.ip.boff, 0, sizeof(int) * in.ip.nblk, stream));
+        }
         HIP_OK(hipStreamSynchronize(stream));
-        in.grid_valid = true;
+        in.grid_clear = true;
         return MPM_OK;
     }
 
@@ -777,8 +792,8 @@ struct mpm_engine {
         if (dirty)
             if (int rc = sort_all()) return rc;
         for (int i = 0; i < ninst; ++i)
-            if (!inst[i].grid_valid)
-                if (int rc = rebuild_grid(i)) return rc;
+            if (!inst[i].grid_clear)
+                if (int rc = clear_grid(i)) return rc;
         return MPM_OK;
     }
 
@@ -879,6 +894,9 @@ struct mpm_engine {
         s.ledger[1] += out_mass;
         for (int a = 0; a < 3; ++a) s.ledger[28 + a] += out_mom[a];
         in.ip.particle_mass -= out_mass;
+        float mmax = 0.0f;
+        for (size_t q2 = 0; q2 < keep; ++q2) mmax = std::max(mmax, k.sh[SMASS * keep + q2]);
+        in.ip.mass_max = mmax;
         if (int rc = upload_params(i)) return rc;
         return upload(i, k);
     }
@@ -895,7 +913,7 @@ struct mpm_engine {
                 }
                 hipGraph_t g = nullptr;
                 HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                for (int j = 0; j < k; ++j) launch_step(stream, B, launch.traction_wgs, pel, nullptr, nullptr);
+                for (int j = 0; j < k; ++j) launch_step(stream, B, launch.traction_wgs, pel);
                 HIP_OK(hipStreamEndCapture(stream, &g));
                 hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
@@ -905,24 +923,41 @@ struct mpm_engine {
             }
             HIP_OK(hipGraphLaunch(graph, stream));
         } else {
+            if (profiling) {
+                // Start stamps all ones (atomic minima), end stamps zero (atomic maxima).
+                for (int j = 0; j < k; ++j) {
+                    h_ktime[4 * j] = h_ktime[4 * j + 2] = ~0ull;
+                    h_ktime[4 * j + 1] = h_ktime[4 * j + 3] = 0ull;
+                }
+                HIP_OK(hipMemcpyAsync(d_ktime, h_ktime, sizeof(unsigned long long) * 4 * k, hipMemcpyHostToDevice,
+                                      stream));
+            }
             for (int j = 0; j < k; ++j) {
                 profiled[j] = profiling && profile_seq++ % (uint64_t)profile_every == 0;
-                launch_step(stream, B, launch.traction_wgs, pel, profiled[j] ? ev0[j] : nullptr,
-                            profiled[j] ? ev1[j] : nullptr);
+                BatchDev Bj = B;
+                Bj.ktime = profiled[j] ? d_ktime + 4 * j : nullptr;
+                launch_step(stream, Bj, launch.traction_wgs, pel);
             }
         }
         HIP_OK(hipGetLastError());
         return MPM_OK;
     }
+    // Kernel times of the profiled steps of the last enqueue (stream synchronised).
+    // A launch whose workgroups all found their instance idle leaves no end stamp
+    // and is not counted.
     int collect_profile(int k) {
         if (!profiling) return MPM_OK;
+        HIP_OK(hipMemcpyAsync(h_ktime, d_ktime, sizeof(unsigned long long) * 4 * k, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        const double ms_per_tick = 1.0 / clock_khz;
         for (int j = 0; j < k; ++j) {
             if (!profiled[j]) continue;
             profiled[j] = false;
-            float ms = 0.0f;
-            HIP_OK(hipEventElapsedTime(&ms, ev0[j], ev1[j]));
-            g2p2g_ms += ms;
-            g2p2g_launches += 1;
+            const unsigned long long* t = h_ktime + 4 * j;
+            if (t[1] <= t[0] || t[3] <= t[2]) continue;
+            p2g_ms += (double)(t[1] - t[0]) * ms_per_tick;
+            g2p_ms += (double)(t[3] - t[2]) * ms_per_tick;
+            profiled_launches += 1;
         }
         return MPM_OK;
     }
@@ -1043,7 +1078,7 @@ struct mpm_engine {
             t.push_back({shared[b], sizeof(float) * NSF * cap});
             t.push_back({ids[b], sizeof(uint32_t) * cap});
             t.push_back({refkey[b], sizeof(uint64_t) * cap});
-            t.push_back({gmem[b], sizeof(float) * 7 * na});
+            t.push_back({gmem[b], sizeof(long long) * kAuditFields * na});
             t.push_back({gflag[b], sizeof(int) * (size_t)total_blocks});
         }
         t.push_back({d_pel, sizeof(PelletState) * (size_t)std::max(total_pel, 1)});
@@ -1088,7 +1123,7 @@ struct mpm_engine {
         std::vector<mpm_launch> cands;
         for (int k : {8, 32})
             for (int g : {0, 1})
-                for (int m : {1, 2}) cands.push_back(mpm_launch{k, g, m, launch.traction_wgs});
+                for (int m : {1, 2}) cands.push_back(mpm_launch{k, g, m, launch.traction_wgs, launch.g2p_occupancy});
         mpm_launch best = launch;
         double best_ms = std::numeric_limits<double>::infinity();
         for (const mpm_launch& c : cands) {
@@ -1223,6 +1258,11 @@ int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pell
     pel.resize(n_pellets);
     in.ip.npel = (int)n_pellets;
     in.ip.particle_mass = mass;
+    {
+        float mmax = 0.0f;
+        for (size_t p = 0; p < n; ++p) mmax = std::max(mmax, h.sh[SMASS * n + p]);
+        in.ip.mass_max = mmax;
+    }
     in.ip.pel_contact_dt = contact_dt;
     if (int rc = upload_params(i)) return rc;
     if (int rc = upload(i, h)) return rc;
@@ -1264,7 +1304,7 @@ int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pell
         energies_f64(in.mat, in.cfg.gravity, *ps, pellets, n_pellets, &ke, &pe, &el);
         s.ledger[5] = ke + pe + el;
     }
-    in.grid_valid = false;
+    in.grid_clear = false;
     dirty = true;
     graph_stale = true;
     return push();
@@ -2044,14 +2084,17 @@ void mpm_enable_profiling(mpm_engine* e, int on) {
     e->profiling = on > 0;
     e->profile_every = on > 0 ? on : 1;
     e->profile_seq = 0;
-    e->g2p2g_ms = 0.0;
-    e->g2p2g_launches = 0;
+    e->g2p_ms = 0.0;
+    e->p2g_ms = 0.0;
+    e->profiled_launches = 0;
 }
 
-int mpm_profile_g2p2g(mpm_engine* e, double* avg_ms, uint64_t* launches) {
+int mpm_profile_kernels(mpm_engine* e, double* g2p_avg_ms, double* p2g_avg_ms, uint64_t* launches) {
     if (!e) return MPM_ERR_INVALID;
-    if (avg_ms) *avg_ms = e->g2p2g_launches ? e->g2p2g_ms / (double)e->g2p2g_launches : 0.0;
-    if (launches) *launches = e->g2p2g_launches;
+    const double n = (double)e->profiled_launches;
+    if (g2p_avg_ms) *g2p_avg_ms = n > 0 ? e->g2p_ms / n : 0.0;
+    if (p2g_avg_ms) *p2g_avg_ms = n > 0 ? e->p2g_ms / n : 0.0;
+    if (launches) *launches = e->profiled_launches;
     return MPM_OK;
 }
 
@@ -2130,9 +2173,10 @@ int mpm_get_launch(const mpm_engine* e, mpm_launch* out) {
 
 int mpm_set_launch(mpm_engine* e, const mpm_launch* l) {
     if (!e || !l) return MPM_ERR_INVALID;
-    // More than two cell blocks per work item would not fit the G2P2G scatter tile.
+    // More than two cell blocks per work item would not fit the LDS tiles.
     if (l->steps_per_sync < 1 || l->steps_per_sync > kMaxStepsPerSync || l->merge_blocks < 1 ||
-        l->merge_blocks > 2 || l->traction_wgs < 1 || l->traction_wgs > 65536)
+        l->merge_blocks > 2 || l->traction_wgs < 1 || l->traction_wgs > 65536 || l->g2p_occupancy < 2 ||
+        l->g2p_occupancy > 3)
         return e->fail(MPM_ERR_INVALID, "launch shape out of range");
     bool resort = l->merge_blocks != e->launch.merge_blocks;
     e->launch = *l;

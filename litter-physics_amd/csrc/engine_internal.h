@@ -17,9 +17,15 @@
 //    change within a step and live in a separate slab pair swapped by the sort.
 //  * grid: each instance's dense padded node array (grid.rs:12-21) re-laid in
 //    4x4x4-node blocks of 64 consecutive nodes (one wavefront per block); the
-//    instances' blocks are concatenated.  Two accumulator sets hold mass, APIC
-//    momentum sum(w (m v + m C d)) and the dt-free stress sum(w V0 tau d); the
-//    reference's P2G momentum is mom - dt (4/dx^2) force.
+//    instances' blocks are concatenated.  The particle-to-grid deposit of a
+//    trial step (solver.rs:632-661) runs once the step's dt is known and sums
+//    mass and momentum w (m v + (m C - dt (4/dx^2) V0 tau) d) as exact 64-bit
+//    integers in units of a per-instance power of two (DepositScale), so the
+//    sums do not depend on the order of the atomics or on how the particles
+//    are grouped into workgroups.  Two accumulator sets alternate from trial
+//    to trial: the grid update reads one and clears the other.  The energy
+//    audit deposits seven fields instead: mass, the APIC momentum
+//    w (m v + m C d) and the stress sum w V0 tau d.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -63,6 +69,9 @@ enum { P_ESCAPED = 1, P_BUDGET = 2 };
 enum { E_MIN_STEP = 1, E_MIN_AFTER_REJECT = 2 };
 
 constexpr int kMaxSpheres = 64;
+// Accumulator fields of the deposit: mass and momentum, or (audit) mass, APIC
+// momentum and the stress sum.
+constexpr int kDepositFields = 4, kAuditFields = 7;
 constexpr int kLedger = 31;  // doubles of mpm_ledger
 constexpr int kGridVals = 12;
 // Node blocks per grid-update workgroup; instance block ranges are multiples of it.
@@ -92,6 +101,7 @@ struct InstParams {
     double gravity[3];
     double wall_friction, contact_k, contact_damping, contact_friction;
     double particle_mass;   // sum of the stored particle masses (f64)
+    float mass_max;         // largest stored particle mass (f32, as the kernels hold it)
     double pel_contact_dt;  // min over the instance's pellets of 0.2 sqrt(m / k)
     int poff, pcap;         // particle range of the slabs
     int boff, nblk;         // node-block range (nblk a multiple of kGridWgBlocks)
@@ -109,6 +119,7 @@ struct StepStatus {
     double grid_momentum[3];     // sum of m v' over nodes after the walls
     float max_speed;
     float max_disp;
+    float bound[2];              // deposit bounds of the trial state (particle_bounds)
     unsigned long long fail;     // (particle << 8) | code of the first failure
     int grid_nonfinite;
     int escaped;
@@ -122,6 +133,15 @@ struct InstState {
     double time, target, dt, limit, dt_scale, max_speed, min_dt_used, max_dt_used;
     double trial_dt;  // > 0: one bare trial step of this size (try_step), no time advance
     double pel_vmax;  // largest pellet speed of the committed state
+    // Deposit scales of the running trial (powers of two; the integer sums hold
+    // value x scale): mass, momentum with the stress impulse, and for the audit
+    // the APIC momentum and the stress sum; wall-traction reactions.
+    float sc_m, sc_p, sc_mom, sc_f, sc_t;
+    // Largest particle_bounds of the committed state (set from the accepted
+    // trial, or by the set-up kernel after a state upload or sort) and largest
+    // particle speed found by the set-up kernel.
+    float bound[2];
+    float setup_speed;
     unsigned long long step_count, limited_steps, rejected_steps;
     StepStatus st;
     int n;            // particle count
@@ -137,12 +157,21 @@ struct InstState {
     int pad;
 };
 
-// Per-pellet coupling accumulators (rigid.rs:350-363).
+// Exact fixed-point sum of f64 values in units of 2^-kFixBits, as a two's
+// complement 128-bit integer in two words: (hi, lo) with lo carrying into hi.
+// Integer sums do not depend on the order of the additions, so atomics and
+// partial sums give the same total whatever the scheduling.
+constexpr int kFixBits = 100;
+struct Fix128 {
+    unsigned long long lo;
+    long long hi;
+};
+
+// Per-pellet coupling accumulators (rigid.rs:350-363): impulse (3), angular
+// impulse (3) and grid energy, exact fixed point; constrained node count.
 struct PelletAccum {
-    double impulse[3];
-    double angular[3];
-    double grid_energy;
-    double constrained;
+    Fix128 sum[7];
+    unsigned long long constrained;
 };
 
 // Pellet state (rigid.rs:30-49), f64; orientation quaternion (i, j, k, w).
@@ -172,27 +201,20 @@ struct PelletTrial {
     int pad;
 };
 
+// One accumulator set: field f of node n at v[f * na + n] (0 mass, 1-3 momentum,
+// 4-6 the audit's stress sum).
 struct GridPtrs {
-    float* m;
-    float* px;
-    float* py;
-    float* pz;
-    float* fx;
-    float* fy;
-    float* fz;
+    long long* v;
     int* flag;
 };
 
 // Set-up record of one work item for one slab, written by the step that
-// produced the slab (or by nearwall_kernel after a sort): the box of the
-// stencil bases of the item's particles, and the bounds the deposit scales
-// start from.  G2P2G reads it to lay out its tiles without a pass over the
+// produced the slab (or by setup_kernel after a state upload or sort): the box
+// of the stencil bases of the item's particles.  The deposit and the gather of
+// the next trial lay out their LDS tiles from it without a pass over the
 // particles first; a particle outside the box takes the untiled path.
 struct ItemPre {
     int lo[3], hi[3];  // stencil-base box of the particles inside the padded grid (lo > hi: none)
-    float m;           // largest particle mass
-    float v0;          // largest V0
-    float rows[4];     // per wavefront: largest V0 max_a sum_b |tau_ab| of the slab's stresses
 };
 
 // Device pointers of the whole batch.
@@ -206,11 +228,16 @@ struct BatchDev {
     float* slab[2];
     float* shared[2];
     int sh;       // shared-slab selector (host controlled)
-    float* gmem[2];  // 7 * na floats each
-    int* gflag[2];   // one per node block
-    float* gv;       // 3 * na
-    float* trS;      // 3 * na
-    float* trT;      // 3 * na
+    int g2p_occupancy;  // G2P workgroups resident per CU (2 or 3: its register budget)
+    // Kernel timing (null: off): the deposit's and the G2P's earliest workgroup
+    // start and latest workgroup end on the device's constant clock,
+    // {p2g start, p2g end, g2p start, g2p end}, set up as {max, 0, max, 0}.
+    unsigned long long* ktime;
+    long long* gmem[2];  // kAuditFields * na integer sums each
+    int* gflag[2];       // one per node block
+    float* gv;           // 3 * na
+    long long* trS;      // 3 * na, units of the traction scale
+    long long* trT;      // 3 * na, units of the traction scale
     uint32_t* ids;     // current shared-slab ids and reference-order keys
     uint64_t* refkey;
     int* nw_list[2];  // per slab, instance ranges like the particles
@@ -224,8 +251,8 @@ struct BatchDev {
     double* apart;    // audit partials, field-major like gpart (NULL: no audit)
     double* audit;    // audit sums per instance, kAuditOut each (NULL: no audit)
     int* gbad;
-    float* pmax;      // g2p2g partials per work item
-    double* pdiss;
+    float* pmax;      // g2p partials per work item: speed, displacement, the two deposit bounds
+    Fix128* pdiss;    // g2p plastic-dissipation partials per work item
     unsigned long long* pfail;
     int* pesc;
     PelletState* pel;
@@ -253,14 +280,7 @@ constexpr int kLogStride = 25;
 
 __host__ __device__ inline GridPtrs grid_set(const BatchDev& B, int set) {
     GridPtrs g;
-    float* m = B.gmem[set];
-    g.m = m;
-    g.px = m + B.na;
-    g.py = m + 2 * B.na;
-    g.pz = m + 3 * B.na;
-    g.fx = m + 4 * B.na;
-    g.fy = m + 5 * B.na;
-    g.fz = m + 6 * B.na;
+    g.v = B.gmem[set];
     g.flag = B.gflag[set];
     return g;
 }
@@ -285,17 +305,15 @@ const uint16_t* mlp_device_fragments(const mpm_mlp* mlp);
 const float* mlp_device_biases(const mpm_mlp* mlp);
 
 // Host launchers (kernels.hip).
-int g2p2g_chunk();
-// One trial step of every running instance: control, traction, grid update,
-// pellet coupling, fused G2P2G, pellet integration, reductions, commit and the
-// 32-step order keys.
-void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, hipEvent_t g2p2g_start,
-                 hipEvent_t g2p2g_stop);
-void launch_p2g(hipStream_t s, const BatchDev& B, int inst, int gcur);
+int g2p_chunk();
+// One trial step of every running instance: control, deposit, traction, grid
+// update, pellet coupling, G2P, pellet integration, reductions and commit.
+void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets);
 // Order keys of the instances whose refresh is pending (the flags are left set).
 void launch_refkeys(hipStream_t s, const BatchDev& B);
-// Near-wall lists of every instance's committed slab (counters cleared by the caller).
-void launch_nearwall(hipStream_t s, const BatchDev& B);
+// Near-wall lists, set-up records, deposit bounds and largest speed of every
+// instance's committed slab (list counters, bounds and speed cleared by the caller).
+void launch_setup(hipStream_t s, const BatchDev& B);
 // Device-order keys of every slab entry: (instance << cell_bits) | 4^3-cell block
 // key; entries past an instance's particle count sort to the end of its range.
 void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t* keys, uint32_t* order);

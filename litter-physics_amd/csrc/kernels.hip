@@ -3,19 +3,22 @@
 // One trial step of every running instance (core/src/mpm/solver.rs:413-611) is a
 // fixed launch sequence with no host round trip:
 //   control      -- advance_while's dt choice (solver.rs:421-443) and the stability
-//                   limit (477-498) per instance; pellet geometry for the coupling
+//                   limit (477-498) per instance, the deposit scales of the trial;
+//                   pellet geometry for the coupling
+//   p2g          -- deposit_particles (solver.rs:632-661) as exact integer sums
 //   traction     -- wall_traction_to_grid (solver.rs:725-819) over the near-wall lists
 //   grid_update  -- grid_update (solver.rs:822-892), one wavefront per 4x4x4-node block
 //   couple       -- couple_grid (rigid.rs:385-438) for each instance's pellets in order
-//   g2p2g        -- grid_to_particle with update_paste / update_liquid (solver.rs:895-977,
-//                   constitutive.rs:456-589) fused with the next step's deposit
-//                   (solver.rs:632-661) into the other accumulator set
+//   g2p          -- grid_to_particle with update_paste / update_liquid (solver.rs:895-977,
+//                   constitutive.rs:456-589) into the trial slab
 //   pellet       -- Pellet::apply_impulse and ::integrate (rigid.rs:251-334) on trial copies
-//   finalize     -- fixed-order reduction of the per-workgroup partials per instance
+//   finalize     -- reduction of the per-workgroup partials per instance
 //   commit       -- accept/reject, commit and ledgers (solver.rs:444-467, 593-611, 980-998)
 //   refkeys      -- the 32-step reference order keys (solver.rs:501-517)
-// The deposit is split into the dt-free APIC momentum and the dt-free stress sum
-// so that it can run before the next step's dt is known.
+// Every sum whose order of additions varies with the scheduling is an exact
+// integer (the deposit, the wall reactions, the pellets' coupling sums and the
+// plastic dissipation), so a trajectory is a function of the state alone: the
+// same on every run and after a restore, whatever the particle order.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -27,8 +30,9 @@ namespace mpm {
 namespace {
 
 constexpr int kChunk = 1024;
-constexpr int kGatherCap = 768;
-constexpr int kScatterCap = 1152;  // 9 x 9 x 13 nodes: two z-adjacent cell blocks plus one cell each side
+// Nodes of a work item's LDS tile: the stencils of two z-adjacent 4^3-cell
+// blocks span 7 x 7 x 11 nodes, with room for the particles' drift between sorts.
+constexpr int kTileCap = 768;
 constexpr double kDtRecovery = 1.25;              // solver.rs:45
 constexpr unsigned long long kSortInterval = 32;  // solver.rs:39
 
@@ -87,6 +91,16 @@ __device__ inline int instance_of(const BatchDev& B, int p) {
             hi = mid - 1;
     }
     return lo;
+}
+
+// Kernel timing on the device's constant clock: the earliest workgroup start
+// and the latest workgroup end of a launch, one atomic per workgroup each.
+// kernel_end is called after the workgroup's last barrier.
+__device__ inline void kernel_start(unsigned long long* slot) {
+    if (threadIdx.x == 0) atomicMin(slot, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+__device__ inline void kernel_end(unsigned long long* slot) {
+    if (threadIdx.x == 0) atomicMax(slot, (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 
 // Reference order key (clamped cell index, z fastest, then id; solver.rs:501-517),
@@ -163,6 +177,15 @@ __global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
             z.fail = ~0ull;
             s.st = z;
             s.nw_count[s.cur ^ 1] = 0;
+            // Deposit scales of this trial from the committed state's bounds
+            // (particle_bounds): no contribution reaches 2^29 units.
+            const double dt = s.dt, b1 = s.bound[0], b2 = s.bound[1];
+            const double gmax = fmax(fmax(fabs(ip.gravity[0]), fabs(ip.gravity[1])), fabs(ip.gravity[2]));
+            s.sc_m = deposit_scale((double)ip.mass_max);
+            s.sc_p = deposit_scale(b1 + dt * b2);
+            s.sc_mom = deposit_scale(b1);
+            s.sc_f = deposit_scale(b2 * 0.25 * ip.dx * ip.dx);
+            s.sc_t = deposit_scale(dt * (b2 + 7.0 * (double)ip.mass_max * gmax));
         }
     }
     __syncthreads();
@@ -201,13 +224,139 @@ __global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
     }
 }
 
+// ------------------------------------------------------------ deposit
+
+// deposit_particles (solver.rs:632-661) of every running instance's committed
+// state, at the start of the trial once its dt is known.  One workgroup per
+// work item: the item's particles deposit into an LDS tile of exact 64-bit
+// integer sums over the nodes of their stencils (the box of the committed
+// slab's set-up record), flushed with one 64-bit global atomic per touched
+// node and field.  Particles outside the tile, and items whose tile exceeds
+// the LDS budget, deposit with global atomics; a particle gives the same
+// integers either way (deposit_particle).  At the 32-step cadence it also
+// writes the reference-order keys of the committed positions.
+// NF = kDepositFields, or kAuditFields for the energy audit.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void p2g_kernel(BatchDev B) {
+    constexpr int R = kChunk / kBlock;
+    __shared__ __align__(16) unsigned long long ss[NF * kTileCap];
+    if (B.ktime) kernel_start(B.ktime);
+    const int4 item = B.work[blockIdx.x];
+    const int inst = item.x;
+    const InstState& is = B.is[inst];
+    if (!is.active) return;
+    const InstParams& ip = B.ip[inst];
+    const Params P = ip.P;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const size_t cap = B.cap, na = B.na;
+    const float dt = (float)is.dt;
+    const float* __restrict__ src = B.slab[is.cur];
+    const float* __restrict__ shared = B.shared[B.sh];
+    GridPtrs out = grid_set(B, is.gcur);
+    out.v += (size_t)ip.boff * 64;
+    out.flag += ip.boff;
+    const ItemPre pre = B.wpre[is.cur][blockIdx.x];
+    Tile t;
+    {
+        bool ok = true;
+        int sz = 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ok = ok && pre.lo[a] <= pre.hi[a];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            t.org[a] = ok ? pre.lo[a] : 0;
+            t.gd[a] = ok ? pre.hi[a] - pre.lo[a] + 3 : 0;
+            sz *= t.gd[a];
+        }
+        t.tiled = ok && sz <= kTileCap;
+    }
+    if (t.tiled)
+        for (int q = tid; q < NF * kTileCap / 2; q += kBlock)
+            reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
+    __syncthreads();
+    const int rot = (lane * 5) % 9;
+    const int s12 = __mul24(t.gd[1], t.gd[2]);
+    const int end = item.y + item.z;
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        const int p = item.y + r * kBlock + tid;
+        if (p >= end) continue;
+        const float x = src[PX * cap + p], y = src[PY * cap + p], z = src[PZ * cap + p];
+        if (is.refsort) B.refkey[p] = (ref_cell(ip, x, y, z) << 32) | (uint64_t)B.ids[p];
+        Stencil s;
+        if (!make_stencil(P, x, y, z, s)) continue;  // outside the padded grid: no deposit (solver.rs:636)
+        const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
+        float v[3], C[9], tau[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = src[(VX + a) * cap + p];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) C[i] = src[(C00 + i) * cap + p];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) tau[i] = src[(TXX + i) * cap + p];
+        const Deposit<NF> q = make_deposit<NF>(P, is, dt, m, v0, v, C, tau);
+        bool in_tile = t.tiled;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int li = s.base[a] - t.org[a];
+            in_tile = in_tile && li >= 0 && li + 2 < t.gd[a];
+        }
+        if (in_tile) {
+            TileSink<NF, kTileCap> sink;
+            sink.ss = ss;
+            sink.s12 = s12;
+            sink.sd2 = t.gd[2];
+            sink.row0 = __mul24(__mul24(s.base[0] - t.org[0], t.gd[1]) + (s.base[1] - t.org[1]), t.gd[2]) +
+                        (s.base[2] - t.org[2]);
+            sink.row = 0;
+            deposit_particle<NF>(s, q, rot, sink);
+        } else {
+            GlobalSink sink;
+            sink.v = out.v;
+            sink.na = na;
+            sink.P = &P;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sink.base[a] = s.base[a];
+            flag_stencil(P, out.flag, s);
+            deposit_particle<NF>(s, q, rot, sink);
+        }
+    }
+    __syncthreads();
+    if (!t.tiled) {
+        if (B.ktime) kernel_end(B.ktime + 1);
+        return;
+    }
+    // Flush: the tile's exact sums into the global ones.  A node with a
+    // contribution has mass (mass rounds up), so the mass word marks it.
+    const int sz = t.gd[0] * s12;
+    const float ig2 = 1.0f / (float)t.gd[2], ig1 = 1.0f / (float)t.gd[1];
+    for (int q = tid; q < sz; q += kBlock) {
+        const unsigned long long qm = ss[q];
+        if (qm == 0ull) continue;
+        const int rest = div_small(q, ig2), k = q - rest * t.gd[2];
+        const int i = div_small(rest, ig1), j = rest - i * t.gd[1];
+        const int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
+        atomicAdd(reinterpret_cast<unsigned long long*>(out.v + ni), qm);
+#pragma unroll
+        for (int f = 1; f < NF; ++f) {
+            const unsigned long long qv = ss[f * kTileCap + q];
+            if (qv) atomicAdd(reinterpret_cast<unsigned long long*>(out.v + f * na + ni), qv);
+        }
+        out.flag[ni >> 6] = 1;
+    }
+    if (B.ktime) {
+        __syncthreads();
+        kernel_end(B.ktime + 1);
+    }
+}
+
 // ------------------------------------------------------------ traction
 
 // wall_traction_to_grid (solver.rs:725-819) over the near-wall lists of the
 // running instances, concatenated.  Each deposit adds its reaction to the
 // Coulomb budget (trT) of its node, which also gives the signed traction
 // impulse (trS, kept separately only on grids two cells deep); the grid update
-// books the telescoped kinetic-energy jump per node.
+// books the telescoped kinetic-energy jump per node.  Reactions are exact
+// integers in units of the instance's traction scale.
 __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
     const int total = B.counters[1];
     const float* shared = B.shared[B.sh];
@@ -237,7 +386,9 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
         int p = B.nw_list[s.cur][ip.poff + local];
         const float* slab = B.slab[s.cur];
         const float dt = (float)s.dt;
-        GridPtrs g = grid_set(B, s.gcur);
+        const long long* gsum = B.gmem[s.gcur];
+        const double um = 1.0 / (double)s.sc_m, up = 1.0 / (double)s.sc_p;
+        const float tscale = s.sc_t;
         const size_t noff = (size_t)ip.boff * 64;
         float pos[3] = {slab[PX * cap + p], slab[PY * cap + p], slab[PZ * cap + p]};
         const float tnn[3] = {slab[TXX * cap + p], slab[TYY * cap + p], slab[TZZ * cap + p]};
@@ -264,12 +415,12 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
             int plane = face == 0.0f ? 1 : P.cells[axis] + 1;
             float inward = face == 0.0f ? 1.0f : -1.0f;
             float weight_momentum = mass * g_a * dt;
-            const float* mom = axis == 0 ? g.px : (axis == 1 ? g.py : g.pz);
-            const float* frc = axis == 0 ? g.fx : (axis == 1 ? g.fy : g.fz);
+            const long long* mom = gsum + (size_t)(1 + axis) * na;
             // The column's three nodes: all their grid reads first (wall-plane node
-            // mass, momentum and force, deposit node mass), then the deposits, so a
+            // mass and momentum, deposit node mass), then the deposits, so a
             // thread waits for one memory round trip rather than two per node.
-            float w[3], pm[3], pmo[3], pfr[3], dm[3];
+            float w[3], pm[3], pmo[3];
+            long long dm[3];
             size_t nis[3];
 #pragma unroll
             for (int off = 0; off < 3; ++off) {
@@ -277,7 +428,8 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
                           sc = axis == 2 ? off : cv;
                 int co[3] = {st.base[0] + sa, st.base[1] + sb, st.base[2] + sc};
                 w[off] = 0.0f;
-                pm[off] = pmo[off] = pfr[off] = dm[off] = 0.0f;
+                pm[off] = pmo[off] = 0.0f;
+                dm[off] = 0;
                 nis[off] = 0;
                 if (co[axis] <= 1 || co[axis] > P.cells[axis]) continue;
                 // sel3: the stencil stays in registers (a runtime index would place it in scratch).
@@ -288,22 +440,25 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
                 pc[axis] = plane;
                 const size_t pi = noff + node_index(P, pc[0], pc[1], pc[2]);
                 nis[off] = noff + node_index(P, co[0], co[1], co[2]);
-                pm[off] = g.m[pi];
-                pmo[off] = mom[pi];
-                pfr[off] = frc[pi];
-                dm[off] = g.m[nis[off]];
+                pm[off] = (float)((double)gsum[pi] * um);
+                pmo[off] = (float)((double)mom[pi] * up);
+                dm[off] = gsum[nis[off]];
             }
 #pragma unroll
             for (int off = 0; off < 3; ++off) {
                 if (w[off] == 0.0f || pm[off] <= 0.0f) continue;
-                float trial = (pmo[off] - dt * k4 * pfr[off]) / pm[off] + g_a * dt;
+                float trial = pmo[off] / pm[off] + g_a * dt;
                 if (trial * inward >= 0.0f) continue;
                 float distance = st.d[axis][off] * P.dx;
                 float reaction = (normal_stress * distance * force_scale + weight_momentum) * w[off] * inward;
                 if (reaction <= 0.0f) continue;
-                if (dm[off] <= 0.0f) continue;
-                atomicAdd(B.trT + axis * na + nis[off], reaction);
-                if (P.cells[axis] <= 2) atomicAdd(B.trS + axis * na + nis[off], reaction * inward);
+                if (dm[off] <= 0) continue;
+                const int units = round_int(reaction * tscale);
+                atomicAdd(reinterpret_cast<unsigned long long*>(B.trT + axis * na + nis[off]),
+                          (unsigned long long)(long long)units);
+                if (P.cells[axis] <= 2)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(B.trS + axis * na + nis[off]),
+                              (unsigned long long)(long long)(face == 0.0f ? units : -units));
             }
         }
     }
@@ -314,8 +469,10 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
 // grid_update (solver.rs:822-892).  kGridWgBlocks 64-node blocks of one
 // instance per workgroup, kGridWgBlocks / 4 per wavefront; ledger sums go to
 // per-workgroup partials (no same-address atomics), reduced once per
-// workgroup.  The other accumulator set is cleared on the way for the trial
-// deposit.
+// workgroup.  The integer sums of the trial's deposit convert exactly to f64
+// (they stay below 2^53) and round once to f32.  The other accumulator set,
+// which the previous trial deposited into, is cleared on the way for the next
+// trial's deposit.
 // AUDIT: also the grid kinetic energy sum 1/2 m |v|^2 over nodes with mass at
 // the stages where the reference's audit takes it (solver.rs:540-625,
 // audit.rs:360-398): the APIC transport momentum alone (the accumulated
@@ -336,6 +493,9 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     const float dt = (float)s.dt;
     const size_t na = B.na;
     const float k4 = 4.0f * P.inv_dx * P.inv_dx;
+    constexpr int NF = AUDIT ? kAuditFields : kDepositFields;
+    const double um = 1.0 / (double)s.sc_m, up = 1.0 / (double)(AUDIT ? s.sc_mom : s.sc_p),
+                 uf = 1.0 / (double)s.sc_f, ut = 1.0 / (double)s.sc_t;
     double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
@@ -369,29 +529,26 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
         node_coords(P, (b - ip.boff) << 6, bc[0], bc[1], bc[2]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) wall[j] = wall[j] || bc[a] == 0 || (bc[a] >> 2) == (P.cells[a] >> 2);
-        fm[j] = A.m[n];
-        fpx[j] = A.px[n];
-        fpy[j] = A.py[n];
-        fpz[j] = A.pz[n];
-        ffx[j] = A.fx[n];
-        ffy[j] = A.fy[n];
-        ffz[j] = A.fz[n];
+        fm[j] = (float)((double)A.v[n] * um);
+        fpx[j] = (float)((double)A.v[na + n] * up);
+        fpy[j] = (float)((double)A.v[2 * na + n] * up);
+        fpz[j] = (float)((double)A.v[3 * na + n] * up);
+        if (AUDIT) {
+            ffx[j] = (float)((double)A.v[4 * na + n] * uf);
+            ffy[j] = (float)((double)A.v[5 * na + n] * uf);
+            ffz[j] = (float)((double)A.v[6 * na + n] * uf);
+        }
         if (wall[j])
 #pragma unroll
-            for (int a = 0; a < 3; ++a) ftb[j][a] = B.trT[a * na + n];
+            for (int a = 0; a < 3; ++a) ftb[j][a] = (float)((double)B.trT[a * na + n] * ut);
     }
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
         const size_t n = ((size_t)b << 6) | l;
         if (cf[j]) {
-            C.m[n] = 0.0f;
-            C.px[n] = 0.0f;
-            C.py[n] = 0.0f;
-            C.pz[n] = 0.0f;
-            C.fx[n] = 0.0f;
-            C.fy[n] = 0.0f;
-            C.fz[n] = 0.0f;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) C.v[f * na + n] = 0;
             if (l == 0) C.flag[b] = 0;
         }
     }
@@ -408,7 +565,14 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
             B.gv[2 * na + n] = 0.0f;
             continue;
         }
-        float p[3] = {fpx[j] - dt * k4 * ffx[j], fpy[j] - dt * k4 * ffy[j], fpz[j] - dt * k4 * ffz[j]};
+        // The audit deposits the APIC momentum and the stress sum apart; the
+        // reference's P2G momentum is their combination (solver.rs:643-655).
+        float p[3] = {fpx[j], fpy[j], fpz[j]};
+        if (AUDIT) {
+            p[0] = fpx[j] - dt * k4 * ffx[j];
+            p[1] = fpy[j] - dt * k4 * ffy[j];
+            p[2] = fpz[j] - dt * k4 * ffz[j];
+        }
         float v[3], tb[3];
         int c[3];
         node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
@@ -421,8 +585,9 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
             tb[a] = ftb[j][a];
             float D;
             if (P.cells[a] <= 2) {
-                D = wall_block ? B.trS[a * na + n] : 0.0f;
-                if (D != 0.0f) B.trS[a * na + n] = 0.0f;
+                const long long ds = wall_block ? B.trS[a * na + n] : 0;
+                D = (float)((double)ds * ut);
+                if (ds != 0) B.trS[a * na + n] = 0;
             } else {
                 D = c[a] == 2 ? tb[a] : -tb[a];
             }
@@ -431,7 +596,7 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
                 vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
                 vals[3 + a] += D;
             }
-            if (tb[a] != 0.0f) B.trT[a * na + n] = 0.0f;
+            if (tb[a] != 0.0f) B.trT[a * na + n] = 0;
             v[a] = (p[a] + D) / m + P.gravity[a] * dt;
             if (AUDIT) {
                 const double md = (double)m;
@@ -517,7 +682,8 @@ __global__ __launch_bounds__(64) void couple_kernel(BatchDev B) {
     if (!A.flag[b]) return;
     const size_t n = ((size_t)b << 6) | l;
     const size_t na = B.na;
-    const double m = A.m[n];
+    // The node mass as the grid update holds it (f32 of the exact sum).
+    const double m = (double)(float)((double)A.v[n] * (1.0 / (double)s.sc_m));
     int ci, cj, ck;
     node_coords(ip.P, (int)((size_t)(b - ip.boff) << 6) | l, ci, cj, ck);
     const double dx = ip.dx;
@@ -588,15 +754,17 @@ __global__ __launch_bounds__(64) void couple_kernel(BatchDev B) {
             double dp[3] = {(nv[0] - v[0]) * m, (nv[1] - v[1]) * m, (nv[2] - v[2]) * m};
             double e = 0.5 * m *
                        ((nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) - (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+            // Exact fixed-point sums: the pellet's totals do not depend on the
+            // order in which the nodes reach them.
             PelletAccum* acc = B.pacc + pi;
-            atomicAdd(&acc->impulse[0], -dp[0]);
-            atomicAdd(&acc->impulse[1], -dp[1]);
-            atomicAdd(&acc->impulse[2], -dp[2]);
-            atomicAdd(&acc->angular[0], -(r[1] * dp[2] - r[2] * dp[1]));
-            atomicAdd(&acc->angular[1], -(r[2] * dp[0] - r[0] * dp[2]));
-            atomicAdd(&acc->angular[2], -(r[0] * dp[1] - r[1] * dp[0]));
-            atomicAdd(&acc->grid_energy, e);
-            atomicAdd(&acc->constrained, 1.0);
+            fix_atomic_add(&acc->sum[0], fix_from(-dp[0]));
+            fix_atomic_add(&acc->sum[1], fix_from(-dp[1]));
+            fix_atomic_add(&acc->sum[2], fix_from(-dp[2]));
+            fix_atomic_add(&acc->sum[3], fix_from(-(r[1] * dp[2] - r[2] * dp[1])));
+            fix_atomic_add(&acc->sum[4], fix_from(-(r[2] * dp[0] - r[0] * dp[2])));
+            fix_atomic_add(&acc->sum[5], fix_from(-(r[0] * dp[1] - r[1] * dp[0])));
+            fix_atomic_add(&acc->sum[6], fix_from(e));
+            atomicAdd(&acc->constrained, 1ull);
             v[0] = nv[0];
             v[1] = nv[1];
             v[2] = nv[2];
@@ -631,8 +799,8 @@ __global__ void pellet_kernel(BatchDev B) {
     for (int a = 0; a < 4; ++a) q[a] = p.q[a];
     double before = pellet_kinetic(p.mass, vel, q, L, p.inertia);
     for (int a = 0; a < 3; ++a) {
-        vel[a] = vel[a] + acc.impulse[a] / p.mass;
-        L[a] = L[a] + acc.angular[a];
+        vel[a] = vel[a] + fix_value(acc.sum[a]) / p.mass;
+        L[a] = L[a] + fix_value(acc.sum[3 + a]);
     }
     t.energy = pellet_kinetic(p.mass, vel, q, L, p.inertia) - before;
     // rigid.rs:279-334
@@ -754,29 +922,26 @@ __device__ inline void block_append_rounds(unsigned bits, int first, int* list, 
     }
 }
 
-// grid_to_particle + constitutive update (solver.rs:895-977) fused with the
-// next step's particle-to-grid deposit into the other accumulator set.  A
-// workgroup owns one work item: the particles of one or two z-adjacent cell
-// blocks of one instance.  It stages the node velocities of their stencils in
-// LDS and deposits every updated particle straight from registers into a
-// fixed-point LDS tile (exact 64-bit integer sums, TileScale) one cell wider on
-// each side than those stencils (an accepted step moves a particle by at most
-// one cell); the tile is flushed with one global atomic per touched node and
-// field.  Particles that leave the tile or exceed its scales, and work items
-// whose tiles exceed the LDS budget, deposit with global atomics.
-// MLP: the return-map surrogate is on.
-template <int CHUNK, int GCAP, int SCAP, bool MLP>
-__global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
-    constexpr int R = CHUNK / kBlock;
-    __shared__ float4 sg[GCAP];
-    __shared__ __align__(16) unsigned long long ss[7 * SCAP];
-    __shared__ float sbound[4][3];
+// grid_to_particle with the constitutive update (solver.rs:895-977,
+// constitutive.rs:456-589) into the trial slab.  One workgroup per work item:
+// the node velocities of the item's stencils (the box of the committed slab's
+// set-up record) are staged in LDS, and each particle gathers v and C,
+// advects, and updates its deformation and stress.  The step's reductions go
+// to per-item partials: the largest speed, displacement and deposit bounds,
+// the plastic dissipation as an exact fixed-point sum, the first failure by
+// particle id.  The set-up record and near-wall list of the trial slab are
+// written for the next trial.  MLP: the return-map surrogate is on.  WGS:
+// workgroups per CU the register budget allows (the launch shape picks it).
+template <bool MLP, int WGS>
+__global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
+    constexpr int R = kChunk / kBlock;
+    __shared__ float4 sg[kTileCap];
     __shared__ int snbox[4][6];
-    __shared__ float snrow[4];
-    __shared__ double sdiss[4];
-    __shared__ float sred[4][2];
+    __shared__ float sred[4][4];
+    __shared__ Fix128 sdiss[4];
     __shared__ unsigned long long sfail[4];
     __shared__ int sesc[4];
+    if (B.ktime) kernel_start(B.ktime + 2);
     const int4 item = B.work[blockIdx.x];
     const int inst = item.x;
     InstState& is = B.is[inst];
@@ -798,46 +963,28 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
     const float* __restrict__ shared = B.shared[B.sh];
     const size_t noff = (size_t)ip.boff * 64;
     const float* __restrict__ gv = B.gv + noff;
-    GridPtrs out = grid_set(B, is.gcur ^ 1);
-    out.m += noff;
-    out.px += noff;
-    out.py += noff;
-    out.pz += noff;
-    out.fx += noff;
-    out.fy += noff;
-    out.fz += noff;
-    out.flag += ip.boff;
     int* nw_list = B.nw_list[is.cur ^ 1] + ip.poff;
     int* nw_count = &is.nw_count[is.cur ^ 1];
 
-    // The committed slab's set-up record gives the tiles: the gather tile spans
-    // the stencil bases of the item's particles, the scatter tile one cell more
-    // on each side (an accepted step moves a particle by at most one cell).
-    const ItemPre* __restrict__ pre = B.wpre[is.cur] + blockIdx.x;
-    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    // The committed slab's set-up record: the box of the stencil bases of the
+    // item's particles, whose nodes the gather tile holds.
+    const ItemPre pre = B.wpre[is.cur][blockIdx.x];
     Tile t;
-    bool stiled;
     {
         bool ok = true;
-        int gsz = 1, ssz = 1;
+        int gsz = 1;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) ok = ok && pre->lo[a] <= pre->hi[a];
+        for (int a = 0; a < 3; ++a) ok = ok && pre.lo[a] <= pre.hi[a];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            t.org[a] = ok ? pre->lo[a] : 0;
-            t.gd[a] = ok ? pre->hi[a] - pre->lo[a] + 3 : 0;
-            t.sd[a] = t.gd[a] + 2;
+            t.org[a] = ok ? pre.lo[a] : 0;
+            t.gd[a] = ok ? pre.hi[a] - pre.lo[a] + 3 : 0;
             gsz *= t.gd[a];
-            ssz *= t.sd[a];
         }
-        t.tiled = ok && gsz <= GCAP;
-        stiled = t.tiled && ssz <= SCAP;
+        t.tiled = ok && gsz <= kTileCap;
     }
-    const int so[3] = {t.org[0] - 1, t.org[1] - 1, t.org[2] - 1};
-    const int sd[3] = {t.sd[0], t.sd[1], t.sd[2]};
-    const int ssz = sd[0] * sd[1] * sd[2];
 
-    // Phase A: the current positions (and the order keys at the sort cadence).
+    // Phase A: the current positions and the gather tile.
     float px[R], py[R], pz[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -847,12 +994,8 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             px[r] = src[PX * cap + p];
             py[r] = src[PY * cap + p];
             pz[r] = src[PZ * cap + p];
-            // The committed step hit the 32-step cadence: order keys of the committed positions.
-            if (is.refsort) B.refkey[p] = (ref_cell(ip, px[r], py[r], pz[r]) << 32) | (uint64_t)B.ids[p];
         }
     }
-    // Gather tile, and the largest node speed per axis for the momentum scales.
-    float bnd[3] = {0.0f, 0.0f, 0.0f};
     if (t.tiled) {
         int gsz = t.gd[0] * t.gd[1] * t.gd[2];
         const float ig2 = 1.0f / (float)t.gd[2], ig1 = 1.0f / (float)t.gd[1];
@@ -860,65 +1003,20 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             const int rest = div_small(q, ig2), k = q - rest * t.gd[2];
             const int i = div_small(rest, ig1), j = rest - i * t.gd[1];
             int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
-            const float4 g = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
-            sg[q] = g;
-            bnd[0] = fmaxf(bnd[0], fabsf(g.x));
-            bnd[1] = fmaxf(bnd[1], fabsf(g.y));
-            bnd[2] = fmaxf(bnd[2], fabsf(g.z));
+            sg[q] = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
         }
     }
-    if (stiled)
-        // The whole tile region, 16 bytes per store: no division by the tile size per word.
-        for (int q = tid; q < 7 * SCAP / 2; q += kBlock) reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        bnd[i] = wave_fmax_uniform(bnd[i]);
-        if (lane == 0) sbound[wave][i] = bnd[i];
-    }
-    const float wave_rows = pre->rows[wv];  // this wavefront's committed stress bound
     __syncthreads();
-    // Deposit scales.  The new velocity is a convex combination of the tile's
-    // node velocities and |C_ab| dx <= 2 max |v_a| (sum_i w_i |d_i| <= 1/2 cell),
-    // so a momentum bound is at most 10 m max|v_a|; the stress bound allows the
-    // committed stresses to grow eightfold in the step.  A workgroup whose
-    // committed stresses are all zero (a state at rest) takes an estimate of
-    // what the step's velocity gradient can build instead: elastic moduli times
-    // dt |C| for the paste, the pressure and viscous response for the liquid.
-    TileScale tsc;
-    {
-        float b[6];
-        b[0] = pre->m;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            b[1 + i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
-                                                     int, fmaxf(fmaxf(sbound[0][i], sbound[1][i]),
-                                                                fmaxf(sbound[2][i], sbound[3][i])))));
-        b[4] = fmaxf(fmaxf(pre->rows[0], pre->rows[1]), fmaxf(pre->rows[2], pre->rows[3]));
-        b[5] = pre->v0;
-        tile_scale(b[0], tsc.sm, tsc.lm);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) tile_scale(10.25f * b[0] * b[1 + a], tsc.sp[a], tsc.lp[a]);
-        float rows = b[4];
-        if (!(rows > 0.0f)) {
-            const float cmax = 2.0f * fmaxf(fmaxf(b[1], b[2]), b[3]) * P.inv_dx;
-            const float rate = P.kind == 0 ? 3.0f * (2.0f * P.G + 3.0f * fabsf(P.lam)) * dt
-                                           : 9.0f * P.bulk * dt + 15.0f * P.visc;
-            rows = b[5] * rate * cmax;
-        }
-        tile_scale(12.0f * P.dx * rows, tsc.sf, tsc.lf);
-    }
 
-    // Phase B: per particle, G2P, advection, constitutive update, writes to the
-    // trial slab, and the deposit of the updated particle.
-    float speed = 0.0f, disp = 0.0f;
-    double diss = 0.0;
+    // Phase B: per particle, G2P, advection, constitutive update and writes to
+    // the trial slab.
+    float speed = 0.0f, disp = 0.0f, bnd1 = 0.0f, bnd2 = 0.0f;
+    Fix128 diss = {0ull, 0ll};
     unsigned long long fail = ~0ull;
     int escaped = 0;
     unsigned long long mlp_tried = 0, mlp_taken = 0;
-    const int rot = (lane * 5) % 9;
     // The trial slab's set-up record, gathered round by round as uniform values.
     int nlo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    float nrows = 0.0f;
     unsigned nwbits = 0u;  // near-wall flags of the thread's updated particles, bit r: round r
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
@@ -929,14 +1027,14 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         int rc = F_OK;
         const float x = px[r], y = py[r], z = pz[r];
         float v[3] = {0.0f, 0.0f, 0.0f}, C[9], Ht[9], tau[6];
-        bool dep = false;  // the updated particle deposits (stencil s2)
+        bool dep = false;  // the updated particle lies inside the padded grid (stencil s2)
         Stencil s2;
         float jn = 0.0f;
         PasteTrial ts;
         // The particle's remaining slab reads, issued before the gather so that
         // their memory latency overlaps it (the barrier keeps the scheduler from
         // sinking them to their first use).
-        float Hp[9], pm = 0.0f, pv0 = 0.0f, pe = 0.0f, prow = 0.0f;
+        float Hp[9], pm = 0.0f, pv0 = 0.0f, pe = 0.0f;
         if (p < chunk_end) {
             pm = shared[SMASS * cap + p];
             pv0 = shared[SVOL0 * cap + p];
@@ -952,9 +1050,6 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             if (!make_stencil(P, x, y, z, s)) {
                 // Outside the padded grid: frozen for this trial (solver.rs:905-917).
                 for (int f = 0; f < NPF; ++f) dst[f * cap + p] = src[f * cap + p];
-                float tf[6];
-                for (int i = 0; i < 6; ++i) tf[i] = src[(TXX + i) * cap + p];
-                prow = stress_rows(pv0, tf);
                 escaped += 1;
             } else {
                 live = true;
@@ -972,7 +1067,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 const float k4 = 4.0f * P.inv_dx;
 #pragma unroll
                 for (int i = 0; i < 9; ++i) C[i] = Bm[i] * k4;
-                float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                float vn = particle_speed(v);
                 speed = fmaxf(speed, vn);
                 disp = fmaxf(disp, vn * dt * P.inv_dx);
                 if (P.kind == 0) {
@@ -1049,7 +1144,7 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             }
         }
         // Finish: exact return where the surrogate was not used or not accepted,
-        // trial-slab writes and the deposit.
+        // and the trial-slab writes.
         if (live) {
             if (P.kind == 0) {
                 float inc = 0.0f, Hn[9];
@@ -1065,14 +1160,14 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 if (rc == F_OK) {
 #pragma unroll
                     for (int i = 0; i < 9; ++i) dst[(H00 + i) * cap + p] = Hn[i];
-                    diss += (double)(q * inc) * (double)pv0;
+                    fix_add(diss, fix_from((double)(q * inc) * (double)pv0));
                     dst[EPS * cap + p] = pe + inc;
                 }
             } else if (rc == F_OK) {
                 dst[JM1 * cap + p] = jn;
             }
             if (rc != F_OK) {
-                unsigned long long f = ((unsigned long long)(p - ip.poff) << 8) | (unsigned long long)rc;
+                unsigned long long f = ((unsigned long long)B.ids[p] << 8) | (unsigned long long)rc;
                 fail = f < fail ? f : fail;
             } else {
                 const float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
@@ -1086,7 +1181,10 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
-                prow = stress_rows(pv0, tau);
+                float b1, b2;
+                particle_bounds(P, pm, pv0, v, C, tau, b1, b2);
+                bnd1 = fmaxf(bnd1, b1);
+                bnd2 = fmaxf(bnd2, b2);
                 if (make_stencil(P, nx, ny, nz, s2)) {
                     dep = true;
                     nw = near_wall(P, nx, ny, nz);
@@ -1095,34 +1193,12 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
                 }
             }
         }
-        // Deposit.  The stress scale derives from the committed stresses of the
-        // workgroup, which the new ones need not follow (a state may store any
-        // Kirchhoff stress): a wavefront whose new stresses all lie more than
-        // 2^20 below its own committed ones deposits through the global path, at
-        // full f32 resolution.
-        {
-            bool in_tile = dep && stiled;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int li = s2.base[a] - so[a];
-                in_tile = in_tile && li >= 0 && li + 2 < sd[a];
-            }
-            float rf = 0.0f;
-            const bool fits = in_tile && tile_fits(tsc, P, pm, v, C, tau, pv0, rf);
-            const float rfw = wave_fmax_uniform(fits ? rf : 0.0f);  // DPP: no LDS permutes next to the deposit
-            const bool coarse = rfw > 0.0f && rfw < wave_rows * (12.0f * 0x1p-20f) * P.dx;
-            if (fits && !coarse)
-                scatter_lds<SCAP>(ss, so, sd, P, s2, tsc, pm, v, C, tau, pv0, rot);
-            else if (dep)
-                scatter_particle(P, out, s2, pm, v[0], v[1], v[2], C, tau, pv0);
-        }
         nwbits |= nw ? (1u << r) : 0u;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             nlo[a] = min(nlo[a], wave_min_uniform(dep ? s2.base[a] : INT_MAX));
             nhi[a] = max(nhi[a], wave_max_uniform(dep ? s2.base[a] : INT_MIN));
         }
-        nrows = fmaxf(nrows, wave_fmax_uniform(prow));
         // Log of exact returns (training data of the surrogate).
         if (B.rlog) {
             const bool lg = live && P.kind == 0 && yielded && rc == F_OK;
@@ -1168,39 +1244,20 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
         }
     }
     block_append_rounds<R>(nwbits, chunk0, nw_list, nw_count);
-    // Flush the tile: one global atomic per touched node and field, each sum
-    // converted exactly to f64 (|sum| < 2^40) and scaled back by 2^-k.
-    if (stiled) {
-        const double um = 1.0 / tsc.sm, ux = 1.0 / tsc.sp[0], uy = 1.0 / tsc.sp[1], uz = 1.0 / tsc.sp[2],
-                     uf = 1.0 / tsc.sf;
-        const float is2 = 1.0f / (float)sd[2], is1 = 1.0f / (float)sd[1];
-        for (int q = tid; q < ssz; q += kBlock) {
-            const long long qm = (long long)ss[q];
-            if (qm == 0) continue;
-            const int rest = div_small(q, is2), k = q - rest * sd[2];
-            const int i = div_small(rest, is1), j = rest - i * sd[1];
-            int ni = node_index(P, so[0] + i, so[1] + j, so[2] + k);
-            atomicAdd(out.m + ni, (float)((double)qm * um));
-            atomicAdd(out.px + ni, (float)((double)(long long)ss[1 * SCAP + q] * ux));
-            atomicAdd(out.py + ni, (float)((double)(long long)ss[2 * SCAP + q] * uy));
-            atomicAdd(out.pz + ni, (float)((double)(long long)ss[3 * SCAP + q] * uz));
-            atomicAdd(out.fx + ni, (float)((double)(long long)ss[4 * SCAP + q] * uf));
-            atomicAdd(out.fy + ni, (float)((double)(long long)ss[5 * SCAP + q] * uf));
-            atomicAdd(out.fz + ni, (float)((double)(long long)ss[6 * SCAP + q] * uf));
-            out.flag[ni >> 6] = 1;
-        }
-    }
     // Per-work-item partials of the step reductions.
-    // Maxima and the dissipation sum by DPP (VALU), not LDS permutes.
     speed = wave_fmax_uniform(speed);
     disp = wave_fmax_uniform(disp);
-    diss = wave_sum_last(diss);
+    bnd1 = wave_fmax_uniform(bnd1);
+    bnd2 = wave_fmax_uniform(bnd2);
+    diss = wave_fix_sum(diss);
     fail = wave_min_u64(fail);
     escaped = wave_sum_i(escaped);
-    if (lane == 63) sdiss[wave] = diss;
     if (lane == 0) {
+        sdiss[wave] = diss;
         sred[wave][0] = speed;
         sred[wave][1] = disp;
+        sred[wave][2] = bnd1;
+        sred[wave][3] = bnd2;
         sfail[wave] = fail;
         sesc[wave] = escaped;
 #pragma unroll
@@ -1208,7 +1265,6 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             snbox[wave][a] = nlo[a];
             snbox[wave][3 + a] = nhi[a];
         }
-        snrow[wave] = nrows;
     }
     __syncthreads();
     if (tid == 0) {
@@ -1218,27 +1274,23 @@ __global__ __launch_bounds__(kBlock, 2) void g2p2g_kernel(BatchDev B) {
             rec.lo[a] = min(min(snbox[0][a], snbox[1][a]), min(snbox[2][a], snbox[3][a]));
             rec.hi[a] = max(max(snbox[0][3 + a], snbox[1][3 + a]), max(snbox[2][3 + a], snbox[3][3 + a]));
         }
-        rec.m = pre->m;
-        rec.v0 = pre->v0;
-        for (int w = 0; w < 4; ++w) rec.rows[w] = snrow[w];
         B.wpre[is.cur ^ 1][blockIdx.x] = rec;
-        float s0 = 0.0f, s1 = 0.0f;
-        double d = 0.0;
+        float red[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        Fix128 d = {0ull, 0ll};
         unsigned long long f = ~0ull;
         int e = 0;
         for (int w = 0; w < 4; ++w) {
-            s0 = fmaxf(s0, sred[w][0]);
-            s1 = fmaxf(s1, sred[w][1]);
-            d += sdiss[w];
+            for (int k = 0; k < 4; ++k) red[k] = fmaxf(red[k], sred[w][k]);
+            fix_add(d, sdiss[w]);
             f = sfail[w] < f ? sfail[w] : f;
             e += sesc[w];
         }
-        B.pmax[2 * blockIdx.x] = s0;
-        B.pmax[2 * blockIdx.x + 1] = s1;
+        for (int k = 0; k < 4; ++k) B.pmax[4 * blockIdx.x + k] = red[k];
         B.pdiss[blockIdx.x] = d;
         B.pfail[blockIdx.x] = f;
         B.pesc[blockIdx.x] = e;
     }
+    if (B.ktime) kernel_end(B.ktime + 3);
 }
 
 // ------------------------------------------------------------ reductions and commit
@@ -1264,6 +1316,9 @@ __device__ inline void commit_instance(InstState& s, const InstParams& ip, const
         code = F_DISPLACEMENT;
     s.last_failure = code;
     s.refsort = 0;
+    // Every trial deposited into its own accumulator set; the next one takes
+    // the other (cleared by this trial's grid update).
+    s.gcur ^= 1;
     const bool trial_only = s.trial_dt > 0.0;
     if (code) {
         if (trial_only) {
@@ -1281,8 +1336,9 @@ __device__ inline void commit_instance(InstState& s, const InstParams& ip, const
     }
     const double dt = s.dt;
     s.cur ^= 1;
-    s.gcur ^= 1;
     s.committed = 1;
+    s.bound[0] = st.bound[0];
+    s.bound[1] = st.bound[1];
     // commit, solver.rs:992-997
     L[27] += st.dissipation;
     for (int a = 0; a < 3; ++a) L[9 + a] += st.traction_impulse[a] + st.wall_impulse[a];
@@ -1370,30 +1426,32 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
             bad += xb[u];
         }
     }
-    float smax = 0.0f, dmax = 0.0f;
+    float pmx[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // speed, displacement, deposit bounds
+    Fix128 diss = {0ull, 0ll};
     unsigned long long f = ~0ull;
     int esc = 0;
     const int w1 = s.w_off + s.w_cnt;
     for (int q0 = s.w_off + tid; q0 < w1; q0 += 8 * 256) {
-        float xs[8], xd[8];
-        double xe[8];
+        float4 xs[8];
+        Fix128 xe[8];
         unsigned long long xf[8];
         int xc[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int q = q0 + u * 256;
             const bool in = q < w1;
-            xs[u] = in ? B.pmax[2 * q] : 0.0f;
-            xd[u] = in ? B.pmax[2 * q + 1] : 0.0f;
-            xe[u] = in ? B.pdiss[q] : 0.0;
+            xs[u] = in ? reinterpret_cast<const float4*>(B.pmax)[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            xe[u] = in ? B.pdiss[q] : Fix128{0ull, 0ll};
             xf[u] = in ? B.pfail[q] : ~0ull;
             xc[u] = in ? B.pesc[q] : 0;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            smax = fmaxf(smax, xs[u]);
-            dmax = fmaxf(dmax, xd[u]);
-            acc[kGridVals] += xe[u];
+            pmx[0] = fmaxf(pmx[0], xs[u].x);
+            pmx[1] = fmaxf(pmx[1], xs[u].y);
+            pmx[2] = fmaxf(pmx[2], xs[u].z);
+            pmx[3] = fmaxf(pmx[3], xs[u].w);
+            fix_add(diss, xe[u]);
             f = xf[u] < f ? xf[u] : f;
             esc += xc[u];
         }
@@ -1410,11 +1468,11 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         pel[0] += B.pel[pi].mass;
         for (int c = 0; c < 3; ++c) {
             pel[1 + c] += t.wall_impulse[c];
-            pel[4 + c] += a.impulse[c];
-            pel[7 + c] += a.angular[c];
+            pel[4 + c] += fix_value(a.sum[c]);
+            pel[7 + c] += fix_value(a.sum[3 + c]);
         }
         pel[10] += t.wall_work;
-        pel[11] += a.grid_energy;
+        pel[11] += fix_value(a.sum[6]);
         pel[12] += t.energy;
         pbad |= t.bad;
         vmax = fmax(vmax, sqrt(t.vel[0] * t.vel[0] + t.vel[1] * t.vel[1] + t.vel[2] * t.vel[2]));
@@ -1424,11 +1482,12 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     // combines the four wavefronts in order.
     const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-    for (int i = 0; i <= kGridVals; ++i) acc[i] = wave_sum_last(acc[i]);
+    for (int i = 0; i < kGridVals; ++i) acc[i] = wave_sum_last(acc[i]);
 #pragma unroll
     for (int i = 0; i < kPelSums; ++i) pel[i] = wave_sum_last(pel[i]);
-    smax = wave_max(smax);
-    dmax = wave_max(dmax);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pmx[k] = wave_max(pmx[k]);
+    diss = wave_fix_sum(diss);
     f = wave_min_u64(f);
     bad = wave_sum_i(bad);
     esc = wave_sum_i(esc);
@@ -1436,19 +1495,20 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     vmax = wave_max_d(vmax);
     __shared__ double ssum[4][kGridVals + 1 + kPelSums];
     __shared__ double svm[4];
-    __shared__ float sfm[4][2];
+    __shared__ float sfm[4][4];
+    __shared__ Fix128 sfx[4];
     __shared__ unsigned long long sfl[4];
     __shared__ int sint[4][3];
     if (lane == 63) {
 #pragma unroll
-        for (int i = 0; i <= kGridVals; ++i) ssum[wave][i] = acc[i];
+        for (int i = 0; i < kGridVals; ++i) ssum[wave][i] = acc[i];
 #pragma unroll
         for (int i = 0; i < kPelSums; ++i) ssum[wave][kGridVals + 1 + i] = pel[i];
     }
     if (lane == 0) {
         svm[wave] = vmax;
-        sfm[wave][0] = smax;
-        sfm[wave][1] = dmax;
+        for (int k = 0; k < 4; ++k) sfm[wave][k] = pmx[k];
+        sfx[wave] = diss;
         sfl[wave] = f;
         sint[wave][0] = bad;
         sint[wave][1] = esc;
@@ -1456,8 +1516,12 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     }
     __syncthreads();
     double tot[kGridVals + 1];
-    if (tid == 0)
-        for (int i = 0; i <= kGridVals; ++i) tot[i] = ((ssum[0][i] + ssum[1][i]) + ssum[2][i]) + ssum[3][i];
+    if (tid == 0) {
+        for (int i = 0; i < kGridVals; ++i) tot[i] = ((ssum[0][i] + ssum[1][i]) + ssum[2][i]) + ssum[3][i];
+        Fix128 d = sfx[0];
+        for (int w = 1; w < 4; ++w) fix_add(d, sfx[w]);
+        tot[kGridVals] = fix_value(d);
+    }
     if (B.audit) {
         // Audit stage sums of this instance, in the same fixed order.
         for (int i = 0; i < kAuditVals; ++i) {
@@ -1485,6 +1549,8 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         st.dissipation = tot[kGridVals];
         st.max_speed = fmaxf(fmaxf(sfm[0][0], sfm[1][0]), fmaxf(sfm[2][0], sfm[3][0]));
         st.max_disp = fmaxf(fmaxf(sfm[0][1], sfm[1][1]), fmaxf(sfm[2][1], sfm[3][1]));
+        st.bound[0] = fmaxf(fmaxf(sfm[0][2], sfm[1][2]), fmaxf(sfm[2][2], sfm[3][2]));
+        st.bound[1] = fmaxf(fmaxf(sfm[0][3], sfm[1][3]), fmaxf(sfm[2][3], sfm[3][3]));
         unsigned long long fl = sfl[0];
         for (int w = 1; w < 4; ++w) fl = sfl[w] < fl ? sfl[w] : fl;
         st.fail = fl;
@@ -1527,180 +1593,30 @@ __global__ __launch_bounds__(kBlock) void refkey_kernel(BatchDev B) {
                       (uint64_t)B.ids[p];
 }
 
-// ------------------------------------------------------------ setup kernels
+// ------------------------------------------------------------ set-up
 
-// deposit_particles (solver.rs:632-661) of one instance's committed state into
-// accumulator set `gset`, for a state that was set from outside (the reference
-// rebuilds the grid from the particles at the start of a step).  One workgroup
-// per work item, as in the fused kernel: the particles' own values give exact
-// deposit scales, so every particle whose stencil stays in the tile deposits
-// there; the tile is flushed with one global atomic per touched node and field.
-template <int CHUNK, int SCAP>
-__global__ __launch_bounds__(kBlock, 2) void setup_deposit_kernel(BatchDev B, int inst, int gset) {
-    constexpr int R = CHUNK / kBlock;
-    __shared__ __align__(16) unsigned long long ss[7 * SCAP];
+// Set-up of every instance's committed slab after a state upload or a sort,
+// one workgroup per work item: the near-wall list, the item's set-up record
+// (ItemPre), and the largest deposit bounds and speed of the particles inside
+// the padded grid, with the same arithmetic as the trial step that writes a
+// state (particle_bounds, particle_speed), so an uploaded state is stepped
+// exactly as the same state reached by stepping.  The maxima are atomic
+// maxima of non-negative floats as integers; the caller clears them and the
+// list counters first.
+__global__ __launch_bounds__(kBlock) void setup_kernel(BatchDev B) {
     __shared__ int sbox[4][6];
-    __shared__ float sbound[4][5];
-    __shared__ int sgeo[7];
-    const int4 item = B.work[blockIdx.x];
-    if (item.x != inst) return;
-    const InstState& is = B.is[inst];
-    const InstParams& ip = B.ip[inst];
-    const Params P = ip.P;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const size_t cap = B.cap;
-    const float* __restrict__ src = B.slab[is.cur];
-    const float* __restrict__ shared = B.shared[B.sh];
-    const size_t noff = (size_t)ip.boff * 64;
-    GridPtrs out = grid_set(B, gset);
-    out.m += noff;
-    out.px += noff;
-    out.py += noff;
-    out.pz += noff;
-    out.fx += noff;
-    out.fy += noff;
-    out.fz += noff;
-    out.flag += ip.boff;
-    const int end = item.y + item.z;
-
-    // Stencil-base box and the exact bounds tile_fits compares.
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    float bnd[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int p = item.y + r * kBlock + tid;
-        if (p >= end) continue;
-        Stencil s;
-        if (!make_stencil(P, src[PX * cap + p], src[PY * cap + p], src[PZ * cap + p], s)) continue;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = min(lo[a], s.base[a]);
-            hi[a] = max(hi[a], s.base[a]);
-        }
-        const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
-        const float v[3] = {src[VX * cap + p], src[VY * cap + p], src[VZ * cap + p]};
-        float C[9], tau[6];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) C[i] = src[(C00 + i) * cap + p];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) tau[i] = src[(TXX + i) * cap + p];
-        const float mdx = m * P.dx;
-        bnd[0] = fmaxf(bnd[0], m);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            bnd[1 + a] = fmaxf(bnd[1 + a], fabsf(m * v[a]) + 1.5f * (fabsf(C[3 * a] * mdx) + fabsf(C[3 * a + 1] * mdx) +
-                                                                   fabsf(C[3 * a + 2] * mdx)));
-        const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]),
-                    r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
-                    r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
-        bnd[4] = fmaxf(bnd[4], 1.5f * (v0 * P.dx * fmaxf(fmaxf(r0, r1), r2)));
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min_i(lo[a]);
-        hi[a] = wave_max_i(hi[a]);
-    }
-#pragma unroll
-    for (int i = 0; i < 5; ++i) bnd[i] = wave_max(bnd[i]);
-    if (lane == 0) {
-        for (int a = 0; a < 3; ++a) {
-            sbox[wave][a] = lo[a];
-            sbox[wave][3 + a] = hi[a];
-        }
-        for (int i = 0; i < 5; ++i) sbound[wave][i] = bnd[i];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int ssz = 1;
-        bool ok = true;
-        for (int a = 0; a < 3; ++a) {
-            const int L = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
-            const int H = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
-            ok = ok && L <= H;
-            sgeo[a] = ok ? L : 0;
-            sgeo[3 + a] = ok ? H - L + 3 : 0;
-            ssz *= ok ? H - L + 3 : 0;
-        }
-        sgeo[6] = ok && ssz <= SCAP;
-    }
-    __syncthreads();
-    const int so[3] = {sgeo[0], sgeo[1], sgeo[2]};
-    const int sd[3] = {sgeo[3], sgeo[4], sgeo[5]};
-    const bool stiled = sgeo[6] != 0;
-    const int ssz = sd[0] * sd[1] * sd[2];
-    if (stiled)
-        // The whole tile region, 16 bytes per store: no division by the tile size per word.
-        for (int q = tid; q < 7 * SCAP / 2; q += kBlock) reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
-    TileScale tsc;
-    {
-        float b[5];
-#pragma unroll
-        for (int i = 0; i < 5; ++i)
-            b[i] = fmaxf(fmaxf(sbound[0][i], sbound[1][i]), fmaxf(sbound[2][i], sbound[3][i]));
-        tile_scale(b[0], tsc.sm, tsc.lm);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) tile_scale(b[1 + a], tsc.sp[a], tsc.lp[a]);
-        tile_scale(b[4], tsc.sf, tsc.lf);
-    }
-    __syncthreads();
-    const int rot = (lane * 5) % 9;
-#pragma unroll 1
-    for (int r = 0; r < R; ++r) {
-        const int p = item.y + r * kBlock + tid;
-        if (p >= end) continue;
-        Stencil s;
-        if (!make_stencil(P, src[PX * cap + p], src[PY * cap + p], src[PZ * cap + p], s)) continue;
-        const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
-        const float v[3] = {src[VX * cap + p], src[VY * cap + p], src[VZ * cap + p]};
-        float C[9], tau[6];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) C[i] = src[(C00 + i) * cap + p];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) tau[i] = src[(TXX + i) * cap + p];
-        float rf;
-        if (stiled && tile_fits(tsc, P, m, v, C, tau, v0, rf))
-            scatter_lds<SCAP>(ss, so, sd, P, s, tsc, m, v, C, tau, v0, rot);
-        else
-            scatter_particle(P, out, s, m, v[0], v[1], v[2], C, tau, v0);
-    }
-    __syncthreads();
-    if (stiled) {
-        const double um = 1.0 / tsc.sm, ux = 1.0 / tsc.sp[0], uy = 1.0 / tsc.sp[1], uz = 1.0 / tsc.sp[2],
-                     uf = 1.0 / tsc.sf;
-        const float is2 = 1.0f / (float)sd[2], is1 = 1.0f / (float)sd[1];
-        for (int q = tid; q < ssz; q += kBlock) {
-            const long long qm = (long long)ss[q];
-            if (qm == 0) continue;
-            const int rest = div_small(q, is2), k = q - rest * sd[2];
-            const int i = div_small(rest, is1), j = rest - i * sd[1];
-            int ni = node_index(P, so[0] + i, so[1] + j, so[2] + k);
-            atomicAdd(out.m + ni, (float)((double)qm * um));
-            atomicAdd(out.px + ni, (float)((double)(long long)ss[1 * SCAP + q] * ux));
-            atomicAdd(out.py + ni, (float)((double)(long long)ss[2 * SCAP + q] * uy));
-            atomicAdd(out.pz + ni, (float)((double)(long long)ss[3 * SCAP + q] * uz));
-            atomicAdd(out.fx + ni, (float)((double)(long long)ss[4 * SCAP + q] * uf));
-            atomicAdd(out.fy + ni, (float)((double)(long long)ss[5 * SCAP + q] * uf));
-            atomicAdd(out.fz + ni, (float)((double)(long long)ss[6 * SCAP + q] * uf));
-            out.flag[ni >> 6] = 1;
-        }
-    }
-}
-
-// Near-wall lists of the committed slabs and the work items' set-up records
-// (ItemPre) for them, one workgroup per work item.
-__global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
-    __shared__ int sbox[4][6];
-    __shared__ float sbnd[4][3];
     const int4 item = B.work[blockIdx.x];
     InstState& s = B.is[item.x];
     const InstParams& ip = B.ip[item.x];
+    const Params P = ip.P;
     const float* slab = B.slab[s.cur];
     const float* shared = B.shared[B.sh];
+    const size_t cap = B.cap;
     int* list = B.nw_list[s.cur] + ip.poff;
     int* count = &s.nw_count[s.cur];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    float m = 0.0f, v0 = 0.0f, rows = 0.0f;
+    float b1m = 0.0f, b2m = 0.0f, spd = 0.0f;
     // Near-wall flags of the thread's particles (bit r: round r), appended once
     // per workgroup below: one atomic on the instance's counter per work item.
     unsigned nwbits = 0u;
@@ -1709,20 +1625,24 @@ __global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
         const int p = base + threadIdx.x;
         bool nw = false;
         if (p < item.y + item.z) {
-            const float x = slab[PX * B.cap + p], y = slab[PY * B.cap + p], z = slab[PZ * B.cap + p];
-            nw = near_wall(ip.P, x, y, z);
+            const float x = slab[PX * cap + p], y = slab[PY * cap + p], z = slab[PZ * cap + p];
+            nw = near_wall(P, x, y, z);
             Stencil st;
-            if (make_stencil(ip.P, x, y, z, st))
+            if (make_stencil(P, x, y, z, st)) {
                 for (int a = 0; a < 3; ++a) {
                     lo[a] = min(lo[a], st.base[a]);
                     hi[a] = max(hi[a], st.base[a]);
                 }
-            float tau[6];
-            for (int i = 0; i < 6; ++i) tau[i] = slab[(TXX + i) * B.cap + p];
-            const float pv0 = shared[SVOL0 * B.cap + p];
-            m = fmaxf(m, shared[SMASS * B.cap + p]);
-            v0 = fmaxf(v0, pv0);
-            rows = fmaxf(rows, stress_rows(pv0, tau));
+                float v[3], C[9], tau[6];
+                for (int a = 0; a < 3; ++a) v[a] = slab[(VX + a) * cap + p];
+                for (int i = 0; i < 9; ++i) C[i] = slab[(C00 + i) * cap + p];
+                for (int i = 0; i < 6; ++i) tau[i] = slab[(TXX + i) * cap + p];
+                float b1, b2;
+                particle_bounds(P, shared[SMASS * cap + p], shared[SVOL0 * cap + p], v, C, tau, b1, b2);
+                b1m = fmaxf(b1m, b1);
+                b2m = fmaxf(b2m, b2);
+                spd = fmaxf(spd, particle_speed(v));
+            }
         }
         nwbits |= nw ? (1u << r) : 0u;
     }
@@ -1731,17 +1651,17 @@ __global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
         lo[a] = wave_min_uniform(lo[a]);
         hi[a] = wave_max_uniform(hi[a]);
     }
-    m = wave_fmax_uniform(m);
-    v0 = wave_fmax_uniform(v0);
-    rows = wave_fmax_uniform(rows);
+    b1m = wave_fmax_uniform(b1m);
+    b2m = wave_fmax_uniform(b2m);
+    spd = wave_fmax_uniform(spd);
     if (lane == 0) {
         for (int a = 0; a < 3; ++a) {
             sbox[wave][a] = lo[a];
             sbox[wave][3 + a] = hi[a];
         }
-        sbnd[wave][0] = m;
-        sbnd[wave][1] = v0;
-        sbnd[wave][2] = rows;
+        atomicMax(reinterpret_cast<unsigned*>(&s.bound[0]), __float_as_uint(b1m));
+        atomicMax(reinterpret_cast<unsigned*>(&s.bound[1]), __float_as_uint(b2m));
+        atomicMax(reinterpret_cast<unsigned*>(&s.setup_speed), __float_as_uint(spd));
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1750,9 +1670,6 @@ __global__ __launch_bounds__(kBlock) void nearwall_kernel(BatchDev B) {
             r.lo[a] = min(min(sbox[0][a], sbox[1][a]), min(sbox[2][a], sbox[3][a]));
             r.hi[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
         }
-        r.m = fmaxf(fmaxf(sbnd[0][0], sbnd[1][0]), fmaxf(sbnd[2][0], sbnd[3][0]));
-        r.v0 = fmaxf(fmaxf(sbnd[0][1], sbnd[1][1]), fmaxf(sbnd[2][1], sbnd[3][1]));
-        for (int w = 0; w < 4; ++w) r.rows[w] = sbnd[w][2];
         B.wpre[s.cur][blockIdx.x] = r;
     }
 }
@@ -1841,25 +1758,30 @@ inline int blocks_for(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
 
-int g2p2g_chunk() { return kChunk; }
+int g2p_chunk() { return kChunk; }
 
-void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, hipEvent_t g2p2g_start,
-                 hipEvent_t g2p2g_stop) {
+void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets) {
     control_kernel<<<1, 256, 0, s>>>(B);
+    if (B.nwork > 0) {
+        if (B.apart)
+            p2g_kernel<kAuditFields><<<B.nwork, kBlock, 0, s>>>(B);
+        else
+            p2g_kernel<kDepositFields><<<B.nwork, kBlock, 0, s>>>(B);
+    }
     traction_kernel<<<traction_wgs, kBlock, 0, s>>>(B);
     if (B.apart)
         grid_update_kernel<true><<<B.ngwg, 256, 0, s>>>(B);
     else
         grid_update_kernel<false><<<B.ngwg, 256, 0, s>>>(B);
     if (pellets) couple_kernel<<<B.ngwg * kGridWgBlocks, 64, 0, s>>>(B);
-    if (g2p2g_start) (void)hipEventRecord(g2p2g_start, s);
     if (B.nwork > 0) {
         if (B.mlp_fwd)
-            g2p2g_kernel<kChunk, kGatherCap, kScatterCap, true><<<B.nwork, kBlock, 0, s>>>(B);
+            g2p_kernel<true, 2><<<B.nwork, kBlock, 0, s>>>(B);
+        else if (B.g2p_occupancy >= 3)
+            g2p_kernel<false, 3><<<B.nwork, kBlock, 0, s>>>(B);
         else
-            g2p2g_kernel<kChunk, kGatherCap, kScatterCap, false><<<B.nwork, kBlock, 0, s>>>(B);
+            g2p_kernel<false, 2><<<B.nwork, kBlock, 0, s>>>(B);
     }
-    if (g2p2g_stop) (void)hipEventRecord(g2p2g_stop, s);
     if (pellets) pellet_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
     finalize_kernel<<<B.ninst, 256, 0, s>>>(B);
     if (pellets) pellet_commit_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
@@ -1869,12 +1791,8 @@ void launch_refkeys(hipStream_t s, const BatchDev& B) {
     if (B.nwork > 0) refkey_kernel<<<B.nwork, kBlock, 0, s>>>(B);
 }
 
-void launch_p2g(hipStream_t s, const BatchDev& B, int inst, int gcur) {
-    if (B.nwork > 0) setup_deposit_kernel<kChunk, kScatterCap><<<B.nwork, kBlock, 0, s>>>(B, inst, gcur);
-}
-
-void launch_nearwall(hipStream_t s, const BatchDev& B) {
-    if (B.nwork > 0) nearwall_kernel<<<B.nwork, kBlock, 0, s>>>(B);
+void launch_setup(hipStream_t s, const BatchDev& B) {
+    if (B.nwork > 0) setup_kernel<<<B.nwork, kBlock, 0, s>>>(B);
 }
 
 void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t* keys, uint32_t* order) {

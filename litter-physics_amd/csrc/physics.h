@@ -1,8 +1,8 @@
 // Device-side physics of the MLS-MPM substep, shared by the kernels:
 // stencils (grid.rs:151-183), the constitutive updates (constitutive.rs:280-589),
-// the wall velocity law (solver.rs:1145-1155) and the gather/scatter helpers
-// of the fused grid-to-particle-to-grid kernel.  f32 throughout, f64 where a
-// reduction needs it, exact fixed point in the LDS deposit tile.
+// the wall velocity law (solver.rs:1145-1155), the deposit and gather helpers
+// and exact fixed-point sums.  f32 throughout, f64 where a reduction needs it,
+// exact integers for the deposit and for the sums whose order varies.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -74,58 +74,6 @@ __device__ inline unsigned long long wave_min_u64(unsigned long long v) {
 }
 
 __device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
-
-// ------------------------------------------------------------ scatter
-
-// deposit_particles (solver.rs:632-661) for one particle: mass w m, momentum
-// w (m v + m C d) and stress w V0 tau d, d = x_node - x_p in metres.
-__device__ inline void scatter_particle(const Params& P, GridPtrs g, const Stencil& s, float m, float vx, float vy,
-                                        float vz, const float* C, const float* tau, float v0) {
-    // Precompute m C dx and V0 tau dx (d in cells times dx is metres).
-    float mc[9], vt[6];
-    float mdx = m * P.dx, vdx = v0 * P.dx;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
-    float mvx = m * vx, mvy = m * vy, mvz = m * vz;
-    // Block flags of the (up to 8) node blocks the stencil touches.
-    int b0[3], b1[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        b0[a] = s.base[a] >> 2;
-        b1[a] = (s.base[a] + 2) >> 2;
-    }
-    for (int bi = b0[0]; bi <= b1[0]; ++bi)
-        for (int bj = b0[1]; bj <= b1[1]; ++bj)
-            for (int bk = b0[2]; bk <= b1[2]; ++bk) g.flag[(bi * P.nb[1] + bj) * P.nb[2] + bk] = 1;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            float wab = s.w[0][a] * s.w[1][b];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float w = wab * s.w[2][c];
-                float dx = s.d[0][a], dy = s.d[1][b], dz = s.d[2][c];
-                int n = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
-                float px = w * (mvx + mc[0] * dx + mc[1] * dy + mc[2] * dz);
-                float py = w * (mvy + mc[3] * dx + mc[4] * dy + mc[5] * dz);
-                float pz = w * (mvz + mc[6] * dx + mc[7] * dy + mc[8] * dz);
-                float fx = w * (vt[0] * dx + vt[3] * dy + vt[4] * dz);
-                float fy = w * (vt[3] * dx + vt[1] * dy + vt[5] * dz);
-                float fz = w * (vt[4] * dx + vt[5] * dy + vt[2] * dz);
-                atomicAdd(g.m + n, w * m);
-                atomicAdd(g.px + n, px);
-                atomicAdd(g.py + n, py);
-                atomicAdd(g.pz + n, pz);
-                atomicAdd(g.fx + n, fx);
-                atomicAdd(g.fy + n, fy);
-                atomicAdd(g.fz + n, fz);
-            }
-        }
-    }
-}
 
 // ------------------------------------------------------------ constitutive
 
@@ -596,14 +544,6 @@ __device__ inline float wave_fmax_uniform(float v) {
 // Three VALU instructions instead of an integer division sequence.
 __device__ inline int div_small(int q, float inv) { return (int)(((float)q + 0.5f) * inv); }
 
-// V0 max_a sum_b |tau_ab| of a symmetric stress (xx, yy, zz, xy, xz, yz): the
-// per-particle quantity the deposit's stress scale bounds.
-__device__ inline float stress_rows(float v0, const float* tau) {
-    const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]), r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
-                r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
-    return v0 * fmaxf(fmaxf(r0, r1), r2);
-}
-
 // Block-wide sum of NV doubles held by every thread; thread 0 gets the totals.
 template <int NV>
 __device__ inline void block_sum(double* vals, double* scratch) {
@@ -624,15 +564,13 @@ __device__ inline void block_sum(double* vals, double* scratch) {
     }
 }
 
-// ------------------------------------------------------------ fused G2P2G
+// ------------------------------------------------------------ tiles
 
-// Tile geometry of one workgroup's chunk of particles.  Gather tile: the nodes
-// of every current stencil; scatter tile: one cell wider on each side, since an
-// accepted step moves a particle by at most one cell.
+// LDS tile of one work item: the nodes of the stencils of its particles (the
+// box of their stencil bases from the set-up record, plus two nodes).
 struct Tile {
-    int org[3];  // gather origin (lowest stencil base)
-    int gd[3];   // gather dims
-    int sd[3];   // scatter dims (origin org - 1)
+    int org[3];  // origin (lowest stencil base)
+    int gd[3];   // dims
     int tiled;
 };
 
@@ -690,54 +628,140 @@ __device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil
 // would place the stencil in scratch memory).
 __device__ inline float sel3(int i, float x0, float x1, float x2) { return i == 0 ? x0 : (i == 1 ? x1 : x2); }
 
-// Fixed-point deposit tile.  Each of the seven fields (m, the three APIC
-// momentum sums and the three stress sums) accumulates in LDS as a 64-bit
-// integer count of 2^-k units, with k chosen per workgroup and field so that
-// no admitted contribution exceeds 2^30 units: the at most 1024 contributions a
-// node receives from one workgroup sum below 2^40, so the tile sums are exact
-// and independent of the order of the atomics.  CDNA4's LDS integer atomics
-// also run well ahead of its f64 ones under the address conflicts of a deposit.
-// A particle whose bound exceeds what the scales admit (or is not finite)
-// deposits with global atomics instead.
-struct TileScale {
-    float sm, sp[3], sf;  // 2^k of the mass, the momentum components and the three stress sums
-    float lm, lp[3], lf;  // largest particle bound (tile_fits) each scale admits; 0 admits only zero
-};
+// ------------------------------------------------------------ exact sums
+
+// Exact fixed point (Fix128, units of 2^-kFixBits) of a finite double with
+// |x| < 2^27, truncated toward zero below the unit.
+__device__ inline Fix128 fix_from(double x) {
+    const bool neg = x < 0.0;
+    const double a = fabs(x) * 0x1p100;  // kFixBits
+    const double hd = floor(a * 0x1p-64);
+    const double ld = a - hd * 0x1p64;  // a mod 2^64: a subset of a's mantissa bits, exact
+    unsigned long long lo = (unsigned long long)ld, hi = (unsigned long long)hd;
+    if (neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+    }
+    return Fix128{lo, (long long)hi};
+}
+
+__device__ inline void fix_add(Fix128& a, const Fix128& b) {
+    const unsigned long long lo = a.lo + b.lo;
+    a.hi += b.hi + (lo < a.lo ? 1ll : 0ll);
+    a.lo = lo;
+}
+
+// The double nearest the fixed-point value (two roundings, a function of the
+// integer alone).
+__device__ inline double fix_value(const Fix128& a) {
+    const bool neg = a.hi < 0;
+    unsigned long long lo = a.lo, hi = (unsigned long long)a.hi;
+    if (neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+    }
+    const double v = ((double)hi * 0x1p64 + (double)lo) * 0x1p-100;
+    return neg ? -v : v;
+}
+
+// 128-bit atomic add from two 64-bit ones: the low word's wrap-arounds reach
+// the high word as carries, each counted once, so the total is exact in any order.
+__device__ inline void fix_atomic_add(Fix128* dst, const Fix128& b) {
+    const unsigned long long old = atomicAdd(&dst->lo, b.lo);
+    const unsigned long long carry = old + b.lo < old ? 1ull : 0ull;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&dst->hi), (unsigned long long)b.hi + carry);
+}
+
+// Sum over the wavefront (every lane gets it).
+__device__ inline Fix128 wave_fix_sum(Fix128 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        Fix128 w;
+        w.lo = __shfl_xor(v.lo, o, 64);
+        w.hi = __shfl_xor(v.hi, o, 64);
+        fix_add(v, w);
+    }
+    return v;
+}
+
+// ------------------------------------------------------------ deposit
 
 // Largest weight of the quadratic B-spline stencil: 0.75^3.
 constexpr float kMaxWeight = 0.421875f;
 
-// 2^k with k such that kMaxWeight * bound * 2^k < 2^30 (so no contribution of a
-// particle within the bound reaches 2^30 units), and the bound it admits.
-__device__ inline void tile_scale(float bound, float& s, float& lim) {
-    if (!(bound > 0.0f) || !(bound <= FLT_MAX)) {
-        s = 1.0f;
-        lim = 0.0f;
-        return;
-    }
-    int e;
-    frexpf(kMaxWeight * bound, &e);  // kMaxWeight * bound < 2^e
-    s = ldexpf(1.0f, min(30 - e, 126));
-    lim = bound;
+// Per-particle bounds of the deposit (solver.rs:643-655): every momentum
+// contribution w (m v_a + sum_b A_ab d_b), A = m C - dt (4/dx^2) V0 tau and
+// |d_b| <= 1.5 dx, is at most kMaxWeight (b1 + dt b2) with
+//   b1 = m (max_a |v_a| + 1.5 dx max_a sum_b |C_ab|),
+//   b2 = (6/dx) V0 max_a sum_b |tau_ab|,
+// and every wall-traction reaction at most kMaxWeight dt (b2 + 7 m |g|).  No
+// contraction: the trial step that writes a state and the set-up kernel that
+// rereads it get the same bits.
+__device__ inline void particle_bounds(const Params& P, float m, float v0, const float* v, const float* C,
+                                       const float* tau, float& b1, float& b2) {
+#pragma clang fp contract(off)
+    const float vm = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fabsf(v[2]));
+    float cm = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) cm = fmaxf(cm, (fabsf(C[3 * a]) + fabsf(C[3 * a + 1])) + fabsf(C[3 * a + 2]));
+    const float r0 = (fabsf(tau[0]) + fabsf(tau[3])) + fabsf(tau[4]),
+                r1 = (fabsf(tau[3]) + fabsf(tau[1])) + fabsf(tau[5]),
+                r2 = (fabsf(tau[4]) + fabsf(tau[5])) + fabsf(tau[2]);
+    b1 = m * (vm + (1.5f * P.dx) * cm);
+    b2 = (6.0f * P.inv_dx) * (v0 * fmaxf(fmaxf(r0, r1), r2));
 }
 
-// The quantities tile_fits compares with TileScale's limits: m; per momentum
-// component m |v_a| + 1.5 sum_b |m dx C_ab| (|node - particle| <= 1.5 cells per
-// axis); rf = 1.5 v0 dx max_a sum_b |tau_ab| for the stress sums.  Each stencil
-// contribution is at most kMaxWeight times its bound.
-__device__ inline bool tile_fits(const TileScale& ts, const Params& P, float m, const float* v, const float* C,
-                                 const float* tau, float v0, float& rf) {
+// |v| as the stability limit takes it (no contraction, as particle_bounds).
+__device__ inline float particle_speed(const float* v) {
+#pragma clang fp contract(off)
+    return sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+}
+
+// 2^k with kMaxWeight-weighted contributions of a particle within `bound`
+// below 2^29 units (bound < 2^e): a quarter of the i32 range left for the
+// f32 rounding of the contributions.  1 for a zero bound (all contributions 0).
+__device__ inline float deposit_scale(double bound) {
+    if (!(bound > 0.0) || !(bound < 1e300)) return 1.0f;
+    int e;
+    frexp((double)kMaxWeight * bound, &e);
+    return ldexpf(1.0f, max(min(29 - e, 126), -126));
+}
+
+// One particle's deposit in scaled units.  NF = 4: mass and momentum with the
+// stress impulse (the reference's P2G once dt is known); NF = 7 (audit): mass,
+// the APIC momentum w (m v + m C d) and the stress sum w V0 tau d.
+template <int NF>
+struct Deposit {
+    float ms;     // m sc_m
+    float mv[3];  // m v_a times the momentum scale
+    float A[9];   // momentum matrix times dx times its scale (d in cells)
+    float T[6];   // audit: V0 tau dx sc_f (xx yy zz xy xz yz)
+};
+
+template <int NF>
+__device__ inline Deposit<NF> make_deposit(const Params& P, const InstState& is, float dt, float m, float v0,
+                                           const float* v, const float* C, const float* tau) {
+#pragma clang fp contract(off)
+    Deposit<NF> q;
+    q.ms = m * is.sc_m;
+    const float sp = NF == 7 ? is.sc_mom : is.sc_p;
     const float mdx = m * P.dx;
-    bool ok = m <= ts.lm;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-        ok = ok && fabsf(m * v[a]) + 1.5f * (fabsf(C[3 * a] * mdx) + fabsf(C[3 * a + 1] * mdx) +
-                                             fabsf(C[3 * a + 2] * mdx)) <= ts.lp[a];
-    const float vdx = v0 * P.dx;
-    const float r0 = fabsf(tau[0]) + fabsf(tau[3]) + fabsf(tau[4]), r1 = fabsf(tau[3]) + fabsf(tau[1]) + fabsf(tau[5]),
-                r2 = fabsf(tau[4]) + fabsf(tau[5]) + fabsf(tau[2]);
-    rf = 1.5f * (vdx * fmaxf(fmaxf(r0, r1), r2));
-    return ok && rf <= ts.lf;
+    for (int a = 0; a < 3; ++a) q.mv[a] = (m * v[a]) * sp;
+    if constexpr (NF == 7) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) q.A[i] = (C[i] * mdx) * sp;
+        const float vdx = (v0 * P.dx) * is.sc_f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) q.T[i] = tau[i] * vdx;
+    } else {
+        // dt (4/dx^2) V0 tau dx = dt (4/dx) V0 tau
+        const float k = (dt * (4.0f * P.inv_dx)) * v0;
+        const int sym[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+#pragma unroll
+        for (int i = 0; i < 9; ++i) q.A[i] = (C[i] * mdx - tau[sym[i]] * k) * sp;
+    }
+    return q;
 }
 
 // Round to the nearest integer (halves up) in one conversion.
@@ -747,59 +771,94 @@ __device__ inline int round_int(float x) {
     return q;
 }
 
-__device__ inline void tile_add(unsigned long long* t, int q) {
-    atomicAdd(t, (unsigned long long)(long long)q);
-}
-
-// deposit_particles for one particle into the fixed-point LDS tile.  Mass
-// contributions round up, so every node the particle reaches with a nonzero
-// weight is marked occupied, as its f64 mass would be.  Lanes walk the nine
-// (a, b) stencil columns starting at a lane-dependent column `rot` so that
-// particles of one cell, which share most nodes, hit different LDS words.  The
-// kernels use rot = 5 lane mod 9: the eight lanes of a cell start at eight
-// different columns, and the next cell's lanes at the same columns shifted by
-// four, which measured fewer LDS conflicts than lane mod 8 (2.4 % less kernel time).
-template <int SCAP>
-__device__ inline void scatter_lds(unsigned long long* ss, const int* so, const int* sd, const Params& P,
-                                   const Stencil& s, const TileScale& ts, float m, const float* v, const float* C,
-                                   const float* tau, float v0, int rot) {
-    float mc[9], vt[6];
-    const float mdx = m * P.dx, vdx = v0 * P.dx * ts.sf;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) mc[i] = C[i] * mdx * ts.sp[i / 3];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) vt[i] = tau[i] * vdx;
-    const float ms = m * ts.sm;
-    float mv0 = m * v[0] * ts.sp[0], mv1 = m * v[1] * ts.sp[1], mv2 = m * v[2] * ts.sp[2];
-    int li = s.base[0] - so[0], lj = s.base[1] - so[1], lk = s.base[2] - so[2];
-    // Tile offsets are far below 2^23: 24-bit multiplies run at full rate.
-    const int s12 = __mul24(sd[1], sd[2]);
-    const int row0 = __mul24(__mul24(li, sd[1]) + lj, sd[2]) + lk;
+// deposit_particles (solver.rs:632-661) of one particle: the integer
+// contribution of every field to every stencil node, handed to `sink(field, a,
+// b, c, value)`.  Mass rounds up, so every node the particle reaches with a
+// nonzero weight is marked occupied, as its f64 mass would be; momentum
+// rounds half up.  The tile and the global paths both come through here, so a
+// particle gives the same integers whichever path it takes.  Lanes walk the
+// nine (a, b) stencil columns starting at column `rot` so that particles of
+// one cell, which share most nodes, hit different LDS words (rot = 5 lane mod 9
+// measured fewest conflicts).
+template <int NF, class Sink>
+__device__ inline void deposit_particle(const Stencil& s, const Deposit<NF>& q, int rot, Sink& sink) {
+#pragma clang fp contract(off)
     for (int kk = 0; kk < 9; ++kk) {
         int ab = kk + rot;
         ab = ab >= 9 ? ab - 9 : ab;
-        int a = ab >= 6 ? 2 : (ab >= 3 ? 1 : 0);
-        int b = ab - 3 * a;
-        float wab = sel3(a, s.w[0][0], s.w[0][1], s.w[0][2]) * sel3(b, s.w[1][0], s.w[1][1], s.w[1][2]);
-        float dx = sel3(a, s.d[0][0], s.d[0][1], s.d[0][2]), dy = sel3(b, s.d[1][0], s.d[1][1], s.d[1][2]);
-        const int row = row0 + (int)__umul24((unsigned)a, (unsigned)s12) + (int)__umul24((unsigned)b, (unsigned)sd[2]);
-        float px0 = mv0 + mc[0] * dx + mc[1] * dy, py0 = mv1 + mc[3] * dx + mc[4] * dy,
-              pz0 = mv2 + mc[6] * dx + mc[7] * dy;
-        float fx0 = vt[0] * dx + vt[3] * dy, fy0 = vt[3] * dx + vt[1] * dy, fz0 = vt[4] * dx + vt[5] * dy;
+        const int a = ab >= 6 ? 2 : (ab >= 3 ? 1 : 0);
+        const int b = ab - 3 * a;
+        const float wab = sel3(a, s.w[0][0], s.w[0][1], s.w[0][2]) * sel3(b, s.w[1][0], s.w[1][1], s.w[1][2]);
+        const float dx = sel3(a, s.d[0][0], s.d[0][1], s.d[0][2]), dy = sel3(b, s.d[1][0], s.d[1][1], s.d[1][2]);
+        const float p0 = fmaf(q.A[1], dy, fmaf(q.A[0], dx, q.mv[0])), p1 = fmaf(q.A[4], dy, fmaf(q.A[3], dx, q.mv[1])),
+                    p2 = fmaf(q.A[7], dy, fmaf(q.A[6], dx, q.mv[2]));
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+        if constexpr (NF == 7) {
+            // rows of the symmetric stress: (xx xy xz), (xy yy yz), (xz yz zz)
+            f0 = fmaf(q.T[3], dy, q.T[0] * dx);
+            f1 = fmaf(q.T[1], dy, q.T[3] * dx);
+            f2 = fmaf(q.T[5], dy, q.T[4] * dx);
+        }
+        sink.column(a, b);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            float w = wab * s.w[2][c];
-            float dz = s.d[2][c];
-            int idx = row + c;
-            atomicAdd(ss + 0 * SCAP + idx, (unsigned long long)(unsigned)(int)ceilf(w * ms));
-            tile_add(ss + 1 * SCAP + idx, round_int(w * (px0 + mc[2] * dz)));
-            tile_add(ss + 2 * SCAP + idx, round_int(w * (py0 + mc[5] * dz)));
-            tile_add(ss + 3 * SCAP + idx, round_int(w * (pz0 + mc[8] * dz)));
-            tile_add(ss + 4 * SCAP + idx, round_int(w * (fx0 + vt[4] * dz)));
-            tile_add(ss + 5 * SCAP + idx, round_int(w * (fy0 + vt[5] * dz)));
-            tile_add(ss + 6 * SCAP + idx, round_int(w * (fz0 + vt[2] * dz)));
+            const float w = wab * s.w[2][c];
+            const float dz = s.d[2][c];
+            sink.add(0, c, (int)ceilf(w * q.ms));
+            sink.add(1, c, round_int(w * fmaf(q.A[2], dz, p0)));
+            sink.add(2, c, round_int(w * fmaf(q.A[5], dz, p1)));
+            sink.add(3, c, round_int(w * fmaf(q.A[8], dz, p2)));
+            if constexpr (NF == 7) {
+                sink.add(4, c, round_int(w * fmaf(q.T[4], dz, f0)));
+                sink.add(5, c, round_int(w * fmaf(q.T[5], dz, f1)));
+                sink.add(6, c, round_int(w * fmaf(q.T[2], dz, f2)));
+            }
         }
     }
+}
+
+// Deposit into the LDS tile: NF fields of CAP 64-bit words (the exact integer
+// sums of one workgroup), node (i, j, k) of the tile at (i sd1 + j) sd2 + k.
+template <int NF, int CAP>
+struct TileSink {
+    unsigned long long* ss;
+    int row0, s12, sd2, row;
+    __device__ inline void column(int a, int b) {
+        row = row0 + (int)__umul24((unsigned)a, (unsigned)s12) + (int)__umul24((unsigned)b, (unsigned)sd2);
+    }
+    __device__ inline void add(int f, int c, int v) {
+        atomicAdd(ss + f * CAP + row + c, (unsigned long long)(long long)v);
+    }
+};
+
+// Deposit straight into the global accumulators (64-bit atomics), for particles
+// outside their workgroup's tile.
+struct GlobalSink {
+    long long* v;  // the accumulator set, already offset to the instance's first node
+    size_t na;
+    const Params* P;
+    int base[3];
+    int ni[3];
+    __device__ inline void column(int a, int b) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ni[c] = node_index(*P, base[0] + a, base[1] + b, base[2] + c);
+    }
+    __device__ inline void add(int f, int c, int val) {
+        if (val) atomicAdd(reinterpret_cast<unsigned long long*>(v + f * na + ni[c]), (unsigned long long)(long long)val);
+    }
+};
+
+// Occupancy flags of the (up to 8) node blocks a stencil touches.
+__device__ inline void flag_stencil(const Params& P, int* flag, const Stencil& s) {
+    int b0[3], b1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b0[a] = s.base[a] >> 2;
+        b1[a] = (s.base[a] + 2) >> 2;
+    }
+    for (int bi = b0[0]; bi <= b1[0]; ++bi)
+        for (int bj = b0[1]; bj <= b1[1]; ++bj)
+            for (int bk = b0[2]; bk <= b1[2]; ++bk) flag[(bi * P.nb[1] + bj) * P.nb[2] + bk] = 1;
 }
 
 __device__ inline int wave_min_i(int v) {
