@@ -312,7 +312,7 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
 // Order keys of the instances whose refresh is pending (the flags are left set).
 void launch_refkeys(hipStream_t s, const BatchDev& B);
 // Near-wall lists, set-up records, deposit bounds and largest speed of every
-// instance's committed slab (list counters, bounds and speed cleared by the caller).
+// instance's committed slab (list counters cleared by the caller).
 void launch_setup(hipStream_t s, const BatchDev& B);
 // Device-order keys of every slab entry: (instance << cell_bits) | 4^3-cell block
 // key; entries past an instance's particle count sort to the end of its range.

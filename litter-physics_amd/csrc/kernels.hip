@@ -1600,9 +1600,9 @@ __global__ __launch_bounds__(kBlock) void refkey_kernel(BatchDev B) {
 // (ItemPre), and the largest deposit bounds and speed of the particles inside
 // the padded grid, with the same arithmetic as the trial step that writes a
 // state (particle_bounds, particle_speed), so an uploaded state is stepped
-// exactly as the same state reached by stepping.  The maxima are atomic
-// maxima of non-negative floats as integers; the caller clears them and the
-// list counters first.
+// exactly as the same state reached by stepping.  The maxima go to the
+// per-item partials (speed, -, b1, b2), which setup_reduce_kernel reduces per
+// instance; the caller clears the list counters first.
 __global__ __launch_bounds__(kBlock) void setup_kernel(BatchDev B) {
     __shared__ int sbox[4][6];
     const int4 item = B.work[blockIdx.x];
@@ -1654,14 +1654,15 @@ __global__ __launch_bounds__(kBlock) void setup_kernel(BatchDev B) {
     b1m = wave_fmax_uniform(b1m);
     b2m = wave_fmax_uniform(b2m);
     spd = wave_fmax_uniform(spd);
+    __shared__ float smx[4][3];
     if (lane == 0) {
         for (int a = 0; a < 3; ++a) {
             sbox[wave][a] = lo[a];
             sbox[wave][3 + a] = hi[a];
         }
-        atomicMax(reinterpret_cast<unsigned*>(&s.bound[0]), __float_as_uint(b1m));
-        atomicMax(reinterpret_cast<unsigned*>(&s.bound[1]), __float_as_uint(b2m));
-        atomicMax(reinterpret_cast<unsigned*>(&s.setup_speed), __float_as_uint(spd));
+        smx[wave][0] = spd;
+        smx[wave][1] = b1m;
+        smx[wave][2] = b2m;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1671,6 +1672,33 @@ __global__ __launch_bounds__(kBlock) void setup_kernel(BatchDev B) {
             r.hi[a] = max(max(sbox[0][3 + a], sbox[1][3 + a]), max(sbox[2][3 + a], sbox[3][3 + a]));
         }
         B.wpre[s.cur][blockIdx.x] = r;
+        float mx[3];
+        for (int k = 0; k < 3; ++k) mx[k] = fmaxf(fmaxf(smx[0][k], smx[1][k]), fmaxf(smx[2][k], smx[3][k]));
+        reinterpret_cast<float4*>(B.pmax)[blockIdx.x] = make_float4(mx[0], 0.0f, mx[1], mx[2]);
+    }
+}
+
+// The set-up maxima of every instance from its items' partials: one
+// workgroup per instance.
+__global__ __launch_bounds__(256) void setup_reduce_kernel(BatchDev B) {
+    InstState& s = B.is[blockIdx.x];
+    float m[3] = {0.0f, 0.0f, 0.0f};
+    for (int q = s.w_off + threadIdx.x; q < s.w_off + s.w_cnt; q += 256) {
+        const float4 v = reinterpret_cast<const float4*>(B.pmax)[q];
+        m[0] = fmaxf(m[0], v.x);
+        m[1] = fmaxf(m[1], v.z);
+        m[2] = fmaxf(m[2], v.w);
+    }
+    __shared__ float sm[4][3];
+    for (int k = 0; k < 3; ++k) m[k] = wave_max(m[k]);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) sm[threadIdx.x >> 6][k] = m[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; ++k) m[k] = fmaxf(fmaxf(sm[0][k], sm[1][k]), fmaxf(sm[2][k], sm[3][k]));
+        s.setup_speed = m[0];
+        s.bound[0] = m[1];
+        s.bound[1] = m[2];
     }
 }
 
@@ -1793,6 +1821,7 @@ void launch_refkeys(hipStream_t s, const BatchDev& B) {
 
 void launch_setup(hipStream_t s, const BatchDev& B) {
     if (B.nwork > 0) setup_kernel<<<B.nwork, kBlock, 0, s>>>(B);
+    setup_reduce_kernel<<<B.ninst, 256, 0, s>>>(B);
 }
 
 void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t* keys, uint32_t* order) {

@@ -1,10 +1,9 @@
 """Research runs on the GPU engine through run_json (core/src/lib.rs:24-49),
 restating core/src/research/tests.rs, and against the f64 oracle.
 
-Resume identity: the reference resumes bit for bit.  The engine's grid sums
-are f32 atomics whose order varies from run to run, so a resumed segment
-equals the uninterrupted one to that round-off: identical step, limiter and
-rejection counts and record times, states within 1e-6 relative.
+Resume identity: the reference resumes bit for bit (research/tests.rs:101-129),
+and so does the engine: every sum whose order of additions varies is an exact
+integer, so a trajectory is a function of the state alone.
 """
 import copy
 import json
@@ -108,22 +107,11 @@ def test_full_state_json_round_trip_matches_uninterrupted_trajectory(engine):
     cp_text = E.checkpoint_encode(json.dumps(req), sim.state(), sim.pellets(), sim.ledger(), sim.history(), 2)
     resumed = run(E, req, json.loads(cp_text))
     full = run(E, req)
-    a, b = resumed["checkpoint"], full["checkpoint"]
-    for key in ("step_count", "limited_steps", "rejected_steps", "next_record_index", "dt_scale"):
-        assert a["state"][key] == b["state"][key], key
+    assert resumed["checkpoint"] == full["checkpoint"]
+    assert resumed["observables"] == full["observables"]
     assert full["observables"]["limited_steps"] > 0
-    assert a["time_s"] == pytest.approx(b["time_s"], rel=1e-12)
-    pa, pb = a["state"]["particles"], b["state"]["particles"]
-    assert pa["id"] == pb["id"]
-    for key in ("position", "velocity", "deformation", "kirchhoff"):
-        assert rel(pa[key], pb[key]) < 1e-6, key
-    assert pa["mass"] == pb["mass"]
     assert len(resumed["frames"]) == 1
-    assert resumed["frames"][0]["time_s"] == pytest.approx(full["frames"][2]["time_s"], rel=1e-12)
-    assert rel(resumed["frames"][0]["positions_m"], full["frames"][2]["positions_m"]) < 1e-6
-    for key, v in full["observables"].items():
-        if key in ("step_count", "limited_steps", "rejected_steps", "particle_count"):
-            assert resumed["observables"][key] == v, key
+    assert resumed["frames"][0] == full["frames"][2]
 
 
 def test_corrupt_checkpoint_histories_and_requests_are_rejected(engine):
