@@ -305,6 +305,19 @@ int mpm_mlp_forward(mpm_mlp* mlp, size_t n, const float* x, float* y);
  * with respect to the master parameters, from the MFMA backward pass. */
 int mpm_mlp_loss_grad(mpm_mlp* mlp, size_t n, const float* x, const float* t,
                       const float* loss_weights, double* loss, float* grad);
+/* mpm_mlp_loss_grad for callers that cannot allocate device memory: code under
+ * HIP graph capture, training loops that forbid allocation.  No counterpart in
+ * the reference.  x (n x 12), t (n x 7), loss_weights (7; NULL: all ones),
+ * loss (1 double; may be NULL) and grad (mpm_mlp_param_count() floats; may be
+ * NULL) are device pointers on the network's device, which must be current;
+ * the caller's writes to x, t and loss_weights must be complete.  The call
+ * allocates nothing: the per-workgroup partial gradients and losses live in a
+ * fixed array of the library on each GPU (64 partials, 2.46 MB, held until
+ * the process ends).  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU race.  n is at most INT32_MAX / 16 (MPM_ERR_CAPACITY
+ * beyond).  Returns when the results are written. */
+int mpm_mlp_loss_grad_noalloc(mpm_mlp* mlp, size_t n, const float* x, const float* t, const float* loss_weights,
+                              double* loss, float* grad);
 /* Adam on the device (cosine decay from lr to lr/100), minibatches of `batch`
  * samples in an order shuffled per epoch from SplitMix64(seed); losses[e]
  * (may be NULL) receives the mean loss of epoch e. */

@@ -212,6 +212,8 @@ def lib():
         "mpm_mlp_set_params": (C.c_int, [C.c_void_p, fp]),
         "mpm_mlp_forward": (C.c_int, [C.c_void_p, C.c_size_t, fp, fp]),
         "mpm_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_size_t, fp, fp, fp, dp, fp]),
+        "mpm_mlp_loss_grad_noalloc": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p]),
         "mpm_mlp_train": (C.c_int, [C.c_void_p, C.c_size_t, fp, fp, fp, C.c_int, C.c_int, C.c_double, C.c_uint64,
                                     dp]),
         "mpm_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int]),
@@ -253,6 +255,7 @@ EXPORTED = [
     "mpm_instance_particle_count", "mpm_instance_set_state", "mpm_instance_get_state", "mpm_batch_advance",
     "mpm_get_launch", "mpm_set_launch", "mpm_autotune", "mpm_mlp_param_count", "mpm_mlp_create", "mpm_mlp_destroy",
     "mpm_mlp_last_error", "mpm_mlp_get_params", "mpm_mlp_set_params", "mpm_mlp_forward", "mpm_mlp_loss_grad",
+    "mpm_mlp_loss_grad_noalloc",
     "mpm_mlp_train", "mpm_set_surrogate", "mpm_surrogate_stats", "mpm_log_returns", "mpm_take_returns",
     "mpm_validate", "mpm_audit_step", "mpm_protocol_version", "mpm_string_free", "mpm_run_json", "mpm_run_json_batch", "mpm_research_validate", "mpm_research_observables",
     "mpm_checkpoint_encode", "mpm_checkpoint_particle_count", "mpm_checkpoint_decode",
@@ -625,6 +628,11 @@ class Network:
         loss = C.c_double()
         self._check(lib().mpm_mlp_loss_grad(self.h, x.shape[0], _p(x), _p(t), _p(w), C.byref(loss), _p(g)))
         return loss.value, g
+
+    def loss_grad_device(self, n, x_ptr, t_ptr, loss_ptr=None, grad_ptr=None, weights_ptr=None):
+        """mpm_mlp_loss_grad_noalloc on device pointers (integers): inputs x (n x 12) and
+        t (n x 7) f32, optional loss weights (7 f32), outputs loss (1 f64) and grad (f32)."""
+        return self._check(lib().mpm_mlp_loss_grad_noalloc(self.h, n, x_ptr, t_ptr, weights_ptr, loss_ptr, grad_ptr))
 
     def train(self, x, t, weights=None, epochs=1, batch=4096, lr=1e-3, seed=0):
         x, t = _f32(x, 12), _f32(t, 7)

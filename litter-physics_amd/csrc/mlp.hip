@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,8 @@ using namespace mpm;
 namespace {
 
 constexpr int kGradWgs = 512;  // workgroups (one wavefront each) of the gradient kernel
+// Largest batch of mpm_mlp_loss_grad_noalloc: sample indices stay in int.
+constexpr size_t kNoAllocMaxSamples = (size_t)INT32_MAX / 16;
 
 __global__ void mlp_pack_kernel(const float* __restrict__ p, uint16_t* fwd, uint16_t* bwd, float* bias) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,8 +128,9 @@ __global__ __launch_bounds__(64) void mlp_grad_kernel(const uint16_t* __restrict
                 const int u = 4 * grp + e;
                 if (u < kMlpOut) {
                     float r = t.y[e] - T[src * kMlpOut + u];
-                    loss += lw[u] * r * r;
-                    dy[e] = 2.0f * lw[u] * r * inv_n;
+                    const float wu = lw ? lw[u] : 1.0f;
+                    loss += wu * r * r;
+                    dy[e] = 2.0f * wu * r * inv_n;
                 }
             }
         }
@@ -224,6 +228,42 @@ __global__ void mlp_adam_kernel(float* __restrict__ p, float* __restrict__ m, fl
     }
 }
 
+// Loss of a batch from its per-workgroup partials, in a fixed order: sum / n.
+__global__ void mlp_loss_sum_kernel(const float* __restrict__ lpart, int nparts, double inv_n, double* loss) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double l = 0.0;
+    for (int w = 0; w < nparts; ++w) l += lpart[w];
+    *loss = l * inv_n;
+}
+
+// Scratch of mpm_mlp_loss_grad_noalloc: kNoAllocWgs partial gradients of
+// kMlpParams floats, then kNoAllocWgs partial losses (2.46 MB).  Each GPU
+// holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns on that GPU (scratch_of).
+constexpr int kNoAllocWgs = 64;
+constexpr size_t kNoAllocFloats = (size_t)kNoAllocWgs * kMlpParams + kNoAllocWgs;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// resolved once per device.
+hipError_t scratch_of(int device, float** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<float*>(ptr);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
 uint64_t splitmix(uint64_t& s) {
     s += 0x9E3779B97F4A7C15ull;
     uint64_t z = s;
@@ -247,6 +287,8 @@ struct mpm_mlp {
     float* gsum = nullptr;
     float* lw = nullptr;
     hipStream_t stream = nullptr;
+    int device = 0;
+    float* noalloc_scratch = nullptr;  // this device's `scratch` (scratch_of)
     uint64_t steps = 0;
     std::string err;
 
@@ -332,7 +374,10 @@ int mpm_mlp_create(const float* params, uint64_t seed, mpm_mlp** out) {
         delete mlp;
         return e == hipSuccess ? MPM_OK : MPM_ERR_DEVICE;
     };
-    hipError_t e = hipStreamCreate(&mlp->stream);
+    hipError_t e = hipGetDevice(&mlp->device);
+    // The no-allocation gradient's scratch on this device, resolved at set-up.
+    if (e == hipSuccess) e = scratch_of(mlp->device, &mlp->noalloc_scratch);
+    if (e == hipSuccess) e = hipStreamCreate(&mlp->stream);
     if (e == hipSuccess) e = hipMalloc(&mlp->p, sizeof(float) * kMlpParams);
     if (e == hipSuccess) e = hipMalloc(&mlp->m, sizeof(float) * kMlpParams);
     if (e == hipSuccess) e = hipMalloc(&mlp->v, sizeof(float) * kMlpParams);
@@ -421,6 +466,38 @@ int mpm_mlp_loss_grad(mpm_mlp* mlp, size_t n, const float* x, const float* t, co
     double l = 0.0;
     for (float v : lp) l += v;
     if (loss) *loss = l / (double)n;
+    return MPM_OK;
+}
+
+int mpm_mlp_loss_grad_noalloc(mpm_mlp* mlp, size_t n, const float* x, const float* t, const float* loss_weights,
+                              double* loss, float* grad) {
+    if (!mlp || !x || !t || n == 0) {
+        if (mlp) mlp->err = "mpm_mlp_loss_grad_noalloc: null network, inputs or empty batch";
+        return MPM_ERR_INVALID;
+    }
+    if (n > kNoAllocMaxSamples) {
+        mlp->err = "mpm_mlp_loss_grad_noalloc: batch of " + std::to_string(n) + " samples exceeds the maximum of " +
+                   std::to_string(kNoAllocMaxSamples);
+        return MPM_ERR_CAPACITY;
+    }
+    int dev = -1;
+    MLP_OK(hipGetDevice(&dev));
+    if (dev != mlp->device || !mlp->noalloc_scratch) {
+        mlp->err = "mpm_mlp_loss_grad_noalloc: the network's device is not the current device";
+        return MPM_ERR_INVALID;
+    }
+    float* gpart = mlp->noalloc_scratch;
+    float* lpart = gpart + (size_t)kNoAllocWgs * kMlpParams;
+    const int tiles = (int)((n + 15) / 16);
+    const int wgs = std::max(1, std::min(kNoAllocWgs, tiles));
+    mlp_grad_kernel<<<wgs, 64, 0, mlp->stream>>>(mlp->fwd, mlp->bwd, mlp->bias, x, t, nullptr, (int)n,
+                                                 1.0f / (float)n, loss_weights, gpart, lpart);
+    if (grad)
+        mlp_adam_kernel<<<(kMlpParams + 255) / 256, 256, 0, mlp->stream>>>(nullptr, nullptr, nullptr, gpart, wgs, 0.0f,
+                                                                           1.0f, 1.0f, grad);
+    if (loss) mlp_loss_sum_kernel<<<1, 64, 0, mlp->stream>>>(lpart, wgs, 1.0 / (double)n, loss);
+    MLP_OK(hipGetLastError());
+    MLP_OK(hipStreamSynchronize(mlp->stream));
     return MPM_OK;
 }
 
