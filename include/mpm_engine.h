@@ -261,10 +261,10 @@ int mpm_instance_get_state(mpm_engine* engine, size_t instance, mpm_particles* p
  * per instance) taking at most max_steps accepted steps each (0: no limit).
  * status[i] (may be NULL): 0 target reached, 1 step limit reached,
  * MPM_ERR_MIN_STEP when the instance aborted (its state stays consistent, as in
- * the reference).  done_ms[i] (may be NULL): wall time in ms from the start of
- * the call until instance i stopped, at the resolution of one host
- * synchronisation.  Returns MPM_OK, or a negative code for misuse or a device
- * failure. */
+ * the reference).  done_ms[i] (may be NULL): time in ms from the start of the
+ * call to the end of instance i's last accepted step, on the device clock (an
+ * instance that took no step: the host time at which it was found stopped).
+ * Returns MPM_OK, or a negative code for misuse or a device failure. */
 int mpm_batch_advance(mpm_engine* engine, const double* targets, uint64_t max_steps, int* status,
                       double* done_ms);
 

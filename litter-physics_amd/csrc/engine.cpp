@@ -376,6 +376,7 @@ struct mpm_engine {
     bool graph_stale = true;
     // Kernel timing: four device-clock stamps per enqueued step (BatchDev::ktime).
     unsigned long long* d_ktime = nullptr;
+    unsigned long long* d_run_clock = nullptr;  // device clock at the start of run()
     unsigned long long* h_ktime = nullptr;  // pinned
     double clock_khz = 0.0;
     bool profiling = false;
@@ -582,6 +583,7 @@ struct mpm_engine {
         // Kernel timing stamps (the kernels write them; no event records, which
         // would leave the GPU idle for microseconds between the kernels they separate).
         if (int rc = dalloc(&d_ktime, (size_t)4 * kMaxStepsPerSync)) return rc;
+        if (int rc = dalloc(&d_run_clock, 1)) return rc;
         HIP_OK(hipHostMalloc((void**)&h_ktime, sizeof(unsigned long long) * 4 * kMaxStepsPerSync,
                              hipHostMallocDefault));
         {
@@ -640,6 +642,7 @@ struct mpm_engine {
         (void)hipFree(rlog_count);
         (void)hipFree(sort_temp);
         (void)hipFree(d_ktime);
+        (void)hipFree(d_run_clock);
         if (h_ktime) (void)hipHostFree(h_ktime);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -981,6 +984,15 @@ struct mpm_engine {
     // limit).  status/done_ms per instance (may be null).
     int run(int k_sync, int* status, double* done_ms, int max_rounds = -1) {
         auto t0 = std::chrono::steady_clock::now();
+        // Completion times come from the device clock: the run's start here, each
+        // instance's last accepted step in its commit stamp.
+        unsigned long long start_clock = 0;
+        if (done_ms) {
+            launch_clock(stream, d_run_clock);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(&start_clock, d_run_clock, sizeof(start_clock), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+        }
         std::vector<char> recorded(ninst, 0);
         int since_sort = 0, rounds = 0;
         bool first_error = true;
@@ -1033,7 +1045,10 @@ struct mpm_engine {
                     first_error = false;
                 }
                 if (status) status[i] = st;
-                if (done_ms) done_ms[i] = ms;
+                if (done_ms)
+                    done_ms[i] = s.commit_clock > start_clock && clock_khz > 0.0
+                                     ? (double)(s.commit_clock - start_clock) / clock_khz
+                                     : ms;
             }
             if (dirty || since_sort >= kLocalitySortInterval) {
                 if (changed)

@@ -152,6 +152,7 @@ struct InstState {
     int budget;       // accepted steps left before a pause (host-set)
     int w_off, w_cnt; // work items
     int nw_base;      // inclusive end of the instance's range in the near-wall prefix (traction)
+    unsigned long long commit_clock;  // device constant clock at the last accepted step
     int refsort;      // the last accepted step hit the 32-step cadence; its order keys are pending
     int committed;    // a trial step was accepted since the state was set
     int pad;
@@ -309,6 +310,8 @@ int g2p_chunk();
 // One trial step of every running instance: control, deposit, traction, grid
 // update, pellet coupling, G2P, pellet integration, reductions and commit.
 void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets);
+// The device's constant clock into *out (the start of a timed run).
+void launch_clock(hipStream_t s, unsigned long long* out);
 // Order keys of the instances whose refresh is pending (the flags are left set).
 void launch_refkeys(hipStream_t s, const BatchDev& B);
 // Near-wall lists, set-up records, deposit bounds and largest speed of every

@@ -499,157 +499,161 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
-    constexpr int J = kGridWgBlocks / 4;  // node blocks per wavefront
-    // All loads of the wavefront's blocks are issued before any result is used:
-    // the flags of both sets, then the sums and traction of the touched blocks,
-    // so a wavefront waits for two memory round trips instead of two per block.
-    int cf[J], af[J];
+    // The workgroup's blocks in groups of 16, four per wavefront, each group's
+    // loads issued together.
+    for (int grp = 0; grp < kGridWgBlocks / 16; ++grp) {
+        constexpr int J = 4;  // node blocks per wavefront and group
+        // All loads of the wavefront's blocks are issued before any result is used:
+        // the flags of both sets, then the sums and traction of the touched blocks,
+        // so a wavefront waits for two memory round trips instead of two per block.
+        int cf[J], af[J];
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
-        cf[j] = C.flag[b];
-        af[j] = A.flag[b];
-    }
-    float fm[J], fpx[J], fpy[J], fpz[J], ffx[J], ffy[J], ffz[J], ftb[J][3];
-    bool wall[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
-        const size_t n = ((size_t)b << 6) | l;
-        fm[j] = fpx[j] = fpy[j] = fpz[j] = ffx[j] = ffy[j] = ffz[j] = 0.0f;
-        wall[j] = false;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ftb[j][a] = 0.0f;
-        if (!af[j]) continue;
-        // The traction buffers can be nonzero only where wall_traction_to_grid
-        // deposits: a particle's wall image lands its deposits on node 2 or node
-        // `cells` of the wall axis, in the first block or the block holding node
-        // `cells` along that axis.
-        int bc[3];
-        node_coords(P, (b - ip.boff) << 6, bc[0], bc[1], bc[2]);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) wall[j] = wall[j] || bc[a] == 0 || (bc[a] >> 2) == (P.cells[a] >> 2);
-        fm[j] = (float)((double)A.v[n] * um);
-        fpx[j] = (float)((double)A.v[na + n] * up);
-        fpy[j] = (float)((double)A.v[2 * na + n] * up);
-        fpz[j] = (float)((double)A.v[3 * na + n] * up);
-        if (AUDIT) {
-            ffx[j] = (float)((double)A.v[4 * na + n] * uf);
-            ffy[j] = (float)((double)A.v[5 * na + n] * uf);
-            ffz[j] = (float)((double)A.v[6 * na + n] * uf);
+        for (int j = 0; j < J; ++j) {
+            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
+            cf[j] = C.flag[b];
+            af[j] = A.flag[b];
         }
-        if (wall[j])
+        float fm[J], fpx[J], fpy[J], fpz[J], ffx[J], ffy[J], ffz[J], ftb[J][3];
+        bool wall[J];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) ftb[j][a] = (float)((double)B.trT[a * na + n] * ut);
-    }
+        for (int j = 0; j < J; ++j) {
+            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
+            const size_t n = ((size_t)b << 6) | l;
+            fm[j] = fpx[j] = fpy[j] = fpz[j] = ffx[j] = ffy[j] = ffz[j] = 0.0f;
+            wall[j] = false;
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
-        const size_t n = ((size_t)b << 6) | l;
-        if (cf[j]) {
+            for (int a = 0; a < 3; ++a) ftb[j][a] = 0.0f;
+            if (!af[j]) continue;
+            // The traction buffers can be nonzero only where wall_traction_to_grid
+            // deposits: a particle's wall image lands its deposits on node 2 or node
+            // `cells` of the wall axis, in the first block or the block holding node
+            // `cells` along that axis.
+            int bc[3];
+            node_coords(P, (b - ip.boff) << 6, bc[0], bc[1], bc[2]);
 #pragma unroll
-            for (int f = 0; f < NF; ++f) C.v[f * na + n] = 0;
-            if (l == 0) C.flag[b] = 0;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        if (!af[j]) continue;
-        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
-        const size_t n = ((size_t)b << 6) | l;
-        const bool wall_block = wall[j];
-        const float m = fm[j];
-        if (!(m > 0.0f)) {
-            B.gv[n] = 0.0f;
-            B.gv[na + n] = 0.0f;
-            B.gv[2 * na + n] = 0.0f;
-            continue;
-        }
-        // The audit deposits the APIC momentum and the stress sum apart; the
-        // reference's P2G momentum is their combination (solver.rs:643-655).
-        float p[3] = {fpx[j], fpy[j], fpz[j]};
-        if (AUDIT) {
-            p[0] = fpx[j] - dt * k4 * ffx[j];
-            p[1] = fpy[j] - dt * k4 * ffy[j];
-            p[2] = fpz[j] - dt * k4 * ffz[j];
-        }
-        float v[3], tb[3];
-        int c[3];
-        node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            // trT holds a node's summed reactions; the signed impulse follows from
-            // the face the node serves (node 2 the low face, inward +1; node
-            // `cells` the high face, -1).  A grid two cells deep puts both faces
-            // on node 2 and keeps the signed sum in trS.
-            tb[a] = ftb[j][a];
-            float D;
-            if (P.cells[a] <= 2) {
-                const long long ds = wall_block ? B.trS[a * na + n] : 0;
-                D = (float)((double)ds * ut);
-                if (ds != 0) B.trS[a * na + n] = 0;
-            } else {
-                D = c[a] == 2 ? tb[a] : -tb[a];
-            }
-            if (D != 0.0f) {
-                // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
-                vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
-                vals[3 + a] += D;
-            }
-            if (tb[a] != 0.0f) B.trT[a * na + n] = 0;
-            v[a] = (p[a] + D) / m + P.gravity[a] * dt;
+            for (int a = 0; a < 3; ++a) wall[j] = wall[j] || bc[a] == 0 || (bc[a] >> 2) == (P.cells[a] >> 2);
+            fm[j] = (float)((double)A.v[n] * um);
+            fpx[j] = (float)((double)A.v[na + n] * up);
+            fpy[j] = (float)((double)A.v[2 * na + n] * up);
+            fpz[j] = (float)((double)A.v[3 * na + n] * up);
             if (AUDIT) {
-                const double md = (double)m;
-                const double pt = a == 0 ? (double)fpx[j] : (a == 1 ? (double)fpy[j] : (double)fpz[j]);
-                avals[0] += 0.5 * pt * pt / md;
-                avals[1] += 0.5 * (double)p[a] * (double)p[a] / md;
-                avals[2] += 0.5 * ((double)p[a] + (double)D) * ((double)p[a] + (double)D) / md;
-                avals[3] += 0.5 * md * (double)v[a] * (double)v[a];
+                ffx[j] = (float)((double)A.v[4 * na + n] * uf);
+                ffy[j] = (float)((double)A.v[5 * na + n] * uf);
+                ffz[j] = (float)((double)A.v[6 * na + n] * uf);
+            }
+            if (wall[j])
+#pragma unroll
+                for (int a = 0; a < 3; ++a) ftb[j][a] = (float)((double)B.trT[a * na + n] * ut);
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
+            const size_t n = ((size_t)b << 6) | l;
+            if (cf[j]) {
+#pragma unroll
+                for (int f = 0; f < NF; ++f) C.v[f * na + n] = 0;
+                if (l == 0) C.flag[b] = 0;
             }
         }
-        if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
-            bad += 1;
-            v[0] = v[1] = v[2] = 0.0f;
-        } else {
-            float before[3] = {v[0], v[1], v[2]};
-            for (int axis = 0; axis < 3; ++axis) {
-                int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
-                if (sign == 0) continue;
-                float ns = v[axis] * (float)sign;
-                if (ns >= 0.0f) continue;
-                vals[6] -= 0.5 * (double)m * (double)ns * (double)ns;
-                v[axis] = 0.0f;
-                double tn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
-                double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                vals[7] += 0.5 * (double)m * (tn - vn);
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (!af[j]) continue;
+            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
+            const size_t n = ((size_t)b << 6) | l;
+            const bool wall_block = wall[j];
+            const float m = fm[j];
+            if (!(m > 0.0f)) {
+                B.gv[n] = 0.0f;
+                B.gv[na + n] = 0.0f;
+                B.gv[2 * na + n] = 0.0f;
+                continue;
             }
-            for (int axis = 0; axis < 3; ++axis) {
-                float reaction = tb[axis];
-                if (reaction <= 0.0f) continue;
-                float normal = v[axis];
-                float t[3] = {v[0], v[1], v[2]};
-                t[axis] = 0.0f;
-                double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
-                reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
-                double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
-                vals[7] += 0.5 * (double)m * (tn - rn);
-                v[0] = t[0];
-                v[1] = t[1];
-                v[2] = t[2];
-                v[axis] = normal;
+            // The audit deposits the APIC momentum and the stress sum apart; the
+            // reference's P2G momentum is their combination (solver.rs:643-655).
+            float p[3] = {fpx[j], fpy[j], fpz[j]};
+            if (AUDIT) {
+                p[0] = fpx[j] - dt * k4 * ffx[j];
+                p[1] = fpy[j] - dt * k4 * ffy[j];
+                p[2] = fpz[j] - dt * k4 * ffz[j];
             }
+            float v[3], tb[3];
+            int c[3];
+            node_coords(P, (int)((size_t)(b - ip.boff) << 6) | l, c[0], c[1], c[2]);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                vals[a] += ((double)v[a] - (double)before[a]) * (double)m;
-                vals[9 + a] += (double)v[a] * (double)m;
-                if (AUDIT) avals[4] += 0.5 * (double)m * (double)v[a] * (double)v[a];
+                // trT holds a node's summed reactions; the signed impulse follows from
+                // the face the node serves (node 2 the low face, inward +1; node
+                // `cells` the high face, -1).  A grid two cells deep puts both faces
+                // on node 2 and keeps the signed sum in trS.
+                tb[a] = ftb[j][a];
+                float D;
+                if (P.cells[a] <= 2) {
+                    const long long ds = wall_block ? B.trS[a * na + n] : 0;
+                    D = (float)((double)ds * ut);
+                    if (ds != 0) B.trS[a * na + n] = 0;
+                } else {
+                    D = c[a] == 2 ? tb[a] : -tb[a];
+                }
+                if (D != 0.0f) {
+                    // Telescoped kinetic-energy jump of this node's deposits (grid.rs:265-295).
+                    vals[8] += (double)D * ((double)p[a] / (double)m + (double)D / (2.0 * (double)m));
+                    vals[3 + a] += D;
+                }
+                if (tb[a] != 0.0f) B.trT[a * na + n] = 0;
+                v[a] = (p[a] + D) / m + P.gravity[a] * dt;
+                if (AUDIT) {
+                    const double md = (double)m;
+                    const double pt = a == 0 ? (double)fpx[j] : (a == 1 ? (double)fpy[j] : (double)fpz[j]);
+                    avals[0] += 0.5 * pt * pt / md;
+                    avals[1] += 0.5 * (double)p[a] * (double)p[a] / md;
+                    avals[2] += 0.5 * ((double)p[a] + (double)D) * ((double)p[a] + (double)D) / md;
+                    avals[3] += 0.5 * md * (double)v[a] * (double)v[a];
+                }
             }
+            if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) {
+                bad += 1;
+                v[0] = v[1] = v[2] = 0.0f;
+            } else {
+                float before[3] = {v[0], v[1], v[2]};
+                for (int axis = 0; axis < 3; ++axis) {
+                    int sign = c[axis] <= 1 ? 1 : (c[axis] > P.cells[axis] ? -1 : 0);
+                    if (sign == 0) continue;
+                    float ns = v[axis] * (float)sign;
+                    if (ns >= 0.0f) continue;
+                    vals[6] -= 0.5 * (double)m * (double)ns * (double)ns;
+                    v[axis] = 0.0f;
+                    double tn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                    reduce_tangential(v, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * (-ns));
+                    double vn = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                    vals[7] += 0.5 * (double)m * (tn - vn);
+                }
+                for (int axis = 0; axis < 3; ++axis) {
+                    float reaction = tb[axis];
+                    if (reaction <= 0.0f) continue;
+                    float normal = v[axis];
+                    float t[3] = {v[0], v[1], v[2]};
+                    t[axis] = 0.0f;
+                    double tn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                    int sign = 2 * c[axis] <= P.cells[axis] + 2 ? 1 : -1;
+                    reduce_tangential(t, P.face_mu[2 * axis + (sign > 0 ? 0 : 1)] * reaction / m);
+                    double rn = (double)t[0] * t[0] + (double)t[1] * t[1] + (double)t[2] * t[2];
+                    vals[7] += 0.5 * (double)m * (tn - rn);
+                    v[0] = t[0];
+                    v[1] = t[1];
+                    v[2] = t[2];
+                    v[axis] = normal;
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    vals[a] += ((double)v[a] - (double)before[a]) * (double)m;
+                    vals[9 + a] += (double)v[a] * (double)m;
+                    if (AUDIT) avals[4] += 0.5 * (double)m * (double)v[a] * (double)v[a];
+                }
+            }
+            B.gv[n] = v[0];
+            B.gv[na + n] = v[1];
+            B.gv[2 * na + n] = v[2];
         }
-        B.gv[n] = v[0];
-        B.gv[na + n] = v[1];
-        B.gv[2 * na + n] = v[2];
     }
     int nbad = wave_sum_i(bad);
     if (l == 0) sbad[wave] = nbad;
@@ -984,18 +988,7 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         t.tiled = ok && gsz <= kTileCap;
     }
 
-    // Phase A: the current positions and the gather tile.
-    float px[R], py[R], pz[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        int p = chunk0 + r * kBlock + tid;
-        px[r] = py[r] = pz[r] = 0.0f;
-        if (p < chunk_end) {
-            px[r] = src[PX * cap + p];
-            py[r] = src[PY * cap + p];
-            pz[r] = src[PZ * cap + p];
-        }
-    }
+    // Phase A: the gather tile.
     if (t.tiled) {
         int gsz = t.gd[0] * t.gd[1] * t.gd[2];
         const float ig2 = 1.0f / (float)t.gd[2], ig1 = 1.0f / (float)t.gd[1];
@@ -1025,7 +1018,7 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         // Trial: gather, advection, trial deformation (or the whole liquid update).
         bool live = false, yielded = false;
         int rc = F_OK;
-        const float x = px[r], y = py[r], z = pz[r];
+        float x = 0.0f, y = 0.0f, z = 0.0f;
         float v[3] = {0.0f, 0.0f, 0.0f}, C[9], Ht[9], tau[6];
         bool dep = false;  // the updated particle lies inside the padded grid (stencil s2)
         Stencil s2;
@@ -1036,6 +1029,9 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         // sinking them to their first use).
         float Hp[9], pm = 0.0f, pv0 = 0.0f, pe = 0.0f;
         if (p < chunk_end) {
+            x = src[PX * cap + p];
+            y = src[PY * cap + p];
+            z = src[PZ * cap + p];
             pm = shared[SMASS * cap + p];
             pv0 = shared[SVOL0 * cap + p];
             pe = src[(P.kind == 0 ? EPS : JM1) * cap + p];
@@ -1337,6 +1333,7 @@ __device__ inline void commit_instance(InstState& s, const InstParams& ip, const
     const double dt = s.dt;
     s.cur ^= 1;
     s.committed = 1;
+    s.commit_clock = (unsigned long long)__builtin_amdgcn_s_memrealtime();
     s.bound[0] = st.bound[0];
     s.bound[1] = st.bound[1];
     // commit, solver.rs:992-997
@@ -1814,6 +1811,12 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
     finalize_kernel<<<B.ninst, 256, 0, s>>>(B);
     if (pellets) pellet_commit_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
 }
+
+__global__ void clock_kernel(unsigned long long* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = (unsigned long long)__builtin_amdgcn_s_memrealtime();
+}
+
+void launch_clock(hipStream_t s, unsigned long long* out) { clock_kernel<<<1, 64, 0, s>>>(out); }
 
 void launch_refkeys(hipStream_t s, const BatchDev& B) {
     if (B.nwork > 0) refkey_kernel<<<B.nwork, kBlock, 0, s>>>(B);

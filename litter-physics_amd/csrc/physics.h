@@ -574,27 +574,65 @@ struct Tile {
     int tiled;
 };
 
-__device__ inline void gather_global(const Params& P, const float* __restrict__ gv, size_t na, const Stencil& s,
-                                     float* v, float* B) {
+// grid_to_particle's sums (solver.rs:905-930): v = sum_i w_i v_i and
+// B = sum_i w_i v_i d_i^T over the 27 stencil nodes (d in cells).  The weights
+// and offsets factor by axis, so the sums run axis by axis: over z for each
+// (x, y) column, then y, then x -- about 290 VALU operations instead of 440.
+// `node(a, b, c)` returns the velocity of stencil node (a, b, c).
+template <class Node>
+__device__ inline void gather_sums(const Stencil& s, const Node& node, float* v, float* B) {
+    float wdz[3], wdy[3], wdx[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+        wdz[c] = s.w[2][c] * s.d[2][c];
+        wdy[c] = s.w[1][c] * s.d[1][c];
+        wdx[c] = s.w[0][c] * s.d[0][c];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        v[i] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) B[3 * i + j] = 0.0f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float T[3] = {0.0f, 0.0f, 0.0f}, Ty[3] = {0.0f, 0.0f, 0.0f}, Tz[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            float wab = s.w[0][a] * s.w[1][b];
+            float S[3] = {0.0f, 0.0f, 0.0f}, Sz[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                float w = wab * s.w[2][c];
-                int ni = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
-                float nv[3] = {gv[ni] * w, gv[na + ni] * w, gv[2 * na + ni] * w};
-                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
+                const float3 g = node(a, b, c);
+                S[0] = fmaf(s.w[2][c], g.x, S[0]);
+                S[1] = fmaf(s.w[2][c], g.y, S[1]);
+                S[2] = fmaf(s.w[2][c], g.z, S[2]);
+                Sz[0] = fmaf(wdz[c], g.x, Sz[0]);
+                Sz[1] = fmaf(wdz[c], g.y, Sz[1]);
+                Sz[2] = fmaf(wdz[c], g.z, Sz[2]);
+            }
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    v[i] += nv[i];
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
-                }
+            for (int i = 0; i < 3; ++i) {
+                T[i] = fmaf(s.w[1][b], S[i], T[i]);
+                Ty[i] = fmaf(wdy[b], S[i], Ty[i]);
+                Tz[i] = fmaf(s.w[1][b], Sz[i], Tz[i]);
             }
         }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            v[i] = fmaf(s.w[0][a], T[i], v[i]);
+            B[3 * i] = fmaf(wdx[a], T[i], B[3 * i]);
+            B[3 * i + 1] = fmaf(s.w[0][a], Ty[i], B[3 * i + 1]);
+            B[3 * i + 2] = fmaf(s.w[0][a], Tz[i], B[3 * i + 2]);
+        }
+    }
+}
+
+__device__ inline void gather_global(const Params& P, const float* __restrict__ gv, size_t na, const Stencil& s,
+                                     float* v, float* B) {
+    gather_sums(s, [&](int a, int b, int c) {
+        const int ni = node_index(P, s.base[0] + a, s.base[1] + b, s.base[2] + c);
+        return make_float3(gv[ni], gv[na + ni], gv[2 * na + ni]);
+    }, v, B);
 }
 
 __device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil& s, float* v, float* B) {
@@ -602,26 +640,10 @@ __device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil
     // Tile offsets are far below 2^23: 24-bit multiplies run at full rate.
     const int row0 = __mul24(__mul24(li, t.gd[1]) + lj, t.gd[2]) + lk;
     const int s12 = __mul24(t.gd[1], t.gd[2]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            float wab = s.w[0][a] * s.w[1][b];
-            const int row = row0 + a * s12 + b * t.gd[2];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float w = wab * s.w[2][c];
-                float4 g = sg[row + c];
-                float nv[3] = {g.x * w, g.y * w, g.z * w};
-                float d[3] = {s.d[0][a], s.d[1][b], s.d[2][c]};
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    v[i] += nv[i];
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) B[3 * i + j] += nv[i] * d[j];
-                }
-            }
-        }
+    gather_sums(s, [&](int a, int b, int c) {
+        const float4 g = sg[row0 + a * s12 + b * t.gd[2] + c];
+        return make_float3(g.x, g.y, g.z);
+    }, v, B);
 }
 
 // Select among three register values by a runtime index (no array indexing, which

@@ -123,3 +123,41 @@ def test_pellet_bed_full_size_invariants(engine):
     out = check_invariants(sim, st, 40, dx, domain)
     assert (out.plastic >= 0.0).all()
     print(f"pellet bed: t = {sim.history().time:.3e} s", flush=True)
+
+
+def test_full_size_256_instance_sweep_invariants(engine):
+    # The batched calibration sweep at full size: 256 paste slumps on grids of
+    # 16^3..64^3 cells over a 100 mm cube, 40 x 40 mm columns 20-80 mm tall,
+    # yield stress U[20, 300] Pa, K U[0.5, 20], n U[0.3, 1] from seed 4, about
+    # 11.3M particles, stepped as one batch for 40 substeps each (across a
+    # locality sort).  Every instance keeps its particle set, closes its mass
+    # ledger, matches its momentum ledger to f32 round-off, takes no rejected
+    # step and stays finite inside its grid.
+    import bench
+    E = engine
+    specs = bench.sweep_specs(E)
+    total = sum(len(s["state"]) for s in specs)
+    print(f"sweep placed: {len(specs)} instances, {total} particles", flush=True)
+    assert len(specs) == 256 and total > 10_000_000
+    batch = E.Batch(specs)
+    status, done_ms = batch.advance(1e9, max_steps=40)
+    assert list(status) == [1] * 256
+    print(f"40 steps each; done at {np.min(done_ms):.1f}..{np.max(done_ms):.1f} ms", flush=True)
+    for i, s in enumerate(specs):
+        h, led = batch.history(i), batch.ledger(i)
+        assert h.step_count == 40 and h.rejected_steps == 0, i
+        out = batch.state(i).by_id()
+        assert np.array_equal(out.id, np.sort(np.asarray(s["state"].id))), i
+        mass = np.asarray(s["state"].mass, dtype=np.float64)
+        assert abs(out.mass.sum() + led.outflow_mass - led.initial_mass) <= 1e-10 * led.initial_mass, i
+        mom = (out.vel * out.mass[:, None]).sum(0)
+        resid = (mom - np.array(led.initial_momentum[:]) - np.array(led.gravity_impulse[:]) -
+                 np.array(led.wall_impulse[:]) - np.array(led.pellet_wall_impulse[:]) +
+                 np.array(led.outflow_momentum[:]))
+        assert np.linalg.norm(resid) < 1e-5 * np.linalg.norm(led.gravity_impulse), i
+        assert np.isfinite(out.pos).all() and np.isfinite(out.vel).all() and np.isfinite(out.kirchhoff).all(), i
+        dx, dom = s["spacing"], s["domain"]
+        for a in range(3):
+            assert (out.pos[:, a] >= -0.5 * dx).all() and (out.pos[:, a] < dom[a] + 0.5 * dx).all(), i
+        assert (out.plastic >= 0.0).all(), i
+        del out, mass
