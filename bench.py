@@ -41,8 +41,9 @@ P2G_BYTES_PER_PARTICLE = 4 * 23
 G2P_PASTE_BYTES_PER_PARTICLE = 4 * (15 + 31)
 G2P_LIQUID_BYTES_PER_PARTICLE = 4 * (6 + 22)
 PEAK_HBM = 8.0e12
-# The reference's largest published CPU rate of this path (slump
-# refinement at dx 2.5 mm, 4096 particles x 10000 substeps on an Apple M4, f64).
+# The reference's largest published CPU rate of this path: its slump refinement
+# at dx 2.5 mm, 4096 particles x 10000 substeps in 7.9454 s on an Apple M4, f64
+# (docs/formal/data/refinement-report.json).
 REFERENCE_PUBLISHED = 5.16e6
 REFERENCE_BASIS = ("reference's largest published CPU rate: slump refinement dx 2.5 mm, 4096 particles x 10000 "
                    "substeps, Apple M4, f64 (5.16e6 particle-substeps/s)")
@@ -254,7 +255,8 @@ def run_liquid(args):
 def bed(E, with_pellets, seed=2):
     """96^3 grid, dx 1.5 mm, paste bed 96 x 96 x 20 cells; yield stress U[20, 200] Pa
     (shear) from `seed`; 512 pellets r 1.5 mm, L 8 mm, rho 650 at random positions
-    above the bed and random orientations from `seed`."""
+    in a band from 1.5 mm below to 3 mm above the bed surface (so that they touch the
+    bed within the first steps) and random orientations from `seed`."""
     rng = np.random.default_rng(seed)
     dx = 0.0015
     cells = [96, 96, 96]
@@ -267,7 +269,7 @@ def bed(E, with_pellets, seed=2):
         top = 20 * dx
         for _ in range(512):
             pos = [rng.uniform(0.006, domain[0] - 0.006), rng.uniform(0.006, domain[1] - 0.006),
-                   rng.uniform(top + 0.006, domain[2] - 0.006)]
+                   rng.uniform(top - 0.0015, top + 0.003)]
             q = rng.normal(size=4)
             q /= np.linalg.norm(q)
             pellets.append(E.new_pellet(0.0015, 0.008, 650.0, pos, orientation=list(q)))
@@ -289,8 +291,9 @@ def run_pellets(args):
     return line("particle-substeps/s, paste bed 1.5M particles + 512 coupled pellets", n * accepted / elapsed,
                 "particle-substeps/s", accepted, args.warmup, elapsed, "f32",
                 {"workload": "HB paste bed 96x96x20 cells x 8 particles (1,474,560) on a 96^3 grid, dx 1.5 mm, "
-                             "yield stress U[20,200] Pa from seed 2, 512 pellets r 1.5 mm L 8 mm rho 650 dropped "
-                             "from random positions and orientations (seed 2), two-way coupled, CFL 0.4",
+                             "yield stress U[20,200] Pa from seed 2, 512 pellets r 1.5 mm L 8 mm rho 650 at random "
+                             "positions in a band from 1.5 mm below to 3 mm above the bed surface and random "
+                             "orientations (seed 2), two-way coupled, CFL 0.4",
                  "particles": n, "pellets": len(pellets), "grid_cells": cells, "rejected_steps": h.rejected_steps,
                  "dt_last_s": h.max_dt_used, "coupling_impulse": list(led.coupling_impulse),
                  "launch": sim.launch().as_dict()},
@@ -502,10 +505,17 @@ def run_copy(args):
                  "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes})
 
 
+# Timed steps per workload: the substep counts of the configurations they follow.
+DEFAULT_STEPS = {"headline": 500, "liquid": 1000, "pellets": 1000, "surrogate": 500, "sweep": 300,
+                 "surrogate_sweep": 300, "copy": 10}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 500 for the headline and surrogate, 1000 for liquid and "
+                         "pellets, 300 for the sweeps)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "liquid", "pellets", "surrogate", "sweep", "surrogate_sweep", "copy"])
@@ -518,6 +528,8 @@ def main():
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
               "(the MLS-MPM substep is measured on a single MI355X).", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = DEFAULT_STEPS[args.workload]
     run = {"headline": run_headline, "liquid": run_liquid, "pellets": run_pellets, "surrogate": run_surrogate,
            "sweep": run_sweep, "surrogate_sweep": run_surrogate_sweep, "copy": run_copy}[args.workload]
     print(json.dumps(run(args)), flush=True)

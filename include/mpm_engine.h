@@ -275,12 +275,18 @@ typedef struct {
     int merge_blocks;    /* cell blocks per work item along z (1 or 2) */
     int traction_wgs;    /* workgroups of the wall-traction kernel */
     int g2p_occupancy;   /* G2P workgroups per CU its register budget allows (2 or 3) */
+    int p2g_block;       /* threads of a deposit workgroup (256 or 512) */
 } mpm_launch;
 int mpm_get_launch(const mpm_engine* engine, mpm_launch* out);
 int mpm_set_launch(mpm_engine* engine, const mpm_launch* launch);
-/* Time `steps` trial steps of every candidate launch shape on the current
- * state, restore the state, keep the fastest and return it in *chosen (may be
- * NULL).  The state, ledgers and histories are unchanged afterwards. */
+/* Pick the launch shape for the current state: time max(steps, 64) trial
+ * steps (two of the longest synchronisation intervals) per candidate from the
+ * same state, restoring it each time.  First the kernel shapes (cell blocks per
+ * work item, G2P occupancy, deposit workgroup size) by the device time of the
+ * deposit and G2P kernels, then the synchronisation interval and graph replay
+ * by wall time.  Keeps the fastest and returns it in *chosen (may be NULL).
+ * The state, ledgers, histories and the caller's kernel timing are unchanged
+ * afterwards. */
 int mpm_autotune(mpm_engine* engine, int steps, mpm_launch* chosen);
 
 /* ---- Learned surrogate of the Herschel-Bulkley return map.  No counterpart

@@ -101,7 +101,7 @@ class StepReport(C.Structure):
 class Launch(C.Structure):
     """mpm_launch: launch shape of the stepping loop (the autotuner's search space)."""
     _fields_ = [("steps_per_sync", C.c_int), ("use_graph", C.c_int), ("merge_blocks", C.c_int),
-                ("traction_wgs", C.c_int), ("g2p_occupancy", C.c_int)]
+                ("traction_wgs", C.c_int), ("g2p_occupancy", C.c_int), ("p2g_block", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

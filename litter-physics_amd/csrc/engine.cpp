@@ -369,7 +369,7 @@ struct mpm_engine {
     int cell_bits = 0, end_bit = 0;
     bool dirty = true;  // work items, set-up records and near-wall lists must be rebuilt
     // launch
-    mpm_launch launch{32, 0, 2, 2048, 2};
+    mpm_launch launch{32, 0, 2, 2048, 2, 512};
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -426,6 +426,7 @@ struct mpm_engine {
         }
         B.sh = sh;
         B.g2p_occupancy = launch.g2p_occupancy;
+        B.p2g_block = launch.p2g_block;
         B.gv = gv;
         B.trS = trS;
         B.trT = trT;
@@ -468,6 +469,9 @@ struct mpm_engine {
         return MPM_OK;
     }
     int push() {
+        // Any host change may have touched the pellets (state upload, restore):
+        // the next trial recomputes their geometry.
+        for (int i = 0; i < ninst; ++i) h_is[i].geo_stale = 1;
         HIP_OK(hipMemcpyAsync(d_is, h_is, sizeof(InstState) * ninst, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
         return MPM_OK;
@@ -1123,59 +1127,105 @@ struct mpm_engine {
         snap.bufs.clear();
     }
 
-    // Time `steps` trial steps of every instance under each candidate launch shape,
-    // from the same state, and keep the fastest.
+    // Time max(steps, kMinTuneSteps) trial steps of every instance under each
+    // candidate launch shape, from the same state, and keep the fastest.  Two
+    // stages:
+    //  1. the kernel shapes -- cell blocks per work item (tile size and workgroup
+    //     count), the G2P register budget (workgroups per CU) and the deposit
+    //     workgroup size -- by the device time of the deposit and G2P kernels,
+    //     stamped on the device clock in stream mode (the kernels these choices
+    //     change; no host noise);
+    //  2. the synchronisation interval and graph replay with those kernels, by
+    //     wall time, best of kRepeats (the first run of a graph candidate
+    //     includes its instantiation).
+    // The count spans two of the longest synchronisation intervals, so the
+    // candidates differ in how often the host synchronises as they will in use.
     int autotune(int steps, mpm_launch* chosen) {
         if (int rc = pull()) return rc;
         if (int rc = prepare()) return rc;
         if (int rc = flush_refkeys()) return rc;
         Snapshot snap;
         int rc = snapshot(snap);
-        bool prof = profiling;
-        profiling = false;
-        // Candidates: synchronisation interval, graph replay, cell blocks per
-        // work item (tile size and workgroup count) and the G2P register budget
-        // (workgroups per CU).  Each is timed `kRepeats` times from the same
-        // state (the first run of a graph candidate includes its instantiation),
-        // and its fastest run counts.
+        // The caller's kernel timing, kept across the tuner's own.
+        const bool prof = profiling;
+        const int prof_every = profile_every;
+        const uint64_t prof_seq = profile_seq, prof_launches = profiled_launches;
+        const double prof_g2p = g2p_ms, prof_p2g = p2g_ms;
+        constexpr int kMinTuneSteps = 64;
         constexpr int kRepeats = 3;
-        std::vector<mpm_launch> cands;
-        for (int k : {8, 32})
-            for (int g : {0, 1})
-                for (int m : {1, 2})
-                    for (int o : {2, 3}) cands.push_back(mpm_launch{k, g, m, launch.traction_wgs, o});
+        const int timed = std::max(steps, kMinTuneSteps);
+        // One timed run of `c` from the snapshot; wall ms in *wall, mean kernel
+        // ms per step in *kern (profiled runs).
+        auto trial = [&](const mpm_launch& c, bool prof_run, double* wall, double* kern) -> int {
+            if (int r = restore(snap)) return r;
+            launch = c;
+            graph_stale = true;
+            if (int r = prepare()) return r;
+            for (int i = 0; i < ninst; ++i) {
+                InstState& s = h_is[i];
+                s.target = std::numeric_limits<double>::max();
+                s.mode = s.n > 0 ? M_RUN : M_DONE;
+                s.pause = 0;
+                s.trial_dt = 0.0;
+                s.budget = timed;
+            }
+            if (int r = push()) return r;
+            profiling = prof_run;
+            profile_every = 1;
+            g2p_ms = p2g_ms = 0.0;
+            profiled_launches = 0;
+            auto t0 = std::chrono::steady_clock::now();
+            int r = run(c.steps_per_sync, nullptr, nullptr);
+            *wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            *kern = profiled_launches ? (g2p_ms + p2g_ms) / (double)profiled_launches
+                                      : std::numeric_limits<double>::infinity();
+            profiling = false;
+            return r;
+        };
         mpm_launch best = launch;
-        double best_ms = std::numeric_limits<double>::infinity();
-        for (const mpm_launch& c : cands) {
-            double cand_ms = std::numeric_limits<double>::infinity();
-            for (int rep = 0; rep < kRepeats && !rc; ++rep) {
-                if ((rc = restore(snap))) break;
-                launch = c;
-                if (rep == 0) graph_stale = true;
-                if ((rc = prepare())) break;
-                for (int i = 0; i < ninst; ++i) {
-                    InstState& s = h_is[i];
-                    s.target = std::numeric_limits<double>::max();
-                    s.mode = s.n > 0 ? M_RUN : M_DONE;
-                    s.pause = 0;
-                    s.trial_dt = 0.0;
-                    s.budget = steps;
+        {
+            double best_kern = std::numeric_limits<double>::infinity();
+            for (int m : {1, 2})
+                for (int o : {2, 3})
+                    for (int pb : {256, 512}) {
+                        const mpm_launch c{32, 0, m, launch.traction_wgs, o, pb};
+                        double wall = 0.0, kern = 0.0;
+                        if ((rc = trial(c, true, &wall, &kern))) break;
+                        if (kern < best_kern) {
+                            best_kern = kern;
+                            best = c;
+                        }
+                    }
+        }
+        if (!rc) {
+            double best_ms = std::numeric_limits<double>::infinity();
+            const mpm_launch kernels = best;
+            for (int k : {8, 32})
+                for (int g : {0, 1}) {
+                    mpm_launch c = kernels;
+                    c.steps_per_sync = k;
+                    c.use_graph = g;
+                    double cand_ms = std::numeric_limits<double>::infinity();
+                    for (int rep = 0; rep < kRepeats && !rc; ++rep) {
+                        double wall = 0.0, kern = 0.0;
+                        if ((rc = trial(c, false, &wall, &kern))) break;
+                        cand_ms = std::min(cand_ms, wall);
+                    }
+                    if (rc) break;
+                    if (cand_ms < best_ms) {
+                        best_ms = cand_ms;
+                        best = c;
+                    }
                 }
-                if ((rc = push())) break;
-                auto t0 = std::chrono::steady_clock::now();
-                if ((rc = run(std::min(c.steps_per_sync, steps), nullptr, nullptr))) break;
-                cand_ms = std::min(cand_ms, std::chrono::duration<double, std::milli>(
-                                                std::chrono::steady_clock::now() - t0).count());
-            }
-            if (rc) break;
-            if (cand_ms < best_ms) {
-                best_ms = cand_ms;
-                best = c;
-            }
         }
         if (!rc) rc = restore(snap);
         release(snap);
         profiling = prof;
+        profile_every = prof_every;
+        profile_seq = prof_seq;
+        profiled_launches = prof_launches;
+        g2p_ms = prof_g2p;
+        p2g_ms = prof_p2g;
         launch = best;
         graph_stale = true;
         dirty = true;
@@ -2200,7 +2250,7 @@ int mpm_set_launch(mpm_engine* e, const mpm_launch* l) {
     // More than two cell blocks per work item would not fit the LDS tiles.
     if (l->steps_per_sync < 1 || l->steps_per_sync > kMaxStepsPerSync || l->merge_blocks < 1 ||
         l->merge_blocks > 2 || l->traction_wgs < 1 || l->traction_wgs > 65536 || l->g2p_occupancy < 2 ||
-        l->g2p_occupancy > 3)
+        l->g2p_occupancy > 3 || (l->p2g_block != 256 && l->p2g_block != 512))
         return e->fail(MPM_ERR_INVALID, "launch shape out of range");
     bool resort = l->merge_blocks != e->launch.merge_blocks;
     e->launch = *l;

@@ -155,7 +155,8 @@ struct InstState {
     unsigned long long commit_clock;  // device constant clock at the last accepted step
     int refsort;      // the last accepted step hit the 32-step cadence; its order keys are pending
     int committed;    // a trial step was accepted since the state was set
-    int pad;
+    int geo_stale;    // the host changed the state: pellet geometry and coupling sums need recomputing
+    double pel_box[6];  // union of the pellets' boxes at the start of the trial (lo, hi)
 };
 
 // Exact fixed-point sum of f64 values in units of 2^-kFixBits, as a two's
@@ -230,6 +231,7 @@ struct BatchDev {
     float* shared[2];
     int sh;       // shared-slab selector (host controlled)
     int g2p_occupancy;  // G2P workgroups resident per CU (2 or 3: its register budget)
+    int p2g_block;      // threads of a deposit workgroup (256 or 512)
     // Kernel timing (null: off): the deposit's and the G2P's earliest workgroup
     // start and latest workgroup end on the device's constant clock,
     // {p2g start, p2g end, g2p start, g2p end}, set up as {max, 0, max, 0}.

@@ -122,105 +122,162 @@ __device__ inline uint64_t ref_cell(const InstParams& ip, float x, float y, floa
 
 // ------------------------------------------------------------ control
 
-// Simulation::stability_limit, solver.rs:477-498.
-__device__ inline double stability_limit(const InstParams& ip, const InstState& s) {
-    double limit = ip.cfl * ip.dx / (ip.wave + s.max_speed);
-    if (ip.diff > 0.0) limit = fmin(limit, 0.25 * ip.dx * ip.dx / ip.diff);
-    if (ip.npel > 0) {
-        // min over pellets of cfl dx / |v_k| is cfl dx / max_k |v_k|
-        limit = fmin(limit, ip.pel_contact_dt);
-        if (s.pel_vmax > 0.0) limit = fmin(limit, ip.cfl * ip.dx / s.pel_vmax);
+// A pellet's start-of-step geometry for the coupling kernel (its axis, angular
+// velocity and box) from its committed state `p`; coupling sums cleared.
+__device__ inline PelletGeo pellet_geometry(const BatchDev& B, int k, const PelletState& p) {
+    PelletGeo g;
+    const double xh[3] = {1.0, 0.0, 0.0};
+    qrot(p.q, xh, g.axis);
+    pellet_omega(p.q, p.L, p.inertia, g.omega);
+    double half = 0.0;
+    for (int q = 0; q < p.spheres; ++q) half = fmax(half, fabs(p.offsets[q]));
+    for (int a = 0; a < 3; ++a) {
+        g.pos[a] = p.pos[a];
+        g.vel[a] = p.vel[a];
+        double reach = fabs(g.axis[a]) * half + p.radius;
+        g.lo[a] = p.pos[a] - reach;
+        g.hi[a] = p.pos[a] + reach;
     }
-    return limit;
+    g.radius = p.radius;
+    g.spheres = p.spheres;
+    g.inst = p.inst;
+    B.pgeo[k] = g;
+    PelletAccum z = {};
+    B.pacc[k] = z;
+    return g;
 }
 
-__global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
-    for (int t = threadIdx.x; t < B.ninst; t += 256) {
-        InstState& s = B.is[t];
-        const InstParams& ip = B.ip[t];
-        s.active = 0;
-        if (s.mode == M_RUN) {
-            if (s.trial_dt > 0.0) {
-                // A bare trial step of a given size (solver.rs:535-537).
-                s.dt = s.trial_dt;
-                s.limit = 0.0;
-                s.limited = 0;
-                s.active = 1;
-            } else if (!(s.time < s.target)) {
-                s.time = s.target;  // solver.rs:470
-                s.mode = M_DONE;
-            } else if (s.budget <= 0) {
-                s.mode = M_PAUSE;
-                s.pause |= P_BUDGET;
+// advance_while's choice of the next trial (solver.rs:425-443) for one
+// instance, and the trial's deposit scales.  Returns the instance's near-wall
+// count if it runs a trial, else 0.  Every field is read before the first
+// store, so the loads go out together.
+__device__ inline int control_instance(InstState& s, const InstParams& ip) {
+    const int mode = s.mode, budget = s.budget, cur = s.cur, nwc = s.nw_count[s.cur];
+    const double trial_dt = s.trial_dt, time = s.time, target = s.target, dt_scale = s.dt_scale;
+    const double max_speed = s.max_speed, pel_vmax = s.pel_vmax;
+    const double b1 = s.bound[0], b2 = s.bound[1];
+    const double cfl = ip.cfl, dx = ip.dx, wave = ip.wave, diff = ip.diff, pel_contact_dt = ip.pel_contact_dt;
+    const double max_dt = ip.max_dt, min_dt = ip.min_dt;
+    const int npel = ip.npel;
+    const double mass_max = (double)ip.mass_max;
+    const double gmax = fmax(fmax(fabs(ip.gravity[0]), fabs(ip.gravity[1])), fabs(ip.gravity[2]));
+    int active = 0;
+    bool set_dt = false;
+    double dt = 0.0;
+    if (mode == M_RUN) {
+        if (trial_dt > 0.0) {
+            // A bare trial step of a given size (solver.rs:535-537).
+            dt = trial_dt;
+            set_dt = true;
+            s.limit = 0.0;
+            s.limited = 0;
+            active = 1;
+        } else if (!(time < target)) {
+            s.time = target;  // solver.rs:470
+            s.mode = M_DONE;
+        } else if (budget <= 0) {
+            s.mode = M_PAUSE;
+            s.pause |= P_BUDGET;
+        } else {
+            // solver.rs:425-443, with the stability limit of solver.rs:477-498
+            const double remaining = target - time;
+            double limit = cfl * dx / (wave + max_speed);
+            if (diff > 0.0) limit = fmin(limit, 0.25 * dx * dx / diff);
+            if (npel > 0) {
+                // min over pellets of cfl dx / |v_k| is cfl dx / max_k |v_k|
+                limit = fmin(limit, pel_contact_dt);
+                if (pel_vmax > 0.0) limit = fmin(limit, cfl * dx / pel_vmax);
+            }
+            const double scaled = dt_scale * limit;
+            const int limited = scaled < max_dt;
+            dt = fmin(fmin(scaled, max_dt), remaining);
+            if (remaining - dt < 1e-12 * fmax(target, 1.0)) dt = remaining;
+            set_dt = true;
+            if (!isfinite(dt) || dt <= 0.0 || (dt < min_dt && remaining > min_dt)) {
+                s.mode = M_ERROR;
+                s.err = E_MIN_STEP;
             } else {
-                // solver.rs:425-443
-                double remaining = s.target - s.time;
-                double limit = stability_limit(ip, s);
-                double scaled = s.dt_scale * limit;
-                int limited = scaled < ip.max_dt;
-                double dt = fmin(fmin(scaled, ip.max_dt), remaining);
-                if (remaining - dt < 1e-12 * fmax(s.target, 1.0)) dt = remaining;
-                if (!isfinite(dt) || dt <= 0.0 || (dt < ip.min_dt && remaining > ip.min_dt)) {
-                    s.mode = M_ERROR;
-                    s.err = E_MIN_STEP;
-                    s.dt = dt;
-                } else {
-                    s.dt = dt;
-                    s.limit = limit;
-                    s.limited = limited;
-                    s.active = 1;
-                }
+                s.limit = limit;
+                s.limited = limited;
+                active = 1;
             }
         }
-        if (s.active) {
-            StepStatus z = {};
-            z.fail = ~0ull;
-            s.st = z;
-            s.nw_count[s.cur ^ 1] = 0;
-            // Deposit scales of this trial from the committed state's bounds
-            // (particle_bounds): no contribution reaches 2^29 units.
-            const double dt = s.dt, b1 = s.bound[0], b2 = s.bound[1];
-            const double gmax = fmax(fmax(fabs(ip.gravity[0]), fabs(ip.gravity[1])), fabs(ip.gravity[2]));
-            s.sc_m = deposit_scale((double)ip.mass_max);
-            s.sc_p = deposit_scale(b1 + dt * b2);
-            s.sc_mom = deposit_scale(b1);
-            s.sc_f = deposit_scale(b2 * 0.25 * ip.dx * ip.dx);
-            s.sc_t = deposit_scale(dt * (b2 + 7.0 * (double)ip.mass_max * gmax));
-        }
+    }
+    s.active = active;
+    if (set_dt) s.dt = dt;
+    if (!active) return 0;
+    StepStatus z = {};
+    z.fail = ~0ull;
+    s.st = z;
+    s.nw_count[cur ^ 1] = 0;
+    // Deposit scales of this trial from the committed state's bounds
+    // (particle_bounds): no contribution reaches 2^29 units.
+    s.sc_m = deposit_scale(mass_max);
+    s.sc_p = deposit_scale(b1 + dt * b2);
+    s.sc_mom = deposit_scale(b1);
+    s.sc_f = deposit_scale(b2 * 0.25 * dx * dx);
+    s.sc_t = deposit_scale(dt * (b2 + 7.0 * mass_max * gmax));
+    return nwc;
+}
+
+// Step control of every instance.  Thread t takes a contiguous run of
+// instances; a block scan of their near-wall counts gives each running
+// instance its range in the concatenated near-wall list (in instance order),
+// which the traction kernel walks.  Pellet geometry is recomputed here only for
+// instances whose pellets the host has touched since the last finalize (which
+// keeps it current otherwise).
+__global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
+    __shared__ int sscan[256];
+    __shared__ int sstale;
+    const int tid = threadIdx.x;
+    const int per = (B.ninst + 255) / 256;
+    const int t0 = min(tid * per, B.ninst), t1 = min(t0 + per, B.ninst);
+    if (tid == 0) sstale = 0;
+    int local = 0;
+    bool stale = false;
+    for (int t = t0; t < t1; ++t) {
+        InstState& s = B.is[t];
+        const int nwc = control_instance(s, B.ip[t]);
+        local += nwc;
+        stale = stale || (s.active && s.geo_stale && B.ip[t].npel > 0);
+    }
+    sscan[tid] = local;
+    __syncthreads();
+    if (stale) sstale = 1;
+    for (int o = 1; o < 256; o <<= 1) {
+        const int v = tid >= o ? sscan[tid - o] : 0;
+        __syncthreads();
+        sscan[tid] += v;
+        __syncthreads();
+    }
+    int acc = sscan[tid] - local;
+    for (int t = t0; t < t1; ++t) {
+        InstState& s = B.is[t];
+        acc += s.active ? s.nw_count[s.cur] : 0;
+        s.nw_base = acc;
+    }
+    if (tid == 255) B.counters[1] = sscan[255];
+    if (!sstale) return;
+    // Start-of-step pellet geometry and zeroed coupling accumulators of the
+    // stale running instances; their coupling kernel skips no node block.
+    __syncthreads();
+    for (int k = tid; k < B.npel; k += 256) {
+        const int inst = B.pel[k].inst;
+        if (inst < 0) continue;
+        const InstState& s = B.is[inst];
+        if (!s.active || !s.geo_stale) continue;
+        const PelletState p = B.pel[k];
+        pellet_geometry(B, k, p);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < B.ninst; ++t) {
-            InstState& s = B.is[t];
-            acc += s.active ? s.nw_count[s.cur] : 0;
-            s.nw_base = acc;
-        }
-        B.counters[1] = acc;
-    }
-    // Start-of-step pellet geometry and zeroed coupling accumulators.
-    for (int k = threadIdx.x; k < B.npel; k += 256) {
-        const PelletState& p = B.pel[k];
-        if (p.inst < 0 || !B.is[p.inst].active) continue;
-        PelletGeo g;
-        const double xh[3] = {1.0, 0.0, 0.0};
-        qrot(p.q, xh, g.axis);
-        pellet_omega(p.q, p.L, p.inertia, g.omega);
-        double half = 0.0;
-        for (int q = 0; q < p.spheres; ++q) half = fmax(half, fabs(p.offsets[q]));
+    for (int t = t0; t < t1; ++t) {
+        InstState& s = B.is[t];
+        if (!s.active || !s.geo_stale) continue;
         for (int a = 0; a < 3; ++a) {
-            g.pos[a] = p.pos[a];
-            g.vel[a] = p.vel[a];
-            double reach = fabs(g.axis[a]) * half + p.radius;
-            g.lo[a] = p.pos[a] - reach;
-            g.hi[a] = p.pos[a] + reach;
+            s.pel_box[a] = -INFINITY;
+            s.pel_box[3 + a] = INFINITY;
         }
-        g.radius = p.radius;
-        g.spheres = p.spheres;
-        g.inst = p.inst;
-        B.pgeo[k] = g;
-        PelletAccum z = {};
-        B.pacc[k] = z;
+        s.geo_stale = 0;
     }
 }
 
@@ -235,10 +292,12 @@ __global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
 // the LDS budget, deposit with global atomics; a particle gives the same
 // integers either way (deposit_particle).  At the 32-step cadence it also
 // writes the reference-order keys of the committed positions.
-// NF = kDepositFields, or kAuditFields for the energy audit.
-template <int NF>
-__global__ __launch_bounds__(kBlock) void p2g_kernel(BatchDev B) {
-    constexpr int R = kChunk / kBlock;
+// NF = kDepositFields, or kAuditFields for the energy audit.  BS: threads per
+// workgroup, 256 (four rounds of a full item) or 512 (two rounds, the tile's
+// clearing and flush over twice the threads); the launch shape picks it.
+template <int NF, int BS>
+__global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
+    constexpr int R = kChunk / BS;
     __shared__ __align__(16) unsigned long long ss[NF * kTileCap];
     if (B.ktime) kernel_start(B.ktime);
     const int4 item = B.work[blockIdx.x];
@@ -271,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void p2g_kernel(BatchDev B) {
         t.tiled = ok && sz <= kTileCap;
     }
     if (t.tiled)
-        for (int q = tid; q < NF * kTileCap / 2; q += kBlock)
+        for (int q = tid; q < NF * kTileCap / 2; q += BS)
             reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
     __syncthreads();
     const int rot = (lane * 5) % 9;
@@ -279,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void p2g_kernel(BatchDev B) {
     const int end = item.y + item.z;
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
-        const int p = item.y + r * kBlock + tid;
+        const int p = item.y + r * BS + tid;
         if (p >= end) continue;
         const float x = src[PX * cap + p], y = src[PY * cap + p], z = src[PZ * cap + p];
         if (is.refsort) B.refkey[p] = (ref_cell(ip, x, y, z) << 32) | (uint64_t)B.ids[p];
@@ -329,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void p2g_kernel(BatchDev B) {
     // contribution has mass (mass rounds up), so the mass word marks it.
     const int sz = t.gd[0] * s12;
     const float ig2 = 1.0f / (float)t.gd[2], ig1 = 1.0f / (float)t.gd[1];
-    for (int q = tid; q < sz; q += kBlock) {
+    for (int q = tid; q < sz; q += BS) {
         const unsigned long long qm = ss[q];
         if (qm == 0ull) continue;
         const int rest = div_small(q, ig2), k = q - rest * t.gd[2];
@@ -673,9 +732,14 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
 
 // couple_grid (rigid.rs:385-438) for the pellets of the node block's instance in
 // pellet index order, so a node sees the pellets one after another as in the
-// sequential loop over pellets.  The wavefront first tests 64 pellets at a time
-// against the block's node box, then walks the hits in index order.  f64 with the
-// reference's operation order; node positions are (i - 1) dx exactly.
+// sequential loop over pellets.  A block outside the union of the pellets'
+// boxes returns at once.  The wavefront tests 256 pellets at a time against the
+// block's node box (the boxes' loads issued together), then walks the hits in
+// index order.  f64 with the reference's operation order; node positions are
+// (i - 1) dx exactly.  Each pellet's impulse, angular impulse and energy
+// change over the block's constrained nodes are summed across the wavefront in
+// lane order (a fixed order) and added to the pellet's exact fixed-point sums
+// by one lane, so the totals do not depend on the order of the blocks.
 __global__ __launch_bounds__(64) void couple_kernel(BatchDev B) {
     const int b = blockIdx.x, l = threadIdx.x;
     const int inst = B.blk_inst[b];
@@ -684,14 +748,7 @@ __global__ __launch_bounds__(64) void couple_kernel(BatchDev B) {
     if (!s.active || ip.npel == 0) return;
     GridPtrs A = grid_set(B, s.gcur);
     if (!A.flag[b]) return;
-    const size_t n = ((size_t)b << 6) | l;
-    const size_t na = B.na;
-    // The node mass as the grid update holds it (f32 of the exact sum).
-    const double m = (double)(float)((double)A.v[n] * (1.0 / (double)s.sc_m));
-    int ci, cj, ck;
-    node_coords(ip.P, (int)((size_t)(b - ip.boff) << 6) | l, ci, cj, ck);
     const double dx = ip.dx;
-    const double node[3] = {((double)ci - 1.0) * dx, ((double)cj - 1.0) * dx, ((double)ck - 1.0) * dx};
     // Node box of the block (node coordinates 4 bi .. 4 bi + 3 per axis).
     double blo[3], bhi[3];
     {
@@ -702,77 +759,117 @@ __global__ __launch_bounds__(64) void couple_kernel(BatchDev B) {
             bhi[a] = ((double)c0[a] + 2.0) * dx;
         }
     }
+    if (bhi[0] < s.pel_box[0] || blo[0] > s.pel_box[3] || bhi[1] < s.pel_box[1] || blo[1] > s.pel_box[4] ||
+        bhi[2] < s.pel_box[2] || blo[2] > s.pel_box[5])
+        return;
+    const size_t n = ((size_t)b << 6) | l;
+    const size_t na = B.na;
+    // The node mass as the grid update holds it (f32 of the exact sum).
+    const double m = (double)(float)((double)A.v[n] * (1.0 / (double)s.sc_m));
+    int ci, cj, ck;
+    node_coords(ip.P, (int)((size_t)(b - ip.boff) << 6) | l, ci, cj, ck);
+    const double node[3] = {((double)ci - 1.0) * dx, ((double)cj - 1.0) * dx, ((double)ck - 1.0) * dx};
     double v[3] = {B.gv[n], B.gv[na + n], B.gv[2 * na + n]};
     bool changed = false;
-    for (int c0 = 0; c0 < ip.npel; c0 += 64) {
-        bool hit = false;
-        if (c0 + l < ip.npel) {
-            const PelletGeo& g = B.pgeo[ip.pel_off + c0 + l];
-            hit = !(bhi[0] < g.lo[0] || blo[0] > g.hi[0] || bhi[1] < g.lo[1] || blo[1] > g.hi[1] ||
-                    bhi[2] < g.lo[2] || blo[2] > g.hi[2]);
+    constexpr int kGroups = 4;
+    for (int c0 = 0; c0 < ip.npel; c0 += 64 * kGroups) {
+        unsigned long long masks[kGroups];
+        {
+            double lo[kGroups][3], hi[kGroups][3];
+            bool in[kGroups];
+#pragma unroll
+            for (int u = 0; u < kGroups; ++u) {
+                const int k = c0 + 64 * u + l;
+                in[u] = k < ip.npel;
+                const PelletGeo& g = B.pgeo[ip.pel_off + (in[u] ? k : 0)];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    lo[u][a] = g.lo[a];
+                    hi[u][a] = g.hi[a];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kGroups; ++u) {
+                const bool hit = in[u] && !(bhi[0] < lo[u][0] || blo[0] > hi[u][0] || bhi[1] < lo[u][1] ||
+                                            blo[1] > hi[u][1] || bhi[2] < lo[u][2] || blo[2] > hi[u][2]);
+                masks[u] = __ballot(hit);
+            }
         }
-        unsigned long long mask = __ballot(hit);
-        while (mask) {
-            int k = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            const int pi = ip.pel_off + c0 + k;
-            const PelletGeo& g = B.pgeo[pi];
-            if (!(m > 0.0)) continue;
-            if (node[0] < g.lo[0] || node[0] > g.hi[0] || node[1] < g.lo[1] || node[1] > g.hi[1] ||
-                node[2] < g.lo[2] || node[2] > g.hi[2])
-                continue;
-            const double* offs = B.pel[pi].offsets;
-            double best = INFINITY, nrm[3] = {0.0, 0.0, 1.0};
-            for (int q = 0; q < g.spheres; ++q) {
-                double o = offs[q];
-                double dlt[3] = {node[0] - (g.pos[0] + g.axis[0] * o), node[1] - (g.pos[1] + g.axis[1] * o),
-                                 node[2] - (g.pos[2] + g.axis[2] * o)};
-                double dist = sqrt(dlt[0] * dlt[0] + dlt[1] * dlt[1] + dlt[2] * dlt[2]);
-                double phi = dist - g.radius;
-                if (phi < best) {
-                    best = phi;
-                    if (dist > 1e-12) {
-                        nrm[0] = dlt[0] / dist;
-                        nrm[1] = dlt[1] / dist;
-                        nrm[2] = dlt[2] / dist;
-                    } else {
-                        nrm[0] = 0.0;
-                        nrm[1] = 0.0;
-                        nrm[2] = 1.0;
+        for (int u = 0; u < kGroups; ++u) {
+            unsigned long long mask = masks[u];
+            while (mask) {
+                const int k = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const int pi = ip.pel_off + c0 + 64 * u + k;
+                const PelletGeo& g = B.pgeo[pi];
+                bool act = !(m <= 0.0) && !(node[0] < g.lo[0] || node[0] > g.hi[0] || node[1] < g.lo[1] ||
+                                        node[1] > g.hi[1] || node[2] < g.lo[2] || node[2] > g.hi[2]);
+                double sums[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                if (act) {
+                    const double* offs = B.pel[pi].offsets;
+                    double best = INFINITY, nrm[3] = {0.0, 0.0, 1.0};
+                    for (int q = 0; q < g.spheres; ++q) {
+                        double o = offs[q];
+                        double dlt[3] = {node[0] - (g.pos[0] + g.axis[0] * o), node[1] - (g.pos[1] + g.axis[1] * o),
+                                         node[2] - (g.pos[2] + g.axis[2] * o)};
+                        double dist = sqrt(dlt[0] * dlt[0] + dlt[1] * dlt[1] + dlt[2] * dlt[2]);
+                        double phi = dist - g.radius;
+                        if (phi < best) {
+                            best = phi;
+                            if (dist > 1e-12) {
+                                nrm[0] = dlt[0] / dist;
+                                nrm[1] = dlt[1] / dist;
+                                nrm[2] = dlt[2] / dist;
+                            } else {
+                                nrm[0] = 0.0;
+                                nrm[1] = 0.0;
+                                nrm[2] = 1.0;
+                            }
+                        }
+                    }
+                    double r[3] = {node[0] - g.pos[0], node[1] - g.pos[1], node[2] - g.pos[2]};
+                    double body[3] = {g.vel[0] + (g.omega[1] * r[2] - g.omega[2] * r[1]),
+                                      g.vel[1] + (g.omega[2] * r[0] - g.omega[0] * r[2]),
+                                      g.vel[2] + (g.omega[0] * r[1] - g.omega[1] * r[0])};
+                    double rel[3] = {v[0] - body[0], v[1] - body[1], v[2] - body[2]};
+                    double ns = rel[0] * nrm[0] + rel[1] * nrm[1] + rel[2] * nrm[2];
+                    act = !(best > 0.0) && !(ns >= 0.0);  // rigid.rs:401-410: outside, or separating
+                    if (act) {
+                        double t[3] = {rel[0] - nrm[0] * ns, rel[1] - nrm[1] * ns, rel[2] - nrm[2] * ns};
+                        double ts = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+                        double red = ip.wall_friction * (-ns);
+                        double f = (ts <= red || ts < 1e-14) ? 0.0 : (ts - red) / ts;
+                        double nv[3] = {body[0] + t[0] * f, body[1] + t[1] * f, body[2] + t[2] * f};
+                        double dp[3] = {(nv[0] - v[0]) * m, (nv[1] - v[1]) * m, (nv[2] - v[2]) * m};
+                        sums[0] = -dp[0];
+                        sums[1] = -dp[1];
+                        sums[2] = -dp[2];
+                        sums[3] = -(r[1] * dp[2] - r[2] * dp[1]);
+                        sums[4] = -(r[2] * dp[0] - r[0] * dp[2]);
+                        sums[5] = -(r[0] * dp[1] - r[1] * dp[0]);
+                        sums[6] = 0.5 * m *
+                                  ((nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) -
+                                   (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+                        v[0] = nv[0];
+                        v[1] = nv[1];
+                        v[2] = nv[2];
+                        changed = true;
+                    }
+                }
+                const unsigned long long cm = __ballot(act);
+                if (cm) {
+#pragma unroll
+                    for (int i = 0; i < 7; ++i) sums[i] = wave_sum_last(sums[i]);
+                    if (l == 63) {
+                        PelletAccum* acc = B.pacc + pi;
+                        Fix128 add[7];
+#pragma unroll
+                        for (int i = 0; i < 7; ++i) add[i] = fix_from(sums[i]);
+                        fix_atomic_add_n<7>(acc->sum, add);
+                        atomicAdd(&acc->constrained, (unsigned long long)__popcll(cm));
                     }
                 }
             }
-            if (best > 0.0) continue;
-            double r[3] = {node[0] - g.pos[0], node[1] - g.pos[1], node[2] - g.pos[2]};
-            double body[3] = {g.vel[0] + (g.omega[1] * r[2] - g.omega[2] * r[1]),
-                              g.vel[1] + (g.omega[2] * r[0] - g.omega[0] * r[2]),
-                              g.vel[2] + (g.omega[0] * r[1] - g.omega[1] * r[0])};
-            double rel[3] = {v[0] - body[0], v[1] - body[1], v[2] - body[2]};
-            double ns = rel[0] * nrm[0] + rel[1] * nrm[1] + rel[2] * nrm[2];
-            if (ns >= 0.0) continue;
-            double t[3] = {rel[0] - nrm[0] * ns, rel[1] - nrm[1] * ns, rel[2] - nrm[2] * ns};
-            double ts = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-            double red = ip.wall_friction * (-ns);
-            double f = (ts <= red || ts < 1e-14) ? 0.0 : (ts - red) / ts;
-            double nv[3] = {body[0] + t[0] * f, body[1] + t[1] * f, body[2] + t[2] * f};
-            double dp[3] = {(nv[0] - v[0]) * m, (nv[1] - v[1]) * m, (nv[2] - v[2]) * m};
-            double e = 0.5 * m *
-                       ((nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) - (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-            // Exact fixed-point sums: the pellet's totals do not depend on the
-            // order in which the nodes reach them.
-            PelletAccum* acc = B.pacc + pi;
-            fix_atomic_add(&acc->sum[0], fix_from(-dp[0]));
-            fix_atomic_add(&acc->sum[1], fix_from(-dp[1]));
-            fix_atomic_add(&acc->sum[2], fix_from(-dp[2]));
-            fix_atomic_add(&acc->sum[3], fix_from(-(r[1] * dp[2] - r[2] * dp[1])));
-            fix_atomic_add(&acc->sum[4], fix_from(-(r[2] * dp[0] - r[0] * dp[2])));
-            fix_atomic_add(&acc->sum[5], fix_from(-(r[0] * dp[1] - r[1] * dp[0])));
-            fix_atomic_add(&acc->sum[6], fix_from(e));
-            atomicAdd(&acc->constrained, 1ull);
-            v[0] = nv[0];
-            v[1] = nv[1];
-            v[2] = nv[2];
-            changed = true;
         }
     }
     if (changed) {
@@ -1392,11 +1489,25 @@ __device__ inline double block_total(double v, double* sd) {
 }
 
 // One workgroup per running instance: fixed-order reduction of its grid and
-// particle partials and of its pellets' step sums, then the accept/reject.
+// particle partials and of its pellets' step sums, then the accept/reject.  The
+// instance state is staged in LDS (one round trip for all of it), committed
+// there by one thread and written back by all.  The instance's pellets are then
+// committed (rigid.rs:334, solver.rs:600-609) and their geometry for the next
+// trial computed from the committed state, with cleared coupling sums.
 __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     __shared__ double sd[4];
+    __shared__ InstState ls;
+    static_assert(sizeof(InstState) % 4 == 0, "InstState is staged as 32-bit words");
+    constexpr int kWords = sizeof(InstState) / 4;
     const int inst = blockIdx.x, tid = threadIdx.x;
-    InstState& s = B.is[inst];
+    InstState& gs = B.is[inst];
+    {
+        const unsigned* src = reinterpret_cast<const unsigned*>(&gs);
+        unsigned* dst = reinterpret_cast<unsigned*>(&ls);
+        for (int w = tid; w < kWords; w += 256) dst[w] = src[w];
+    }
+    __syncthreads();
+    InstState& s = ls;
     if (!s.active) return;
     const InstParams& ip = B.ip[inst];
     const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
@@ -1557,23 +1668,58 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         const double pv = fmax(fmax(svm[0], svm[1]), fmax(svm[2], svm[3]));
         commit_instance(s, ip, pel, pv);
     }
-}
-
-// Commit every accepted instance's trial pellets (rigid.rs:334, solver.rs:600-609).
-__global__ void pellet_commit_kernel(BatchDev B) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= B.npel) return;
-    PelletState& p = B.pel[k];
-    if (p.inst < 0) return;
-    const InstState& s = B.is[p.inst];
-    if (!s.active || s.last_failure != 0) return;
-    const PelletTrial& t = B.ptrial[k];
-    for (int c = 0; c < 3; ++c) {
-        p.pos[c] = t.pos[c];
-        p.vel[c] = t.vel[c];
-        p.L[c] = t.L[c];
+    __syncthreads();
+    if (ip.npel > 0) {
+        // The instance's pellets: commit an accepted trial, then the next
+        // trial's geometry and cleared sums, and the union of the pellets' boxes.
+        const bool accepted = s.last_failure == 0;
+        double box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int k = tid; k < ip.npel; k += 256) {
+            const int pi = ip.pel_off + k;
+            PelletState p = B.pel[pi];
+            if (accepted) {
+                const PelletTrial& t = B.ptrial[pi];
+                for (int c = 0; c < 3; ++c) {
+                    p.pos[c] = t.pos[c];
+                    p.vel[c] = t.vel[c];
+                    p.L[c] = t.L[c];
+                }
+                for (int c = 0; c < 4; ++c) p.q[c] = t.q[c];
+                PelletState& dst = B.pel[pi];
+                for (int c = 0; c < 3; ++c) {
+                    dst.pos[c] = p.pos[c];
+                    dst.vel[c] = p.vel[c];
+                    dst.L[c] = p.L[c];
+                }
+                for (int c = 0; c < 4; ++c) dst.q[c] = p.q[c];
+            }
+            const PelletGeo g = pellet_geometry(B, pi, p);
+            for (int a = 0; a < 3; ++a) {
+                box[a] = fmin(box[a], g.lo[a]);
+                box[3 + a] = fmax(box[3 + a], g.hi[a]);
+            }
+        }
+        __shared__ double sbox[4][6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            box[a] = -wave_max_d(-box[a]);
+            box[3 + a] = wave_max_d(box[3 + a]);
+        }
+        if (lane == 0)
+            for (int a = 0; a < 6; ++a) sbox[wave][a] = box[a];
+        __syncthreads();
+        if (tid < 3) {
+            s.pel_box[tid] = fmin(fmin(sbox[0][tid], sbox[1][tid]), fmin(sbox[2][tid], sbox[3][tid]));
+            s.pel_box[3 + tid] =
+                fmax(fmax(sbox[0][3 + tid], sbox[1][3 + tid]), fmax(sbox[2][3 + tid], sbox[3][3 + tid]));
+        }
+        __syncthreads();
     }
-    for (int c = 0; c < 4; ++c) p.q[c] = t.q[c];
+    {
+        const unsigned* src = reinterpret_cast<const unsigned*>(&ls);
+        unsigned* dst = reinterpret_cast<unsigned*>(&gs);
+        for (int w = tid; w < kWords; w += 256) dst[w] = src[w];
+    }
 }
 
 // Order keys at the 32-step cadence, for the instances whose accepted step hit
@@ -1789,9 +1935,11 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
     control_kernel<<<1, 256, 0, s>>>(B);
     if (B.nwork > 0) {
         if (B.apart)
-            p2g_kernel<kAuditFields><<<B.nwork, kBlock, 0, s>>>(B);
+            p2g_kernel<kAuditFields, 256><<<B.nwork, 256, 0, s>>>(B);
+        else if (B.p2g_block == 512)
+            p2g_kernel<kDepositFields, 512><<<B.nwork, 512, 0, s>>>(B);
         else
-            p2g_kernel<kDepositFields><<<B.nwork, kBlock, 0, s>>>(B);
+            p2g_kernel<kDepositFields, 256><<<B.nwork, 256, 0, s>>>(B);
     }
     traction_kernel<<<traction_wgs, kBlock, 0, s>>>(B);
     if (B.apart)
@@ -1809,7 +1957,6 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
     }
     if (pellets) pellet_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
     finalize_kernel<<<B.ninst, 256, 0, s>>>(B);
-    if (pellets) pellet_commit_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
 }
 
 __global__ void clock_kernel(unsigned long long* out) {

@@ -98,7 +98,8 @@ def test_six_million_particle_dam_break_invariants(engine):
 
 def test_pellet_bed_full_size_invariants(engine):
     # 1.5M-particle paste bed (96 x 96 x 20 cells x 8) on a 96^3 grid, dx 1.5 mm,
-    # with 512 rigid pellets above it: 40 substeps.
+    # with 512 rigid pellets in a band from 1.5 mm below to 3 mm above its
+    # surface, so that they are coupled from the first step: 40 substeps.
     E = engine
     rng = np.random.default_rng(2)
     dx = 0.0015
@@ -113,7 +114,7 @@ def test_pellet_bed_full_size_invariants(engine):
     pellets = []
     for _ in range(512):
         pos = [rng.uniform(0.006, domain[0] - 0.006), rng.uniform(0.006, domain[1] - 0.006),
-               rng.uniform(20 * dx + 0.006, domain[2] - 0.006)]
+               rng.uniform(20 * dx - 0.0015, 20 * dx + 0.003)]
         q = rng.normal(size=4)
         pellets.append(E.new_pellet(0.0015, 0.008, 650.0, pos, orientation=list(q / np.linalg.norm(q))))
     assert len(st) == 1_474_560
@@ -122,6 +123,9 @@ def test_pellet_bed_full_size_invariants(engine):
     assert sim.advance_steps(40) == 40
     out = check_invariants(sim, st, 40, dx, domain)
     assert (out.plastic >= 0.0).all()
+    impulse = np.array(sim.ledger().coupling_impulse[:])
+    assert np.isfinite(impulse).all() and np.abs(impulse).max() > 0.0, impulse
+    print(f"pellet bed: coupling impulse {impulse}", flush=True)
     print(f"pellet bed: t = {sim.history().time:.3e} s", flush=True)
 
 
