@@ -273,7 +273,7 @@ typedef struct {
     int steps_per_sync;  /* trial steps enqueued between host synchronisations (1..256) */
     int use_graph;       /* 1: replay the steps as one captured HIP graph */
     int merge_blocks;    /* cell blocks per work item along z (1 or 2) */
-    int traction_wgs;    /* workgroups of the wall-traction kernel */
+    int traction_wgs;    /* workgroups of the wall-traction kernel; 0: sized from the near-wall lists */
     int g2p_occupancy;   /* G2P workgroups per CU its register budget allows (2 or 3) */
     int p2g_block;       /* threads of a deposit workgroup (256 or 512) */
 } mpm_launch;

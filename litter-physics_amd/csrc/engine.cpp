@@ -369,10 +369,11 @@ struct mpm_engine {
     int cell_bits = 0, end_bit = 0;
     bool dirty = true;  // work items, set-up records and near-wall lists must be rebuilt
     // launch
-    mpm_launch launch{32, 0, 2, 2048, 2, 512};
+    mpm_launch launch{32, 0, 2, 0, 2, 512};
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    int graph_twgs = 0;  // traction workgroups the captured graph launches
     bool graph_stale = true;
     // Kernel timing: four device-clock stamps per enqueued step (BatchDev::ktime).
     unsigned long long* d_ktime = nullptr;
@@ -906,24 +907,46 @@ struct mpm_engine {
         return upload(i, k);
     }
 
+    // Workgroups of the wall-traction kernel: the launch shape's count, or (0)
+    // enough threads for one pass over the near-wall lists as of the last
+    // synchronisation (nine per particle, a quarter spare), rounded up to a power
+    // of two so that a captured graph is rebuilt only when the lists change
+    // much.  The kernel's grid-stride loop covers any excess.
+    int traction_workgroups() const {
+        if (launch.traction_wgs > 0) return launch.traction_wgs;
+        long long nw = 0;
+        for (int i = 0; i < ninst; ++i)
+            if (h_is[i].mode == M_RUN) nw += h_is[i].nw_count[h_is[i].cur];
+        const long long want = (9 * nw + 9 * nw / 4 + 255) / 256;
+        int w = 64;
+        while (w < want && w < 65536) w <<= 1;
+        return w;
+    }
+
     // Enqueue `k` trial steps of every running instance.
     int enqueue(int k) {
         BatchDev B = dev();
         bool pel = any_pellets();
+        const int twgs = traction_workgroups();
         if (launch.use_graph && !profiling && !auditing) {
-            if (graph_stale || graph_steps != k || !graph) {
+            if (graph_stale || graph_steps != k || graph_twgs != twgs || !graph) {
                 if (graph) {
                     HIP_OK(hipGraphExecDestroy(graph));
                     graph = nullptr;
                 }
                 hipGraph_t g = nullptr;
                 HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                for (int j = 0; j < k; ++j) launch_step(stream, B, launch.traction_wgs, pel);
+                for (int j = 0; j < k; ++j) {
+                    BatchDev Bj = B;
+                    Bj.fuse_control = j + 1 < k;
+                    launch_step(stream, Bj, twgs, pel, j == 0);
+                }
                 HIP_OK(hipStreamEndCapture(stream, &g));
                 hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 HIP_OK(e);
                 graph_steps = k;
+                graph_twgs = twgs;
                 graph_stale = false;
             }
             HIP_OK(hipGraphLaunch(graph, stream));
@@ -941,7 +964,8 @@ struct mpm_engine {
                 profiled[j] = profiling && profile_seq++ % (uint64_t)profile_every == 0;
                 BatchDev Bj = B;
                 Bj.ktime = profiled[j] ? d_ktime + 4 * j : nullptr;
-                launch_step(stream, Bj, launch.traction_wgs, pel);
+                Bj.fuse_control = j + 1 < k;
+                launch_step(stream, Bj, twgs, pel, j == 0);
             }
         }
         HIP_OK(hipGetLastError());
@@ -2249,7 +2273,7 @@ int mpm_set_launch(mpm_engine* e, const mpm_launch* l) {
     if (!e || !l) return MPM_ERR_INVALID;
     // More than two cell blocks per work item would not fit the LDS tiles.
     if (l->steps_per_sync < 1 || l->steps_per_sync > kMaxStepsPerSync || l->merge_blocks < 1 ||
-        l->merge_blocks > 2 || l->traction_wgs < 1 || l->traction_wgs > 65536 || l->g2p_occupancy < 2 ||
+        l->merge_blocks > 2 || l->traction_wgs < 0 || l->traction_wgs > 65536 || l->g2p_occupancy < 2 ||
         l->g2p_occupancy > 3 || (l->p2g_block != 256 && l->p2g_block != 512))
         return e->fail(MPM_ERR_INVALID, "launch shape out of range");
     bool resort = l->merge_blocks != e->launch.merge_blocks;

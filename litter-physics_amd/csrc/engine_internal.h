@@ -232,6 +232,7 @@ struct BatchDev {
     int sh;       // shared-slab selector (host controlled)
     int g2p_occupancy;  // G2P workgroups resident per CU (2 or 3: its register budget)
     int p2g_block;      // threads of a deposit workgroup (256 or 512)
+    int fuse_control;   // finalize also chooses the next trial (control_instance) for every instance
     // Kernel timing (null: off): the deposit's and the G2P's earliest workgroup
     // start and latest workgroup end on the device's constant clock,
     // {p2g start, p2g end, g2p start, g2p end}, set up as {max, 0, max, 0}.
@@ -249,7 +250,7 @@ struct BatchDev {
     ItemPre* wpre[2];  // per slab, one per work item
     int nwork;        // number of work items
     int ngwg;         // grid-update workgroups (all instances)
-    int* counters;    // [0] unused, [1] near-wall entries of the running instances
+    int* counters;    // [0] finalize workgroups done (the last resets it), [1] near-wall entries of the running instances
     double* gpart;    // grid partials, field-major: gpart[v * ngwg + workgroup]
     double* apart;    // audit partials, field-major like gpart (NULL: no audit)
     double* audit;    // audit sums per instance, kAuditOut each (NULL: no audit)
@@ -311,7 +312,10 @@ const float* mlp_device_biases(const mpm_mlp* mlp);
 int g2p_chunk();
 // One trial step of every running instance: control, deposit, traction, grid
 // update, pellet coupling, G2P, pellet integration, reductions and commit.
-void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets);
+// One trial step.  control: launch the control kernel first (the first step
+// after the host touched the instance states); otherwise the previous step's
+// finalize chose this trial (launched with B.fuse_control set).
+void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, bool control);
 // The device's constant clock into *out (the start of a timed run).
 void launch_clock(hipStream_t s, unsigned long long* out);
 // Order keys of the instances whose refresh is pending (the flags are left set).

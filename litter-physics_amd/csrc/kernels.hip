@@ -123,23 +123,27 @@ __device__ inline uint64_t ref_cell(const InstParams& ip, float x, float y, floa
 // ------------------------------------------------------------ control
 
 // A pellet's start-of-step geometry for the coupling kernel (its axis, angular
-// velocity and box) from its committed state `p`; coupling sums cleared.
-__device__ inline PelletGeo pellet_geometry(const BatchDev& B, int k, const PelletState& p) {
+// velocity and box) from its committed pose and momenta (pos, vel, L, q) and
+// its constant shape in `p`; coupling sums cleared.
+__device__ inline PelletGeo pellet_geometry(const BatchDev& B, int k, const PelletState& p, const double* pos,
+                                            const double* vel, const double* L, const double* q) {
     PelletGeo g;
     const double xh[3] = {1.0, 0.0, 0.0};
-    qrot(p.q, xh, g.axis);
-    pellet_omega(p.q, p.L, p.inertia, g.omega);
+    qrot(q, xh, g.axis);
+    pellet_omega(q, L, p.inertia, g.omega);
+    const int spheres = p.spheres;
+    const double radius = p.radius;
     double half = 0.0;
-    for (int q = 0; q < p.spheres; ++q) half = fmax(half, fabs(p.offsets[q]));
+    for (int i = 0; i < spheres; ++i) half = fmax(half, fabs(p.offsets[i]));
     for (int a = 0; a < 3; ++a) {
-        g.pos[a] = p.pos[a];
-        g.vel[a] = p.vel[a];
-        double reach = fabs(g.axis[a]) * half + p.radius;
-        g.lo[a] = p.pos[a] - reach;
-        g.hi[a] = p.pos[a] + reach;
+        g.pos[a] = pos[a];
+        g.vel[a] = vel[a];
+        double reach = fabs(g.axis[a]) * half + radius;
+        g.lo[a] = pos[a] - reach;
+        g.hi[a] = pos[a] + reach;
     }
-    g.radius = p.radius;
-    g.spheres = p.spheres;
+    g.radius = radius;
+    g.spheres = spheres;
     g.inst = p.inst;
     B.pgeo[k] = g;
     PelletAccum z = {};
@@ -266,8 +270,8 @@ __global__ __launch_bounds__(256) void control_kernel(BatchDev B) {
         if (inst < 0) continue;
         const InstState& s = B.is[inst];
         if (!s.active || !s.geo_stale) continue;
-        const PelletState p = B.pel[k];
-        pellet_geometry(B, k, p);
+        const PelletState& p = B.pel[k];
+        pellet_geometry(B, k, p, p.pos, p.vel, p.L, p.q);
     }
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
@@ -1488,6 +1492,51 @@ __device__ inline double block_total(double v, double* sd) {
     return t;
 }
 
+// The last finalize workgroup to finish (a counter of finished workgroups,
+// with fences around it) writes every instance's range in the concatenated
+// near-wall list, as the control kernel's scan does, and resets the counter.
+// Every thread of every finalize workgroup calls it once.
+__device__ inline void near_wall_prefix_last(const BatchDev& B) {
+    __shared__ int slast;
+    __shared__ int scarry;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int done = atomicAdd(&B.counters[0], 1);
+        slast = done == B.ninst - 1;
+        scarry = 0;
+    }
+    __syncthreads();
+    if (!slast) return;
+    __threadfence();
+    __shared__ int sscan[256];
+    const int tid = threadIdx.x;
+    for (int t0 = 0; t0 < B.ninst; t0 += 256) {
+        const int t = t0 + tid;
+        int c = 0;
+        if (t < B.ninst) {
+            const InstState& s = B.is[t];
+            c = s.active ? s.nw_count[s.cur] : 0;
+        }
+        sscan[tid] = c;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int v = tid >= o ? sscan[tid - o] : 0;
+            __syncthreads();
+            sscan[tid] += v;
+            __syncthreads();
+        }
+        if (t < B.ninst) B.is[t].nw_base = scarry + sscan[tid];
+        __syncthreads();
+        if (tid == 255) scarry += sscan[255];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        B.counters[1] = scarry;
+        B.counters[0] = 0;
+    }
+}
+
 // One workgroup per running instance: fixed-order reduction of its grid and
 // particle partials and of its pellets' step sums, then the accept/reject.  The
 // instance state is staged in LDS (one round trip for all of it), committed
@@ -1508,7 +1557,10 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     }
     __syncthreads();
     InstState& s = ls;
-    if (!s.active) return;
+    if (!s.active) {
+        if (B.fuse_control && B.ninst > 1) near_wall_prefix_last(B);
+        return;
+    }
     const InstParams& ip = B.ip[inst];
     const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
     double acc[kGridVals + 1];
@@ -1676,24 +1728,31 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         double box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
         for (int k = tid; k < ip.npel; k += 256) {
             const int pi = ip.pel_off + k;
-            PelletState p = B.pel[pi];
+            PelletState& p = B.pel[pi];
+            double pos[3], vel[3], L[3], q[4];
             if (accepted) {
                 const PelletTrial& t = B.ptrial[pi];
                 for (int c = 0; c < 3; ++c) {
-                    p.pos[c] = t.pos[c];
-                    p.vel[c] = t.vel[c];
-                    p.L[c] = t.L[c];
+                    pos[c] = t.pos[c];
+                    vel[c] = t.vel[c];
+                    L[c] = t.L[c];
                 }
-                for (int c = 0; c < 4; ++c) p.q[c] = t.q[c];
-                PelletState& dst = B.pel[pi];
+                for (int c = 0; c < 4; ++c) q[c] = t.q[c];
                 for (int c = 0; c < 3; ++c) {
-                    dst.pos[c] = p.pos[c];
-                    dst.vel[c] = p.vel[c];
-                    dst.L[c] = p.L[c];
+                    p.pos[c] = pos[c];
+                    p.vel[c] = vel[c];
+                    p.L[c] = L[c];
                 }
-                for (int c = 0; c < 4; ++c) dst.q[c] = p.q[c];
+                for (int c = 0; c < 4; ++c) p.q[c] = q[c];
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    pos[c] = p.pos[c];
+                    vel[c] = p.vel[c];
+                    L[c] = p.L[c];
+                }
+                for (int c = 0; c < 4; ++c) q[c] = p.q[c];
             }
-            const PelletGeo g = pellet_geometry(B, pi, p);
+            const PelletGeo g = pellet_geometry(B, pi, p, pos, vel, L, q);
             for (int a = 0; a < 3; ++a) {
                 box[a] = fmin(box[a], g.lo[a]);
                 box[3 + a] = fmax(box[3 + a], g.hi[a]);
@@ -1715,11 +1774,22 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         }
         __syncthreads();
     }
+    // The next trial of this instance (what the control kernel would choose).
+    // A batch of one is its own near-wall prefix.
+    if (B.fuse_control && tid == 0) {
+        const int nwc = control_instance(s, ip);
+        if (B.ninst == 1) {
+            s.nw_base = nwc;
+            B.counters[1] = nwc;
+        }
+    }
+    __syncthreads();
     {
         const unsigned* src = reinterpret_cast<const unsigned*>(&ls);
         unsigned* dst = reinterpret_cast<unsigned*>(&gs);
         for (int w = tid; w < kWords; w += 256) dst[w] = src[w];
     }
+    if (B.fuse_control && B.ninst > 1) near_wall_prefix_last(B);
 }
 
 // Order keys at the 32-step cadence, for the instances whose accepted step hit
@@ -1931,8 +2001,8 @@ inline int blocks_for(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 int g2p_chunk() { return kChunk; }
 
-void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets) {
-    control_kernel<<<1, 256, 0, s>>>(B);
+void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellets, bool control) {
+    if (control) control_kernel<<<1, 256, 0, s>>>(B);
     if (B.nwork > 0) {
         if (B.apart)
             p2g_kernel<kAuditFields, 256><<<B.nwork, 256, 0, s>>>(B);
