@@ -304,7 +304,10 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
     constexpr int R = kChunk / BS;
     __shared__ __align__(16) unsigned long long ss[NF * kTileCap];
     if (B.ktime) kernel_start(B.ktime);
-    const int4 item = B.work[blockIdx.x];
+    // The item and the instance fields the rounds use, read once into uniform
+    // registers (read through references, they would be reloaded after every
+    // global store in the loop).
+    const int4 item = uniform_copy(B.work[blockIdx.x]);
     const int inst = item.x;
     const InstState& is = B.is[inst];
     if (!is.active) return;
@@ -313,6 +316,8 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
     const int tid = threadIdx.x, lane = tid & 63;
     const size_t cap = B.cap, na = B.na;
     const float dt = (float)is.dt;
+    const int refsort = is.refsort;
+    const DepositScales scales{is.sc_m, is.sc_p, is.sc_mom, is.sc_f};
     const float* __restrict__ src = B.slab[is.cur];
     const float* __restrict__ shared = B.shared[B.sh];
     GridPtrs out = grid_set(B, is.gcur);
@@ -344,10 +349,8 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
     for (int r = 0; r < R; ++r) {
         const int p = item.y + r * BS + tid;
         if (p >= end) continue;
+        // All of the particle's reads go out together, before any of them is used.
         const float x = src[PX * cap + p], y = src[PY * cap + p], z = src[PZ * cap + p];
-        if (is.refsort) B.refkey[p] = (ref_cell(ip, x, y, z) << 32) | (uint64_t)B.ids[p];
-        Stencil s;
-        if (!make_stencil(P, x, y, z, s)) continue;  // outside the padded grid: no deposit (solver.rs:636)
         const float m = shared[SMASS * cap + p], v0 = shared[SVOL0 * cap + p];
         float v[3], C[9], tau[6];
 #pragma unroll
@@ -356,7 +359,11 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
         for (int i = 0; i < 9; ++i) C[i] = src[(C00 + i) * cap + p];
 #pragma unroll
         for (int i = 0; i < 6; ++i) tau[i] = src[(TXX + i) * cap + p];
-        const Deposit<NF> q = make_deposit<NF>(P, is, dt, m, v0, v, C, tau);
+        __builtin_amdgcn_sched_barrier(0);
+        if (refsort) B.refkey[p] = (ref_cell(ip, x, y, z) << 32) | (uint64_t)B.ids[p];
+        Stencil s;
+        if (!make_stencil(P, x, y, z, s)) continue;  // outside the padded grid: no deposit (solver.rs:636)
+        const Deposit<NF> q = make_deposit<NF>(P, scales, dt, m, v0, v, C, tau);
         bool in_tile = t.tiled;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {

@@ -650,6 +650,21 @@ __device__ inline void gather_lds(const float4* sg, const Tile& t, const Stencil
     }, v, B);
 }
 
+// A wavefront-uniform copy of a struct read from memory: every 32-bit word goes
+// through readfirstlane, so the compiler knows the fields are uniform and keeps
+// them, and every address derived from them, in scalar registers.
+template <class T>
+__device__ inline T uniform_copy(const T& src) {
+    static_assert(sizeof(T) % 4 == 0, "uniform_copy copies 32-bit words");
+    int w[sizeof(T) / 4];
+    __builtin_memcpy(w, &src, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
 // Select among three register values by a runtime index (no array indexing, which
 // would place the stencil in scratch memory).
 __device__ inline float sel3(int i, float x0, float x1, float x2) { return i == 0 ? x0 : (i == 1 ? x1 : x2); }
@@ -761,6 +776,11 @@ __device__ inline float deposit_scale(double bound) {
     return ldexpf(1.0f, max(min(29 - e, 126), -126));
 }
 
+// The trial's deposit scales (InstState::sc_*), read once per workgroup.
+struct DepositScales {
+    float sc_m, sc_p, sc_mom, sc_f;
+};
+
 // One particle's deposit in scaled units.  NF = 4: mass and momentum with the
 // stress impulse (the reference's P2G once dt is known); NF = 7 (audit): mass,
 // the APIC momentum w (m v + m C d) and the stress sum w V0 tau d.
@@ -773,7 +793,7 @@ struct Deposit {
 };
 
 template <int NF>
-__device__ inline Deposit<NF> make_deposit(const Params& P, const InstState& is, float dt, float m, float v0,
+__device__ inline Deposit<NF> make_deposit(const Params& P, const DepositScales& is, float dt, float m, float v0,
                                            const float* v, const float* C, const float* tau) {
 #pragma clang fp contract(off)
     Deposit<NF> q;
