@@ -18,6 +18,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -364,6 +365,15 @@ struct mpm_engine {
     uint64_t* run_unique = nullptr;
     uint32_t* run_count = nullptr;
     int* d_nruns = nullptr;
+    // Pinned staging of the run list and the work items (one copy each way per sort).
+    int* h_nruns = nullptr;
+    uint64_t* h_uq = nullptr;
+    uint32_t* h_cnt = nullptr;
+    size_t h_run_cap = 0;
+    size_t last_nruns = 0;  // run count of the last sort: the size of the next first copy
+    int4* h_work = nullptr;
+    std::vector<int4> work_build, work_sorted;  // the sort's work-item lists, kept between sorts
+    std::vector<int> work_hist;
     void* sort_temp = nullptr;
     size_t sort_bytes = 0;
     int cell_bits = 0, end_bit = 0;
@@ -464,22 +474,30 @@ struct mpm_engine {
         return B;
     }
 
-    int pull() {
-        HIP_OK(hipMemcpyAsync(h_is, d_is, sizeof(InstState) * ninst, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+    // Wait for the stream by polling it.  A blocking synchronisation sleeps, and
+    // its wake-up cost tens of microseconds of idle GPU per host round trip.
+    int sync() {
+        hipError_t e;
+        while ((e = hipStreamQuery(stream)) == hipErrorNotReady) std::this_thread::yield();
+        HIP_OK(e);
         return MPM_OK;
     }
-    int push() {
+    int pull() {
+        HIP_OK(hipMemcpyAsync(h_is, d_is, sizeof(InstState) * ninst, hipMemcpyDeviceToHost, stream));
+        return sync();
+    }
+    // Upload the mirror.  Without `wait`, the caller must not touch the mirror
+    // before a later synchronisation of the stream (a pull() is one).
+    int push(bool wait = true) {
         // Any host change may have touched the pellets (state upload, restore):
         // the next trial recomputes their geometry.
         for (int i = 0; i < ninst; ++i) h_is[i].geo_stale = 1;
         HIP_OK(hipMemcpyAsync(d_is, h_is, sizeof(InstState) * ninst, hipMemcpyHostToDevice, stream));
-        HIP_OK(hipStreamSynchronize(stream));
-        return MPM_OK;
+        return wait ? sync() : MPM_OK;
     }
     int upload_params(int i) {
         HIP_OK(hipMemcpyAsync(d_ip + i, &inst[i].ip, sizeof(InstParams), hipMemcpyHostToDevice, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+        if (int rc_ = sync()) return rc_;
         return MPM_OK;
     }
 
@@ -579,10 +597,12 @@ struct mpm_engine {
         if (int rc = dalloc(&run_unique, cap, false)) return rc;
         if (int rc = dalloc(&run_count, cap, false)) return rc;
         if (int rc = dalloc(&d_nruns, 1)) return rc;
+        HIP_OK(hipHostMalloc((void**)&h_nruns, sizeof(int), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc((void**)&h_work, sizeof(int4) * std::max<size_t>(work_cap, 1), hipHostMallocDefault));
         if (int rc = dalloc(&mlp_stats, 2)) return rc;
         size_t sort_need = 0, rle_need = 0;
         HIP_OK(sort_temp_bytes((int)cap, end_bit, &sort_need));
-        HIP_OK(rle_temp_bytes((int)cap, &rle_need));
+        HIP_OK(rle_temp_bytes((int)cap, end_bit, &rle_need));
         sort_bytes = std::max(sort_need, rle_need);
         HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
         // Kernel timing stamps (the kernels write them; no event records, which
@@ -642,6 +662,10 @@ struct mpm_engine {
         (void)hipFree(run_unique);
         (void)hipFree(run_count);
         (void)hipFree(d_nruns);
+        if (h_nruns) (void)hipHostFree(h_nruns);
+        if (h_uq) (void)hipHostFree(h_uq);
+        if (h_cnt) (void)hipHostFree(h_cnt);
+        if (h_work) (void)hipHostFree(h_work);
         (void)hipFree(mlp_stats);
         (void)hipFree(rlog);
         (void)hipFree(rlog_count);
@@ -655,14 +679,32 @@ struct mpm_engine {
     // Order keys still pending for instances whose last accepted step hit the
     // 32-step cadence and that took no trial step since (the next trial's G2P2G
     // writes them otherwise).  The mirror must be current.
-    int flush_refkeys() {
+    // With `upload` false the cleared flags reach the device with the caller's
+    // next push (nothing launched before it reads them).
+    int flush_refkeys(bool upload = true) {
         bool any = false;
         for (int i = 0; i < ninst; ++i) any = any || h_is[i].refsort;
         if (!any) return MPM_OK;
         launch_refkeys(stream, dev());
         HIP_OK(hipGetLastError());
         for (int i = 0; i < ninst; ++i) h_is[i].refsort = 0;
-        return push();
+        return upload ? push() : MPM_OK;
+    }
+
+    // Pinned staging for at least n runs (contents are not kept).  It grows to
+    // twice the request: pinned allocations cost far more than the copies.
+    int reserve_runs(size_t n) {
+        if (n <= h_run_cap) return MPM_OK;
+        n = std::min(cap, 2 * n);
+        if (h_uq) (void)hipHostFree(h_uq);
+        if (h_cnt) (void)hipHostFree(h_cnt);
+        h_uq = nullptr;
+        h_cnt = nullptr;
+        h_run_cap = 0;
+        HIP_OK(hipHostMalloc((void**)&h_uq, sizeof(uint64_t) * n, hipHostMallocDefault));
+        HIP_OK(hipHostMalloc((void**)&h_cnt, sizeof(uint32_t) * n, hipHostMallocDefault));
+        h_run_cap = n;
+        return MPM_OK;
     }
 
     // Sort every instance's particles by 4^3-cell block (device order, for
@@ -670,27 +712,42 @@ struct mpm_engine {
     // of the reference order travel with the particles.  The mirror must be
     // current; it is current again on return.
     int sort_all() {
-        if (int rc = flush_refkeys()) return rc;
+        if (int rc = flush_refkeys(false)) return rc;
         BatchDev B = dev();
-        launch_sort_keys(stream, B, cell_bits, keys[0], order[0]);
+        launch_sort_keys(stream, B, cell_bits, end_bit, keys[0], order[0]);
         HIP_OK(hipGetLastError());
+        // A state set from the host is sorted by cell; the periodic locality
+        // sorts order by cell block only (two radix passes at 1M particles in one
+        // instance instead of three): within a block the stable order of the
+        // previous sort stays, which the particles' drift barely disturbs.
+        const int begin_bit = dirty ? 0 : 6;
         HIP_OK(launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)cap,
-                                 end_bit));
+                                 begin_bit, end_bit));
         launch_permute(stream, B, order[1], shared[sh], shared[sh ^ 1], ids[sh], ids[sh ^ 1], refkey[sh],
                        refkey[sh ^ 1]);
         HIP_OK(hipGetLastError());
-        HIP_OK(launch_block_runs(stream, sort_temp, sort_bytes, (int)cap, keys[1], keys[0], run_unique, run_count,
+        HIP_OK(launch_block_runs(stream, sort_temp, sort_bytes, (int)cap, end_bit, keys[1], run_unique, run_count,
                                  d_nruns));
-        int nruns = 0;
-        HIP_OK(hipMemcpyAsync(&nruns, d_nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
-        std::vector<uint64_t> uq(nruns);
-        std::vector<uint32_t> cnt(nruns);
-        if (nruns) {
-            HIP_OK(hipMemcpyAsync(uq.data(), run_unique, sizeof(uint64_t) * nruns, hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipMemcpyAsync(cnt.data(), run_count, sizeof(uint32_t) * nruns, hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipStreamSynchronize(stream));
+        // The run list comes back with its count in one round trip: the first
+        // `guess` runs (the last sort's count and a margin), and the rest in a
+        // second copy only if the count outgrew the guess.
+        const size_t guess = std::min(cap, last_nruns + last_nruns / 4 + 1024);
+        if (int rc = reserve_runs(guess)) return rc;
+        HIP_OK(hipMemcpyAsync(h_nruns, d_nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(h_uq, run_unique, sizeof(uint64_t) * guess, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(h_cnt, run_count, sizeof(uint32_t) * guess, hipMemcpyDeviceToHost, stream));
+        if (int rc_ = sync()) return rc_;
+        const int nruns = *h_nruns;
+        if (nruns < 0 || (size_t)nruns > cap) return fail(MPM_ERR_DEVICE, "run-length encoding count out of range");
+        if ((size_t)nruns > guess) {
+            if (int rc = reserve_runs((size_t)nruns)) return rc;
+            HIP_OK(hipMemcpyAsync(h_uq, run_unique, sizeof(uint64_t) * nruns, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipMemcpyAsync(h_cnt, run_count, sizeof(uint32_t) * nruns, hipMemcpyDeviceToHost, stream));
+            if (int rc_ = sync()) return rc_;
         }
+        last_nruns = (size_t)nruns;
+        const uint64_t* uq = h_uq;
+        const uint32_t* cnt = h_cnt;
         sh ^= 1;
         for (int i = 0; i < ninst; ++i) h_is[i].cur ^= 1;
         // Work items: runs of one cell block, merged along z up to launch.merge_blocks
@@ -698,9 +755,10 @@ struct mpm_engine {
         const int chunk = g2p_chunk();
         const int shift = cell_bits - 6;
         const uint64_t bmask = (1ull << shift) - 1;
-        std::vector<int4> work;
+        std::vector<int4>& work = work_build;
+        work.clear();
         work.reserve(nruns + cap / chunk + 1);
-        int start = 0, ci = -1, cstart = -1, ccount = 0, cblocks = 0;
+        int start = 0, ci = -1, cstart = -1, ccount = 0, cblocks = 0, cbz = 1, zpos = 0;
         uint64_t prev = 0;
         auto flush = [&]() {
             if (cstart >= 0) work.push_back(make_int4(ci, cstart, ccount, 0));
@@ -709,17 +767,25 @@ struct mpm_engine {
             cblocks = 0;
         };
         for (int r = 0; r < nruns; ++r) {
-            int c = (int)cnt[r];
-            int in = (int)(uq[r] >> shift);
-            uint64_t blk = uq[r] & bmask;
+            const int c = (int)cnt[r];
+            const int in = (int)(uq[r] >> shift);
+            const uint64_t blk = uq[r] & bmask;
             if (blk == bmask || in >= ninst) {
                 flush();
                 start += c;
                 continue;
             }
-            uint64_t cbz = (uint64_t)inst[in].cb[2];
-            bool adjacent = cstart >= 0 && ci == in && cblocks < launch.merge_blocks && blk == prev + 1 &&
-                            blk % cbz != 0 && ccount + c <= chunk;
+            // z position of the block in its column: counted along a run of
+            // consecutive blocks, a 64-bit modulo only where the run breaks.
+            if (in == ci && blk == prev + 1) {
+                zpos = zpos + 1 == cbz ? 0 : zpos + 1;
+            } else {
+                cbz = inst[in].cb[2];
+                zpos = (int)(blk % (uint64_t)cbz);
+            }
+            const bool adjacent =
+                cstart >= 0 && ci == in && cblocks < launch.merge_blocks && blk == prev + 1 && zpos != 0 &&
+                ccount + c <= chunk;
             if (!adjacent) flush();
             if (c > chunk) {
                 for (int off = 0; off < c; off += chunk)
@@ -728,11 +794,11 @@ struct mpm_engine {
                 ccount += c;
                 cblocks += 1;
             } else {
-                ci = in;
                 cstart = start;
                 ccount = c;
                 cblocks = 1;
             }
+            ci = in;
             prev = blk;
             start += c;
         }
@@ -741,12 +807,29 @@ struct mpm_engine {
         // grows with its particle count, and workgroups start in index order, so
         // the small items fill the end of the launch instead of a few large ones
         // running on alone (1M-particle column: 173 us of modelled launch time
-        // becomes 151 us).  Each instance's items stay contiguous.
-        for (size_t b = 0; b < work.size();) {
-            size_t e = b;
-            while (e < work.size() && work[e].x == work[b].x) ++e;
-            std::stable_sort(work.begin() + b, work.begin() + e, [](const int4& p, const int4& q) { return p.z > q.z; });
-            b = e;
+        // becomes 151 us).  Each instance's items stay contiguous.  A stable
+        // counting sort on the particle count (at most one chunk); the list is
+        // already grouped by instance and each group is ordered on its own.
+        {
+            std::vector<int>& hist = work_hist;
+            hist.assign((size_t)chunk + 2, 0);
+            std::vector<int4>& out = work_sorted;
+            out.resize(work.size());
+            for (size_t b = 0; b < work.size();) {
+                size_t e = b;
+                while (e < work.size() && work[e].x == work[b].x) ++e;
+                for (size_t w = b; w < e; ++w) hist[chunk - work[w].z + 1] += 1;
+                int run = (int)b;
+                for (int k = 0; k <= chunk + 1; ++k) {
+                    const int h = hist[k];
+                    hist[k] = run;
+                    run += h;
+                }
+                for (size_t w = b; w < e; ++w) out[hist[chunk - work[w].z + 1]++] = work[w];
+                std::fill(hist.begin(), hist.end(), 0);
+                b = e;
+            }
+            work.swap(out);
         }
         if (work.size() > work_cap) return fail(MPM_ERR_CAPACITY, "work list overflow");
         nwork = (int)work.size();
@@ -763,16 +846,16 @@ struct mpm_engine {
             InstState& s = h_is[i];
             s.nw_count[s.cur] = 0;
         }
-        if (nwork) HIP_OK(hipMemcpyAsync(d_work, work.data(), sizeof(int4) * nwork, hipMemcpyHostToDevice, stream));
-        if (int rc = push()) return rc;
+        if (nwork) {
+            std::memcpy(h_work, work.data(), sizeof(int4) * nwork);
+            HIP_OK(hipMemcpyAsync(d_work, h_work, sizeof(int4) * nwork, hipMemcpyHostToDevice, stream));
+        }
+        // The pull below synchronises the stream before the host touches the
+        // mirror or the staged work list again.
+        if (int rc = push(false)) return rc;
         launch_setup(stream, dev());
         HIP_OK(hipGetLastError());
         if (int rc = pull()) return rc;
-        // The committed speed as the trial steps compute it (particle_speed in
-        // f32), so that a state set from outside takes the same dt as the same
-        // state reached by stepping.
-        for (int i = 0; i < ninst; ++i) h_is[i].max_speed = (double)h_is[i].setup_speed;
-        if (int rc = push()) return rc;
         dirty = false;
         graph_stale = true;
         return MPM_OK;
@@ -789,7 +872,7 @@ struct mpm_engine {
                 HIP_OK(hipMemsetAsync(gmem[g] + f * na + noff, 0, sizeof(long long) * nn, stream));
             HIP_OK(hipMemsetAsync(gflag[g] + in.ip.boff, 0, sizeof(int) * in.ip.nblk, stream));
         }
-        HIP_OK(hipStreamSynchronize(stream));
+        if (int rc_ = sync()) return rc_;
         in.grid_clear = true;
         return MPM_OK;
     }
@@ -823,7 +906,7 @@ struct mpm_engine {
                                   hipMemcpyDeviceToHost, stream));
         HIP_OK(hipMemcpyAsync(h.id.data(), ids[sh] + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipMemcpyAsync(h.ref.data(), refkey[sh] + off, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+        if (int rc_ = sync()) return rc_;
         return MPM_OK;
     }
     // Write instance i's particles into both slabs (and the current shared slab).
@@ -842,7 +925,7 @@ struct mpm_engine {
             HIP_OK(hipMemcpyAsync(refkey[sh] + off, h.ref.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice,
                                   stream));
         }
-        HIP_OK(hipStreamSynchronize(stream));
+        if (int rc_ = sync()) return rc_;
         h_is[i].n = (int)n;
         dirty = true;
         return MPM_OK;
@@ -853,7 +936,7 @@ struct mpm_engine {
         if (in.ip.npel) {
             HIP_OK(hipMemcpyAsync(out.data(), d_pel + in.ip.pel_off, sizeof(PelletState) * in.ip.npel,
                                   hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipStreamSynchronize(stream));
+            if (int rc_ = sync()) return rc_;
         }
         return MPM_OK;
     }
@@ -971,13 +1054,20 @@ struct mpm_engine {
         HIP_OK(hipGetLastError());
         return MPM_OK;
     }
-    // Kernel times of the profiled steps of the last enqueue (stream synchronised).
-    // A launch whose workgroups all found their instance idle leaves no end stamp
-    // and is not counted.
+    // Enqueue the copy of the last enqueue's stamps (read by collect_profile
+    // after the stream has been synchronised).
+    int fetch_profile(int k) {
+        if (profiling)
+            HIP_OK(hipMemcpyAsync(h_ktime, d_ktime, sizeof(unsigned long long) * 4 * k, hipMemcpyDeviceToHost,
+                                  stream));
+        return MPM_OK;
+    }
+    // Kernel times of the profiled steps of the last enqueue, from the stamps
+    // fetch_profile copied (stream synchronised since).  A launch whose
+    // workgroups all found their instance idle leaves no end stamp and is not
+    // counted.
     int collect_profile(int k) {
         if (!profiling) return MPM_OK;
-        HIP_OK(hipMemcpyAsync(h_ktime, d_ktime, sizeof(unsigned long long) * 4 * k, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
         const double ms_per_tick = 1.0 / clock_khz;
         for (int j = 0; j < k; ++j) {
             if (!profiled[j]) continue;
@@ -1019,7 +1109,7 @@ struct mpm_engine {
             launch_clock(stream, d_run_clock);
             HIP_OK(hipGetLastError());
             HIP_OK(hipMemcpyAsync(&start_clock, d_run_clock, sizeof(start_clock), hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipStreamSynchronize(stream));
+            if (int rc_ = sync()) return rc_;
         }
         std::vector<char> recorded(ninst, 0);
         int since_sort = 0, rounds = 0;
@@ -1038,9 +1128,10 @@ struct mpm_engine {
             // No launches past the last budgeted step (they would find nothing to do).
             const int k = std::max(1, std::min(k_sync, most));
             if (int rc = enqueue(k)) return rc;
-            HIP_OK(hipStreamSynchronize(stream));
-            if (int rc = collect_profile(k)) return rc;
+            // One host round trip per round: the stamps and the mirror together.
+            if (int rc = fetch_profile(k)) return rc;
             if (int rc = pull()) return rc;
+            if (int rc = collect_profile(k)) return rc;
             since_sort += k;
             rounds += 1;
             double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1134,7 +1225,7 @@ struct mpm_engine {
             snap.bufs.push_back({b, t.second});
             HIP_OK(hipMemcpyAsync(b, t.first, t.second, hipMemcpyDeviceToDevice, stream));
         }
-        HIP_OK(hipStreamSynchronize(stream));
+        if (int rc_ = sync()) return rc_;
         return MPM_OK;
     }
     int restore(const Snapshot& snap) {
@@ -1351,7 +1442,7 @@ int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pell
     if (in.pel_cap) {
         HIP_OK(hipMemcpyAsync(d_pel + in.ip.pel_off, pel.data(), sizeof(PelletState) * in.pel_cap,
                               hipMemcpyHostToDevice, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+        if (int rc_ = sync()) return rc_;
     }
     pel.resize(n_pellets);
     in.ip.npel = (int)n_pellets;
@@ -1717,9 +1808,9 @@ int mpm_try_step(mpm_engine* e, double dt, mpm_step_report* report) {
     }
     if (int rc = e->push()) return rc;
     if (int rc = e->enqueue(1)) return rc;
-    if (hipStreamSynchronize(e->stream) != hipSuccess) return e->fail(MPM_ERR_DEVICE, "device failure in try_step");
-    if (int rc = e->collect_profile(1)) return rc;
+    if (int rc = e->fetch_profile(1)) return rc;
     if (int rc = e->pull()) return rc;
+    if (int rc = e->collect_profile(1)) return rc;
     InstState& s = e->h_is[0];
     int code = s.last_failure;
     if (report) {

@@ -325,21 +325,25 @@ void launch_refkeys(hipStream_t s, const BatchDev& B);
 void launch_setup(hipStream_t s, const BatchDev& B);
 // Device-order keys of every slab entry: (instance << cell_bits) | 4^3-cell block
 // key; entries past an instance's particle count sort to the end of its range.
-void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t* keys, uint32_t* order);
+// Keys of at most 32 bits (end_bit <= 32) are stored as 32-bit words in `keys`.
+void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, int end_bit, uint64_t* keys, uint32_t* order);
 // Gather every slab entry into the sorted order (each instance from its
 // committed slab into the other one; shared slabs, ids and order keys likewise).
 void launch_permute(hipStream_t s, const BatchDev& B, const uint32_t* order, const float* shared_src,
                     float* shared_dst, const uint32_t* id_src, uint32_t* id_dst, const uint64_t* ref_src,
                     uint64_t* ref_dst);
 // Temporary storage of the radix sort and of the run-length encoding (the
-// size is written to *bytes); the sort and encode calls return hipcub's status.
+// size is written to *bytes); the sort and encode calls return rocPRIM's status.
+// Keys are 32-bit words when end_bit <= 32, 64-bit otherwise.  The sort orders
+// by bits [begin_bit, end_bit) of the key, stably.
 hipError_t sort_temp_bytes(int n, int end_bit, size_t* bytes);
-hipError_t rle_temp_bytes(int n, size_t* bytes);
+hipError_t rle_temp_bytes(int n, int end_bit, size_t* bytes);
 hipError_t launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in,
-                             uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit);
+                             uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int n, int begin_bit,
+                             int end_bit);
 // Runs of equal (instance, cell block) in the sorted keys.
-hipError_t launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
-                             uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns);
+hipError_t launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, int end_bit,
+                             const uint64_t* sorted_keys, uint64_t* unique, uint32_t* counts, int* nruns);
 void launch_paste_batch(hipStream_t s, const Params& P, float dt, int n, const float* h_in, float* h_out,
                         float* tau, float* inc, float* dd, int* codes);
 void launch_liquid_batch(hipStream_t s, const Params& P, float dt, int n, const float* jm1, const float* c,

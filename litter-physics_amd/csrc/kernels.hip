@@ -20,7 +20,9 @@
 // plastic dissipation), so a trajectory is a function of the state alone: the
 // same on every run and after a restore, whatever the particle order.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include "mlp.h"
 #include "physics.h"
@@ -1916,7 +1918,11 @@ __global__ __launch_bounds__(256) void setup_reduce_kernel(BatchDev B) {
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 0; k < 3; ++k) m[k] = fmaxf(fmaxf(sm[0][k], sm[1][k]), fmaxf(sm[2][k], sm[3][k]));
+        // The committed speed as the trial steps compute it (particle_speed in
+        // f32), so that a state set from outside takes the same dt as the same
+        // state reached by stepping.
         s.setup_speed = m[0];
+        s.max_speed = (double)m[0];
         s.bound[0] = m[1];
         s.bound[1] = m[2];
     }
@@ -1924,8 +1930,10 @@ __global__ __launch_bounds__(256) void setup_reduce_kernel(BatchDev B) {
 
 // Device order: 4x4x4-cell blocks (compact workgroup tiles), instances in
 // range order; entries past an instance's particle count go to the end of its
-// range.  Radix sort is stable, so equal keys keep their order.
-__global__ void sort_keys_kernel(BatchDev B, int cell_bits, uint64_t* keys, uint32_t* order) {
+// range.  Radix sort is stable, so equal keys keep their order.  K: 32-bit keys
+// when (instance, block, cell) fits in 32 bits (half the sort's traffic), else 64.
+template <typename K>
+__global__ void sort_keys_kernel(BatchDev B, int cell_bits, K* keys, uint32_t* order) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (int)B.cap) return;
     const int inst = instance_of(B, p);
@@ -1946,14 +1954,17 @@ __global__ void sort_keys_kernel(BatchDev B, int cell_bits, uint64_t* keys, uint
         uint64_t within = ((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3);
         key = blk * 64 + within;
     }
-    keys[p] = ((uint64_t)inst << cell_bits) | key;
+    keys[p] = (K)(((uint64_t)inst << cell_bits) | key);
     order[p] = (uint32_t)p;
 }
 
-__global__ void block_ids_kernel(int n, const uint64_t* __restrict__ keys, uint64_t* out) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) out[p] = keys[p] >> 6;
-}
+// (instance, cell block) of a sorted key: the run-length encoding's input.
+struct BlockOfKey {
+    template <typename K>
+    __host__ __device__ uint64_t operator()(K k) const {
+        return (uint64_t)(k >> 6);
+    }
+};
 
 // Gather every slab entry into sorted order: from each instance's committed
 // slab into its other slab (the host then flips the instance's selector), and
@@ -2051,8 +2062,12 @@ void launch_setup(hipStream_t s, const BatchDev& B) {
     setup_reduce_kernel<<<B.ninst, 256, 0, s>>>(B);
 }
 
-void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, uint64_t* keys, uint32_t* order) {
-    sort_keys_kernel<<<blocks_for(B.cap), kBlock, 0, s>>>(B, cell_bits, keys, order);
+void launch_sort_keys(hipStream_t s, const BatchDev& B, int cell_bits, int end_bit, uint64_t* keys, uint32_t* order) {
+    if (end_bit <= 32)
+        sort_keys_kernel<uint32_t>
+            <<<blocks_for(B.cap), kBlock, 0, s>>>(B, cell_bits, reinterpret_cast<uint32_t*>(keys), order);
+    else
+        sort_keys_kernel<uint64_t><<<blocks_for(B.cap), kBlock, 0, s>>>(B, cell_bits, keys, order);
 }
 
 void launch_permute(hipStream_t s, const BatchDev& B, const uint32_t* order, const float* shared_src,
@@ -2062,31 +2077,57 @@ void launch_permute(hipStream_t s, const BatchDev& B, const uint32_t* order, con
                                                         ref_dst);
 }
 
-hipError_t sort_temp_bytes(int n, int end_bit, size_t* bytes) {
-    *bytes = 0;
-    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
+// rocPRIM's radix sort picks a merge sort at up to 2^20 keys, whatever the key's
+// width: one pass per doubling of the sorted runs, 20 launches at 1M particles.
+// A merge-sort limit of 0 takes the onesweep radix sort instead, one pass per
+// 8 bits of the sorted key range.  (Its gfx950 tiles of 6144 keys give a 1M-key
+// pass 163 workgroups; tiles of 2048 keys and 7-bit digits measured 41 us per
+// pass against 28 us.)
+using OnesweepOnly =
+    rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config, rocprim::default_config, 0>;
+
+template <typename K>
+hipError_t radix_sort_as(hipStream_t s, void* temp, size_t& bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                         const uint32_t* vals_in, uint32_t* vals_out, int n, int begin_bit, int end_bit) {
+    return rocprim::radix_sort_pairs<OnesweepOnly>(temp, bytes, reinterpret_cast<const K*>(keys_in),
+                                                   reinterpret_cast<K*>(keys_out), vals_in, vals_out, n,
+                                                   (unsigned)begin_bit, (unsigned)end_bit, s);
 }
 
-hipError_t rle_temp_bytes(int n, size_t* bytes) {
+template <typename K>
+hipError_t block_runs_as(hipStream_t s, void* temp, size_t& bytes, int n, const uint64_t* sorted_keys,
+                         uint64_t* unique, uint32_t* counts, int* nruns) {
+    auto blocks = rocprim::make_transform_iterator(reinterpret_cast<const K*>(sorted_keys), BlockOfKey());
+    return rocprim::run_length_encode(temp, bytes, blocks, (unsigned)n, unique, counts, nruns, s);
+}
+
+hipError_t sort_temp_bytes(int n, int end_bit, size_t* bytes) {
     *bytes = 0;
-    return hipcub::DeviceRunLengthEncode::Encode(nullptr, *bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                 (uint32_t*)nullptr, (int*)nullptr, n);
+    if (end_bit <= 32)
+        return radix_sort_as<uint32_t>(nullptr, nullptr, *bytes, nullptr, nullptr, nullptr, nullptr, n, 0, end_bit);
+    return radix_sort_as<uint64_t>(nullptr, nullptr, *bytes, nullptr, nullptr, nullptr, nullptr, n, 0, end_bit);
+}
+
+hipError_t rle_temp_bytes(int n, int end_bit, size_t* bytes) {
+    *bytes = 0;
+    if (end_bit <= 32) return block_runs_as<uint32_t>(nullptr, nullptr, *bytes, n, nullptr, nullptr, nullptr, nullptr);
+    return block_runs_as<uint64_t>(nullptr, nullptr, *bytes, n, nullptr, nullptr, nullptr, nullptr);
 }
 
 hipError_t launch_radix_sort(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* keys_in,
-                             uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit) {
-    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, end_bit,
-                                              s);
+                             uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int n, int begin_bit,
+                             int end_bit) {
+    if (end_bit <= 32)
+        return radix_sort_as<uint32_t>(s, temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, begin_bit,
+                                       end_bit);
+    return radix_sort_as<uint64_t>(s, temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, begin_bit, end_bit);
 }
 
-hipError_t launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, const uint64_t* sorted_keys,
-                             uint64_t* ids_tmp, uint64_t* unique, uint32_t* counts, int* nruns) {
+hipError_t launch_block_runs(hipStream_t s, void* temp, size_t temp_bytes, int n, int end_bit,
+                             const uint64_t* sorted_keys, uint64_t* unique, uint32_t* counts, int* nruns) {
     if (n <= 0) return hipSuccess;
-    block_ids_kernel<<<blocks_for(n), kBlock, 0, s>>>(n, sorted_keys, ids_tmp);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return hipcub::DeviceRunLengthEncode::Encode(temp, temp_bytes, ids_tmp, unique, counts, nruns, n, s);
+    if (end_bit <= 32) return block_runs_as<uint32_t>(s, temp, temp_bytes, n, sorted_keys, unique, counts, nruns);
+    return block_runs_as<uint64_t>(s, temp, temp_bytes, n, sorted_keys, unique, counts, nruns);
 }
 
 void launch_paste_batch(hipStream_t s, const Params& P, float dt, int n, const float* h_in, float* h_out, float* tau,
