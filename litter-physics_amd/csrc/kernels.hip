@@ -1036,6 +1036,30 @@ __device__ inline void block_append_rounds(unsigned bits, int first, int* list, 
     }
 }
 
+// The slab reads of one particle's G2P update: position, mass, V0, the plastic
+// strain (paste) or J - 1 (liquid) and, for a paste, H = F - I.  Nothing is
+// read for p >= end.
+struct G2pIn {
+    float x, y, z, pm, pv0, pe, H[9];
+};
+__device__ inline void g2p_load(const float* __restrict__ src, const float* __restrict__ shared, size_t cap, int kind,
+                                int p, int end, G2pIn& o) {
+    o.x = o.y = o.z = o.pm = o.pv0 = o.pe = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) o.H[i] = 0.0f;
+    if (p >= end) return;
+    o.x = src[PX * cap + p];
+    o.y = src[PY * cap + p];
+    o.z = src[PZ * cap + p];
+    o.pm = shared[SMASS * cap + p];
+    o.pv0 = shared[SVOL0 * cap + p];
+    o.pe = src[(kind == 0 ? EPS : JM1) * cap + p];
+    if (kind == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) o.H[i] = src[(H00 + i) * cap + p];
+    }
+}
+
 // grid_to_particle with the constitutive update (solver.rs:895-977,
 // constitutive.rs:456-589) into the trial slab.  One workgroup per work item:
 // the node velocities of the item's stencils (the box of the committed slab's
@@ -1046,7 +1070,8 @@ __device__ inline void block_append_rounds(unsigned bits, int first, int* list, 
 // particle id.  The set-up record and near-wall list of the trial slab are
 // written for the next trial.  MLP: the return-map surrogate is on.  WGS:
 // workgroups per CU the register budget allows (the launch shape picks it).
-template <bool MLP, int WGS>
+// LOG: the log of exact returns may be on (it keeps C live through the update).
+template <bool MLP, int WGS, bool LOG = MLP>
 __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
     constexpr int R = kChunk / kBlock;
     __shared__ float4 sg[kTileCap];
@@ -1098,6 +1123,12 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         t.tiled = ok && gsz <= kTileCap;
     }
 
+    // The first round's particle reads go out before the tile is staged, so
+    // their memory latency overlaps the staging; each later round's reads go
+    // out a round ahead (below).
+    G2pIn cur;
+    g2p_load(src, shared, cap, P.kind, chunk0 + tid, chunk_end, cur);
+
     // Phase A: the gather tile.
     if (t.tiled) {
         int gsz = t.gd[0] * t.gd[1] * t.gd[2];
@@ -1124,6 +1155,10 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
         const int p = chunk0 + r * kBlock + tid;
+        // The next round's reads go out now: their latency overlaps this round's
+        // gather and update (the registers for them fit in the occupancy's budget).
+        G2pIn nxt;
+        g2p_load(src, shared, cap, P.kind, p + kBlock, r + 1 < R ? chunk_end : 0, nxt);
         bool nw = false;
         // Trial: gather, advection, trial deformation (or the whole liquid update).
         bool live = false, yielded = false;
@@ -1134,22 +1169,16 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         Stencil s2;
         float jn = 0.0f;
         PasteTrial ts;
-        // The particle's remaining slab reads, issued before the gather so that
-        // their memory latency overlaps it (the barrier keeps the scheduler from
-        // sinking them to their first use).
-        float Hp[9], pm = 0.0f, pv0 = 0.0f, pe = 0.0f;
-        if (p < chunk_end) {
-            x = src[PX * cap + p];
-            y = src[PY * cap + p];
-            z = src[PZ * cap + p];
-            pm = shared[SMASS * cap + p];
-            pv0 = shared[SVOL0 * cap + p];
-            pe = src[(P.kind == 0 ? EPS : JM1) * cap + p];
-            if (P.kind == 0) {
+        // This round's reads, issued a round earlier (the barrier keeps the
+        // scheduler from sinking the next round's reads to their first use).
+        float Hp[9];
+        x = cur.x;
+        y = cur.y;
+        z = cur.z;
+        const float pm = cur.pm, pv0 = cur.pv0, pe = cur.pe;
 #pragma unroll
-                for (int i = 0; i < 9; ++i) Hp[i] = src[(H00 + i) * cap + p];
-            }
-        }
+        for (int i = 0; i < 9; ++i) Hp[i] = cur.H[i];
+        cur = nxt;
         __builtin_amdgcn_sched_barrier(0);
         if (p < chunk_end) {
             Stencil s;
@@ -1185,6 +1214,29 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
                         for (int j = 0; j < 3; ++j)
                             Ht[3 * i + j] = H[3 * i + j] + dt * C[3 * i + j] +
                                             dt * (C[3 * i] * H[j] + C[3 * i + 1] * H[3 + j] + C[3 * i + 2] * H[6 + j]);
+                }
+                // Advection and every write and reduction that needs no
+                // constitutive result go out before the update, so that x, v and
+                // C are dead through it (fewer registers live at the peak).  A
+                // failed update rejects the whole trial, so these writes, the
+                // set-up box and the near-wall flag are then never used.
+                const float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
+                dst[PX * cap + p] = nx;
+                dst[PY * cap + p] = ny;
+                dst[PZ * cap + p] = nz;
+                dst[VX * cap + p] = v[0];
+                dst[VY * cap + p] = v[1];
+                dst[VZ * cap + p] = v[2];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
+                bnd1 = fmaxf(bnd1, particle_bound_motion(P, pm, v, C));
+                if (make_stencil(P, nx, ny, nz, s2)) {
+                    dep = true;
+                    nw = near_wall(P, nx, ny, nz);
+                } else {
+                    escaped += 1;
+                }
+                if (P.kind == 0) {
                     rc = paste_trial(P, Ht, ts);
                     yielded = rc == F_OK && paste_yields(P, ts);
                 } else {
@@ -1276,27 +1328,9 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
                 unsigned long long f = ((unsigned long long)B.ids[p] << 8) | (unsigned long long)rc;
                 fail = f < fail ? f : fail;
             } else {
-                const float nx = x + v[0] * dt, ny = y + v[1] * dt, nz = z + v[2] * dt;
-                dst[PX * cap + p] = nx;
-                dst[PY * cap + p] = ny;
-                dst[PZ * cap + p] = nz;
-                dst[VX * cap + p] = v[0];
-                dst[VY * cap + p] = v[1];
-                dst[VZ * cap + p] = v[2];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) dst[(C00 + i) * cap + p] = C[i];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) dst[(TXX + i) * cap + p] = tau[i];
-                float b1, b2;
-                particle_bounds(P, pm, pv0, v, C, tau, b1, b2);
-                bnd1 = fmaxf(bnd1, b1);
-                bnd2 = fmaxf(bnd2, b2);
-                if (make_stencil(P, nx, ny, nz, s2)) {
-                    dep = true;
-                    nw = near_wall(P, nx, ny, nz);
-                } else {
-                    escaped += 1;
-                }
+                bnd2 = fmaxf(bnd2, particle_bound_stress(P, pv0, tau));
             }
         }
         nwbits |= nw ? (1u << r) : 0u;
@@ -1306,7 +1340,7 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
             nhi[a] = max(nhi[a], wave_max_uniform(dep ? s2.base[a] : INT_MIN));
         }
         // Log of exact returns (training data of the surrogate).
-        if (B.rlog) {
+        if (LOG && B.rlog) {
             const bool lg = live && P.kind == 0 && yielded && rc == F_OK;
             const unsigned long long mk = __ballot(lg);
             if (mk) {
@@ -2038,6 +2072,8 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
     if (B.nwork > 0) {
         if (B.mlp_fwd)
             g2p_kernel<true, 2><<<B.nwork, kBlock, 0, s>>>(B);
+        else if (B.rlog)
+            g2p_kernel<false, 2, true><<<B.nwork, kBlock, 0, s>>>(B);
         else if (B.g2p_occupancy >= 3)
             g2p_kernel<false, 3><<<B.nwork, kBlock, 0, s>>>(B);
         else

@@ -746,18 +746,25 @@ constexpr float kMaxWeight = 0.421875f;
 // and every wall-traction reaction at most kMaxWeight dt (b2 + 7 m |g|).  No
 // contraction: the trial step that writes a state and the set-up kernel that
 // rereads it get the same bits.
-__device__ inline void particle_bounds(const Params& P, float m, float v0, const float* v, const float* C,
-                                       const float* tau, float& b1, float& b2) {
+__device__ inline float particle_bound_motion(const Params& P, float m, const float* v, const float* C) {
 #pragma clang fp contract(off)
     const float vm = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fabsf(v[2]));
     float cm = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) cm = fmaxf(cm, (fabsf(C[3 * a]) + fabsf(C[3 * a + 1])) + fabsf(C[3 * a + 2]));
+    return m * (vm + (1.5f * P.dx) * cm);
+}
+__device__ inline float particle_bound_stress(const Params& P, float v0, const float* tau) {
+#pragma clang fp contract(off)
     const float r0 = (fabsf(tau[0]) + fabsf(tau[3])) + fabsf(tau[4]),
                 r1 = (fabsf(tau[3]) + fabsf(tau[1])) + fabsf(tau[5]),
                 r2 = (fabsf(tau[4]) + fabsf(tau[5])) + fabsf(tau[2]);
-    b1 = m * (vm + (1.5f * P.dx) * cm);
-    b2 = (6.0f * P.inv_dx) * (v0 * fmaxf(fmaxf(r0, r1), r2));
+    return (6.0f * P.inv_dx) * (v0 * fmaxf(fmaxf(r0, r1), r2));
+}
+__device__ inline void particle_bounds(const Params& P, float m, float v0, const float* v, const float* C,
+                                       const float* tau, float& b1, float& b2) {
+    b1 = particle_bound_motion(P, m, v, C);
+    b2 = particle_bound_stress(P, v0, tau);
 }
 
 // |v| as the stability limit takes it (no contraction, as particle_bounds).
