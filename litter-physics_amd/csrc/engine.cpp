@@ -36,6 +36,7 @@ using mpm_host::eigen3;
 using mpm_host::qmatrix;
 
 constexpr int kMaxStepsPerSync = 256;
+constexpr int kStampsPerStep = 4 * kStampSlots;  // kernel timing stamps of one trial step
 constexpr int kLocalitySortInterval = 32;  // host re-sorts for locality every 32 trial steps
 
 // ------------------------------------------------------------ f64 host math
@@ -607,9 +608,9 @@ struct mpm_engine {
         HIP_OK(hipMalloc(&sort_temp, std::max<size_t>(sort_bytes, 16)));
         // Kernel timing stamps (the kernels write them; no event records, which
         // would leave the GPU idle for microseconds between the kernels they separate).
-        if (int rc = dalloc(&d_ktime, (size_t)4 * kMaxStepsPerSync)) return rc;
+        if (int rc = dalloc(&d_ktime, (size_t)kStampsPerStep * kMaxStepsPerSync)) return rc;
         if (int rc = dalloc(&d_run_clock, 1)) return rc;
-        HIP_OK(hipHostMalloc((void**)&h_ktime, sizeof(unsigned long long) * 4 * kMaxStepsPerSync,
+        HIP_OK(hipHostMalloc((void**)&h_ktime, sizeof(unsigned long long) * kStampsPerStep * kMaxStepsPerSync,
                              hipHostMallocDefault));
         {
             int dev = 0, khz = 0;
@@ -1036,17 +1037,17 @@ struct mpm_engine {
         } else {
             if (profiling) {
                 // Start stamps all ones (atomic minima), end stamps zero (atomic maxima).
-                for (int j = 0; j < k; ++j) {
-                    h_ktime[4 * j] = h_ktime[4 * j + 2] = ~0ull;
-                    h_ktime[4 * j + 1] = h_ktime[4 * j + 3] = 0ull;
-                }
-                HIP_OK(hipMemcpyAsync(d_ktime, h_ktime, sizeof(unsigned long long) * 4 * k, hipMemcpyHostToDevice,
-                                      stream));
+                for (int j = 0; j < k; ++j)
+                    for (int q = 0; q < 4; ++q)
+                        for (int w = 0; w < kStampSlots; ++w)
+                            h_ktime[kStampsPerStep * j + q * kStampSlots + w] = q % 2 == 0 ? ~0ull : 0ull;
+                HIP_OK(hipMemcpyAsync(d_ktime, h_ktime, sizeof(unsigned long long) * kStampsPerStep * k,
+                                      hipMemcpyHostToDevice, stream));
             }
             for (int j = 0; j < k; ++j) {
                 profiled[j] = profiling && profile_seq++ % (uint64_t)profile_every == 0;
                 BatchDev Bj = B;
-                Bj.ktime = profiled[j] ? d_ktime + 4 * j : nullptr;
+                Bj.ktime = profiled[j] ? d_ktime + kStampsPerStep * j : nullptr;
                 Bj.fuse_control = j + 1 < k;
                 launch_step(stream, Bj, twgs, pel, j == 0);
             }
@@ -1058,8 +1059,8 @@ struct mpm_engine {
     // after the stream has been synchronised).
     int fetch_profile(int k) {
         if (profiling)
-            HIP_OK(hipMemcpyAsync(h_ktime, d_ktime, sizeof(unsigned long long) * 4 * k, hipMemcpyDeviceToHost,
-                                  stream));
+            HIP_OK(hipMemcpyAsync(h_ktime, d_ktime, sizeof(unsigned long long) * kStampsPerStep * k,
+                                  hipMemcpyDeviceToHost, stream));
         return MPM_OK;
     }
     // Kernel times of the profiled steps of the last enqueue, from the stamps
@@ -1072,7 +1073,13 @@ struct mpm_engine {
         for (int j = 0; j < k; ++j) {
             if (!profiled[j]) continue;
             profiled[j] = false;
-            const unsigned long long* t = h_ktime + 4 * j;
+            // The earliest start and the latest end over the partial slots.
+            unsigned long long t[4] = {~0ull, 0ull, ~0ull, 0ull};
+            for (int q = 0; q < 4; ++q)
+                for (int w = 0; w < kStampSlots; ++w) {
+                    const unsigned long long v = h_ktime[kStampsPerStep * j + q * kStampSlots + w];
+                    t[q] = q % 2 == 0 ? std::min(t[q], v) : std::max(t[q], v);
+                }
             if (t[1] <= t[0] || t[3] <= t[2]) continue;
             p2g_ms += (double)(t[1] - t[0]) * ms_per_tick;
             g2p_ms += (double)(t[3] - t[2]) * ms_per_tick;

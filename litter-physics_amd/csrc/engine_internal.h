@@ -76,6 +76,8 @@ constexpr int kLedger = 31;  // doubles of mpm_ledger
 constexpr int kGridVals = 12;
 // Node blocks per grid-update workgroup; instance block ranges are multiples of it.
 constexpr int kGridWgBlocks = 16;
+// Kernel timing stamps: partial minima and maxima per timed quantity.
+constexpr int kStampSlots = 16;
 // Energy audit (audit.rs): grid kinetic energies at five stages of one trial
 // (transport, after stress, after traction, after gravity, after walls), and
 // per instance those five plus the trial's plastic-work sum.
@@ -234,8 +236,11 @@ struct BatchDev {
     int p2g_block;      // threads of a deposit workgroup (256 or 512)
     int fuse_control;   // finalize also chooses the next trial (control_instance) for every instance
     // Kernel timing (null: off): the deposit's and the G2P's earliest workgroup
-    // start and latest workgroup end on the device's constant clock,
-    // {p2g start, p2g end, g2p start, g2p end}, set up as {max, 0, max, 0}.
+    // start and latest workgroup end on the device's constant clock, as
+    // kStampSlots partial minima or maxima each (workgroup w stamps slot
+    // w % kStampSlots, so no address takes every workgroup's atomic):
+    // {p2g start, p2g end, g2p start, g2p end} x kStampSlots, set up as
+    // {max, 0, max, 0}.
     unsigned long long* ktime;
     long long* gmem[2];  // kAuditFields * na integer sums each
     int* gflag[2];       // one per node block

@@ -96,13 +96,19 @@ __device__ inline int instance_of(const BatchDev& B, int p) {
 }
 
 // Kernel timing on the device's constant clock: the earliest workgroup start
-// and the latest workgroup end of a launch, one atomic per workgroup each.
-// kernel_end is called after the workgroup's last barrier.
-__device__ inline void kernel_start(unsigned long long* slot) {
-    if (threadIdx.x == 0) atomicMin(slot, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+// and the latest workgroup end of a launch, one atomic per workgroup each,
+// spread over kStampSlots partial slots (a single address taking every
+// workgroup's atomic cost the step 10 us).  kernel_end is called after the
+// workgroup's last barrier.  q: the timed quantity (0..3).
+__device__ inline void kernel_start(unsigned long long* stamps, int q) {
+    if (threadIdx.x == 0)
+        atomicMin(stamps + q * kStampSlots + (blockIdx.x % kStampSlots),
+                  (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
-__device__ inline void kernel_end(unsigned long long* slot) {
-    if (threadIdx.x == 0) atomicMax(slot, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+__device__ inline void kernel_end(unsigned long long* stamps, int q) {
+    if (threadIdx.x == 0)
+        atomicMax(stamps + q * kStampSlots + (blockIdx.x % kStampSlots),
+                  (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 
 // Reference order key (clamped cell index, z fastest, then id; solver.rs:501-517),
@@ -305,7 +311,7 @@ template <int NF, int BS>
 __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
     constexpr int R = kChunk / BS;
     __shared__ __align__(16) unsigned long long ss[NF * kTileCap];
-    if (B.ktime) kernel_start(B.ktime);
+    if (B.ktime) kernel_start(B.ktime, 0);
     // The item and the instance fields the rounds use, read once into uniform
     // registers (read through references, they would be reloaded after every
     // global store in the loop).
@@ -394,7 +400,7 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
     }
     __syncthreads();
     if (!t.tiled) {
-        if (B.ktime) kernel_end(B.ktime + 1);
+        if (B.ktime) kernel_end(B.ktime, 1);
         return;
     }
     // Flush: the tile's exact sums into the global ones.  A node with a
@@ -417,7 +423,7 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
     }
     if (B.ktime) {
         __syncthreads();
-        kernel_end(B.ktime + 1);
+        kernel_end(B.ktime, 1);
     }
 }
 
@@ -1080,7 +1086,7 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
     __shared__ Fix128 sdiss[4];
     __shared__ unsigned long long sfail[4];
     __shared__ int sesc[4];
-    if (B.ktime) kernel_start(B.ktime + 2);
+    if (B.ktime) kernel_start(B.ktime, 2);
     const int4 item = B.work[blockIdx.x];
     const int inst = item.x;
     InstState& is = B.is[inst];
@@ -1430,7 +1436,7 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         B.pfail[blockIdx.x] = f;
         B.pesc[blockIdx.x] = e;
     }
-    if (B.ktime) kernel_end(B.ktime + 3);
+    if (B.ktime) kernel_end(B.ktime, 3);
 }
 
 // ------------------------------------------------------------ reductions and commit
