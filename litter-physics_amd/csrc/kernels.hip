@@ -445,6 +445,12 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
     // their memory round trips.  A thread takes the three nodes of its column
     // along the wall axis; one of them is past the wall-side nodes.
     const long long items = 9LL * total;
+    // A batch of one instance reads the instance's list fields together with
+    // the count (both list counts, selected by the slab): the list entry then
+    // waits for one memory round trip, not three.
+    const InstState& s0 = B.is[0];
+    const int cur0 = s0.cur, nwc00 = s0.nw_count[0], nwc01 = s0.nw_count[1], nwb0 = s0.nw_base;
+    const int poff0 = B.ip[0].poff;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < items;
          t += (long long)gridDim.x * blockDim.x) {
         const int q = (int)(t / 9), col = (int)(t - 9LL * q);
@@ -460,9 +466,11 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
         const InstState& s = B.is[lo];
         const InstParams& ip = B.ip[lo];
         const Params P = ip.P;
-        int local = q - (s.nw_base - s.nw_count[s.cur]);
-        int p = B.nw_list[s.cur][ip.poff + local];
-        const float* slab = B.slab[s.cur];
+        const bool one = B.ninst == 1;
+        const int cur = one ? cur0 : s.cur;
+        const int local = q - (one ? nwb0 - (cur0 ? nwc01 : nwc00) : s.nw_base - s.nw_count[cur]);
+        const int p = B.nw_list[cur][(one ? poff0 : ip.poff) + local];
+        const float* slab = B.slab[cur];
         const float dt = (float)s.dt;
         const long long* gsum = B.gmem[s.gcur];
         const double um = 1.0 / (double)s.sc_m, up = 1.0 / (double)s.sc_p;
@@ -562,7 +570,20 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     __shared__ double ascratch[AUDIT ? 4 * kAuditVals : 1];
     __shared__ int sbad[4];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int inst = B.blk_inst[blockIdx.x * kGridWgBlocks];
+    static_assert(kGridWgBlocks == 16, "four node blocks per wavefront");
+    constexpr int J = 4;  // node blocks per wavefront
+    // The flags of both accumulator sets go out first, with the instance state
+    // (which of the two is the trial's is only known from it), and a batch of
+    // one instance needs no block-to-instance lookup: the sums wait for one
+    // memory round trip instead of three.
+    int fl0[J], fl1[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
+        fl0[j] = B.gflag[0][b];
+        fl1[j] = B.gflag[1][b];
+    }
+    const int inst = B.ninst == 1 ? 0 : B.blk_inst[blockIdx.x * kGridWgBlocks];
     const InstState& s = B.is[inst];
     if (!s.active) return;
     const InstParams& ip = B.ip[inst];
@@ -577,19 +598,16 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
-    // The workgroup's blocks in groups of 16, four per wavefront, each group's
-    // loads issued together.
-    for (int grp = 0; grp < kGridWgBlocks / 16; ++grp) {
-        constexpr int J = 4;  // node blocks per wavefront and group
+    {
+        const int grp = 0;
         // All loads of the wavefront's blocks are issued before any result is used:
         // the flags of both sets, then the sums and traction of the touched blocks,
         // so a wavefront waits for two memory round trips instead of two per block.
         int cf[J], af[J];
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
-            cf[j] = C.flag[b];
-            af[j] = A.flag[b];
+            cf[j] = s.gcur ? fl0[j] : fl1[j];
+            af[j] = s.gcur ? fl1[j] : fl0[j];
         }
         float fm[J], fpx[J], fpy[J], fpz[J], ffx[J], ffy[J], ffz[J], ftb[J][3];
         bool wall[J];
@@ -1599,60 +1617,58 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     constexpr int kWords = sizeof(InstState) / 4;
     const int inst = blockIdx.x, tid = threadIdx.x;
     InstState& gs = B.is[inst];
+    // The partials' ranges come from the parameters and the host-set work-item
+    // range, not from the staged state, so the first batch of partial loads goes
+    // out with the staging: one memory round trip for all of them.
+    const InstParams& ip = B.ip[inst];
+    const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
+    const int w0 = gs.w_off, w1 = gs.w_off + gs.w_cnt;
     {
         const unsigned* src = reinterpret_cast<const unsigned*>(&gs);
         unsigned* dst = reinterpret_cast<unsigned*>(&ls);
         for (int w = tid; w < kWords; w += 256) dst[w] = src[w];
     }
-    __syncthreads();
-    InstState& s = ls;
-    if (!s.active) {
-        if (B.fuse_control && B.ninst > 1) near_wall_prefix_last(B);
-        return;
-    }
-    const InstParams& ip = B.ip[inst];
-    const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
     double acc[kGridVals + 1];
     for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
-    // Each thread sums its partials in index order.  The loads of several
-    // partials are issued together (one memory round trip per batch, not per
-    // partial); a load past the end contributes 0, which leaves a sum unchanged.
+    float pmx[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // speed, displacement, deposit bounds
+    Fix128 diss = {0ull, 0ll};
+    unsigned long long f = ~0ull;
+    int esc = 0;
     int bad = 0;
-    for (int q0 = g0 + tid; q0 < g1; q0 += 4 * 256) {
+    // Each thread sums its partials in index order: grid partials four, work-item
+    // partials eight at a time, the loads of a batch issued together (one memory
+    // round trip per batch, not per partial); a load past the end contributes
+    // the identity, which leaves a sum or an extremum unchanged.
+    for (int it = 0; g0 + it * 4 * 256 < g1 || w0 + it * 8 * 256 < w1; ++it) {
         double x[4][kGridVals];
         int xb[4];
+        const int qg = g0 + it * 4 * 256 + tid;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int q = q0 + u * 256;
+            const int q = qg + u * 256;
 #pragma unroll
             for (int i = 0; i < kGridVals; ++i) x[u][i] = q < g1 ? B.gpart[(size_t)i * B.ngwg + q] : 0.0;
             xb[u] = q < g1 ? B.gbad[q] : 0;
+        }
+        float4 xs[8];
+        Fix128 xe[8];
+        unsigned long long xf[8];
+        int xc[8];
+        const int qw = w0 + it * 8 * 256 + tid;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = qw + u * 256;
+            const bool in = q < w1;
+            xs[u] = in ? reinterpret_cast<const float4*>(B.pmax)[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            xe[u] = in ? B.pdiss[q] : Fix128{0ull, 0ll};
+            xf[u] = in ? B.pfail[q] : ~0ull;
+            xc[u] = in ? B.pesc[q] : 0;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
 #pragma unroll
             for (int i = 0; i < kGridVals; ++i) acc[i] += x[u][i];
             bad += xb[u];
-        }
-    }
-    float pmx[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // speed, displacement, deposit bounds
-    Fix128 diss = {0ull, 0ll};
-    unsigned long long f = ~0ull;
-    int esc = 0;
-    const int w1 = s.w_off + s.w_cnt;
-    for (int q0 = s.w_off + tid; q0 < w1; q0 += 8 * 256) {
-        float4 xs[8];
-        Fix128 xe[8];
-        unsigned long long xf[8];
-        int xc[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int q = q0 + u * 256;
-            const bool in = q < w1;
-            xs[u] = in ? reinterpret_cast<const float4*>(B.pmax)[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            xe[u] = in ? B.pdiss[q] : Fix128{0ull, 0ll};
-            xf[u] = in ? B.pfail[q] : ~0ull;
-            xc[u] = in ? B.pesc[q] : 0;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -1664,6 +1680,12 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
             f = xf[u] < f ? xf[u] : f;
             esc += xc[u];
         }
+    }
+    __syncthreads();
+    InstState& s = ls;
+    if (!s.active) {
+        if (B.fuse_control && B.ninst > 1) near_wall_prefix_last(B);
+        return;
     }
     // Pellet sums (the instance's pellets, in parallel).
     double pel[kPelSums];
@@ -1692,16 +1714,20 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
     for (int i = 0; i < kGridVals; ++i) acc[i] = wave_sum_last(acc[i]);
+    // Without pellets their sums stay zero: the reductions are skipped (a
+    // uniform branch), a third of the reduction work.
+    if (ip.npel > 0) {
 #pragma unroll
-    for (int i = 0; i < kPelSums; ++i) pel[i] = wave_sum_last(pel[i]);
+        for (int i = 0; i < kPelSums; ++i) pel[i] = wave_sum_last(pel[i]);
+        pbad = wave_sum_i(pbad != 0 ? 1 : 0);
+        vmax = wave_max_d(vmax);
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) pmx[k] = wave_max(pmx[k]);
     diss = wave_fix_sum(diss);
     f = wave_min_u64(f);
     bad = wave_sum_i(bad);
     esc = wave_sum_i(esc);
-    pbad = wave_sum_i(pbad != 0 ? 1 : 0);
-    vmax = wave_max_d(vmax);
     __shared__ double ssum[4][kGridVals + 1 + kPelSums];
     __shared__ double svm[4];
     __shared__ float sfm[4][4];
