@@ -96,14 +96,16 @@ __device__ inline int instance_of(const BatchDev& B, int p) {
 }
 
 // Kernel timing on the device's constant clock: the earliest workgroup start
-// and the latest workgroup end of a launch, one atomic per workgroup each,
-// spread over kStampSlots partial slots (a single address taking every
-// workgroup's atomic cost the step 10 us).  kernel_end is called after the
-// workgroup's last barrier.  q: the timed quantity (0..3).
+// and the latest workgroup end of a launch.  Workgroups start in index order,
+// so the first 64 stamp the start (one address); every workgroup stamps its
+// end, spread over kStampSlots partial slots (a single address taking every
+// workgroup's atomic cost the step 10 us; a start address that depends on the
+// workgroup index made G2P's register allocation 44 VGPRs larger).
+// kernel_end is called after the workgroup's last barrier.  q: the timed
+// quantity (0..3).
 __device__ inline void kernel_start(unsigned long long* stamps, int q) {
-    if (threadIdx.x == 0)
-        atomicMin(stamps + q * kStampSlots + (blockIdx.x % kStampSlots),
-                  (unsigned long long)__builtin_amdgcn_s_memrealtime());
+    if (threadIdx.x == 0 && blockIdx.x < 64)
+        atomicMin(stamps + q * kStampSlots, (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 __device__ inline void kernel_end(unsigned long long* stamps, int q) {
     if (threadIdx.x == 0)
@@ -1095,7 +1097,7 @@ __device__ inline void g2p_load(const float* __restrict__ src, const float* __re
 // written for the next trial.  MLP: the return-map surrogate is on.  WGS:
 // workgroups per CU the register budget allows (the launch shape picks it).
 // LOG: the log of exact returns may be on (it keeps C live through the update).
-template <bool MLP, int WGS, bool LOG = MLP>
+template <bool MLP, int WGS, bool LOG>
 __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
     constexpr int R = kChunk / kBlock;
     __shared__ float4 sg[kTileCap];
@@ -1149,9 +1151,10 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
 
     // The first round's particle reads go out before the tile is staged, so
     // their memory latency overlaps the staging; each later round's reads go
-    // out a round ahead (below).
+    // out a round ahead (below).  The surrogate's instance has no registers to
+    // spare for the reads in flight: it reads each round's at its start.
     G2pIn cur;
-    g2p_load(src, shared, cap, P.kind, chunk0 + tid, chunk_end, cur);
+    if (!MLP) g2p_load(src, shared, cap, P.kind, chunk0 + tid, chunk_end, cur);
 
     // Phase A: the gather tile.
     if (t.tiled) {
@@ -1182,7 +1185,10 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         // The next round's reads go out now: their latency overlaps this round's
         // gather and update (the registers for them fit in the occupancy's budget).
         G2pIn nxt;
-        g2p_load(src, shared, cap, P.kind, p + kBlock, r + 1 < R ? chunk_end : 0, nxt);
+        if (MLP)
+            g2p_load(src, shared, cap, P.kind, p, chunk_end, cur);
+        else
+            g2p_load(src, shared, cap, P.kind, p + kBlock, r + 1 < R ? chunk_end : 0, nxt);
         bool nw = false;
         // Trial: gather, advection, trial deformation (or the whole liquid update).
         bool live = false, yielded = false;
@@ -1202,7 +1208,7 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
         const float pm = cur.pm, pv0 = cur.pv0, pe = cur.pe;
 #pragma unroll
         for (int i = 0; i < 9; ++i) Hp[i] = cur.H[i];
-        cur = nxt;
+        if (!MLP) cur = nxt;
         __builtin_amdgcn_sched_barrier(0);
         if (p < chunk_end) {
             Stencil s;
@@ -2102,14 +2108,16 @@ void launch_step(hipStream_t s, const BatchDev& B, int traction_wgs, bool pellet
         grid_update_kernel<false><<<B.ngwg, 256, 0, s>>>(B);
     if (pellets) couple_kernel<<<B.ngwg * kGridWgBlocks, 64, 0, s>>>(B);
     if (B.nwork > 0) {
-        if (B.mlp_fwd)
-            g2p_kernel<true, 2><<<B.nwork, kBlock, 0, s>>>(B);
+        if (B.mlp_fwd && B.rlog)
+            g2p_kernel<true, 2, true><<<B.nwork, kBlock, 0, s>>>(B);
+        else if (B.mlp_fwd)
+            g2p_kernel<true, 2, false><<<B.nwork, kBlock, 0, s>>>(B);
         else if (B.rlog)
             g2p_kernel<false, 2, true><<<B.nwork, kBlock, 0, s>>>(B);
         else if (B.g2p_occupancy >= 3)
-            g2p_kernel<false, 3><<<B.nwork, kBlock, 0, s>>>(B);
+            g2p_kernel<false, 3, false><<<B.nwork, kBlock, 0, s>>>(B);
         else
-            g2p_kernel<false, 2><<<B.nwork, kBlock, 0, s>>>(B);
+            g2p_kernel<false, 2, false><<<B.nwork, kBlock, 0, s>>>(B);
     }
     if (pellets) pellet_kernel<<<(B.npel + 63) / 64, 64, 0, s>>>(B);
     finalize_kernel<<<B.ninst, 256, 0, s>>>(B);
