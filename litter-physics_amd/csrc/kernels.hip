@@ -1348,7 +1348,9 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
                 if (rc == F_OK) {
 #pragma unroll
                     for (int i = 0; i < 9; ++i) dst[(H00 + i) * cap + p] = Hn[i];
-                    fix_add(diss, fix_from((double)(q * inc) * (double)pv0));
+                    // Only a yielded particle dissipates; a wavefront with none skips
+                    // the f64 fixed-point conversion (a zero adds nothing).
+                    if (inc != 0.0f) fix_add(diss, fix_from((double)(q * inc) * (double)pv0));
                     dst[EPS * cap + p] = pe + inc;
                 }
             } else if (rc == F_OK) {
