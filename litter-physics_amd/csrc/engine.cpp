@@ -1325,7 +1325,7 @@ struct mpm_engine {
         if (!rc) {
             double best_ms = std::numeric_limits<double>::infinity();
             const mpm_launch kernels = best;
-            for (int k : {8, 32})
+            for (int k : {8, 32, 64})
                 for (int g : {0, 1}) {
                     mpm_launch c = kernels;
                     c.steps_per_sync = k;
