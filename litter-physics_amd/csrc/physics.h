@@ -865,7 +865,8 @@ __device__ inline void deposit_particle(const Stencil& s, const Deposit<NF>& q, 
         for (int c = 0; c < 3; ++c) {
             const float w = wab * s.w[2][c];
             const float dz = s.d[2][c];
-            sink.add(0, c, (int)ceilf(w * q.ms));
+            // The mass is never negative: its 64-bit word needs no sign extension.
+            sink.add_unsigned(0, c, (unsigned)ceilf(w * q.ms));
             sink.add(1, c, round_int(w * fmaf(q.A[2], dz, p0)));
             sink.add(2, c, round_int(w * fmaf(q.A[5], dz, p1)));
             sink.add(3, c, round_int(w * fmaf(q.A[8], dz, p2)));
@@ -890,6 +891,9 @@ struct TileSink {
     __device__ inline void add(int f, int c, int v) {
         atomicAdd(ss + f * CAP + row + c, (unsigned long long)(long long)v);
     }
+    __device__ inline void add_unsigned(int f, int c, unsigned v) {
+        atomicAdd(ss + f * CAP + row + c, (unsigned long long)v);
+    }
 };
 
 // Deposit straight into the global accumulators (64-bit atomics), for particles
@@ -906,6 +910,9 @@ struct GlobalSink {
     }
     __device__ inline void add(int f, int c, int val) {
         if (val) atomicAdd(reinterpret_cast<unsigned long long*>(v + f * na + ni[c]), (unsigned long long)(long long)val);
+    }
+    __device__ inline void add_unsigned(int f, int c, unsigned val) {
+        if (val) atomicAdd(reinterpret_cast<unsigned long long*>(v + f * na + ni[c]), (unsigned long long)val);
     }
 };
 
