@@ -601,7 +601,6 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
     {
-        const int grp = 0;
         // All loads of the wavefront's blocks are issued before any result is used:
         // the flags of both sets, then the sums and traction of the touched blocks,
         // so a wavefront waits for two memory round trips instead of two per block.
@@ -615,7 +614,7 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
         bool wall[J];
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
+            const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
             const size_t n = ((size_t)b << 6) | l;
             fm[j] = fpx[j] = fpy[j] = fpz[j] = ffx[j] = ffy[j] = ffz[j] = 0.0f;
             wall[j] = false;
@@ -645,7 +644,7 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
         }
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
+            const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
             const size_t n = ((size_t)b << 6) | l;
             if (cf[j]) {
 #pragma unroll
@@ -656,7 +655,7 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             if (!af[j]) continue;
-            const int b = blockIdx.x * kGridWgBlocks + 16 * grp + 4 * j + wave;
+            const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
             const size_t n = ((size_t)b << 6) | l;
             const bool wall_block = wall[j];
             const float m = fm[j];
