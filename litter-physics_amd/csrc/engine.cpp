@@ -339,6 +339,9 @@ struct mpm_engine {
     int* blk_inst = nullptr;
     double* gpart = nullptr;
     int* gbad = nullptr;
+    double* gchunk = nullptr;
+    int* gchunk_bad = nullptr;
+    ItemSlot* islot = nullptr;
     // energy audit (allocated on first use)
     double* apart = nullptr;
     double* d_audit = nullptr;
@@ -360,9 +363,6 @@ struct mpm_engine {
     int nwork = 0;
     size_t work_cap = 0;
     float* pmax = nullptr;
-    Fix128* pdiss = nullptr;
-    unsigned long long* pfail = nullptr;
-    int* pesc = nullptr;
     // sort
     uint64_t* keys[2] = {nullptr, nullptr};
     uint32_t* order[2] = {nullptr, nullptr};
@@ -458,10 +458,10 @@ struct mpm_engine {
         B.gbad = gbad;
         B.apart = auditing ? apart : nullptr;
         B.audit = auditing ? d_audit : nullptr;
+        B.gchunk = gchunk;
+        B.gchunk_bad = gchunk_bad;
+        B.islot = islot;
         B.pmax = pmax;
-        B.pdiss = pdiss;
-        B.pfail = pfail;
-        B.pesc = pesc;
         B.pel = d_pel;
         B.pgeo = d_pgeo;
         B.pacc = d_pacc;
@@ -568,6 +568,17 @@ struct mpm_engine {
         HIP_OK(hipMemcpy(blk_inst, binst.data(), sizeof(int) * binst.size(), hipMemcpyHostToDevice));
         if (int rc = dalloc(&gpart, (size_t)kGridVals * ngwg)) return rc;
         if (int rc = dalloc(&gbad, (size_t)ngwg)) return rc;
+        if (int rc = dalloc(&gchunk, (size_t)ninst * kGridVals * kGridChunks)) return rc;
+        if (int rc = dalloc(&gchunk_bad, (size_t)ninst * kGridChunks)) return rc;
+        if (int rc = dalloc(&islot, (size_t)ninst * kItemSlots)) return rc;
+        {
+            std::vector<ItemSlot> idle((size_t)ninst * kItemSlots);
+            for (ItemSlot& sl : idle) {
+                std::memset(&sl, 0, sizeof(sl));
+                sl.fail = ~0ull;
+            }
+            HIP_OK(hipMemcpy(islot, idle.data(), sizeof(ItemSlot) * idle.size(), hipMemcpyHostToDevice));
+        }
         if (int rc = dalloc(&d_ip, (size_t)ninst)) return rc;
         if (int rc = dalloc(&d_is, (size_t)ninst)) return rc;
         HIP_OK(hipHostMalloc((void**)&h_is, sizeof(InstState) * ninst, hipHostMallocDefault));
@@ -595,9 +606,6 @@ struct mpm_engine {
         if (int rc = dalloc(&d_work, work_cap)) return rc;
         if (int rc = dalloc(&wpre, 2 * work_cap)) return rc;
         if (int rc = dalloc(&pmax, 4 * work_cap)) return rc;
-        if (int rc = dalloc(&pdiss, work_cap)) return rc;
-        if (int rc = dalloc(&pfail, work_cap)) return rc;
-        if (int rc = dalloc(&pesc, work_cap)) return rc;
         if (int rc = dalloc(&run_unique, cap, false)) return rc;
         if (int rc = dalloc(&run_count, cap, false)) return rc;
         if (int rc = dalloc(&d_nruns, 1)) return rc;
@@ -646,6 +654,9 @@ struct mpm_engine {
         (void)hipFree(blk_inst);
         (void)hipFree(gpart);
         (void)hipFree(gbad);
+        (void)hipFree(gchunk);
+        (void)hipFree(gchunk_bad);
+        (void)hipFree(islot);
         (void)hipFree(apart);
         (void)hipFree(d_audit);
         (void)hipFree(d_ip);
@@ -660,9 +671,6 @@ struct mpm_engine {
         (void)hipFree(d_work);
         (void)hipFree(wpre);
         (void)hipFree(pmax);
-        (void)hipFree(pdiss);
-        (void)hipFree(pfail);
-        (void)hipFree(pesc);
         (void)hipFree(run_unique);
         (void)hipFree(run_count);
         (void)hipFree(d_nruns);

@@ -73,7 +73,7 @@ constexpr int kMaxSpheres = 64;
 // momentum and the stress sum.
 constexpr int kDepositFields = 4, kAuditFields = 7;
 constexpr int kLedger = 31;  // doubles of mpm_ledger
-constexpr int kGridVals = 12;
+constexpr int kGridVals = 9;
 // Node blocks per grid-update workgroup; instance block ranges are multiples of it.
 constexpr int kGridWgBlocks = 16;
 // Kernel timing stamps: partial minima and maxima per timed quantity.
@@ -118,7 +118,6 @@ struct StepStatus {
     double friction;             // wall_friction_dissipation increment
     double traction_energy;      // wall_normal_traction_energy increment
     double dissipation;          // plastic dissipation increment
-    double grid_momentum[3];     // sum of m v' over nodes after the walls
     float max_speed;
     float max_disp;
     float bound[2];              // deposit bounds of the trial state (particle_bounds)
@@ -170,6 +169,27 @@ struct Fix128 {
     unsigned long long lo;
     long long hi;
 };
+
+// G2P's step reductions of one instance, reached by order-independent integer
+// atomics: work item w adds into slot w % kItemSlots of its instance, so no
+// address takes every workgroup's atomic.  The maxima are of non-negative
+// floats, as their bit patterns; the dissipation is a Fix128 sum kept in three
+// 64-bit limbs (bits 0-31 and 32-63 of the low word, the high word modulo
+// 2^64), so no carry crosses a word below 2^32 additions.  The finalize kernel
+// reads an instance's slots and leaves them at the identity (0, except fail:
+// all ones) for the next trial.
+constexpr int kItemSlots = 16;
+struct ItemSlot {
+    unsigned max_bits[4];     // speed, displacement, the two deposit bounds
+    unsigned long long fail;  // (particle << 8) | code of the first failure
+    unsigned long long diss[3];
+    int escaped;
+    int pad[3];
+};
+// Grid partials per instance after G2P's first-level reduction: the first
+// kGridChunks work items of an instance each sum one chunk of its grid-update
+// workgroups' partials (unused chunks stay zero).
+constexpr int kGridChunks = 64;
 
 // Per-pellet coupling accumulators (rigid.rs:350-363): impulse (3), angular
 // impulse (3) and grid energy, exact fixed point; constrained node count.
@@ -260,10 +280,12 @@ struct BatchDev {
     double* apart;    // audit partials, field-major like gpart (NULL: no audit)
     double* audit;    // audit sums per instance, kAuditOut each (NULL: no audit)
     int* gbad;
-    float* pmax;      // g2p partials per work item: speed, displacement, the two deposit bounds
-    Fix128* pdiss;    // g2p plastic-dissipation partials per work item
-    unsigned long long* pfail;
-    int* pesc;
+    // Chunk sums of gpart, field-major per instance:
+    // gchunk[(inst * kGridVals + v) * kGridChunks + chunk], and of gbad.
+    double* gchunk;
+    int* gchunk_bad;  // [inst * kGridChunks + chunk]
+    ItemSlot* islot;  // [inst * kItemSlots + slot]
+    float* pmax;      // set-up partials per work item: speed, -, the two deposit bounds
     PelletState* pel;
     PelletGeo* pgeo;
     PelletAccum* pacc;

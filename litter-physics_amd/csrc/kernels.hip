@@ -597,7 +597,7 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     constexpr int NF = AUDIT ? kAuditFields : kDepositFields;
     const double um = 1.0 / (double)s.sc_m, up = 1.0 / (double)(AUDIT ? s.sc_mom : s.sc_p),
                  uf = 1.0 / (double)s.sc_f, ut = 1.0 / (double)s.sc_t;
-    double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double vals[kGridVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double avals[kAuditVals] = {0, 0, 0, 0, 0};
     int bad = 0;
     {
@@ -743,7 +743,6 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     vals[a] += ((double)v[a] - (double)before[a]) * (double)m;
-                    vals[9 + a] += (double)v[a] * (double)m;
                     if (AUDIT) avals[4] += 0.5 * (double)m * (double)v[a] * (double)v[a];
                 }
             }
@@ -1456,10 +1455,47 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
             f = sfail[w] < f ? sfail[w] : f;
             e += sesc[w];
         }
-        for (int k = 0; k < 4; ++k) B.pmax[4 * blockIdx.x + k] = red[k];
-        B.pdiss[blockIdx.x] = d;
-        B.pfail[blockIdx.x] = f;
-        B.pesc[blockIdx.x] = e;
+        // Into one of the instance's step-reduction slots, by non-returning
+        // atomics; identities are skipped (a maximum of +0 or -0 among them:
+        // the maxima start at +0).
+        ItemSlot* sl = B.islot + inst * kItemSlots + blockIdx.x % kItemSlots;
+        for (int k = 0; k < 4; ++k)
+            if (red[k] > 0.0f) atomicMax(&sl->max_bits[k], __float_as_uint(red[k]));
+        if (f != ~0ull) atomicMin(&sl->fail, f);
+        if ((d.lo | (unsigned long long)d.hi) != 0ull) {
+            atomicAdd(&sl->diss[0], d.lo & 0xffffffffull);
+            atomicAdd(&sl->diss[1], d.lo >> 32);
+            atomicAdd(&sl->diss[2], (unsigned long long)d.hi);
+        }
+        if (e) atomicAdd(&sl->escaped, e);
+    }
+    // The instance's first kGridChunks work items also sum the grid update's
+    // partials, chunk by chunk (the grid update ended before this launch), so
+    // the finalize kernel reads kGridChunks records per value instead of one
+    // per grid workgroup: wavefront w takes the values w, w + 4, ..., lane l
+    // the chunk's workgroups l, l + 64, ... in order, then a fixed-order
+    // wavefront sum.
+    {
+        const int j = (int)blockIdx.x - is.w_off;
+        const int K = min(is.w_cnt, kGridChunks);
+        if (j < K) {
+            const int g0 = ip.boff / kGridWgBlocks, ng = ip.nblk / kGridWgBlocks;
+            const int cs = (ng + K - 1) / K;
+            const int c0 = g0 + j * cs, c1 = min(c0 + cs, g0 + ng);
+            for (int v = wave; v <= kGridVals; v += 4) {
+                if (v < kGridVals) {
+                    double a = 0.0;
+                    for (int q = c0 + lane; q < c1; q += 64) a += B.gpart[(size_t)v * B.ngwg + q];
+                    a = wave_sum_last(a);
+                    if (lane == 63) B.gchunk[((size_t)inst * kGridVals + v) * kGridChunks + j] = a;
+                } else {
+                    int nb = 0;
+                    for (int q = c0 + lane; q < c1; q += 64) nb += B.gbad[q];
+                    nb = wave_sum_i(nb);
+                    if (lane == 0) B.gchunk_bad[inst * kGridChunks + j] = nb;
+                }
+            }
+        }
     }
     if (B.ktime) kernel_end(B.ktime, 3);
 }
@@ -1611,196 +1647,146 @@ __device__ inline void near_wall_prefix_last(const BatchDev& B) {
     }
 }
 
-// One workgroup per running instance: fixed-order reduction of its grid and
-// particle partials and of its pellets' step sums, then the accept/reject.  The
-// instance state is staged in LDS (one round trip for all of it), committed
-// there by one thread and written back by all.  The instance's pellets are then
-// committed (rigid.rs:334, solver.rs:600-609) and their geometry for the next
-// trial computed from the committed state, with cleared coupling sums.
+// One workgroup per running instance: the step reductions of its trial, then
+// the accept/reject.  Everything the kernel reads comes in one memory round
+// trip: the instance's state and parameters, staged in LDS by all threads,
+// and, in wavefront 0, the grid chunk sums (lane c: chunk c) and G2P's
+// step-reduction slots (lane s < kItemSlots: slot s), both at addresses that
+// depend on the instance index alone.  Wavefront 0 reduces them (sums by DPP
+// into lane 63, in a fixed order; extrema and integer sums by DPP and
+// shuffles) and clears them for the next trial; lane 63 commits the instance
+// in LDS (commit_instance) and all threads write the state back.  The
+// instance's pellets are then committed (rigid.rs:334, solver.rs:600-609) and
+// their geometry for the next trial computed from the committed state, with
+// cleared coupling sums.
 __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     __shared__ double sd[4];
     __shared__ InstState ls;
-    static_assert(sizeof(InstState) % 4 == 0, "InstState is staged as 32-bit words");
-    constexpr int kWords = sizeof(InstState) / 4;
-    const int inst = blockIdx.x, tid = threadIdx.x;
+    __shared__ InstParams lp;
+    static_assert(sizeof(InstState) % 4 == 0 && sizeof(InstParams) % 4 == 0, "staged as 32-bit words");
+    static_assert(kGridChunks == 64, "one grid chunk per lane of wavefront 0");
+    constexpr int kWords = sizeof(InstState) / 4, kPWords = sizeof(InstParams) / 4;
+    const int inst = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     InstState& gs = B.is[inst];
-    // The partials' ranges come from the parameters and the host-set work-item
-    // range, not from the staged state, so the first batch of partial loads goes
-    // out with the staging: one memory round trip for all of them.
-    const InstParams& ip = B.ip[inst];
-    const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
-    const int w0 = gs.w_off, w1 = gs.w_off + gs.w_cnt;
     {
         const unsigned* src = reinterpret_cast<const unsigned*>(&gs);
         unsigned* dst = reinterpret_cast<unsigned*>(&ls);
         for (int w = tid; w < kWords; w += 256) dst[w] = src[w];
+        const unsigned* psrc = reinterpret_cast<const unsigned*>(&B.ip[inst]);
+        unsigned* pdst = reinterpret_cast<unsigned*>(&lp);
+        for (int w = tid; w < kPWords; w += 256) pdst[w] = psrc[w];
     }
-    double acc[kGridVals + 1];
-    for (int i = 0; i <= kGridVals; ++i) acc[i] = 0.0;
-    float pmx[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // speed, displacement, deposit bounds
-    Fix128 diss = {0ull, 0ll};
-    unsigned long long f = ~0ull;
-    int esc = 0;
+    double acc[kGridVals];
     int bad = 0;
-    // Each thread sums its partials in index order: grid partials four, work-item
-    // partials eight at a time, the loads of a batch issued together (one memory
-    // round trip per batch, not per partial); a load past the end contributes
-    // the identity, which leaves a sum or an extremum unchanged.
-    for (int it = 0; g0 + it * 4 * 256 < g1 || w0 + it * 8 * 256 < w1; ++it) {
-        double x[4][kGridVals];
-        int xb[4];
-        const int qg = g0 + it * 4 * 256 + tid;
+    ItemSlot sl = {{0u, 0u, 0u, 0u}, ~0ull, {0ull, 0ull, 0ull}, 0, {0, 0, 0}};
+    if (wave == 0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int q = qg + u * 256;
-#pragma unroll
-            for (int i = 0; i < kGridVals; ++i) x[u][i] = q < g1 ? B.gpart[(size_t)i * B.ngwg + q] : 0.0;
-            xb[u] = q < g1 ? B.gbad[q] : 0;
-        }
-        float4 xs[8];
-        Fix128 xe[8];
-        unsigned long long xf[8];
-        int xc[8];
-        const int qw = w0 + it * 8 * 256 + tid;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int q = qw + u * 256;
-            const bool in = q < w1;
-            xs[u] = in ? reinterpret_cast<const float4*>(B.pmax)[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            xe[u] = in ? B.pdiss[q] : Fix128{0ull, 0ll};
-            xf[u] = in ? B.pfail[q] : ~0ull;
-            xc[u] = in ? B.pesc[q] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-#pragma unroll
-            for (int i = 0; i < kGridVals; ++i) acc[i] += x[u][i];
-            bad += xb[u];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            pmx[0] = fmaxf(pmx[0], xs[u].x);
-            pmx[1] = fmaxf(pmx[1], xs[u].y);
-            pmx[2] = fmaxf(pmx[2], xs[u].z);
-            pmx[3] = fmaxf(pmx[3], xs[u].w);
-            fix_add(diss, xe[u]);
-            f = xf[u] < f ? xf[u] : f;
-            esc += xc[u];
-        }
+        for (int v = 0; v < kGridVals; ++v) acc[v] = B.gchunk[((size_t)inst * kGridVals + v) * kGridChunks + lane];
+        bad = B.gchunk_bad[inst * kGridChunks + lane];
+        if (lane < kItemSlots) sl = B.islot[inst * kItemSlots + lane];
     }
     __syncthreads();
     InstState& s = ls;
+    const InstParams& ip = lp;
     if (!s.active) {
         if (B.fuse_control && B.ninst > 1) near_wall_prefix_last(B);
         return;
     }
-    // Pellet sums (the instance's pellets, in parallel).
-    double pel[kPelSums];
-    for (int i = 0; i < kPelSums; ++i) pel[i] = 0.0;
-    double vmax = 0.0;
-    int pbad = 0;
-    for (int k = tid; k < ip.npel; k += 256) {
-        const int pi = ip.pel_off + k;
-        const PelletTrial& t = B.ptrial[pi];
-        const PelletAccum& a = B.pacc[pi];
-        pel[0] += B.pel[pi].mass;
-        for (int c = 0; c < 3; ++c) {
-            pel[1 + c] += t.wall_impulse[c];
-            pel[4 + c] += fix_value(a.sum[c]);
-            pel[7 + c] += fix_value(a.sum[3 + c]);
-        }
-        pel[10] += t.wall_work;
-        pel[11] += fix_value(a.sum[6]);
-        pel[12] += t.energy;
-        pbad |= t.bad;
-        vmax = fmax(vmax, sqrt(t.vel[0] * t.vel[0] + t.vel[1] * t.vel[1] + t.vel[2] * t.vel[2]));
-    }
-    // One fixed-order block reduction of everything: each wavefront reduces its
-    // lanes (sums by DPP into lane 63, extrema by shuffles), then thread 0
-    // combines the four wavefronts in order.
-    const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-    for (int i = 0; i < kGridVals; ++i) acc[i] = wave_sum_last(acc[i]);
-    // Without pellets their sums stay zero: the reductions are skipped (a
-    // uniform branch), a third of the reduction work.
+    // Pellet sums (the instance's pellets, in parallel), reduced across the
+    // workgroup into ssum; skipped without pellets (a uniform branch).
+    __shared__ double ssum[4][kPelSums];
+    __shared__ double svm[4];
+    __shared__ int spbad[4];
     if (ip.npel > 0) {
+        double pel[kPelSums];
+        for (int i = 0; i < kPelSums; ++i) pel[i] = 0.0;
+        double vmax = 0.0;
+        int pbad = 0;
+        for (int k = tid; k < ip.npel; k += 256) {
+            const int pi = ip.pel_off + k;
+            const PelletTrial& t = B.ptrial[pi];
+            const PelletAccum& a = B.pacc[pi];
+            pel[0] += B.pel[pi].mass;
+            for (int c = 0; c < 3; ++c) {
+                pel[1 + c] += t.wall_impulse[c];
+                pel[4 + c] += fix_value(a.sum[c]);
+                pel[7 + c] += fix_value(a.sum[3 + c]);
+            }
+            pel[10] += t.wall_work;
+            pel[11] += fix_value(a.sum[6]);
+            pel[12] += t.energy;
+            pbad |= t.bad;
+            vmax = fmax(vmax, sqrt(t.vel[0] * t.vel[0] + t.vel[1] * t.vel[1] + t.vel[2] * t.vel[2]));
+        }
 #pragma unroll
         for (int i = 0; i < kPelSums; ++i) pel[i] = wave_sum_last(pel[i]);
         pbad = wave_sum_i(pbad != 0 ? 1 : 0);
         vmax = wave_max_d(vmax);
+        if (lane == 63)
+            for (int i = 0; i < kPelSums; ++i) ssum[wave][i] = pel[i];
+        if (lane == 0) {
+            svm[wave] = vmax;
+            spbad[wave] = pbad;
+        }
+        __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) pmx[k] = wave_max(pmx[k]);
-    diss = wave_fix_sum(diss);
-    f = wave_min_u64(f);
-    bad = wave_sum_i(bad);
-    esc = wave_sum_i(esc);
-    __shared__ double ssum[4][kGridVals + 1 + kPelSums];
-    __shared__ double svm[4];
-    __shared__ float sfm[4][4];
-    __shared__ Fix128 sfx[4];
-    __shared__ unsigned long long sfl[4];
-    __shared__ int sint[4][3];
-    if (lane == 63) {
-#pragma unroll
-        for (int i = 0; i < kGridVals; ++i) ssum[wave][i] = acc[i];
-#pragma unroll
-        for (int i = 0; i < kPelSums; ++i) ssum[wave][kGridVals + 1 + i] = pel[i];
-    }
-    if (lane == 0) {
-        svm[wave] = vmax;
-        for (int k = 0; k < 4; ++k) sfm[wave][k] = pmx[k];
-        sfx[wave] = diss;
-        sfl[wave] = f;
-        sint[wave][0] = bad;
-        sint[wave][1] = esc;
-        sint[wave][2] = pbad;
-    }
-    __syncthreads();
-    double tot[kGridVals + 1];
-    if (tid == 0) {
-        for (int i = 0; i < kGridVals; ++i) tot[i] = ((ssum[0][i] + ssum[1][i]) + ssum[2][i]) + ssum[3][i];
-        Fix128 d = sfx[0];
-        for (int w = 1; w < 4; ++w) fix_add(d, sfx[w]);
-        tot[kGridVals] = fix_value(d);
-    }
+    const int g0 = ip.boff / kGridWgBlocks, g1 = (ip.boff + ip.nblk) / kGridWgBlocks;
     if (B.audit) {
-        // Audit stage sums of this instance, in the same fixed order.
+        // Audit stage sums of this instance, in a fixed order.
         for (int i = 0; i < kAuditVals; ++i) {
             double a = 0.0;
             for (int q = g0 + tid; q < g1; q += 256) a += B.apart[(size_t)i * B.ngwg + q];
             a = block_total(a, sd);
             if (tid == 0) B.audit[(size_t)inst * kAuditOut + i] = a;
         }
-        if (tid == 0) B.audit[(size_t)inst * kAuditOut + kAuditVals] = tot[kGridVals];
     }
-    if (tid == 0) {
-        for (int i = 0; i < kPelSums; ++i) {
-            const int j = kGridVals + 1 + i;
-            pel[i] = ((ssum[0][j] + ssum[1][j]) + ssum[2][j]) + ssum[3][j];
+    if (wave == 0) {
+#pragma unroll
+        for (int v = 0; v < kGridVals; ++v) acc[v] = wave_sum_last(acc[v]);
+        bad = wave_sum_i(bad);
+        float mx[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mx[k] = wave_fmax_uniform(__uint_as_float(sl.max_bits[k]));
+        const unsigned long long fl = wave_min_u64(sl.fail);
+        const unsigned long long l0 = wave_sum_u64(sl.diss[0]), l1 = wave_sum_u64(sl.diss[1]),
+                                 lh = wave_sum_u64(sl.diss[2]);
+        const int esc = wave_sum_i(sl.escaped);
+        // Cleared for the next trial (the values are in registers).
+#pragma unroll
+        for (int v = 0; v < kGridVals; ++v) B.gchunk[((size_t)inst * kGridVals + v) * kGridChunks + lane] = 0.0;
+        B.gchunk_bad[inst * kGridChunks + lane] = 0;
+        if (lane < kItemSlots) B.islot[inst * kItemSlots + lane] = ItemSlot{{0u, 0u, 0u, 0u}, ~0ull, {0ull, 0ull, 0ull}, 0, {0, 0, 0}};
+        if (lane == 63) {
+            const double diss = fix_value(fix_from_limbs(l0, l1, lh));
+            if (B.audit) B.audit[(size_t)inst * kAuditOut + kAuditVals] = diss;
+            double pel[kPelSums];
+            double pv = 0.0;
+            int pb = 0;
+            for (int i = 0; i < kPelSums; ++i) pel[i] = 0.0;
+            if (ip.npel > 0) {
+                for (int i = 0; i < kPelSums; ++i) pel[i] = ((ssum[0][i] + ssum[1][i]) + ssum[2][i]) + ssum[3][i];
+                pv = fmax(fmax(svm[0], svm[1]), fmax(svm[2], svm[3]));
+                pb = spbad[0] + spbad[1] + spbad[2] + spbad[3];
+            }
+            StepStatus& st = s.st;
+            for (int a = 0; a < 3; ++a) {
+                st.wall_impulse[a] = acc[a];
+                st.traction_impulse[a] = acc[3 + a];
+            }
+            st.projection = acc[6];
+            st.friction = acc[7];
+            st.traction_energy = acc[8];
+            st.dissipation = diss;
+            st.max_speed = mx[0];
+            st.max_disp = mx[1];
+            st.bound[0] = mx[2];
+            st.bound[1] = mx[3];
+            st.fail = fl;
+            st.grid_nonfinite = bad;
+            st.escaped = esc;
+            st.pellet_bad = pb != 0;
+            commit_instance(s, ip, pel, pv);
         }
-        StepStatus& st = s.st;
-        for (int a = 0; a < 3; ++a) {
-            st.wall_impulse[a] = tot[a];
-            st.traction_impulse[a] = tot[3 + a];
-            st.grid_momentum[a] = tot[9 + a];
-        }
-        st.projection = tot[6];
-        st.friction = tot[7];
-        st.traction_energy = tot[8];
-        st.dissipation = tot[kGridVals];
-        st.max_speed = fmaxf(fmaxf(sfm[0][0], sfm[1][0]), fmaxf(sfm[2][0], sfm[3][0]));
-        st.max_disp = fmaxf(fmaxf(sfm[0][1], sfm[1][1]), fmaxf(sfm[2][1], sfm[3][1]));
-        st.bound[0] = fmaxf(fmaxf(sfm[0][2], sfm[1][2]), fmaxf(sfm[2][2], sfm[3][2]));
-        st.bound[1] = fmaxf(fmaxf(sfm[0][3], sfm[1][3]), fmaxf(sfm[2][3], sfm[3][3]));
-        unsigned long long fl = sfl[0];
-        for (int w = 1; w < 4; ++w) fl = sfl[w] < fl ? sfl[w] : fl;
-        st.fail = fl;
-        st.grid_nonfinite = sint[0][0] + sint[1][0] + sint[2][0] + sint[3][0];
-        st.escaped = sint[0][1] + sint[1][1] + sint[2][1] + sint[3][1];
-        st.pellet_bad = (sint[0][2] + sint[1][2] + sint[2][2] + sint[3][2]) != 0;
-        const double pv = fmax(fmax(svm[0], svm[1]), fmax(svm[2], svm[3]));
-        commit_instance(s, ip, pel, pv);
     }
     __syncthreads();
     if (ip.npel > 0) {

@@ -64,6 +64,11 @@ __device__ inline int wave_sum_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 __device__ inline unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -690,6 +695,16 @@ __device__ inline void fix_add(Fix128& a, const Fix128& b) {
     const unsigned long long lo = a.lo + b.lo;
     a.hi += b.hi + (lo < a.lo ? 1ll : 0ll);
     a.lo = lo;
+}
+
+// The Fix128 value of three limb sums l0 + l1 2^32 + h 2^64 (modulo 2^128),
+// each limb a sum of fewer than 2^32 terms below 2^32 (l0, l1) or modulo 2^64 (h).
+__device__ inline Fix128 fix_from_limbs(unsigned long long l0, unsigned long long l1, unsigned long long h) {
+    Fix128 r;
+    r.lo = l0 + (l1 << 32);
+    const unsigned long long carry = r.lo < l0 ? 1ull : 0ull;
+    r.hi = (long long)(h + (l1 >> 32) + carry);
+    return r;
 }
 
 // The double nearest the fixed-point value (two roundings, a function of the
