@@ -431,6 +431,17 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
 
 // ------------------------------------------------------------ traction
 
+// A workgroup's traction deposits merged by node in LDS before they reach
+// memory: open addressing with linear probing on the key axis * na + node.
+// Neighbouring near-wall particles share most of their wall nodes, and every
+// global 64-bit atomic on one address waits for the previous one, so one
+// atomic per node and workgroup instead of one per deposit shortens the
+// drain after the kernel's last wave.  A deposit that finds no free entry
+// within kTrProbes probes goes to memory directly; integer sums do not
+// depend on the order.
+constexpr int kTrHash = 1024, kTrProbes = 32;
+constexpr unsigned kTrEmpty = 0xffffffffu;
+
 // wall_traction_to_grid (solver.rs:725-819) over the near-wall lists of the
 // running instances, concatenated.  Each deposit adds its reaction to the
 // Coulomb budget (trT) of its node, which also gives the signed traction
@@ -441,6 +452,15 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
     const int total = B.counters[1];
     const float* shared = B.shared[B.sh];
     const size_t cap = B.cap, na = B.na;
+    if ((long long)blockIdx.x * blockDim.x >= 9LL * total) return;  // no work in this workgroup
+    __shared__ unsigned hkey[kTrHash];
+    __shared__ unsigned long long hval[kTrHash];
+    const bool merge = 3 * na < (size_t)kTrEmpty;  // keys fit 32 bits
+    for (int i = threadIdx.x; i < kTrHash; i += kBlock) {
+        hkey[i] = kTrEmpty;
+        hval[i] = 0ull;
+    }
+    __syncthreads();
     // One thread per (near-wall particle, stencil column across the wall): a
     // particle's node visits are independent (each reads its own wall-plane node
     // and deposits on its own node), so spreading them over threads overlaps
@@ -542,13 +562,32 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
                 if (reaction <= 0.0f) continue;
                 if (dm[off] <= 0) continue;
                 const int units = round_int(reaction * tscale);
-                atomicAdd(reinterpret_cast<unsigned long long*>(B.trT + axis * na + nis[off]),
-                          (unsigned long long)(long long)units);
+                const size_t key = axis * na + nis[off];
+                bool held = false;
+                if (merge) {
+                    unsigned h = ((unsigned)key * 2654435761u) >> 22;  // 10 bits: kTrHash
+                    for (int probe = 0; probe < kTrProbes; ++probe) {
+                        const unsigned old = atomicCAS(&hkey[h], kTrEmpty, (unsigned)key);
+                        if (old == kTrEmpty || old == (unsigned)key) {
+                            atomicAdd(&hval[h], (unsigned long long)(long long)units);
+                            held = true;
+                            break;
+                        }
+                        h = (h + 1) & (kTrHash - 1);
+                    }
+                }
+                if (!held)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(B.trT + key), (unsigned long long)(long long)units);
                 if (P.cells[axis] <= 2)
                     atomicAdd(reinterpret_cast<unsigned long long*>(B.trS + axis * na + nis[off]),
                               (unsigned long long)(long long)(face == 0.0f ? units : -units));
             }
         }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTrHash; i += kBlock) {
+        const unsigned key = hkey[i];
+        if (key != kTrEmpty) atomicAdd(reinterpret_cast<unsigned long long*>(B.trT) + key, hval[i]);
     }
 }
 
