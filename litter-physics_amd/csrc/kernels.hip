@@ -352,7 +352,7 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
         for (int q = tid; q < NF * kTileCap / 2; q += BS)
             reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
     __syncthreads();
-    const int rot = (lane * 5) % 9;
+    const int rot = (lane * 5) % 9, ra = rot / 3, rb = rot - 3 * ra;
     const int s12 = __mul24(t.gd[1], t.gd[2]);
     const int end = item.y + item.z;
 #pragma unroll 1
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
             sink.row0 = __mul24(__mul24(s.base[0] - t.org[0], t.gd[1]) + (s.base[1] - t.org[1]), t.gd[2]) +
                         (s.base[2] - t.org[2]);
             sink.row = 0;
-            deposit_particle<NF>(s, q, rot, sink);
+            deposit_particle<NF>(s, q, ra, rb, sink);
         } else {
             GlobalSink sink;
             sink.v = out.v;
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) sink.base[a] = s.base[a];
             flag_stencil(P, out.flag, s);
-            deposit_particle<NF>(s, q, rot, sink);
+            deposit_particle<NF>(s, q, ra, rb, sink);
         }
     }
     __syncthreads();

@@ -853,42 +853,55 @@ __device__ inline int round_int(float x) {
 // nonzero weight is marked occupied, as its f64 mass would be; momentum
 // rounds half up.  The tile and the global paths both come through here, so a
 // particle gives the same integers whichever path it takes.  Lanes walk the
-// nine (a, b) stencil columns starting at column `rot` so that particles of
-// one cell, which share most nodes, hit different LDS words (rot = 5 lane mod 9
-// measured fewest conflicts).
+// nine (a, b) stencil columns from column (ra, rb), both indices advancing
+// modulo 3, so that particles of one cell, which share most nodes, hit
+// different LDS words at the same time (ra = rot / 3, rb = rot % 3 with
+// rot = 5 lane mod 9).  The weights and offsets along x and y are rotated
+// once per particle, so the nine unrolled columns index them by constants.
 template <int NF, class Sink>
-__device__ inline void deposit_particle(const Stencil& s, const Deposit<NF>& q, int rot, Sink& sink) {
+__device__ inline void deposit_particle(const Stencil& s, const Deposit<NF>& q, int ra, int rb, Sink& sink) {
 #pragma clang fp contract(off)
-    for (int kk = 0; kk < 9; ++kk) {
-        int ab = kk + rot;
-        ab = ab >= 9 ? ab - 9 : ab;
-        const int a = ab >= 6 ? 2 : (ab >= 3 ? 1 : 0);
-        const int b = ab - 3 * a;
-        const float wab = sel3(a, s.w[0][0], s.w[0][1], s.w[0][2]) * sel3(b, s.w[1][0], s.w[1][1], s.w[1][2]);
-        const float dx = sel3(a, s.d[0][0], s.d[0][1], s.d[0][2]), dy = sel3(b, s.d[1][0], s.d[1][1], s.d[1][2]);
-        const float p0 = fmaf(q.A[1], dy, fmaf(q.A[0], dx, q.mv[0])), p1 = fmaf(q.A[4], dy, fmaf(q.A[3], dx, q.mv[1])),
-                    p2 = fmaf(q.A[7], dy, fmaf(q.A[6], dx, q.mv[2]));
-        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-        if constexpr (NF == 7) {
-            // rows of the symmetric stress: (xx xy xz), (xy yy yz), (xz yz zz)
-            f0 = fmaf(q.T[3], dy, q.T[0] * dx);
-            f1 = fmaf(q.T[1], dy, q.T[3] * dx);
-            f2 = fmaf(q.T[5], dy, q.T[4] * dx);
-        }
-        sink.column(a, b);
+    float wx[3], dx[3], wy[3], dy[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float w = wab * s.w[2][c];
-            const float dz = s.d[2][c];
-            // The mass is never negative: its 64-bit word needs no sign extension.
-            sink.add_unsigned(0, c, (unsigned)ceilf(w * q.ms));
-            sink.add(1, c, round_int(w * fmaf(q.A[2], dz, p0)));
-            sink.add(2, c, round_int(w * fmaf(q.A[5], dz, p1)));
-            sink.add(3, c, round_int(w * fmaf(q.A[8], dz, p2)));
+    for (int i = 0; i < 3; ++i) {
+        const int a = ra + i >= 3 ? ra + i - 3 : ra + i, b = rb + i >= 3 ? rb + i - 3 : rb + i;
+        wx[i] = sel3(a, s.w[0][0], s.w[0][1], s.w[0][2]);
+        dx[i] = sel3(a, s.d[0][0], s.d[0][1], s.d[0][2]);
+        wy[i] = sel3(b, s.w[1][0], s.w[1][1], s.w[1][2]);
+        dy[i] = sel3(b, s.d[1][0], s.d[1][1], s.d[1][2]);
+    }
+    sink.rotate(ra, rb);
+#pragma unroll
+    for (int ka = 0; ka < 3; ++ka) {
+        const float px0 = fmaf(q.A[0], dx[ka], q.mv[0]), px1 = fmaf(q.A[3], dx[ka], q.mv[1]),
+                    px2 = fmaf(q.A[6], dx[ka], q.mv[2]);
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+            const float wab = wx[ka] * wy[kb];
+            const float p0 = fmaf(q.A[1], dy[kb], px0), p1 = fmaf(q.A[4], dy[kb], px1),
+                        p2 = fmaf(q.A[7], dy[kb], px2);
+            float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
             if constexpr (NF == 7) {
-                sink.add(4, c, round_int(w * fmaf(q.T[4], dz, f0)));
-                sink.add(5, c, round_int(w * fmaf(q.T[5], dz, f1)));
-                sink.add(6, c, round_int(w * fmaf(q.T[2], dz, f2)));
+                // rows of the symmetric stress: (xx xy xz), (xy yy yz), (xz yz zz)
+                f0 = fmaf(q.T[3], dy[kb], q.T[0] * dx[ka]);
+                f1 = fmaf(q.T[1], dy[kb], q.T[3] * dx[ka]);
+                f2 = fmaf(q.T[5], dy[kb], q.T[4] * dx[ka]);
+            }
+            sink.column(ka, kb);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float w = wab * s.w[2][c];
+                const float dz = s.d[2][c];
+                // The mass is never negative: its 64-bit word needs no sign extension.
+                sink.add_unsigned(0, c, (unsigned)ceilf(w * q.ms));
+                sink.add(1, c, round_int(w * fmaf(q.A[2], dz, p0)));
+                sink.add(2, c, round_int(w * fmaf(q.A[5], dz, p1)));
+                sink.add(3, c, round_int(w * fmaf(q.A[8], dz, p2)));
+                if constexpr (NF == 7) {
+                    sink.add(4, c, round_int(w * fmaf(q.T[4], dz, f0)));
+                    sink.add(5, c, round_int(w * fmaf(q.T[5], dz, f1)));
+                    sink.add(6, c, round_int(w * fmaf(q.T[2], dz, f2)));
+                }
             }
         }
     }
@@ -900,9 +913,16 @@ template <int NF, int CAP>
 struct TileSink {
     unsigned long long* ss;
     int row0, s12, sd2, row;
-    __device__ inline void column(int a, int b) {
-        row = row0 + (int)__umul24((unsigned)a, (unsigned)s12) + (int)__umul24((unsigned)b, (unsigned)sd2);
+    int oa[3], ob[3];  // row offsets of the rotated columns
+    __device__ inline void rotate(int ra, int rb) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int a = ra + i >= 3 ? ra + i - 3 : ra + i, b = rb + i >= 3 ? rb + i - 3 : rb + i;
+            oa[i] = row0 + (int)__umul24((unsigned)a, (unsigned)s12);
+            ob[i] = (int)__umul24((unsigned)b, (unsigned)sd2);
+        }
     }
+    __device__ inline void column(int ka, int kb) { row = oa[ka] + ob[kb]; }
     __device__ inline void add(int f, int c, int v) {
         atomicAdd(ss + f * CAP + row + c, (unsigned long long)(long long)v);
     }
@@ -919,7 +939,13 @@ struct GlobalSink {
     const Params* P;
     int base[3];
     int ni[3];
-    __device__ inline void column(int a, int b) {
+    int ra, rb;
+    __device__ inline void rotate(int ra_, int rb_) {
+        ra = ra_;
+        rb = rb_;
+    }
+    __device__ inline void column(int ka, int kb) {
+        const int a = ra + ka >= 3 ? ra + ka - 3 : ra + ka, b = rb + kb >= 3 ? rb + kb - 3 : rb + kb;
 #pragma unroll
         for (int c = 0; c < 3; ++c) ni[c] = node_index(*P, base[0] + a, base[1] + b, base[2] + c);
     }
