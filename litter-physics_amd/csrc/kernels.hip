@@ -1782,14 +1782,14 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
     if (wave == 0) {
 #pragma unroll
         for (int v = 0; v < kGridVals; ++v) acc[v] = wave_sum_last(acc[v]);
-        bad = wave_sum_i(bad);
+        bad = wave_sum_i_last(bad);
         float mx[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) mx[k] = wave_fmax_uniform(__uint_as_float(sl.max_bits[k]));
-        const unsigned long long fl = wave_min_u64(sl.fail);
-        const unsigned long long l0 = wave_sum_u64(sl.diss[0]), l1 = wave_sum_u64(sl.diss[1]),
-                                 lh = wave_sum_u64(sl.diss[2]);
-        const int esc = wave_sum_i(sl.escaped);
+        const unsigned long long fl = wave_min_u64_last(sl.fail);
+        const unsigned long long l0 = wave_sum_u64_last(sl.diss[0]), l1 = wave_sum_u64_last(sl.diss[1]),
+                                 lh = wave_sum_u64_last(sl.diss[2]);
+        const int esc = wave_sum_i_last(sl.escaped);
         // Cleared for the next trial (the values are in registers).
 #pragma unroll
         for (int v = 0; v < kGridVals; ++v) B.gchunk[((size_t)inst * kGridVals + v) * kGridChunks + lane] = 0.0;
@@ -1825,6 +1825,15 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
             st.escaped = esc;
             st.pellet_bad = pb != 0;
             commit_instance(s, ip, pel, pv);
+            // Without pellets the next trial's control follows at once, in the
+            // same thread (a batch of one is its own near-wall prefix).
+            if (B.fuse_control && ip.npel == 0) {
+                const int nwc = control_instance(s, ip);
+                if (B.ninst == 1) {
+                    s.nw_base = nwc;
+                    B.counters[1] = nwc;
+                }
+            }
         }
     }
     __syncthreads();
@@ -1881,16 +1890,18 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
         }
         __syncthreads();
     }
-    // The next trial of this instance (what the control kernel would choose).
-    // A batch of one is its own near-wall prefix.
-    if (B.fuse_control && tid == 0) {
-        const int nwc = control_instance(s, ip);
-        if (B.ninst == 1) {
-            s.nw_base = nwc;
-            B.counters[1] = nwc;
+    // The next trial of this instance (what the control kernel would choose),
+    // here after the pellets' commit.
+    if (ip.npel > 0) {
+        if (B.fuse_control && tid == 0) {
+            const int nwc = control_instance(s, ip);
+            if (B.ninst == 1) {
+                s.nw_base = nwc;
+                B.counters[1] = nwc;
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
     {
         const unsigned* src = reinterpret_cast<const unsigned*>(&ls);
         unsigned* dst = reinterpret_cast<unsigned*>(&gs);

@@ -64,11 +64,6 @@ __device__ inline int wave_sum_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 __device__ inline unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -508,12 +503,65 @@ __device__ inline double wave_sum_last(double v) {
     return v;
 }
 
+// The same fixed-order sum for 64-bit and 32-bit integers (wrapping), left in
+// lane 63.
+template <int CTRL, int ROWS>
+__device__ inline unsigned long long dpp_take_u64(unsigned long long v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, ROWS, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, ROWS, 0xF, true);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ inline unsigned long long wave_sum_u64_last(unsigned long long v) {
+    v += dpp_take_u64<0x111, 0xF>(v);
+    v += dpp_take_u64<0x112, 0xF>(v);
+    v += dpp_take_u64<0x114, 0xF>(v);
+    v += dpp_take_u64<0x118, 0xF>(v);
+    v += dpp_take_u64<0x142, 0xA>(v);
+    v += dpp_take_u64<0x143, 0xC>(v);
+    return v;
+}
+__device__ inline int wave_sum_i_last(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, true);
+    return v;
+}
+
 // Wavefront minimum and maximum by the same DPP pattern, returned as a uniform
 // (scalar) value.  Lanes without a DPP source keep their own value.  Every lane
 // of the wavefront must be active.
 template <int CTRL, int ROWS>
 __device__ inline int dpp_self(int v) {
     return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false);
+}
+
+template <int CTRL, int ROWS>
+__device__ inline unsigned long long dpp_self_u64(unsigned long long v) {
+    const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+    const int l = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xF, false);
+    const int h = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xF, false);
+    return ((unsigned long long)(unsigned)h << 32) | (unsigned)l;
+}
+
+// Wavefront minimum of 64-bit integers, left in lane 63.
+__device__ inline unsigned long long wave_min_u64_last(unsigned long long v) {
+    unsigned long long w;
+    w = dpp_self_u64<0x111, 0xF>(v);
+    v = w < v ? w : v;
+    w = dpp_self_u64<0x112, 0xF>(v);
+    v = w < v ? w : v;
+    w = dpp_self_u64<0x114, 0xF>(v);
+    v = w < v ? w : v;
+    w = dpp_self_u64<0x118, 0xF>(v);
+    v = w < v ? w : v;
+    w = dpp_self_u64<0x142, 0xA>(v);
+    v = w < v ? w : v;
+    w = dpp_self_u64<0x143, 0xC>(v);
+    v = w < v ? w : v;
+    return v;
 }
 
 __device__ inline int wave_min_uniform(int v) {
