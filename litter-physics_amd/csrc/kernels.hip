@@ -616,13 +616,17 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     // The flags of both accumulator sets go out first, with the instance state
     // (which of the two is the trial's is only known from it), and a batch of
     // one instance needs no block-to-instance lookup: the sums wait for one
-    // memory round trip instead of three.
+    // memory round trip instead of three.  Every wavefront reads all 32 flags of
+    // the workgroup (lane i < 16: set 0 of block i, 16 <= i < 32: set 1), so
+    // each knows whether any block of the workgroup holds a deposit.
+    const int fb = blockIdx.x * kGridWgBlocks + (l & 15);
+    const int myflag = l < 16 ? B.gflag[0][fb] : (l < 32 ? B.gflag[1][fb] : 0);
+    const unsigned long long fmask = __ballot(myflag != 0);
     int fl0[J], fl1[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const int b = blockIdx.x * kGridWgBlocks + 4 * j + wave;
-        fl0[j] = B.gflag[0][b];
-        fl1[j] = B.gflag[1][b];
+        fl0[j] = (int)((fmask >> (4 * j + wave)) & 1ull);
+        fl1[j] = (int)((fmask >> (16 + 4 * j + wave)) & 1ull);
     }
     const int inst = B.ninst == 1 ? 0 : B.blk_inst[blockIdx.x * kGridWgBlocks];
     const InstState& s = B.is[inst];
@@ -790,17 +794,42 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
             B.gv[2 * na + n] = v[2];
         }
     }
-    int nbad = wave_sum_i(bad);
-    if (l == 0) sbad[wave] = nbad;
-    block_sum<kGridVals>(vals, scratch);
-    if (AUDIT) block_sum<kAuditVals>(avals, ascratch);
-    if (threadIdx.x == 0) {
+    // The workgroup's partials.  A workgroup with no deposit in the trial's set
+    // has nothing to sum (a uniform branch: every wavefront holds the same flag
+    // mask): zeros, without the reduction and its barrier.  Otherwise each
+    // wavefront sums by DPP into lane 63, and thread i combines value i over the
+    // wavefronts in order.
+    const unsigned long long amask = s.gcur ? fmask >> 16 : fmask;
+    if ((amask & 0xffffull) == 0ull) {
+        if (threadIdx.x < kGridVals) B.gpart[(size_t)threadIdx.x * B.ngwg + blockIdx.x] = 0.0;
+        if (threadIdx.x == kGridVals) B.gbad[blockIdx.x] = 0;
+        if (AUDIT && threadIdx.x < kAuditVals) B.apart[(size_t)threadIdx.x * B.ngwg + blockIdx.x] = 0.0;
+        return;
+    }
+    const int nbad = wave_sum_i_last(bad);
 #pragma unroll
-        for (int i = 0; i < kGridVals; ++i) B.gpart[(size_t)i * B.ngwg + blockIdx.x] = vals[i];
-        B.gbad[blockIdx.x] = sbad[0] + sbad[1] + sbad[2] + sbad[3];
+    for (int i = 0; i < kGridVals; ++i) vals[i] = wave_sum_last(vals[i]);
+    if (AUDIT)
+#pragma unroll
+        for (int i = 0; i < kAuditVals; ++i) avals[i] = wave_sum_last(avals[i]);
+    if (l == 63) {
+        sbad[wave] = nbad;
+#pragma unroll
+        for (int i = 0; i < kGridVals; ++i) scratch[wave * kGridVals + i] = vals[i];
         if (AUDIT)
 #pragma unroll
-            for (int i = 0; i < kAuditVals; ++i) B.apart[(size_t)i * B.ngwg + blockIdx.x] = avals[i];
+            for (int i = 0; i < kAuditVals; ++i) ascratch[wave * kAuditVals + i] = avals[i];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < kGridVals)
+        B.gpart[(size_t)t * B.ngwg + blockIdx.x] =
+            (((0.0 + scratch[t]) + scratch[kGridVals + t]) + scratch[2 * kGridVals + t]) + scratch[3 * kGridVals + t];
+    if (t == kGridVals) B.gbad[blockIdx.x] = sbad[0] + sbad[1] + sbad[2] + sbad[3];
+    if (AUDIT && t >= 32 && t < 32 + kAuditVals) {
+        const int i = t - 32;
+        B.apart[(size_t)i * B.ngwg + blockIdx.x] =
+            (((0.0 + ascratch[i]) + ascratch[kAuditVals + i]) + ascratch[2 * kAuditVals + i]) + ascratch[3 * kAuditVals + i];
     }
 }
 

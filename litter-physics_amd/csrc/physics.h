@@ -601,26 +601,6 @@ __device__ inline float wave_fmax_uniform(float v) {
 // Three VALU instructions instead of an integer division sequence.
 __device__ inline int div_small(int q, float inv) { return (int)(((float)q + 0.5f) * inv); }
 
-// Block-wide sum of NV doubles held by every thread; thread 0 gets the totals.
-template <int NV>
-__device__ inline void block_sum(double* vals, double* scratch) {
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) vals[i] = wave_sum_last(vals[i]);
-    if (lane == 63)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) scratch[wave * NV + i] = vals[i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int waves = blockDim.x >> 6;
-        for (int i = 0; i < NV; ++i) {
-            double t = 0.0;
-            for (int w = 0; w < waves; ++w) t += scratch[w * NV + i];
-            vals[i] = t;
-        }
-    }
-}
-
 // ------------------------------------------------------------ tiles
 
 // LDS tile of one work item: the nodes of the stencils of its particles (the
