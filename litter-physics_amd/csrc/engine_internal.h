@@ -75,7 +75,7 @@ constexpr int kDepositFields = 4, kAuditFields = 7;
 constexpr int kLedger = 31;  // doubles of mpm_ledger
 constexpr int kGridVals = 9;
 // Node blocks per grid-update workgroup; instance block ranges are multiples of it.
-constexpr int kGridWgBlocks = 16;
+constexpr int kGridWgBlocks = 4;
 // Kernel timing stamps: partial minima and maxima per timed quantity.
 constexpr int kStampSlots = 16;
 // Energy audit (audit.rs): grid kinetic energies at five stages of one trial

@@ -605,28 +605,32 @@ __global__ __launch_bounds__(kBlock) void traction_kernel(BatchDev B) {
 // audit.rs:360-398): the APIC transport momentum alone (the accumulated
 // sum w (m v + m C d) is exactly the audit's shadow deposit), after the stress
 // impulse, after the wall traction, with gravity's kick, and after the walls.
+// One node block per wavefront (kGridWgBlocks = 4): blocks with deposits lie
+// together, so smaller workgroups spread them over more wavefronts; at eight
+// wavefronts per SIMD the deposit-free ones cost little.
 template <bool AUDIT>
-__global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
+__global__ __launch_bounds__(256, AUDIT ? 4 : 8) void grid_update_kernel(BatchDev B) {
     __shared__ double scratch[4 * kGridVals];
     __shared__ double ascratch[AUDIT ? 4 * kAuditVals : 1];
     __shared__ int sbad[4];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    static_assert(kGridWgBlocks == 16, "four node blocks per wavefront");
-    constexpr int J = 4;  // node blocks per wavefront
+    static_assert(kGridWgBlocks % 4 == 0 && kGridWgBlocks <= 32, "node blocks in whole wavefronts, flags in one ballot");
+    constexpr int J = kGridWgBlocks / 4;  // node blocks per wavefront
     // The flags of both accumulator sets go out first, with the instance state
     // (which of the two is the trial's is only known from it), and a batch of
     // one instance needs no block-to-instance lookup: the sums wait for one
-    // memory round trip instead of three.  Every wavefront reads all 32 flags of
-    // the workgroup (lane i < 16: set 0 of block i, 16 <= i < 32: set 1), so
-    // each knows whether any block of the workgroup holds a deposit.
-    const int fb = blockIdx.x * kGridWgBlocks + (l & 15);
-    const int myflag = l < 16 ? B.gflag[0][fb] : (l < 32 ? B.gflag[1][fb] : 0);
+    // memory round trip instead of three.  Every wavefront reads both flags of
+    // every block of the workgroup (lane i < G = kGridWgBlocks: set 0 of block
+    // i, G <= i < 2G: set 1), so each knows whether any block of the workgroup
+    // holds a deposit.
+    const int fb = blockIdx.x * kGridWgBlocks + l % kGridWgBlocks;
+    const int myflag = l < kGridWgBlocks ? B.gflag[0][fb] : (l < 2 * kGridWgBlocks ? B.gflag[1][fb] : 0);
     const unsigned long long fmask = __ballot(myflag != 0);
     int fl0[J], fl1[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         fl0[j] = (int)((fmask >> (4 * j + wave)) & 1ull);
-        fl1[j] = (int)((fmask >> (16 + 4 * j + wave)) & 1ull);
+        fl1[j] = (int)((fmask >> (kGridWgBlocks + 4 * j + wave)) & 1ull);
     }
     const int inst = B.ninst == 1 ? 0 : B.blk_inst[blockIdx.x * kGridWgBlocks];
     const InstState& s = B.is[inst];
@@ -799,8 +803,8 @@ __global__ __launch_bounds__(256, 4) void grid_update_kernel(BatchDev B) {
     // mask): zeros, without the reduction and its barrier.  Otherwise each
     // wavefront sums by DPP into lane 63, and thread i combines value i over the
     // wavefronts in order.
-    const unsigned long long amask = s.gcur ? fmask >> 16 : fmask;
-    if ((amask & 0xffffull) == 0ull) {
+    const unsigned long long amask = s.gcur ? fmask >> kGridWgBlocks : fmask;
+    if ((amask & ((1ull << kGridWgBlocks) - 1ull)) == 0ull) {
         if (threadIdx.x < kGridVals) B.gpart[(size_t)threadIdx.x * B.ngwg + blockIdx.x] = 0.0;
         if (threadIdx.x == kGridVals) B.gbad[blockIdx.x] = 0;
         if (AUDIT && threadIdx.x < kAuditVals) B.apart[(size_t)threadIdx.x * B.ngwg + blockIdx.x] = 0.0;
