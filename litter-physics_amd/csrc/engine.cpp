@@ -37,10 +37,22 @@ using mpm_host::qmatrix;
 
 constexpr int kMaxStepsPerSync = 256;
 constexpr int kStampsPerStep = 4 * kStampSlots;  // kernel timing stamps of one trial step
-// The host re-sorts for locality every 64 trial steps.  With the onesweep sort
-// (about 450 us at 1M particles) 64 measured 3 % faster than 32 on the paste
-// column and the dam break; 96 gained under 1 % more.
-constexpr int kLocalitySortInterval = 64;
+// Locality sorts.  Between sorts the particles drift out of their work items'
+// cell blocks and the items' tiles grow, so the trial steps slow down, at a
+// rate that depends on the flow.  The host re-sorts when the time the rounds
+// since the last sort lost to that drift -- each round's time per step above
+// the fastest round's since the sort, times its steps -- has reached what a
+// sort costs.  For a loss that grows linearly with the steps since the sort
+// this is the interval that minimises the total, sqrt(2 C / slope).  The cost
+// C is twice the host time of the last periodic sort (the full sort after the
+// host set the state is not representative: it sorts by cell and, the first
+// time, loads the sort's code; until a periodic sort has been timed,
+// kSortCostUs0 + kSortCostUsPerParticle per particle slot).  Twice: sorting
+// somewhat later than at the measured break-even was faster on the paste
+// column, the dam break and the pellet bed.  Sorts stay kSortMinInterval to
+// kSortMaxInterval trial steps apart.
+constexpr int kSortMinInterval = 32, kSortMaxInterval = 1024;
+constexpr double kSortCostUs0 = 300.0, kSortCostUsPerParticle = 2.5e-4;
 
 // ------------------------------------------------------------ f64 host math
 
@@ -378,6 +390,11 @@ struct mpm_engine {
     int4* h_work = nullptr;
     std::vector<int4> work_build, work_sorted;  // the sort's work-item lists, kept between sorts
     std::vector<int> work_hist;
+    // Locality-sort policy (kept across runs): trial steps since the last sort,
+    // that sort's host time, the fastest round since it (per trial step; 0: no
+    // round yet) and the time the rounds since it lost against that pace.
+    int since_sort = 0;
+    double sort_cost_us = 0.0, best_step_us = 0.0, drift_us = 0.0;  // sort_cost_us 0: not timed yet
     void* sort_temp = nullptr;
     size_t sort_bytes = 0;
     int cell_bits = 0, end_bit = 0;
@@ -724,6 +741,8 @@ struct mpm_engine {
     // of the reference order travel with the particles.  The mirror must be
     // current; it is current again on return.
     int sort_all() {
+        const auto t_sort = std::chrono::steady_clock::now();
+        const bool full = dirty;
         if (int rc = flush_refkeys(false)) return rc;
         BatchDev B = dev();
         launch_sort_keys(stream, B, cell_bits, end_bit, keys[0], order[0]);
@@ -870,6 +889,11 @@ struct mpm_engine {
         if (int rc = pull()) return rc;
         dirty = false;
         graph_stale = true;
+        if (!full)
+            sort_cost_us =
+                2.0 * std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t_sort).count();
+        since_sort = 0;
+        best_step_us = drift_us = 0.0;
         return MPM_OK;
     }
 
@@ -1130,7 +1154,7 @@ struct mpm_engine {
             if (int rc_ = sync()) return rc_;
         }
         std::vector<char> recorded(ninst, 0);
-        int since_sort = 0, rounds = 0;
+        int rounds = 0;
         bool first_error = true;
         for (int i = 0; i < ninst; ++i)
             if (h_is[i].mode != M_RUN) recorded[i] = 1;
@@ -1145,11 +1169,16 @@ struct mpm_engine {
             if (!running) break;
             // No launches past the last budgeted step (they would find nothing to do).
             const int k = std::max(1, std::min(k_sync, most));
+            const auto t_round = std::chrono::steady_clock::now();
             if (int rc = enqueue(k)) return rc;
             // One host round trip per round: the stamps and the mirror together.
             if (int rc = fetch_profile(k)) return rc;
             if (int rc = pull()) return rc;
             if (int rc = collect_profile(k)) return rc;
+            const double step_us =
+                std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t_round).count() / k;
+            if (best_step_us <= 0.0 || step_us < best_step_us) best_step_us = step_us;
+            drift_us += (step_us - best_step_us) * k;
             since_sort += k;
             rounds += 1;
             double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1187,11 +1216,12 @@ struct mpm_engine {
                                      ? (double)(s.commit_clock - start_clock) / clock_khz
                                      : ms;
             }
-            if (dirty || since_sort >= kLocalitySortInterval) {
+            const double sort_cost =
+                sort_cost_us > 0.0 ? sort_cost_us : kSortCostUs0 + kSortCostUsPerParticle * (double)cap;
+            if (dirty || since_sort >= kSortMaxInterval || (since_sort >= kSortMinInterval && drift_us >= sort_cost)) {
                 if (changed)
                     if (int rc = push()) return rc;
                 if (int rc = sort_all()) return rc;
-                since_sort = 0;
             } else if (changed) {
                 if (int rc = push()) return rc;
             }

@@ -33,8 +33,13 @@ namespace {
 
 constexpr int kChunk = 1024;
 // Nodes of a work item's LDS tile: the stencils of two z-adjacent 4^3-cell
-// blocks span 7 x 7 x 11 nodes, with room for the particles' drift between sorts.
-constexpr int kTileCap = 768;
+// blocks span 7 x 7 x 11 nodes, with room for the particles' drift between
+// sorts.  An item whose box outgrows it deposits and gathers through global
+// memory at several times the cost, so the cap sets how far the particles may
+// drift before that cliff: 1280 nodes (40 KB of deposit tile, four 256-thread
+// deposit workgroups per CU) against 768 moved the cliff past where the sort
+// policy re-sorts in a fast flow.
+constexpr int kTileCap = 1280;
 constexpr double kDtRecovery = 1.25;              // solver.rs:45
 constexpr unsigned long long kSortInterval = 32;  // solver.rs:39
 
