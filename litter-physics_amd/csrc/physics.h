@@ -134,6 +134,14 @@ __device__ inline int eigen_sym3(float* a, float* v) {
     return off <= 1e-8f * scale ? F_OK : F_EIGEN;
 }
 
+// (x / K)^(1/n) of the Herschel-Bulkley law for x >= 0, by the hardware base-2
+// logarithm and exponential (about 1 ulp each; the library powf is some
+// thirty instructions more per Newton step).  0 for x = 0 and for subnormal x,
+// whose power is below the return's tolerance.
+__device__ inline float hb_pow(float x, float inv_n) {
+    return x > 0.0f ? __builtin_amdgcn_exp2f(inv_n * __builtin_amdgcn_logf(x)) : 0.0f;
+}
+
 // Herschel-Bulkley implicit return (constitutive.rs:280-362), solved for the
 // overstress d = q - sigma_y on [0, q_tr - sigma_y] with the same bracketed Newton;
 // tolerance 2.5e-7 of the trial stress (a few f32 ulps).
@@ -156,7 +164,7 @@ __device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, 
     float lo = 0.0f, hi = avail, d = 0.5f * avail;
     float tol = 2.5e-7f * fmaxf(qtr, 1.0f);
     // pw = (d/K)^(1/n); its derivative in d is (1/n) pw / d, so one power per iteration.
-    float pw = powf(d * inv_k, P.inv_n);
+    float pw = hb_pow(d * inv_k, P.inv_n);
     float r = d - avail + vs * pw;
     for (int it = 0; it < 100; ++it) {
         if (fabsf(r) <= tol || (hi - lo) <= tol) {
@@ -171,7 +179,7 @@ __device__ inline int hb_return(const Params& P, float dt, float qtr, float& q, 
         float deriv = d > 0.0f ? 1.0f + vs * P.inv_n * pw * __builtin_amdgcn_rcpf(d) : INFINITY;
         float nd = d - r * __builtin_amdgcn_rcpf(deriv);
         d = (isfinite(nd) && nd > lo && nd < hi) ? nd : 0.5f * (lo + hi);
-        pw = powf(d * inv_k, P.inv_n);
+        pw = hb_pow(d * inv_k, P.inv_n);
         r = d - avail + vs * pw;
         if (!isfinite(r)) return F_NONFINITE;
     }
@@ -424,13 +432,13 @@ __device__ inline float surrogate_polish(const Params& P, float dt, float qtr, f
     const float s3 = 3.0f * P.G * dt, inv_k = 1.0f / P.K;
     const float avail = qtr - P.ys;
     float d = avail * fminf(fmaxf(y, 0.0f), 1.0f);
-    float pw = powf(d * inv_k, P.inv_n);
+    float pw = hb_pow(d * inv_k, P.inv_n);
     for (int it = 0; it < steps; ++it) {
         const float r = d - avail + s3 * pw;
         const float deriv = d > 0.0f ? 1.0f + s3 * P.inv_n * pw / d : INFINITY;
         const float nd = d - r / deriv;
         d = (isfinite(nd) && nd > 0.0f && nd < avail) ? nd : d;
-        pw = powf(d * inv_k, P.inv_n);
+        pw = hb_pow(d * inv_k, P.inv_n);
     }
     q = P.ys + d;
     return fabsf(d - avail + s3 * pw) / fmaxf(qtr, 1.0f);
