@@ -134,6 +134,11 @@ __device__ inline int eigen_sym3(float* a, float* v) {
     return off <= 1e-8f * scale ? F_OK : F_EIGEN;
 }
 
+// Square root by the hardware instruction (within about 1 ulp; the library's
+// correctly rounded sqrtf is a dozen instructions more), for norms whose last
+// bit does not matter: the stress norm that sets the trial stress, speeds.
+__device__ inline float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // (x / K)^(1/n) of the Herschel-Bulkley law for x >= 0, by the hardware base-2
 // logarithm and exponential (about 1 ulp each; the library powf is some
 // thirty instructions more per Newton step).  0 for x = 0 and for subnormal x,
@@ -279,7 +284,7 @@ __device__ inline int paste_trial(const Params& P, const float* Ht, PasteTrial& 
     s.D[3] = e[3];
     s.D[4] = e[4];
     s.D[5] = e[5];
-    s.dn = sqrtf(sym_norm2(s.D));
+    s.dn = hw_sqrt(sym_norm2(s.D));
     s.qtr = 1.2247448713915890f * 2.0f * P.G * s.dn;
     return F_OK;
 }
@@ -478,7 +483,7 @@ __device__ inline bool near_wall(const Params& P, float x, float y, float z) {
 
 __device__ inline void reduce_tangential(float* v, float reduction) {
     if (reduction == 0.0f) return;
-    float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    float speed = hw_sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (speed <= reduction) {
         v[0] = v[1] = v[2] = 0.0f;
     } else {
@@ -821,7 +826,7 @@ __device__ inline void particle_bounds(const Params& P, float m, float v0, const
 // |v| as the stability limit takes it (no contraction, as particle_bounds).
 __device__ inline float particle_speed(const float* v) {
 #pragma clang fp contract(off)
-    return sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    return hw_sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
 }
 
 // 2^k with kMaxWeight-weighted contributions of a particle within `bound`
