@@ -449,6 +449,42 @@ __device__ inline float surrogate_polish(const Params& P, float dt, float qtr, f
     return fabsf(d - avail + s3 * pw) / fmaxf(qtr, 1.0f);
 }
 
+// log1p and expm1 of the Tait pressure by their Taylor series (Horner) where the
+// argument is small, the common case of a weakly compressible liquid (|J - 1|
+// of a percent or less); the library functions beyond.  The first omitted
+// terms are below 2^-40 |x| (log1p, |x| <= 1/16, ten terms) and 1e-13 |y|
+// (expm1, |y| <= 1/2, twelve terms).
+__device__ inline float tait_log1p(float x) {
+    if (!(fabsf(x) <= 0.0625f)) return log1pf(x);
+    float t = -1.0f / 10.0f;
+    t = fmaf(t, x, 1.0f / 9.0f);
+    t = fmaf(t, x, -1.0f / 8.0f);
+    t = fmaf(t, x, 1.0f / 7.0f);
+    t = fmaf(t, x, -1.0f / 6.0f);
+    t = fmaf(t, x, 1.0f / 5.0f);
+    t = fmaf(t, x, -1.0f / 4.0f);
+    t = fmaf(t, x, 1.0f / 3.0f);
+    t = fmaf(t, x, -1.0f / 2.0f);
+    t = fmaf(t, x, 1.0f);
+    return t * x;
+}
+__device__ inline float tait_expm1(float y) {
+    if (!(fabsf(y) <= 0.5f)) return expm1f(y);
+    float t = 1.0f / 479001600.0f;  // 1/12!
+    t = fmaf(t, y, 1.0f / 39916800.0f);
+    t = fmaf(t, y, 1.0f / 3628800.0f);
+    t = fmaf(t, y, 1.0f / 362880.0f);
+    t = fmaf(t, y, 1.0f / 40320.0f);
+    t = fmaf(t, y, 1.0f / 5040.0f);
+    t = fmaf(t, y, 1.0f / 720.0f);
+    t = fmaf(t, y, 1.0f / 120.0f);
+    t = fmaf(t, y, 1.0f / 24.0f);
+    t = fmaf(t, y, 1.0f / 6.0f);
+    t = fmaf(t, y, 1.0f / 2.0f);
+    t = fmaf(t, y, 1.0f);
+    return t * y;
+}
+
 // update_liquid (constitutive.rs:557-589) on J - 1, with the Tait pressure
 // p = (K/g) expm1(-g log1p(J - 1)); g = 1 gives the reference's K (1/J - 1).
 __device__ inline int liquid_update(const Params& P, float dt, float jm1, const float* C, float& jn, float* tau) {
@@ -458,7 +494,7 @@ __device__ inline int liquid_update(const Params& P, float dt, float jm1, const 
     float J = 1.0f + jn;
     if (!isfinite(jn)) return F_NONFINITE;
     if (J <= 1e-6f) return F_INVERTED;
-    float p = P.gamma == 1.0f ? -P.bulk * jn / J : (P.bulk / P.gamma) * expm1f(-P.gamma * log1pf(jn));
+    float p = P.gamma == 1.0f ? -P.bulk * jn / J : (P.bulk / P.gamma) * tait_expm1(-P.gamma * tait_log1p(jn));
     float c2 = 2.0f * P.visc * J;
     float m3 = tr / 3.0f;
     float pj = -p * J;
