@@ -1231,15 +1231,31 @@ __global__ __launch_bounds__(kBlock, WGS) void g2p_kernel(BatchDev B) {
     G2pIn cur;
     if (!MLP) g2p_load(src, shared, cap, P.kind, chunk0 + tid, chunk_end, cur);
 
-    // Phase A: the gather tile.
+    // Phase A: the gather tile, four nodes per thread at a time with all their
+    // loads issued before the first LDS store (one memory round trip per four).
     if (t.tiled) {
         int gsz = t.gd[0] * t.gd[1] * t.gd[2];
         const float ig2 = 1.0f / (float)t.gd[2], ig1 = 1.0f / (float)t.gd[1];
-        for (int q = tid; q < gsz; q += kBlock) {
-            const int rest = div_small(q, ig2), k = q - rest * t.gd[2];
-            const int i = div_small(rest, ig1), j = rest - i * t.gd[1];
-            int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
-            sg[q] = make_float4(gv[ni], gv[na + ni], gv[2 * na + ni], 0.0f);
+        for (int q0 = tid; q0 < gsz; q0 += 4 * kBlock) {
+            float gx[4], gy[4], gz[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + u * kBlock;
+                gx[u] = gy[u] = gz[u] = 0.0f;
+                if (q < gsz) {
+                    const int rest = div_small(q, ig2), k = q - rest * t.gd[2];
+                    const int i = div_small(rest, ig1), j = rest - i * t.gd[1];
+                    const int ni = node_index(P, t.org[0] + i, t.org[1] + j, t.org[2] + k);
+                    gx[u] = gv[ni];
+                    gy[u] = gv[na + ni];
+                    gz[u] = gv[2 * na + ni];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + u * kBlock;
+                if (q < gsz) sg[q] = make_float4(gx[u], gy[u], gz[u], 0.0f);
+            }
         }
     }
     __syncthreads();
