@@ -353,9 +353,15 @@ __global__ __launch_bounds__(BS) void p2g_kernel(BatchDev B) {
         }
         t.tiled = ok && sz <= kTileCap;
     }
-    if (t.tiled)
-        for (int q = tid; q < NF * kTileCap / 2; q += BS)
-            reinterpret_cast<ulonglong2*>(ss)[q] = make_ulonglong2(0ull, 0ull);
+    // Only the tile's own nodes are cleared (the flush reads no others), two
+    // 64-bit words per store.
+    if (t.tiled) {
+        const int half = (t.gd[0] * t.gd[1] * t.gd[2] + 1) >> 1;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+            for (int q = tid; q < half; q += BS)
+                reinterpret_cast<ulonglong2*>(ss + f * kTileCap)[q] = make_ulonglong2(0ull, 0ull);
+    }
     __syncthreads();
     const int rot = (lane * 5) % 9, ra = rot / 3, rb = rot - 3 * ra;
     const int s12 = __mul24(t.gd[1], t.gd[2]);
