@@ -393,7 +393,7 @@ struct mpm_engine {
     // Locality-sort policy (kept across runs): trial steps since the last sort,
     // that sort's host time, the fastest round since it (per trial step; 0: no
     // round yet) and the time the rounds since it lost against that pace.
-    int since_sort = 0;
+    int since_sort = 0, sorts_since_full = 0;
     double sort_cost_us = 0.0, best_step_us = 0.0, drift_us = 0.0;  // sort_cost_us 0: not timed yet
     void* sort_temp = nullptr;
     size_t sort_bytes = 0;
@@ -750,8 +750,13 @@ struct mpm_engine {
         // A state set from the host is sorted by cell; the periodic locality
         // sorts order by cell block only (two radix passes at 1M particles in one
         // instance instead of three): within a block the stable order of the
-        // previous sort stays, which the particles' drift barely disturbs.
-        const int begin_bit = dirty ? 0 : 6;
+        // previous sort stays.  The drift does disturb it over a long run (the
+        // deposit's LDS atomics of particles of one cell then no longer go to
+        // lanes side by side), so every fourth periodic sort orders by cell again:
+        // 188.5 against 191.5 us per step over 4000 steps of the 1M-particle
+        // column, the same over its first 500.
+        const int begin_bit = (dirty || ++sorts_since_full >= 4) ? 0 : 6;
+        if (begin_bit == 0) sorts_since_full = 0;
         HIP_OK(launch_radix_sort(stream, sort_temp, sort_bytes, keys[0], keys[1], order[0], order[1], (int)cap,
                                  begin_bit, end_bit));
         launch_permute(stream, B, order[1], shared[sh], shared[sh ^ 1], ids[sh], ids[sh ^ 1], refkey[sh],
