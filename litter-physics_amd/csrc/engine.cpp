@@ -1489,7 +1489,11 @@ int mpm_engine::set_state(int i, const mpm_particles* ps, const mpm_pellet* pell
         }
         for (int a = 0; a < 4; ++a) d.q[a] = src.orientation[a];
         d.spheres = std::min(std::max(src.sphere_count, 0), kMaxSpheres);
-        for (int s = 0; s < d.spheres; ++s) d.offsets[s] = src.sphere_offsets[s];
+        d.half = 0.0;
+        for (int s = 0; s < d.spheres; ++s) {
+            d.offsets[s] = src.sphere_offsets[s];
+            d.half = std::max(d.half, std::fabs(d.offsets[s]));
+        }
         contact_dt = std::min(contact_dt, 0.2 * std::sqrt(d.mass / in.cfg.contact_stiffness));
     }
     if (in.pel_cap) {

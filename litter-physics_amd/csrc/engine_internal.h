@@ -205,6 +205,7 @@ struct PelletState {
     double pos[3], q[4], vel[3], L[3];
     int inst, spheres;
     double offsets[kMaxSpheres];
+    double half;  // largest |offset|: the pellet's half-extent along its axis, less the radius
 };
 
 // Start-of-step geometry of a pellet for the coupling kernel.

@@ -147,9 +147,7 @@ __device__ inline PelletGeo pellet_geometry(const BatchDev& B, int k, const Pell
     qrot(q, xh, g.axis);
     pellet_omega(q, L, p.inertia, g.omega);
     const int spheres = p.spheres;
-    const double radius = p.radius;
-    double half = 0.0;
-    for (int i = 0; i < spheres; ++i) half = fmax(half, fabs(p.offsets[i]));
+    const double radius = p.radius, half = p.half;  // half: set with the pellet on the host
     for (int a = 0; a < 3; ++a) {
         g.pos[a] = pos[a];
         g.vel[a] = vel[a];
