@@ -194,7 +194,11 @@ constexpr int kGridChunks = 64;
 // Per-pellet coupling accumulators (rigid.rs:350-363): impulse (3), angular
 // impulse (3) and grid energy, exact fixed point; constrained node count.
 struct PelletAccum {
-    Fix128 sum[7];
+    // Fix128 sums in three 64-bit limbs each (bits 0-31 and 32-63 of the low
+    // word, the high word modulo 2^64, as ItemSlot::diss): added with
+    // non-returning atomics, so a block need not wait for a carry's round trip
+    // before its next pellet.
+    unsigned long long sum[7][3];
     unsigned long long constrained;
 };
 

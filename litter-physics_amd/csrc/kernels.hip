@@ -137,6 +137,11 @@ __device__ inline uint64_t ref_cell(const InstParams& ip, float x, float y, floa
 
 // ------------------------------------------------------------ control
 
+// The double nearest coupling sum i of a pellet.
+__device__ inline double pel_sum(const PelletAccum& a, int i) {
+    return fix_value(fix_from_limbs(a.sum[i][0], a.sum[i][1], a.sum[i][2]));
+}
+
 // A pellet's start-of-step geometry for the coupling kernel (its axis, angular
 // velocity and box) from its committed pose and momenta (pos, vel, L, q) and
 // its constant shape in `p`; coupling sums cleared.
@@ -980,10 +985,13 @@ __global__ __launch_bounds__(64) void couple_kernel(BatchDev B) {
                     for (int i = 0; i < 7; ++i) sums[i] = wave_sum_last(sums[i]);
                     if (l == 63) {
                         PelletAccum* acc = B.pacc + pi;
-                        Fix128 add[7];
 #pragma unroll
-                        for (int i = 0; i < 7; ++i) add[i] = fix_from(sums[i]);
-                        fix_atomic_add_n<7>(acc->sum, add);
+                        for (int i = 0; i < 7; ++i) {
+                            const Fix128 add = fix_from(sums[i]);
+                            atomicAdd(&acc->sum[i][0], add.lo & 0xffffffffull);
+                            atomicAdd(&acc->sum[i][1], add.lo >> 32);
+                            atomicAdd(&acc->sum[i][2], (unsigned long long)add.hi);
+                        }
                         atomicAdd(&acc->constrained, (unsigned long long)__popcll(cm));
                     }
                 }
@@ -1018,8 +1026,8 @@ __global__ void pellet_kernel(BatchDev B) {
     for (int a = 0; a < 4; ++a) q[a] = p.q[a];
     double before = pellet_kinetic(p.mass, vel, q, L, p.inertia);
     for (int a = 0; a < 3; ++a) {
-        vel[a] = vel[a] + fix_value(acc.sum[a]) / p.mass;
-        L[a] = L[a] + fix_value(acc.sum[3 + a]);
+        vel[a] = vel[a] + pel_sum(acc, a) / p.mass;
+        L[a] = L[a] + pel_sum(acc, 3 + a);
     }
     t.energy = pellet_kinetic(p.mass, vel, q, L, p.inertia) - before;
     // rigid.rs:279-334
@@ -1806,11 +1814,11 @@ __global__ __launch_bounds__(256) void finalize_kernel(BatchDev B) {
             pel[0] += B.pel[pi].mass;
             for (int c = 0; c < 3; ++c) {
                 pel[1 + c] += t.wall_impulse[c];
-                pel[4 + c] += fix_value(a.sum[c]);
-                pel[7 + c] += fix_value(a.sum[3 + c]);
+                pel[4 + c] += pel_sum(a, c);
+                pel[7 + c] += pel_sum(a, 3 + c);
             }
             pel[10] += t.wall_work;
-            pel[11] += fix_value(a.sum[6]);
+            pel[11] += pel_sum(a, 6);
             pel[12] += t.energy;
             pbad |= t.bad;
             vmax = fmax(vmax, sqrt(t.vel[0] * t.vel[0] + t.vel[1] * t.vel[1] + t.vel[2] * t.vel[2]));

@@ -797,22 +797,6 @@ __device__ inline double fix_value(const Fix128& a) {
     return neg ? -v : v;
 }
 
-// N 128-bit atomic adds, each from two 64-bit ones: the low word's wrap-arounds
-// reach the high word as carries, each counted once, so the totals are exact in
-// any order.  The low words' atomics are all issued before any carry is needed,
-// so the N additions take one round trip for the low words and one for the high.
-template <int N>
-__device__ inline void fix_atomic_add_n(Fix128* dst, const Fix128* b) {
-    unsigned long long old[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) old[i] = atomicAdd(&dst[i].lo, b[i].lo);
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const unsigned long long carry = old[i] + b[i].lo < old[i] ? 1ull : 0ull;
-        atomicAdd(reinterpret_cast<unsigned long long*>(&dst[i].hi), (unsigned long long)b[i].hi + carry);
-    }
-}
-
 // Sum over the wavefront (every lane gets it).
 __device__ inline Fix128 wave_fix_sum(Fix128 v) {
 #pragma unroll
